@@ -1,0 +1,730 @@
+// libtdp.so: the device layer of the join path behind include/tdp.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/tdp.h"
+#include "tdp_kernels.hpp"
+
+using namespace tdp;
+
+struct tdp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // grow-only scratch for scans
+    std::vector<void *> scan_buf;
+    std::vector<size_t> scan_cap;
+    uint64_t *d_total = nullptr;
+};
+
+struct tdp_tile {
+    tdp_ctx *ctx = nullptr;
+    size_t n_obj = 0, n_vox = 0, n_tri = 0;
+    uint32_t *obj_vox_off = nullptr;
+    float *obj_hd = nullptr;
+    float *vox_box = nullptr;
+    uint32_t *vox_obj = nullptr;
+    uint64_t *vox_tri_off = nullptr;
+    uint32_t *vox_tri_cnt = nullptr;
+    float4 *tri = nullptr;
+    size_t bytes = 0;
+};
+
+struct tdp_join {
+    tdp_ctx *ctx = nullptr;
+    const tdp_tile *t1 = nullptr, *t2 = nullptr;
+    tdp_join_params prm{};
+    size_t N = 0, P = 0;
+    size_t rsz = 4;  // sizeof range value (4 REF32, 8 otherwise)
+    uint32_t *prefix = nullptr, *pair_obj2 = nullptr, *pair_target = nullptr;
+    void *rmin = nullptr, *rmax = nullptr;
+    uint8_t *state = nullptr, *round = nullptr, *scratch = nullptr;
+    int32_t *confirmed = nullptr;
+    void *acc_min = nullptr, *acc_max = nullptr;
+    uint32_t *count = nullptr;
+    uint64_t *offs = nullptr;
+    // voxel-pair lists, double-buffered for compaction
+    uint32_t *v1[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr}, *vpp[2] = {nullptr, nullptr};
+    void *vmin[2] = {nullptr, nullptr}, *vmax[2] = {nullptr, nullptr};
+    uint32_t *keep = nullptr;
+    uint64_t *pos = nullptr;
+    size_t cap_vp = 0;
+    int cur = 0;
+    uint64_t n_vp = 0;
+    int rounds = 0;
+    bool filtered = false;
+    unsigned long long *stats = nullptr;
+    uint64_t filter_vp = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double refine_ms = 0, filter_ms = 0;
+};
+
+// ------------------------------------------------------------ helpers
+
+#define TDP_CHECK(ctx, call)                                                         \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return e_ == hipErrorOutOfMemory ? TDP_ERR_NOMEM : TDP_ERR_HIP;          \
+        }                                                                            \
+    } while (0)
+
+#define TDP_LAUNCHED(ctx)                                                            \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess) {                                                      \
+            (ctx)->err = std::string("kernel launch: ") + hipGetErrorString(e_);     \
+            return TDP_ERR_HIP;                                                      \
+        }                                                                            \
+    } while (0)
+
+static inline unsigned blocks_for(size_t n, size_t per_block) {
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+template <typename T>
+static int dalloc(tdp_ctx *ctx, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    TDP_CHECK(ctx, hipMalloc((void **)p, count * sizeof(T)));
+    return TDP_OK;
+}
+static void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+// exclusive scan of n counters (u32 or u64) into out (u64); *total set on host
+template <typename In>
+static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, uint64_t *total) {
+    if (n == 0) {
+        *total = 0;
+        return TDP_OK;
+    }
+    // level sizes
+    std::vector<size_t> sizes;
+    size_t m = n;
+    while (true) {
+        size_t b = (m + 1023) / 1024;
+        sizes.push_back(b);
+        if (b == 1) break;
+        m = b;
+    }
+    // block-sum buffers: for each level a sums array and an offsets array
+    if (ctx->scan_buf.size() < 2 * sizes.size()) {
+        ctx->scan_buf.resize(2 * sizes.size(), nullptr);
+        ctx->scan_cap.resize(2 * sizes.size(), 0);
+    }
+    for (size_t l = 0; l < sizes.size(); l++)
+        for (int k = 0; k < 2; k++) {
+            size_t idx = 2 * l + k;
+            if (ctx->scan_cap[idx] < sizes[l]) {
+                dfree(ctx->scan_buf[idx]);
+                ctx->scan_buf[idx] = nullptr;
+                TDP_CHECK(ctx, hipMalloc(&ctx->scan_buf[idx], sizes[l] * sizeof(uint64_t)));
+                ctx->scan_cap[idx] = sizes[l];
+            }
+        }
+    if (!ctx->d_total) TDP_CHECK(ctx, hipMalloc((void **)&ctx->d_total, sizeof(uint64_t)));
+    // level 0
+    uint64_t *sums0 = (uint64_t *)ctx->scan_buf[0];
+    k_scan_local<In><<<(unsigned)sizes[0], kBlock, 0, ctx->stream>>>(in, n, out, sums0);
+    TDP_LAUNCHED(ctx);
+    // upper levels: scan the block sums in place into offsets
+    for (size_t l = 0; l + 1 < sizes.size(); l++) {
+        uint64_t *sums = (uint64_t *)ctx->scan_buf[2 * l];
+        uint64_t *offs = (uint64_t *)ctx->scan_buf[2 * l + 1];
+        uint64_t *nsums = (uint64_t *)ctx->scan_buf[2 * (l + 1)];
+        k_scan_local<uint64_t><<<(unsigned)sizes[l + 1], kBlock, 0, ctx->stream>>>(sums, sizes[l],
+                                                                                   offs, nsums);
+        TDP_LAUNCHED(ctx);
+    }
+    // top level has one block: its offsets are the scan itself; propagate down
+    for (size_t l = sizes.size() - 1; l-- > 0;) {
+        uint64_t *offs = (uint64_t *)ctx->scan_buf[2 * l + 1];
+        uint64_t *upper = (uint64_t *)ctx->scan_buf[2 * (l + 1) + 1];
+        if (l + 1 < sizes.size() - 1)
+            k_scan_add<<<blocks_for(sizes[l], kBlock), kBlock, 0, ctx->stream>>>(offs, sizes[l], upper);
+        TDP_LAUNCHED(ctx);
+    }
+    if (sizes.size() > 1) {
+        uint64_t *offs0 = (uint64_t *)ctx->scan_buf[1];
+        k_scan_add<<<blocks_for(n, kBlock), kBlock, 0, ctx->stream>>>(out, n, offs0);
+        TDP_LAUNCHED(ctx);
+    }
+    k_scan_total<In><<<1, 1, 0, ctx->stream>>>(in, out, n, ctx->d_total);
+    TDP_LAUNCHED(ctx);
+    TDP_CHECK(ctx, hipMemcpyAsync(total, ctx->d_total, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    TDP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TDP_OK;
+}
+
+// ------------------------------------------------------------ context
+
+extern "C" int tdp_open(int device, tdp_ctx **out) {
+    if (!out) return TDP_ERR_ARG;
+    *out = nullptr;
+    tdp_ctx *ctx = new tdp_ctx();
+    ctx->device = device;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= device || device < 0) {
+        ctx->err = std::string("no HIP device ") + std::to_string(device) +
+                   (e != hipSuccess ? std::string(": ") + hipGetErrorString(e) : std::string());
+        *out = ctx;  // returned so the caller can read the message
+        return TDP_ERR_HIP;
+    }
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+    if (e != hipSuccess) {
+        ctx->err = std::string("hip init: ") + hipGetErrorString(e);
+        *out = ctx;
+        return TDP_ERR_HIP;
+    }
+    *out = ctx;
+    return TDP_OK;
+}
+
+extern "C" int tdp_close(tdp_ctx *ctx) {
+    if (!ctx) return TDP_ERR_ARG;
+    for (void *p : ctx->scan_buf) dfree(p);
+    dfree(ctx->d_total);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TDP_OK;
+}
+
+extern "C" const char *tdp_last_error(const tdp_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int tdp_device_info(tdp_ctx *ctx, char *name, size_t name_len, int *cu_count) {
+    if (!ctx) return TDP_ERR_ARG;
+    hipDeviceProp_t prop;
+    TDP_CHECK(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    return TDP_OK;
+}
+
+// ------------------------------------------------------------ tiles
+
+extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **out) {
+    if (!ctx || !d || !out) return TDP_ERR_ARG;
+    *out = nullptr;
+    if (!d->obj_vox_off || !d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt ||
+        (d->n_tri && !d->tri) || !d->obj_hd) {
+        ctx->err = "tdp_tile_upload: null array";
+        return TDP_ERR_ARG;
+    }
+    if (d->obj_vox_off[0] != 0 || d->obj_vox_off[d->n_obj] != d->n_vox) {
+        ctx->err = "tdp_tile_upload: obj_vox_off does not span the voxels";
+        return TDP_ERR_ARG;
+    }
+    for (size_t v = 0; v < 5 * d->n_vox; v++)
+        if (d->vox_tri_off[v] + d->vox_tri_cnt[v] > d->n_tri) {
+            ctx->err = "tdp_tile_upload: voxel triangles outside the record array";
+            return TDP_ERR_ARG;
+        }
+    tdp_tile *t = new tdp_tile();
+    t->ctx = ctx;
+    t->n_obj = d->n_obj;
+    t->n_vox = d->n_vox;
+    t->n_tri = d->n_tri;
+    std::vector<uint32_t> vobj(d->n_vox);
+    for (size_t o = 0; o < d->n_obj; o++)
+        for (uint32_t v = d->obj_vox_off[o]; v < d->obj_vox_off[o + 1]; v++) vobj[v] = (uint32_t)o;
+    int rc = TDP_OK;
+    auto up = [&](auto **dst, const auto *src, size_t count) {
+        if (rc != TDP_OK) return;
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
+        rc = dalloc(ctx, dst, count);
+        if (rc != TDP_OK) return;
+        t->bytes += count * sizeof(T);
+        if (count) {
+            hipError_t e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                ctx->err = std::string("tile upload: ") + hipGetErrorString(e);
+                rc = TDP_ERR_HIP;
+            }
+        }
+    };
+    up(&t->obj_vox_off, d->obj_vox_off, d->n_obj + 1);
+    up(&t->obj_hd, d->obj_hd, 10 * d->n_obj);
+    up(&t->vox_box, d->vox_box, 9 * d->n_vox);
+    up(&t->vox_obj, vobj.data(), d->n_vox);
+    up(&t->vox_tri_off, d->vox_tri_off, 5 * d->n_vox);
+    up(&t->vox_tri_cnt, d->vox_tri_cnt, 5 * d->n_vox);
+    up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
+    if (rc != TDP_OK) {
+        tdp_tile_free(t);
+        return rc;
+    }
+    *out = t;
+    return TDP_OK;
+}
+
+extern "C" int tdp_tile_free(tdp_tile *t) {
+    if (!t) return TDP_ERR_ARG;
+    dfree(t->obj_vox_off);
+    dfree(t->obj_hd);
+    dfree(t->vox_box);
+    dfree(t->vox_obj);
+    dfree(t->vox_tri_off);
+    dfree(t->vox_tri_cnt);
+    dfree(t->tri);
+    delete t;
+    return TDP_OK;
+}
+
+extern "C" size_t tdp_tile_bytes(const tdp_tile *t) { return t ? t->bytes : 0; }
+
+// ------------------------------------------------------------ batch ops
+
+extern "C" int tdp_tridist_batch(tdp_ctx *ctx, int precision, const float *S, const float *T,
+                                 size_t n, void *out) {
+    if (!ctx || (n && (!S || !T || !out))) return TDP_ERR_ARG;
+    if (precision < 0 || precision > 2) {
+        ctx->err = "tdp_tridist_batch: bad precision";
+        return TDP_ERR_ARG;
+    }
+    if (n == 0) return TDP_OK;
+    const size_t osz = precision == TDP_PREC_REF32 ? 4 : 8;
+    float *dS = nullptr, *dT = nullptr;
+    void *dO = nullptr;
+    int rc = dalloc(ctx, &dS, 9 * n);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dT, 9 * n);
+    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&dO, n * osz);
+    if (rc == TDP_OK) {
+        hipError_t e = hipMemcpy(dS, S, 9 * n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dT, T, 9 * n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            if (precision == TDP_PREC_REF32)
+                k_tridist_batch<float><<<blocks_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                    n, dS, dT, (float *)dO);
+            else
+                k_tridist_batch<double><<<blocks_for(n, kBlock), kBlock, 0, ctx->stream>>>(
+                    n, dS, dT, (double *)dO);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, dO, n * osz, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_tridist_batch: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    }
+    dfree(dS);
+    dfree(dT);
+    dfree(dO);
+    return rc;
+}
+
+extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri, size_t n_tri,
+                                  const uint64_t *os, size_t P, void *out) {
+    if (!ctx || (P && (!os || !out)) || (n_tri && !tri)) return TDP_ERR_ARG;
+    if (precision < 0 || precision > 2) {
+        ctx->err = "tdp_meshdist_batch: bad precision";
+        return TDP_ERR_ARG;
+    }
+    for (size_t p = 0; p < P; p++)
+        if (os[4 * p] + os[4 * p + 1] > n_tri || os[4 * p + 2] + os[4 * p + 3] > n_tri) {
+            ctx->err = "tdp_meshdist_batch: pair " + std::to_string(p) + " outside the records";
+            return TDP_ERR_ARG;
+        }
+    if (P == 0) return TDP_OK;
+    const size_t osz = precision == TDP_PREC_REF32 ? 4 : 8;
+    float4 *dR = nullptr;
+    uint64_t *dOS = nullptr;
+    void *dO = nullptr;
+    unsigned long long *dst = nullptr;
+    int rc = dalloc(ctx, &dR, 3 * n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dOS, 4 * P);
+    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&dO, 3 * P * osz);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dst, 4);
+    if (rc == TDP_OK) {
+        hipError_t e = hipMemcpy(dR, tri, 48 * n_tri, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dOS, os, 32 * P, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(dst, 0, 32);
+        if (e == hipSuccess) {
+            const unsigned g = blocks_for(P, kWavesPerBlock);
+            if (precision == TDP_PREC_REF32)
+                k_meshdist_batch<float, MODE_REF32><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, (float *)dO, dst);
+            else if (precision == TDP_PREC_FP64)
+                k_meshdist_batch<double, MODE_FP64><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, (double *)dO, dst);
+            else
+                k_meshdist_batch<double, MODE_MIXED><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, (double *)dO, dst);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, dO, 3 * P * osz, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_meshdist_batch: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    }
+    dfree(dR);
+    dfree(dOS);
+    dfree(dO);
+    dfree(dst);
+    return rc;
+}
+
+extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
+                              const int32_t *confirmed, const double *rmin, const double *rmax,
+                              int k, int32_t *status) {
+    if (!ctx || (N && (!prefix || !confirmed || !status))) return TDP_ERR_ARG;
+    if (N == 0) return TDP_OK;
+    const size_t M = prefix[N];
+    if (M && (!rmin || !rmax)) return TDP_ERR_ARG;
+    uint32_t *dp = nullptr;
+    int32_t *dc = nullptr, *ds = nullptr;
+    double *dmn = nullptr, *dmx = nullptr;
+    int rc = dalloc(ctx, &dp, N + 1);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dc, N);
+    if (rc == TDP_OK) rc = dalloc(ctx, &ds, M);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dmn, M);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dmx, M);
+    if (rc == TDP_OK) {
+        hipError_t e = hipMemcpy(dp, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dc, confirmed, 4 * N, hipMemcpyHostToDevice);
+        if (e == hipSuccess && M) e = hipMemcpy(dmn, rmin, 8 * M, hipMemcpyHostToDevice);
+        if (e == hipSuccess && M) e = hipMemcpy(dmx, rmax, 8 * M, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            k_knn_status<<<blocks_for(N, kBlock), kBlock, 0, ctx->stream>>>(N, dp, dc, dmn, dmx, k, ds);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess && M) e = hipMemcpy(status, ds, 4 * M, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_knn_status: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    }
+    dfree(dp); dfree(dc); dfree(ds); dfree(dmn); dfree(dmx);
+    return rc;
+}
+
+// ------------------------------------------------------------ join
+
+extern "C" int tdp_join_destroy(tdp_join *j) {
+    if (!j) return TDP_ERR_ARG;
+    void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->rmin, j->rmax, j->state,
+                    j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
+                    j->offs, j->keep, j->pos, j->stats};
+    for (void *p : ptrs) dfree(p);
+    for (int b = 0; b < 2; b++) {
+        dfree(j->v1[b]); dfree(j->v2[b]); dfree(j->vpp[b]); dfree(j->vmin[b]); dfree(j->vmax[b]);
+    }
+    for (auto &e : j->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete j;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                               const tdp_join_params *prm, size_t N, const uint32_t *prefix,
+                               const uint32_t *cand, tdp_join **out) {
+    if (!ctx || !t1 || !t2 || !prm || !out || (N && !prefix)) return TDP_ERR_ARG;
+    *out = nullptr;
+    if (prm->query != TDP_QUERY_KNN && prm->query != TDP_QUERY_WITHIN) {
+        ctx->err = "tdp_join_create: bad query";
+        return TDP_ERR_ARG;
+    }
+    if (prm->query == TDP_QUERY_KNN && prm->k < 1) {
+        ctx->err = "tdp_join_create: k must be >= 1";
+        return TDP_ERR_ARG;
+    }
+    if (prm->precision < 0 || prm->precision > 2 || prm->hausdorff_level < 0 ||
+        prm->hausdorff_level > 2) {
+        ctx->err = "tdp_join_create: bad precision or hausdorff level";
+        return TDP_ERR_ARG;
+    }
+    if (N > t1->n_obj) {
+        ctx->err = "tdp_join_create: more targets than tile-1 objects";
+        return TDP_ERR_ARG;
+    }
+    const size_t P = N ? prefix[N] : 0;
+    if (P > 0xffffffffull) {
+        ctx->err = "tdp_join_create: too many candidate pairs for 32-bit pair ids";
+        return TDP_ERR_ARG;
+    }
+    std::vector<uint32_t> tgt(P);
+    for (size_t t = 0; t < N; t++) {
+        if (prefix[t] > prefix[t + 1]) {
+            ctx->err = "tdp_join_create: cand_prefix not monotone";
+            return TDP_ERR_ARG;
+        }
+        for (uint32_t i = prefix[t]; i < prefix[t + 1]; i++) tgt[i] = (uint32_t)t;
+    }
+    for (size_t i = 0; i < P; i++)
+        if (cand[i] >= t2->n_obj) {
+            ctx->err = "tdp_join_create: candidate id outside tile 2";
+            return TDP_ERR_ARG;
+        }
+    tdp_join *j = new tdp_join();
+    j->ctx = ctx;
+    j->t1 = t1;
+    j->t2 = t2;
+    j->prm = *prm;
+    j->N = N;
+    j->P = P;
+    j->rsz = prm->precision == TDP_PREC_REF32 ? 4 : 8;
+    int rc = TDP_OK;
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->prefix, N + 1);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->pair_obj2, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->pair_target, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->rmin, P * j->rsz);
+    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->rmax, P * j->rsz);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->state, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->round, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->scratch, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->confirmed, N);
+    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->acc_min, P * j->rsz);
+    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->acc_max, P * j->rsz);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->count, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->offs, P);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->stats, 4);
+    if (rc != TDP_OK) {
+        tdp_join_destroy(j);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(j->prefix, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && P) e = hipMemcpy(j->pair_obj2, cand, 4 * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess && P) e = hipMemcpy(j->pair_target, tgt.data(), 4 * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(j->stats, 0, 32);
+    if (e == hipSuccess) e = hipMemset(j->confirmed, 0, 4 * (N ? N : 1));
+    if (e == hipSuccess) e = hipMemset(j->round, 0xff, P ? P : 1);
+    if (e == hipSuccess) e = hipMemset(j->state, 0, P ? P : 1);
+    for (auto &ev : j->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        ctx->err = std::string("tdp_join_create: ") + hipGetErrorString(e);
+        tdp_join_destroy(j);
+        return TDP_ERR_HIP;
+    }
+    *out = j;
+    return TDP_OK;
+}
+
+static int ensure_vp(tdp_join *j, size_t n) {
+    if (n <= j->cap_vp) return TDP_OK;
+    tdp_ctx *ctx = j->ctx;
+    size_t cap = std::max(n, (size_t)1024);
+    for (int b = 0; b < 2; b++) {
+        dfree(j->v1[b]); dfree(j->v2[b]); dfree(j->vpp[b]); dfree(j->vmin[b]); dfree(j->vmax[b]);
+        j->v1[b] = j->v2[b] = j->vpp[b] = nullptr;
+        j->vmin[b] = j->vmax[b] = nullptr;
+    }
+    dfree(j->keep); dfree(j->pos);
+    j->keep = nullptr; j->pos = nullptr;
+    j->cap_vp = 0;
+    int rc = TDP_OK;
+    for (int b = 0; b < 2 && rc == TDP_OK; b++) {
+        rc = dalloc(ctx, &j->v1[b], cap);
+        if (rc == TDP_OK) rc = dalloc(ctx, &j->v2[b], cap);
+        if (rc == TDP_OK) rc = dalloc(ctx, &j->vpp[b], cap);
+        if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->vmin[b], cap * j->rsz);
+        if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->vmax[b], cap * j->rsz);
+    }
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->keep, cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->pos, cap);
+    if (rc == TDP_OK) j->cap_vp = cap;
+    return rc;
+}
+
+template <typename R>
+static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
+    tdp_ctx *ctx = j->ctx;
+    const size_t P = j->P, N = j->N;
+    const bool within = j->prm.query == TDP_QUERY_WITHIN;
+    const R tau = (R)j->prm.within_dist;
+    R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
+    hipStream_t s = ctx->stream;
+    TDP_CHECK(ctx, hipEventRecord(j->ev[0], s));
+    if (P) {
+        k_aggregate<R><<<blocks_for(P, kWavesPerBlock), kBlock, 0, s>>>(
+            P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
+            j->t2->obj_vox_off, j->t2->vox_box, rmin, rmax);
+        TDP_LAUNCHED(ctx);
+        if (!within) {
+            if (N) {
+                k_filter_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
+                    N, j->prefix, rmin, rmax, j->prm.k, j->state, j->round, j->confirmed,
+                    j->scratch);
+                TDP_LAUNCHED(ctx);
+            }
+        } else {
+            k_filter_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, rmin, rmax, tau,
+                                                                          j->state, j->round);
+            TDP_LAUNCHED(ctx);
+        }
+        k_survivor_count<R><<<blocks_for(P, kWavesPerBlock), kBlock, 0, s>>>(
+            P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, tau, j->state, j->round, j->count);
+        TDP_LAUNCHED(ctx);
+    }
+    uint64_t total = 0;
+    int rc = scan_exclusive<uint32_t>(ctx, j->count, P, j->offs, &total);
+    if (rc != TDP_OK) return rc;
+    rc = ensure_vp(j, total);
+    if (rc != TDP_OK) return rc;
+    j->cur = 0;
+    if (total) {
+        k_survivor_write<R><<<blocks_for(P, kWavesPerBlock), kBlock, 0, s>>>(
+            P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, tau, j->state, j->offs, j->v1[0],
+            j->v2[0], j->vpp[0], (R *)j->vmin[0], (R *)j->vmax[0]);
+        TDP_LAUNCHED(ctx);
+    }
+    TDP_CHECK(ctx, hipEventRecord(j->ev[1], s));
+    TDP_CHECK(ctx, hipStreamSynchronize(s));
+    float ms = 0;
+    TDP_CHECK(ctx, hipEventElapsedTime(&ms, j->ev[0], j->ev[1]));
+    j->filter_ms += ms;
+    j->n_vp = total;
+    j->filter_vp = total;
+    j->filtered = true;
+    if (pairs_out) *pairs_out = total;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_filter(tdp_join *j, uint64_t *pairs_out) {
+    if (!j) return TDP_ERR_ARG;
+    if (j->filtered) {
+        j->ctx->err = "tdp_join_filter: already filtered";
+        return TDP_ERR_STATE;
+    }
+    return j->rsz == 4 ? join_filter_t<float>(j, pairs_out) : join_filter_t<double>(j, pairs_out);
+}
+
+template <typename R, int MODE>
+static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
+    tdp_ctx *ctx = j->ctx;
+    hipStream_t s = ctx->stream;
+    const size_t P = j->P, N = j->N;
+    const bool within = j->prm.query == TDP_QUERY_WITHIN;
+    const R tau = (R)j->prm.within_dist;
+    R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
+    using KT = typename Key<R>::type;
+    int slot = -1;
+    for (int k = 0; k < 5; k++)
+        if (lod == 20 * (k + 1)) slot = k;
+    const uint8_t rnd = (uint8_t)std::min(j->rounds + 1, 254);
+    const int c = j->cur;
+    const uint64_t n = j->n_vp;
+    TDP_CHECK(ctx, hipEventRecord(j->ev[2], s));
+    k_reset_acc<R, KT><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (KT *)j->acc_min,
+                                                                (KT *)j->acc_max);
+    TDP_LAUNCHED(ctx);
+    k_refine<R, MODE><<<blocks_for(n, kWavesPerBlock), kBlock, 0, s>>>(
+        n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], j->t1->vox_tri_off,
+        j->t1->vox_tri_cnt, j->t1->tri, j->t2->vox_tri_off, j->t2->vox_tri_cnt, j->t2->tri,
+        j->t1->vox_obj, j->t2->vox_obj, j->t1->obj_hd, j->t2->obj_hd, slot, is_last,
+        j->prm.hausdorff_level, (KT *)j->acc_min, (KT *)j->acc_max, j->stats);
+    TDP_LAUNCHED(ctx);
+    TDP_CHECK(ctx, hipEventRecord(j->ev[3], s));
+    k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
+                                                              (const KT *)j->acc_max, rmin, rmax);
+    TDP_LAUNCHED(ctx);
+    if (!within) {
+        k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
+            N, j->prefix, rmin, rmax, j->prm.k, j->state, j->round, j->confirmed, j->scratch, rnd);
+    } else {
+        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, rmin, rmax, tau, j->state,
+                                                                     j->round, rnd);
+    }
+    TDP_LAUNCHED(ctx);
+    k_vp_keep<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->vpp[c], (const R *)j->vmin[c],
+                                                          j->state, rmax, j->keep);
+    TDP_LAUNCHED(ctx);
+    uint64_t kept = 0;
+    int rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &kept);
+    if (rc != TDP_OK) return rc;
+    const int o = 1 - c;
+    if (kept) {
+        k_vp_scatter<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+            n, j->keep, j->pos, j->v1[c], j->v2[c], j->vpp[c], (const R *)j->vmin[c],
+            (const R *)j->vmax[c], j->v1[o], j->v2[o], j->vpp[o], (R *)j->vmin[o], (R *)j->vmax[o]);
+        TDP_LAUNCHED(ctx);
+    }
+    TDP_CHECK(ctx, hipStreamSynchronize(s));
+    float ms = 0;
+    TDP_CHECK(ctx, hipEventElapsedTime(&ms, j->ev[2], j->ev[3]));
+    j->refine_ms += ms;
+    j->cur = o;
+    j->n_vp = kept;
+    j->rounds++;
+    if (pairs_out) *pairs_out = kept;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
+    if (!j) return TDP_ERR_ARG;
+    if (!j->filtered) {
+        j->ctx->err = "tdp_join_refine: call tdp_join_filter first";
+        return TDP_ERR_STATE;
+    }
+    if (j->n_vp == 0) {
+        if (pairs_out) *pairs_out = 0;
+        return TDP_OK;
+    }
+    switch (j->prm.precision) {
+        case TDP_PREC_REF32: return join_refine_t<float, MODE_REF32>(j, lod, is_last, pairs_out);
+        case TDP_PREC_FP64: return join_refine_t<double, MODE_FP64>(j, lod, is_last, pairs_out);
+        default: return join_refine_t<double, MODE_MIXED>(j, lod, is_last, pairs_out);
+    }
+}
+
+extern "C" int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin, double *rmax) {
+    if (!j) return TDP_ERR_ARG;
+    tdp_ctx *ctx = j->ctx;
+    const size_t P = j->P;
+    if (P == 0) return TDP_OK;
+    if (report_round)
+        TDP_CHECK(ctx, hipMemcpy(report_round, j->round, P, hipMemcpyDeviceToHost));
+    if (rmin || rmax) {
+        std::vector<char> a(P * j->rsz), b(P * j->rsz);
+        TDP_CHECK(ctx, hipMemcpy(a.data(), j->rmin, P * j->rsz, hipMemcpyDeviceToHost));
+        TDP_CHECK(ctx, hipMemcpy(b.data(), j->rmax, P * j->rsz, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) {
+            double x, y;
+            if (j->rsz == 4) {
+                x = ((const float *)a.data())[p];
+                y = ((const float *)b.data())[p];
+            } else {
+                x = ((const double *)a.data())[p];
+                y = ((const double *)b.data())[p];
+            }
+            if (rmin) rmin[p] = x;
+            if (rmax) rmax[p] = y;
+        }
+    }
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out4) {
+    if (!j || !out4) return TDP_ERR_ARG;
+    unsigned long long s[4];
+    TDP_CHECK(j->ctx, hipMemcpy(s, j->stats, sizeof s, hipMemcpyDeviceToHost));
+    out4[0] = s[0];
+    out4[1] = s[1];
+    out4[2] = s[2];
+    out4[3] = j->filter_vp;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms) {
+    if (!j) return TDP_ERR_ARG;
+    if (refine_ms) *refine_ms = j->refine_ms;
+    if (filter_ms) *filter_ms = j->filter_ms;
+    return TDP_OK;
+}

@@ -1,0 +1,417 @@
+// libtdbase.so: the host join engine behind include/tdbase.h. It generates
+// candidates on the host (octree, OpenMP over targets) and runs the filter and
+// every LOD round on the device through include/tdp.h.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../../../include/tdbase.h"
+#include "../../../include/tdp.h"
+#include "tdb_octree.hpp"
+#include "tdb_tile.hpp"
+
+struct tdb_tile {
+    tdb::Tile t;
+    mutable std::once_flag tree_once;
+    mutable std::unique_ptr<tdb::Octree> tree;
+    const tdb::Octree &octree() const {
+        std::call_once(tree_once, [this] {
+            tree.reset(new tdb::Octree(t.obj_box.data(), t.obj_transform.data(), t.n_obj, 10));
+        });
+        return *tree;
+    }
+};
+
+struct tdb_engine {
+    tdp_ctx *ctx = nullptr;
+    std::string err;
+    std::unordered_map<const tdb_tile *, tdp_tile *> resident;
+};
+
+struct tdb_result {
+    uint64_t n_targets = 0;
+    std::vector<int32_t> target, obj2;
+    std::vector<double> rmin, rmax;
+    std::vector<uint8_t> round;
+    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint64_t stats[6] = {0, 0, 0, 0, 0, 0};
+};
+
+static void set_err(char *err, size_t len, const std::string &msg) {
+    if (err && len) std::snprintf(err, len, "%s", msg.c_str());
+}
+
+using Clock = std::chrono::steady_clock;
+static double ms_since(Clock::time_point a) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+}
+
+extern "C" {
+
+// ------------------------------------------------------------------ tiles
+
+int tdb_tile_load(const char *path, tdb_tile **out, char *err, size_t err_len) {
+    if (!path || !out) return -1;
+    *out = nullptr;
+    std::unique_ptr<tdb_tile> t(new tdb_tile());
+    std::string msg;
+    if (!tdb::read_raw(path, t->t, msg)) {
+        set_err(err, err_len, msg);
+        return -1;
+    }
+    *out = t.release();
+    return 0;
+}
+
+int tdb_tile_save(const tdb_tile *t, const char *path, char *err, size_t err_len) {
+    if (!t || !path) return -1;
+    std::string msg;
+    if (!tdb::write_raw(path, t->t, msg)) {
+        set_err(err, err_len, msg);
+        return -1;
+    }
+    return 0;
+}
+
+void tdb_tile_free(tdb_tile *t) { delete t; }
+
+void tdb_tile_size(const tdb_tile *t, uint64_t *out3) {
+    out3[0] = t->t.n_obj;
+    out3[1] = t->t.n_vox();
+    out3[2] = t->t.n_tri();
+}
+
+void tdb_tile_views(const tdb_tile *t, const uint32_t **obj_vox_off, const float **obj_box,
+                    const float **obj_transform, const float **obj_hd, const float **vox_box,
+                    const uint64_t **vox_tri_off, const uint32_t **vox_tri_cnt,
+                    const float **tri) {
+    if (obj_vox_off) *obj_vox_off = t->t.obj_vox_off.data();
+    if (obj_box) *obj_box = t->t.obj_box.data();
+    if (obj_transform) *obj_transform = t->t.obj_transform.data();
+    if (obj_hd) *obj_hd = t->t.obj_hd.data();
+    if (vox_box) *vox_box = t->t.vox_box.data();
+    if (vox_tri_off) *vox_tri_off = t->t.vox_tri_off.data();
+    if (vox_tri_cnt) *vox_tri_cnt = t->t.vox_tri_cnt.data();
+    if (tri) *tri = t->t.tri.data();
+}
+
+void tdb_gen_defaults(tdb_gen_params *p) {
+    tdb::GenParams g;
+    p->seed = g.seed;
+    p->n_vessels = g.n_vessels;
+    p->nuclei_per_vessel = g.nuclei_per_vessel;
+    p->cube = g.cube;
+    p->vessel_r_min = g.vessel_r_min;
+    p->vessel_r_max = g.vessel_r_max;
+    p->vessel_segments = g.vessel_segments;
+    p->segment_length = g.segment_length;
+    p->turn_max_deg = g.turn_max_deg;
+    p->vessel_sides = g.vessel_sides;
+    p->rings_min = g.rings_min;
+    p->rings_max = g.rings_max;
+    p->nucleus_r_min = g.nucleus_r_min;
+    p->nucleus_r_max = g.nucleus_r_max;
+    p->noise = g.noise;
+    p->subdiv = g.subdiv;
+    p->place_min = g.place_min;
+    p->place_max = g.place_max;
+    p->prototypes = g.prototypes;
+    p->voxel_tris = g.voxel_tris;
+    p->threads = g.threads;
+    p->max_nuclei = 0;
+}
+
+int tdb_generate(const tdb_gen_params *p, tdb_tile **nuclei, tdb_tile **vessels) {
+    if (!p || !nuclei || !vessels) return -1;
+    if (p->n_vessels < 0 || p->nuclei_per_vessel < 0 || p->vessel_segments < 1 ||
+        p->vessel_sides < 3 || p->rings_min < 1 || p->rings_max < p->rings_min ||
+        p->subdiv < 0 || p->subdiv > 6 || p->voxel_tris < 1 || p->prototypes < 1)
+        return -1;
+    tdb::GenParams g;
+    g.seed = p->seed;
+    g.n_vessels = p->n_vessels;
+    g.nuclei_per_vessel = p->nuclei_per_vessel;
+    g.cube = p->cube;
+    g.vessel_r_min = p->vessel_r_min;
+    g.vessel_r_max = p->vessel_r_max;
+    g.vessel_segments = p->vessel_segments;
+    g.segment_length = p->segment_length;
+    g.turn_max_deg = p->turn_max_deg;
+    g.vessel_sides = p->vessel_sides;
+    g.rings_min = p->rings_min;
+    g.rings_max = p->rings_max;
+    g.nucleus_r_min = p->nucleus_r_min;
+    g.nucleus_r_max = p->nucleus_r_max;
+    g.noise = p->noise;
+    g.subdiv = p->subdiv;
+    g.place_min = p->place_min;
+    g.place_max = p->place_max;
+    g.prototypes = p->prototypes;
+    g.voxel_tris = p->voxel_tris;
+    g.threads = p->threads;
+    g.max_nuclei = p->max_nuclei;
+    std::unique_ptr<tdb_tile> n(new tdb_tile()), v(new tdb_tile());
+    tdb::generate(g, n->t, v->t);
+    *nuclei = n.release();
+    *vessels = v.release();
+    return 0;
+}
+
+// ------------------------------------------------------------------ engine
+
+int tdb_engine_open(int device, tdb_engine **out) {
+    if (!out) return -1;
+    tdb_engine *e = new tdb_engine();
+    int rc = tdp_open(device, &e->ctx);
+    if (rc != TDP_OK) {
+        e->err = e->ctx ? tdp_last_error(e->ctx) : "tdp_open failed";
+        if (e->ctx) tdp_close(e->ctx);
+        e->ctx = nullptr;
+        *out = e;  // so the message can be read; the engine cannot join
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+void tdb_engine_close(tdb_engine *e) {
+    if (!e) return;
+    for (auto &kv : e->resident) tdp_tile_free(kv.second);
+    if (e->ctx) tdp_close(e->ctx);
+    delete e;
+}
+
+const char *tdb_engine_error(const tdb_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int tdb_engine_device(tdb_engine *e, char *name, size_t len, int *cu) {
+    if (!e || !e->ctx) return -1;
+    return tdp_device_info(e->ctx, name, len, cu);
+}
+
+void tdb_engine_evict(tdb_engine *e, const tdb_tile *t) {
+    if (!e) return;
+    auto it = e->resident.find(t);
+    if (it == e->resident.end()) return;
+    tdp_tile_free(it->second);
+    e->resident.erase(it);
+}
+
+static int resident(tdb_engine *e, const tdb_tile *t, tdp_tile **out) {
+    auto it = e->resident.find(t);
+    if (it != e->resident.end()) {
+        *out = it->second;
+        return 0;
+    }
+    tdp_tile_desc d;
+    d.n_obj = t->t.n_obj;
+    d.n_vox = t->t.n_vox();
+    d.n_tri = t->t.n_tri();
+    d.obj_vox_off = t->t.obj_vox_off.data();
+    d.obj_hd = t->t.obj_hd.data();
+    d.vox_box = t->t.vox_box.data();
+    d.vox_tri_off = t->t.vox_tri_off.data();
+    d.vox_tri_cnt = t->t.vox_tri_cnt.data();
+    d.tri = t->t.tri.data();
+    int rc = tdp_tile_upload(e->ctx, &d, out);
+    if (rc != TDP_OK) {
+        e->err = tdp_last_error(e->ctx);
+        return rc;
+    }
+    e->resident[t] = *out;
+    return 0;
+}
+
+void tdb_join_defaults(tdb_join_opts *o) {
+    o->query = TDB_QUERY_NN;
+    o->k = 1;
+    o->within_dist = 1000.0f;
+    o->lods = nullptr;
+    o->n_lods = 0;
+    o->hausdorff_level = 2;
+    o->max_objects1 = 0;
+    o->precision = TDP_PREC_MIXED;
+    o->threads = 0;
+}
+
+// LOD list post-processing of parse_args (query_context.h:183-188): sort,
+// then std::unique without erase, which keeps the old tail.
+static std::vector<int> lod_list(const tdb_join_opts *o) {
+    std::vector<int> l;
+    if (o->lods && o->n_lods > 0) l.assign(o->lods, o->lods + o->n_lods);
+    else
+        for (int v = 20; v <= 100; v += 20) l.push_back(v);
+    std::sort(l.begin(), l.end());
+    std::unique(l.begin(), l.end());
+    return l;
+}
+
+static size_t n_targets(const tdb_tile *t1, const tdb_join_opts *o) {
+    size_t n = t1->t.n_obj;
+    if (o->max_objects1 > 0 && o->max_objects1 < n) n = (size_t)o->max_objects1;
+    return n;
+}
+
+// candidate lists of every target (KNNJoin.cpp:552-573, WithinJoin.cpp:79-97)
+static void gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self,
+                           const tdb_join_opts *o, std::vector<uint32_t> &prefix,
+                           std::vector<uint32_t> &ids) {
+    const size_t N = n_targets(t1, o);
+    const tdb::Octree &tree = t2->octree();
+    std::vector<std::vector<uint32_t>> per(N);
+#ifdef _OPENMP
+    const int nt = o->threads > 0 ? o->threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+#endif
+    for (long i = 0; i < (long)N; i++) {
+        std::vector<std::pair<int, tdb::Range>> c;
+        const float *box = &t1->t.obj_box[6 * i];
+        const float *tr = &t1->t.obj_transform[3 * i];
+        if (o->query == TDB_QUERY_NN) tree.knn(box, tr, self ? (int)i : -1, o->k, c);
+        else tree.within(box, tr, self ? (int)i : -1, o->within_dist, c);
+        per[i].reserve(c.size());
+        for (auto &x : c) per[i].push_back((uint32_t)x.first);
+    }
+    prefix.assign(N + 1, 0);
+    for (size_t i = 0; i < N; i++) prefix[i + 1] = prefix[i] + (uint32_t)per[i].size();
+    ids.resize(prefix[N]);
+    for (size_t i = 0; i < N; i++) std::copy(per[i].begin(), per[i].end(), ids.begin() + prefix[i]);
+}
+
+int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
+                       uint32_t *prefix, uint32_t *ids, size_t cap) {
+    if (!t1 || !o) return -1;
+    const bool self = (t2 == nullptr || t2 == t1);
+    std::vector<uint32_t> pf, id;
+    gen_candidates(t1, self ? t1 : t2, self, o, pf, id);
+    if (prefix) std::copy(pf.begin(), pf.end(), prefix);
+    if (ids) std::copy(id.begin(), id.begin() + std::min(cap, id.size()), ids);
+    return (int64_t)id.size();
+}
+
+int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
+             tdb_result **out) {
+    if (!e || !t1 || !o || !out) return -1;
+    *out = nullptr;
+    if (!e->ctx) {
+        e->err = "engine has no device: " + e->err;
+        return TDP_ERR_HIP;
+    }
+    if (o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN) {
+        e->err = "query must be nn or within";
+        return TDP_ERR_ARG;
+    }
+    if (o->query == TDB_QUERY_NN && o->k < 1) {
+        e->err = "--knn must be at least 1";
+        return TDP_ERR_ARG;
+    }
+    const bool self = (t2 == nullptr || t2 == t1);
+    const tdb_tile *T2 = self ? t1 : t2;
+    std::unique_ptr<tdb_result> res(new tdb_result());
+    const auto t_start = Clock::now();
+
+    auto t0 = Clock::now();
+    tdp_tile *d1 = nullptr, *d2 = nullptr;
+    int rc = resident(e, t1, &d1);
+    if (rc == 0) rc = resident(e, T2, &d2);
+    if (rc != 0) return rc;
+    res->timing[6] = ms_since(t0);
+
+    t0 = Clock::now();
+    std::vector<uint32_t> prefix, ids;
+    gen_candidates(t1, T2, self, o, prefix, ids);
+    res->timing[1] = ms_since(t0);
+    const size_t N = prefix.size() - 1, P = ids.size();
+    res->n_targets = N;
+    res->stats[4] = P;
+
+    tdp_join_params prm;
+    prm.query = o->query == TDB_QUERY_NN ? TDP_QUERY_KNN : TDP_QUERY_WITHIN;
+    prm.k = o->k;
+    prm.within_dist = o->within_dist;
+    prm.precision = o->precision;
+    prm.hausdorff_level = o->hausdorff_level;
+    tdp_join *j = nullptr;
+    rc = tdp_join_create(e->ctx, d1, d2, &prm, N, prefix.data(), ids.data(), &j);
+    if (rc != TDP_OK) {
+        e->err = tdp_last_error(e->ctx);
+        return rc;
+    }
+    t0 = Clock::now();
+    uint64_t n_vp = 0;
+    rc = tdp_join_filter(j, &n_vp);
+    res->timing[2] = ms_since(t0);
+    t0 = Clock::now();
+    const std::vector<int> lods = lod_list(o);
+    const int highest = lods.empty() ? 0 : lods.back();
+    for (size_t li = 0; rc == TDP_OK && li < lods.size(); li++) {
+        if (n_vp == 0) break;
+        rc = tdp_join_refine(j, lods[li], lods[li] == highest, &n_vp);
+        res->stats[5]++;
+    }
+    res->timing[3] = ms_since(t0);
+    std::vector<uint8_t> rnd(P);
+    std::vector<double> mn(P), mx(P);
+    if (rc == TDP_OK) rc = tdp_join_results(j, rnd.data(), mn.data(), mx.data());
+    uint64_t st[4] = {0, 0, 0, 0};
+    if (rc == TDP_OK) rc = tdp_join_stats(j, st);
+    if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
+    if (rc != TDP_OK) {
+        e->err = tdp_last_error(e->ctx);
+        tdp_join_destroy(j);
+        return rc;
+    }
+    tdp_join_destroy(j);
+    for (int k = 0; k < 4; k++) res->stats[k] = st[k];
+    // per target: reported pairs ordered by (round, list position)
+    std::vector<uint32_t> order;
+    for (size_t t = 0; t < N; t++) {
+        order.clear();
+        for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++)
+            if (rnd[p] != 255) order.push_back(p);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return rnd[a] < rnd[b]; });
+        for (uint32_t p : order) {
+            res->target.push_back((int32_t)t);
+            res->obj2.push_back((int32_t)ids[p]);
+            res->rmin.push_back(mn[p]);
+            res->rmax.push_back(mx[p]);
+            res->round.push_back(rnd[p]);
+        }
+    }
+    res->timing[0] = ms_since(t_start);
+    *out = res.release();
+    return 0;
+}
+
+uint64_t tdb_result_count(const tdb_result *r) { return r ? r->target.size() : 0; }
+uint64_t tdb_result_targets(const tdb_result *r) { return r ? r->n_targets : 0; }
+
+void tdb_result_get(const tdb_result *r, int32_t *target, int32_t *obj2, double *rmin,
+                    double *rmax, uint8_t *round) {
+    const size_t n = r->target.size();
+    if (target) std::copy(r->target.begin(), r->target.end(), target);
+    if (obj2) std::copy(r->obj2.begin(), r->obj2.end(), obj2);
+    if (rmin) std::copy(r->rmin.begin(), r->rmin.end(), rmin);
+    if (rmax) std::copy(r->rmax.begin(), r->rmax.end(), rmax);
+    if (round) std::copy(r->round.begin(), r->round.begin() + n, round);
+}
+
+void tdb_result_timing(const tdb_result *r, double *out7) {
+    for (int k = 0; k < 7; k++) out7[k] = r->timing[k];
+}
+void tdb_result_stats(const tdb_result *r, uint64_t *out6) {
+    for (int k = 0; k < 6; k++) out6[k] = r->stats[k];
+}
+void tdb_result_free(tdb_result *r) { delete r; }
+
+}  // extern "C"

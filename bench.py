@@ -1,0 +1,158 @@
+"""Benchmark: 10-NN join of 200k synthetic nuclei against 1000 vessels on one
+MI355X (mixed FP32/FP64 form, result lists identical to the FP64 form).
+
+One step = one full `tdbase join -q nn --knn 10` over the resident tiles:
+octree candidates, device filter, and the refinement rounds at LODs
+20/40/60/80/100. The tiles are generated (seeded) and uploaded before the
+timed region. `value` is triangle-pair distances per second over the whole
+join (wall clock); `roofline` is for the refinement kernel, whose device time
+is taken from HIP events recorded on the stream it runs on.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Static FP32 operation count of one triangle pair on the canonical full path
+# of TriDist (9 edge-edge cases + 2 vertex-face cases, no early exit) plus the
+# bound update; add/sub/mul/div/sqrt/compare = 1 (pinned by tests/flops).
+F_TRI = 1138
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tiles_dir, n_targets_total, n_sub, knn, budget_s=15.0):
+    """The oracle (single-threaded C restatement, FP64) on the first targets
+    of the same workload, grown until it has run for about budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    t1 = orc.OracleTile(os.path.join(tiles_dir, "nuclei.dt"))
+    t2 = orc.OracleTile(os.path.join(tiles_dir, "vessels.dt"))
+    n_targets_total = min(n_targets_total, n_sub)
+    m = 20
+    while True:
+        t0 = time.perf_counter()
+        _, _, _, _, st = orc.join(t1, t2, "nn", knn, precision="f64", max_objects1=m)
+        dt = time.perf_counter() - t0
+        if dt >= budget_s / 3 or m >= n_targets_total:
+            break
+        m = min(n_targets_total, int(m * max(2.0, budget_s / 3 / max(dt, 1e-3))))
+    return {"value": st["tri_pairs"] / dt, "unit": "tri-pair dist/s", "cores": 1, "kind": "port",
+            "sample": f"first {m} of {n_targets_total} nuclei vs all vessels, 10-NN, FP64 oracle, "
+                      f"{dt:.1f} s, {st['tri_pairs']} triangle pairs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--vessels", type=int, default=1000)
+    ap.add_argument("--nuclei", type=int, default=200, help="nuclei per vessel")
+    ap.add_argument("--cube", type=float, default=20000.0)
+    ap.add_argument("--knn", type=int, default=10)
+    ap.add_argument("--precision", default="mixed", choices=["ref32", "fp64", "mixed"])
+    ap.add_argument("--compare-fp64", type=int, default=1,
+                    help="also time one plain FP64 join and check identical lists")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the join shards by "
+              f"target objects across GPUs but multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import importlib
+    import tempfile
+    import numpy as np
+    tdp = importlib.import_module("3dpipe_amd")
+
+    t0 = time.perf_counter()
+    nuc, ves = tdp.generate(seed=args.seed, n_vessels=args.vessels,
+                            nuclei_per_vessel=args.nuclei, cube=args.cube)
+    log(f"generated {nuc.n_obj} nuclei ({nuc.n_tri} triangle records) and {ves.n_obj} vessels "
+        f"({ves.n_tri}) in {time.perf_counter() - t0:.1f} s")
+    eng = tdp.Engine(0)
+    name, cus = eng.device()
+    log(f"device {name}, {cus} CUs")
+
+    def run(precision):
+        return eng.join(nuc, ves, query="nn", knn=args.knn, precision=precision)
+
+    for w in range(args.warmup):
+        r = run(args.precision)
+        log(f"warmup {w}: {r.timing_ms['total']:.1f} ms, {r.stats['tri_pairs']} triangle pairs")
+    times, results = [], []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        r = run(args.precision)
+        times.append(time.perf_counter() - t0)
+        results.append(r)
+        log(f"step {s}: {times[-1] * 1e3:.1f} ms (filter {r.timing_ms['filter']:.1f}, refine "
+            f"{r.timing_ms['refine']:.1f}, refine kernels {r.timing_ms['refine_kernels']:.1f})")
+    r = results[-1]
+    ms = 1e3 * sum(times) / len(times)
+    tri_pairs = r.stats["tri_pairs"]
+    refine_kernel_ms = sum(x.timing_ms["refine_kernels"] for x in results) / len(results)
+    launches = max(1, r.stats["rounds"])
+    achieved = F_TRI * tri_pairs / (refine_kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": "10-NN join time (200k nuclei); tri-pair dist/s vs FP32 peak; p50/p99 query latency",
+        "value": tri_pairs / (ms * 1e-3),
+        "unit": "tri-pair dist/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else ("f32" if args.precision == "ref32" else "f64"),
+        "data": "synthetic (seeded icosphere nuclei around tube vessels)",
+        "config": {"workload": f"{args.knn}-NN join nuclei->vessels, {nuc.n_obj} nuclei x "
+                               f"{ves.n_obj} vessels, {args.cube:g}^3 cube, LODs 20-100",
+                   "precision": args.precision, "seed": args.seed,
+                   "triangle_pairs": tri_pairs, "voxel_pairs": r.stats["voxel_pairs"],
+                   "fp64_rechecks": r.stats["fp64_rechecks"],
+                   "object_pairs": r.stats["object_pairs"], "results": int(len(r.target)),
+                   "timing_ms": r.timing_ms},
+        "roofline": {"bound": "valu-fp32", "kernel": "k_refine", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "launches": launches, "avg_launch_ms": refine_kernel_ms / launches,
+                     "flop_per_tri_pair": F_TRI},
+    }
+    if args.compare_fp64 and args.precision != "fp64":
+        t0 = time.perf_counter()
+        rp = run("fp64")
+        fp64_ms = 1e3 * (time.perf_counter() - t0)
+        same = (np.array_equal(rp.target, r.target) and np.array_equal(rp.obj2, r.obj2)
+                and np.array_equal(rp.rmin, r.rmin) and np.array_equal(rp.rmax, r.rmax))
+        out["plain_fp64"] = {"ms": fp64_ms, "speedup": fp64_ms / ms, "identical_results": bool(same),
+                             "refine_kernel_ms": rp.timing_ms["refine_kernels"]}
+        log(f"plain FP64: {fp64_ms:.1f} ms, identical={same}")
+    if args.cpu_baseline:
+        with tempfile.TemporaryDirectory() as d:
+            # the first nuclei of the same workload (identical objects) and
+            # all vessels, written for the oracle
+            n_sub = min(nuc.n_obj, 2000)
+            sub, _ = tdp.generate(seed=args.seed, n_vessels=args.vessels,
+                                  nuclei_per_vessel=args.nuclei, cube=args.cube,
+                                  max_nuclei=n_sub)
+            sub.save(os.path.join(d, "nuclei.dt"))
+            ves.save(os.path.join(d, "vessels.dt"))
+            del sub
+            out["cpu_baseline"] = cpu_baseline(d, nuc.n_obj, n_sub, args.knn)
+            out["cpu_baseline"]["vs_gpu"] = out["value"] / out["cpu_baseline"]["value"]
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,135 @@
+/*
+ * tdbase.h - host layer of the MI355X spatial-join path (libtdbase.so).
+ *
+ * C++ host code (tiles, RAW .dt I/O, the seeded generator, octree candidate
+ * generation, the LOD loop) that drives the device through include/tdp.h. It
+ * is the operator behind the `tdbase join` command of the reference
+ * (src/tools/tdbase.cpp:482-549 -> SpatialJoin::join, SpatialJoin.cpp:463-484
+ * -> nearest_neighbor_chunking / within_chunking). C ABI, integer status
+ * codes (0 ok, < 0 error, message in tdb_engine_error / the err buffer).
+ */
+#ifndef TDBASE_H
+#define TDBASE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdb_tile tdb_tile;
+typedef struct tdb_engine tdb_engine;
+typedef struct tdb_result tdb_result;
+
+/* ---- tiles ---- */
+
+/* Load a RAW .dt tile (type byte 1). Replaces Tile::load
+ * (src/storage/tile_IO.cpp:13-49) with the RAW branch of the HiMesh_Wrapper
+ * constructor (src/spatial/himesh_wrapper.cpp:17-85). COMPRESSED tiles are
+ * rejected with an error. */
+int tdb_tile_load(const char *path, tdb_tile **out, char *err, size_t err_len);
+/* Write a RAW .dt tile, byte-compatible with Tile::dump_raw_stream
+ * (src/storage/tile_IO.cpp:79-172). */
+int tdb_tile_save(const tdb_tile *t, const char *path, char *err, size_t err_len);
+void tdb_tile_free(tdb_tile *t);
+/* n_obj, n_vox, n_tri */
+void tdb_tile_size(const tdb_tile *t, uint64_t *out3);
+/* read-only views of the tile arrays (valid while the tile lives):
+ * obj_vox_off[n_obj+1], obj_box[6n_obj], obj_transform[3n_obj], obj_hd[10n_obj],
+ * vox_box[9n_vox], vox_tri_off[5n_vox], vox_tri_cnt[5n_vox], tri[12n_tri] */
+void tdb_tile_views(const tdb_tile *t, const uint32_t **obj_vox_off, const float **obj_box,
+                    const float **obj_transform, const float **obj_hd, const float **vox_box,
+                    const uint64_t **vox_tri_off, const uint32_t **vox_tri_cnt,
+                    const float **tri);
+
+/* Seeded synthetic data (no counterpart in the reference's join path; the
+ * reference's simulator, src/tools/simulator.cpp, replicates prototype OFF
+ * meshes instead): icosphere nuclei with radial noise, tube vessels along
+ * random polylines, a 5-level LOD ladder, voxels of ~voxel_tris triangles and
+ * conservative per-triangle Hausdorff bounds. */
+typedef struct {
+    uint64_t seed;
+    int n_vessels;            /* 0: nuclei only, uniform in the cube */
+    int nuclei_per_vessel;    /* with n_vessels == 0: total nuclei */
+    double cube;
+    double vessel_r_min, vessel_r_max;
+    int vessel_segments;
+    double segment_length;
+    double turn_max_deg;
+    int vessel_sides;
+    int rings_min, rings_max; /* rings per segment, uniform */
+    double nucleus_r_min, nucleus_r_max;
+    double noise;             /* radial noise amplitude (fraction) */
+    int subdiv;               /* icosphere subdivision level */
+    double place_min, place_max; /* nucleus distance from its vessel's centreline */
+    int prototypes;           /* nucleus shape bank */
+    int voxel_tris;
+    int threads;
+    int64_t max_nuclei;       /* > 0: only the first max_nuclei nuclei (same objects) */
+} tdb_gen_params;
+
+void tdb_gen_defaults(tdb_gen_params *p);
+int tdb_generate(const tdb_gen_params *p, tdb_tile **nuclei, tdb_tile **vessels);
+
+/* ---- engine and join ---- */
+
+/* Engine on one GPU: owns the device context and the device-resident copies
+ * of the tiles it has joined (uploaded once, reused across joins). */
+int tdb_engine_open(int device, tdb_engine **out);
+void tdb_engine_close(tdb_engine *e);
+const char *tdb_engine_error(const tdb_engine *e);
+/* device name (for reports) */
+int tdb_engine_device(tdb_engine *e, char *name, size_t len, int *cu_count);
+/* drop the device copy of a tile (it is uploaded again on next use) */
+void tdb_engine_evict(tdb_engine *e, const tdb_tile *t);
+
+#define TDB_QUERY_NN 0
+#define TDB_QUERY_WITHIN 1
+
+/* the `tdbase join` option set (query_context.h:141-192) */
+typedef struct {
+    int query;               /* TDB_QUERY_NN or TDB_QUERY_WITHIN */
+    int k;                   /* --knn */
+    float within_dist;       /* --within_dist */
+    const int *lods;         /* --lods; NULL or n_lods == 0 -> 20 40 60 80 100 */
+    int n_lods;
+    int hausdorff_level;     /* --hausdorff_level, default 2 */
+    uint64_t max_objects1;   /* --max_objects1 (0 = all) */
+    int precision;           /* TDP_PREC_* of tdp.h: 0 ref32, 1 fp64, 2 mixed */
+    int threads;             /* host threads for candidate generation (0 = all) */
+} tdb_join_opts;
+
+void tdb_join_defaults(tdb_join_opts *o);
+
+/* Join tile1 (targets) against tile2 (NULL = self-join, objects never match
+ * themselves). Replaces SpatialJoin::join for one tile pair
+ * (SpatialJoin.cpp:435-484). */
+int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
+             tdb_result **out);
+
+/* candidate lists alone (octree query_knn_fast / query_within per target):
+ * prefix[n_targets+1] and ids; returns the number of ids (written up to cap) */
+int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
+                       uint32_t *prefix, uint32_t *ids, size_t cap);
+
+/* results: every reported (target, object) pair, grouped by target, in
+ * report order within a target; with the object distance range at report
+ * time and the round (0 = filter, r = r-th LOD round) */
+uint64_t tdb_result_count(const tdb_result *r);
+uint64_t tdb_result_targets(const tdb_result *r);
+void tdb_result_get(const tdb_result *r, int32_t *target, int32_t *obj2, double *rmin,
+                    double *rmax, uint8_t *round);
+/* timing (ms): [0] join total, [1] candidate generation, [2] filter stage,
+ * [3] refinement, [4] filter kernels (device), [5] refine kernels (device),
+ * [6] tile upload */
+void tdb_result_timing(const tdb_result *r, double *out7);
+/* [0] triangle pairs, [1] voxel pairs evaluated, [2] FP64 re-check pairs,
+ * [3] voxel pairs after the filter, [4] candidate object pairs, [5] rounds */
+void tdb_result_stats(const tdb_result *r, uint64_t *out6);
+void tdb_result_free(tdb_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDBASE_H */

@@ -1,0 +1,154 @@
+/*
+ * tdp.h - device layer of the MI355X spatial-join refinement path (libtdp.so).
+ *
+ * A C ABI over hand-written gfx950 HIP kernels: plain pointers and sizes, no
+ * C++ or torch types, integer status codes (0 = ok, < 0 = error; the message
+ * is in tdp_last_error). Nothing here calls exit(). One host thread drives a
+ * context at a time; the library owns all device memory it allocates.
+ *
+ * It replaces the GPU entry points the reference's join engine calls through
+ * src/include/geometry.h (the free functions at geometry.h:209-531 and the
+ * dispatcher geometry_computer, geometry.h:533-572), implemented in
+ * src/geometry/computing_gpu.cu. Each entry point names the reference
+ * interface it stands in for.
+ */
+#ifndef TDP_H
+#define TDP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TDP_OK 0
+#define TDP_ERR_ARG -1     /* invalid argument */
+#define TDP_ERR_HIP -2     /* HIP runtime error (no device, launch failure, ...) */
+#define TDP_ERR_NOMEM -3   /* device or pinned allocation failed */
+#define TDP_ERR_STATE -4   /* call out of order */
+
+/* arithmetic form of the distance path */
+#define TDP_PREC_REF32 0  /* the reference's FP32 arithmetic, bit for bit */
+#define TDP_PREC_FP64 1   /* the same algorithm in FP64 ("plain") */
+#define TDP_PREC_MIXED 2  /* FP32 bulk + FP64 re-check; results identical to FP64 */
+
+#define TDP_QUERY_KNN 0
+#define TDP_QUERY_WITHIN 1
+
+typedef struct tdp_ctx tdp_ctx;
+typedef struct tdp_tile tdp_tile;
+typedef struct tdp_join tdp_join;
+
+/* Context on one HIP device.
+ * Replaces: initialize()/get_gpus()/init_gpu (src/geometry/cuda_common.cu:15-73,
+ * src/include/mygpu.h:22-29) and initialize_refine_stream
+ * (src/geometry/computing_gpu.cu:37-43). */
+int tdp_open(int device, tdp_ctx **out);
+int tdp_close(tdp_ctx *ctx);
+/* message of the last failed call on ctx (never NULL; "" if none) */
+const char *tdp_last_error(const tdp_ctx *ctx);
+/* device name and compute-unit count (for reports) */
+int tdp_device_info(tdp_ctx *ctx, char *name, size_t name_len, int *cu_count);
+
+/* Device-resident tile. Triangles are 48-byte records
+ * {v0.xyz, proxy_hausdorff, v1.xyz, hausdorff, v2.xyz, 0}; voxel boxes are
+ * 9 floats {low.xyz, high.xyz, core.xyz}. Per (voxel, LOD slot 0..4 for LOD
+ * 20..100): the first record index and the record count.
+ * Replaces: allocateVoxelsForAll (computing_gpu.cu:1708-1748) for the voxel
+ * boxes, and the per-chunk packing + H2D of triangles and Hausdorff values
+ * (SpatialJoin.cpp:169-248, computing_gpu.cu:6540-6658): the whole tile is
+ * uploaded once and stays resident for every LOD. */
+typedef struct {
+    size_t n_obj, n_vox, n_tri;
+    const uint32_t *obj_vox_off; /* n_obj + 1 */
+    const float *obj_hd;         /* 10 per object: (hausdorff, proxy) per LOD slot */
+    const float *vox_box;        /* 9 per voxel */
+    const uint64_t *vox_tri_off; /* 5 per voxel */
+    const uint32_t *vox_tri_cnt; /* 5 per voxel */
+    const float *tri;            /* 12 per triangle */
+} tdp_tile_desc;
+
+int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *desc, tdp_tile **out);
+int tdp_tile_free(tdp_tile *tile);
+size_t tdp_tile_bytes(const tdp_tile *tile);
+
+/* Triangle-triangle distance for n independent pairs; S and T are 9 floats
+ * per triangle (host memory), out is n values (host) in the given precision:
+ * REF32 -> float* (bit-identical to the reference's TriDist), FP64 and MIXED
+ * -> double*.
+ * Replaces: TriDist (computing_cpu.cpp:403-439) / TriDist_kernel
+ * (computing_gpu.cu:566-610). */
+int tdp_tridist_batch(tdp_ctx *ctx, int precision, const float *S, const float *T, size_t n,
+                      void *out);
+
+/* Per voxel pair: minimum triangle distance, minimum lower bound
+ * max(d - ph1 - ph2, 0) and minimum upper bound d + h1 + h2 over the n1 x n2
+ * triangle pairs. tri: 12-float records (host), offset_size: 4 x u64 per pair
+ * {offset1, n1, offset2, n2} in records; out: 3 values per pair (float for
+ * REF32, double otherwise).
+ * Replaces: MeshDist_batch_gpu_op_shm_flat_stream (computing_gpu.cu:6540-6658)
+ * and its kernel TriDist_cuda_op_shm_flat_fused (:1069-1152). */
+int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri, size_t n_tri,
+                       const uint64_t *offset_size, size_t n_pairs, void *out);
+
+/* kNN join status (evaluate_knn_kernel, computing_gpu.cu:105-151, via
+ * evaluation_kernel :6905-6950): per target t with candidates
+ * [prefix[t], prefix[t+1]) and confirmed[t] already confirmed:
+ * status 1 = CONFIRMED, 2 = REMOVE, 0 = KEEP. Host arrays; min/max are double
+ * and compared in double (exact for values that came from float). */
+int tdp_knn_status(tdp_ctx *ctx, size_t n_targets, const uint32_t *prefix,
+                   const int32_t *confirmed, const double *rmin, const double *rmax, int k,
+                   int32_t *status);
+
+/* ---- device-resident join (filter + LOD refinement) ---- */
+
+typedef struct {
+    int query;            /* TDP_QUERY_KNN or TDP_QUERY_WITHIN */
+    int k;                /* kNN */
+    float within_dist;    /* within */
+    int precision;        /* TDP_PREC_* */
+    int hausdorff_level;  /* 0, 1 or 2 */
+} tdp_join_params;
+
+/* Candidate lists from the octree: target t of tile 1 (t < n_targets) has
+ * candidates cand_obj[cand_prefix[t] .. cand_prefix[t+1]) of tile 2, in list
+ * order. Replaces the device set-up of mbb_knn_op_gpu_one_kernel /
+ * mbb_within_op_gpu_one_kernel (KNNJoin.cpp:497-851, WithinJoin.cpp:26-258). */
+int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                    const tdp_join_params *params, size_t n_targets, const uint32_t *cand_prefix,
+                    const uint32_t *cand_obj, tdp_join **out);
+
+/* Filter stage on the device: object-level voxel bounds, the filter-time kNN
+ * status / within decisions and compaction of surviving voxel pairs.
+ * Replaces compute_voxel_pair_distance_for_all_streaming_pipeline_reduceIO
+ * (computing_gpu.cu:4417-4862) and ..._within_streaming_pipeline (:5681-6013)
+ * together with the host decisions that follow them. *pairs_out = number of
+ * surviving voxel pairs. */
+int tdp_join_filter(tdp_join *j, uint64_t *pairs_out);
+
+/* One refinement round at `lod` (20..100; other values decode to empty
+ * voxels): triangle distances of every live voxel pair, bound folding,
+ * eviction, then the kNN status walk or the within decision. is_last selects
+ * exact distances ([d,d]). *pairs_out = live voxel pairs after the round.
+ * Replaces one iteration of the LOD loop of nearest_neighbor_chunking
+ * (KNNJoin.cpp:1564-1800) / within_chunking (WithinJoin.cpp:663-802). */
+int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pairs_out);
+
+/* Per candidate pair (in cand_prefix order): the round that reported it
+ * (0 = filter, r = r-th refinement round, 255 = not reported) and its object
+ * distance range when reported (or its last range). */
+int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin, double *rmax);
+
+/* counters: [0] triangle pairs evaluated, [1] voxel pairs evaluated,
+ * [2] FP64 re-check triangle pairs (MIXED), [3] voxel pairs at filter */
+int tdp_join_stats(tdp_join *j, uint64_t *out4);
+/* device time (ms, from HIP events) spent in the refine kernels so far */
+int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
+int tdp_join_destroy(tdp_join *j);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDP_H */

@@ -1,0 +1,376 @@
+"""CPU-side tests: the oracle against the reference's own compiled sources and
+closed forms, the C-ABI libraries (load, exported symbols), the RAW format,
+the generator and the host octree (no GPU calls)."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from conftest import ROOT, random_triangles
+
+
+# ---------------------------------------------------------------- oracle pins
+
+def test_oracle_tridist_matches_reference_build():
+    ref = orc.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(11)
+    S, T = random_triangles(rng, 200_000)
+    want = np.zeros(len(S), np.float32)
+    ref.ref_tridist(orc.ptr(S), orc.ptr(T), len(S), orc.ptr(want))
+    got = orc.tridist(S, T, "f32")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_oracle_meshdist_and_aab_match_reference_build():
+    ref = orc.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(12)
+    tri = rng.uniform(-30, 30, (3000, 9)).astype(np.float32)
+    hd = rng.uniform(0, 2, (3000, 2)).astype(np.float32)
+    os_ = np.stack([rng.integers(0, 2900, 400), rng.integers(1, 60, 400),
+                    rng.integers(0, 2900, 400), rng.integers(1, 60, 400)], 1).astype(np.uint64)
+    os_[:, 1] = np.minimum(os_[:, 1], 3000 - os_[:, 0])
+    os_[:, 3] = np.minimum(os_[:, 3], 3000 - os_[:, 2])
+    want = np.zeros((400, 3), np.float32)
+    ref.ref_meshdist(orc.ptr(tri), orc.ptr(hd), orc.ptr(os_), 400, orc.ptr(want))
+    got = orc.meshdist(tri, hd, os_, "f32")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    a = rng.uniform(-5, 5, (1000, 6)).astype(np.float32)
+    b = rng.uniform(-5, 5, (1000, 6)).astype(np.float32)
+    a[:, 3:] = a[:, :3] + np.abs(a[:, 3:])
+    b[:, 3:] = b[:, :3] + np.abs(b[:, 3:])
+    for i in range(1000):
+        w = np.zeros(2, np.float32)
+        g = np.zeros(2, np.float32)
+        ref.ref_aab_distance(orc.ptr(a[i]), orc.ptr(b[i]), orc.ptr(w))
+        orc.lib().orc_aab_distance(orc.ptr(a[i]), orc.ptr(b[i]), orc.ptr(g))
+        assert np.array_equal(w, g)
+
+
+def _boxsets(rng, n):
+    # integer-grid boxes: many equal distances, so queue tie order matters
+    low = np.floor(rng.uniform(0, 40, (n, 3))).astype(np.float32)
+    high = (low + np.floor(rng.uniform(1, 6, (n, 3)))).astype(np.float32)
+    tr = np.floor((low + high) / 2).astype(np.float32)
+    return low, high, tr
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_oracle_octree_matches_reference_build(seed):
+    """candidate lists of query_knn_fast / query_within, order included"""
+    ref = orc.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(seed)
+    n = 1500
+    low, high, tr = _boxsets(rng, n)
+    ot = orc.lib().orc_boxset_build(n, orc.ptr(low), orc.ptr(high), orc.ptr(tr), 10)
+    rt = ref.ref_boxset_build(n, orc.ptr(low), orc.ptr(high), orc.ptr(tr), 10)
+    cap = 4096
+    for i in range(0, n, 3):
+        for k in (1, 3, 10):
+            out = []
+            for f, h in ((orc.lib().orc_query_knn, ot), (ref.ref_query_knn, rt)):
+                ids = np.zeros(cap, np.int32); mn = np.zeros(cap, np.float32); mx = np.zeros(cap, np.float32)
+                c = f(h, orc.ptr(low[i]), orc.ptr(high[i]), orc.ptr(tr[i]), i, k, orc.ptr(ids),
+                      orc.ptr(mn), orc.ptr(mx), cap)
+                out.append((ids[:c].tolist(), mn[:c].tolist(), mx[:c].tolist()))
+            assert out[0] == out[1], (i, k)
+        out = []
+        for f, h in ((orc.lib().orc_query_within, ot), (ref.ref_query_within, rt)):
+            ids = np.zeros(cap, np.int32); mn = np.zeros(cap, np.float32); mx = np.zeros(cap, np.float32)
+            c = f(h, orc.ptr(low[i]), orc.ptr(high[i]), orc.ptr(tr[i]), i, 4.0, orc.ptr(ids),
+                  orc.ptr(mn), orc.ptr(mx), cap)
+            out.append((ids[:c].tolist(), mn[:c].tolist(), mx[:c].tolist()))
+        assert out[0] == out[1], i
+    ref.ref_boxset_free(rt)
+
+
+def test_oracle_golden_fixtures():
+    """the oracle against vectors written from the reference's own build"""
+    path = os.path.join(ROOT, "tests", "golden", "tridist_ref.npz")
+    z = np.load(path)
+    got = orc.tridist(z["S"], z["T"], "f32")
+    assert np.array_equal(got.view(np.uint32), z["d"].view(np.uint32))
+    z = np.load(os.path.join(ROOT, "tests", "golden", "meshdist_ref.npz"))
+    got = orc.meshdist(z["tri"], z["hd"], z["os"], "f32")
+    assert np.array_equal(got.view(np.uint32), z["out"].view(np.uint32))
+    z = np.load(os.path.join(ROOT, "tests", "golden", "octree_ref.npz"))
+    low, high, tr = z["low"], z["high"], z["tr"]
+    n = len(low)
+    ot = orc.lib().orc_boxset_build(n, orc.ptr(low), orc.ptr(high), orc.ptr(tr), 10)
+    cap = 4096
+    for row in range(len(z["q_index"])):
+        i, k = int(z["q_index"][row]), int(z["q_k"][row])
+        b, e = z["q_prefix"][row], z["q_prefix"][row + 1]
+        ids = np.zeros(cap, np.int32); mn = np.zeros(cap, np.float32); mx = np.zeros(cap, np.float32)
+        c = orc.lib().orc_query_knn(ot, orc.ptr(low[i]), orc.ptr(high[i]), orc.ptr(tr[i]), i, k,
+                                    orc.ptr(ids), orc.ptr(mn), orc.ptr(mx), cap)
+        assert ids[:c].tolist() == z["q_ids"][b:e].tolist()
+
+
+def test_oracle_fp64_closed_forms():
+    """FP64 TriDist against configurations whose distance is known exactly"""
+    rng = np.random.default_rng(13)
+    n = 2000
+    S = np.zeros((n, 9)); T = np.zeros((n, 9)); want = np.zeros(n)
+    for i in range(n):
+        kind = i % 4
+        a = rng.uniform(-5, 5, (3, 3))
+        if kind == 0:      # parallel planes: T = S shifted along the normal, overlapping in projection
+            a[:, 2] = 0
+            h = rng.uniform(0.1, 20)
+            S[i] = a.ravel(); T[i] = (a + [0, 0, h]).ravel(); want[i] = h
+        elif kind == 1:    # vertex above the interior of the other face
+            a[:, 2] = 0
+            c = a.mean(0)
+            h = rng.uniform(0.1, 20)
+            S[i] = a.ravel()
+            T[i] = np.array([c + [0, 0, h], c + [3, 1, h + 9], c + [-2, 4, h + 7]]).ravel()
+            want[i] = h
+        elif kind == 2:    # skew edges crossing in projection: x-axis segment vs y-parallel segment at height h
+            h = rng.uniform(0.1, 20)
+            S[i] = np.array([[-1, 0, 0], [1, 0, 0], [0, -5, -8]]).ravel()
+            T[i] = np.array([[0, -1, h], [0, 1, h], [0, 5, h + 8]]).ravel()
+            want[i] = h
+        else:              # shared vertex
+            S[i] = a.ravel()
+            b = rng.uniform(-5, 5, (3, 3)); b[0] = a[0]
+            T[i] = b.ravel(); want[i] = 0.0
+    got = orc.tridist_dd(S, T)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_fp64_dense_sampling():
+    """FP64 TriDist against the minimum over dense samples of both triangles.
+    Every value the reference's TriDist returns is a distance between two
+    points of the triangles, so it is never below the true distance (up to the
+    sampling resolution). It can be above: the reference's SegPoints computes
+    u from the unclamped line parameter t (computing_cpu.cpp:86-98), so in
+    about a quarter of random pairs the result overestimates the true
+    distance; the oracle reproduces that, as the GPU path must."""
+    rng = np.random.default_rng(14)
+    m = 40
+    u, v = np.meshgrid(np.linspace(0, 1, m + 1), np.linspace(0, 1, m + 1))
+    keep = (u + v) <= 1
+    u, v = u[keep], v[keep]
+    over = 0
+    for _ in range(120):
+        S = rng.uniform(-3, 3, (3, 3)); T = rng.uniform(-3, 3, (3, 3)) + rng.uniform(-6, 6, 3)
+        ps = S[0] + np.outer(u, S[1] - S[0]) + np.outer(v, S[2] - S[0])
+        pt = T[0] + np.outer(u, T[1] - T[0]) + np.outer(v, T[2] - T[0])
+        d2 = ((ps[:, None, :] - pt[None, :, :]) ** 2).sum(-1)
+        sampled = math.sqrt(d2.min())
+        d = orc.tridist_dd(S.ravel(), T.ravel())[0]
+        res = max(np.linalg.norm(S[1] - S[0]), np.linalg.norm(S[2] - S[0]),
+                  np.linalg.norm(T[1] - T[0]), np.linalg.norm(T[2] - T[0])) / m
+        assert sampled - d <= 2 * res + 1e-9
+        over += d > sampled + 1e-9
+    assert 0.05 < over / 120 < 0.5
+
+
+def test_oracle_fp64_agrees_with_fp32_restatement():
+    """the FP64 instantiation is the FP32 text (bit-pinned to the reference)
+    with REAL = double; on well-separated triangles both agree to FP32
+    rounding"""
+    rng = np.random.default_rng(16)
+    S = rng.uniform(-5, 5, (50000, 9)).astype(np.float32)
+    T = (rng.uniform(-5, 5, (50000, 9)) + rng.uniform(20, 60, (50000, 1))).astype(np.float32)
+    d32 = orc.tridist(S, T, "f32").astype(np.float64)
+    d64 = orc.tridist(S, T, "f64")
+    rel = np.abs(d32 - d64) / d64
+    assert np.median(rel) < 1e-6 and np.quantile(rel, 0.999) < 1e-4
+
+
+def test_oracle_meshdist_is_min_of_tridist():
+    """MeshDist per voxel pair = min over its triangle pairs of TriDist and of
+    the two bound expressions, in both forms"""
+    rng = np.random.default_rng(15)
+    tri = rng.uniform(-20, 20, (300, 9)).astype(np.float32)
+    hd = rng.uniform(0, 3, (300, 2)).astype(np.float32)
+    os_ = np.array([[0, 17, 100, 23], [50, 1, 200, 40], [290, 10, 0, 3]], np.uint64)
+    for prec, dt in (("f32", np.float32), ("f64", np.float64)):
+        out = orc.meshdist(tri, hd, os_, prec)
+        for p, (a, na, b, nb) in enumerate(os_.astype(int)):
+            S = np.repeat(tri[a:a + na], nb, 0)
+            T = np.tile(tri[b:b + nb], (na, 1))
+            d = orc.tridist(S, T, prec)
+            ph1 = np.repeat(hd[a:a + na, 0], nb).astype(dt)
+            ph2 = np.tile(hd[b:b + nb, 0], na).astype(dt)
+            h1 = np.repeat(hd[a:a + na, 1], nb).astype(dt)
+            h2 = np.tile(hd[b:b + nb, 1], na).astype(dt)
+            assert out[p, 0] == d.min()
+            assert out[p, 1] == np.maximum((d - ph1) - ph2, 0).min()
+            assert out[p, 2] == ((d + h1) + h2).min()
+
+
+def test_lod_list_quirk():
+    """parse_args sorts the LOD list and calls std::unique without erase"""
+    assert orc.lod_list([20, 20, 40]) == [20, 40, 40]
+    assert orc.lod_list([100, 20, 60, 20]) == [20, 60, 100, 100]
+    assert orc.lod_list([40, 20, 20]) == [20, 40, 40]
+
+
+# ------------------------------------------------------------- C-ABI libs
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(td[pb]_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("tdp.h", "libtdp.so"), ("tdbase.h", "libtdbase.so")])
+def test_library_exports_every_declared_symbol(tdp, header, lib):
+    tdp._load()
+    path = os.path.join(ROOT, "3dpipe_amd", "lib", lib)
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if l.strip())
+    names = _declared(header)
+    assert names, header
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_device_calls_fail_loudly_without_gpu(tdp):
+    """no GPU here: opening the device raises, nothing falls back to the CPU"""
+    import ctypes
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is present")
+    with pytest.raises(tdp.TdpError):
+        tdp.Engine(0)
+    with pytest.raises(tdp.TdpError):
+        tdp.Device(0)
+
+
+# ------------------------------------------------------- RAW format, generator
+
+def test_raw_roundtrip_and_oracle_reads_same_bytes(tdp, small_tiles, tmp_path):
+    pn, pv = small_tiles
+    a = tdp.Tile.load(pn)
+    p2 = str(tmp_path / "again.dt")
+    a.save(p2)
+    assert open(pn, "rb").read() == open(p2, "rb").read()
+    b = tdp.Tile.load(p2)
+    A, B = a.arrays(), b.arrays()
+    for k in A:
+        assert np.array_equal(A[k], B[k]), k
+    ot = orc.OracleTile(pn)
+    assert ot.n == a.n_obj
+    box = np.zeros(9, np.float32)
+    for i in range(a.n_obj):
+        orc.lib().orc_tile_object_box(ot.h, i, orc.ptr(box))
+        assert np.array_equal(box[:6], A["obj_box"][6 * i:6 * i + 6])
+        assert np.array_equal(box[6:], A["obj_transform"][3 * i:3 * i + 3])
+
+
+def test_raw_reader_rejects_bad_files(tdp, tmp_path):
+    p = tmp_path / "comp.dt"
+    p.write_bytes(b"\x00" + b"\x01" * 64)
+    with pytest.raises(tdp.TdpError, match="COMPRESSED"):
+        tdp.Tile.load(str(p))
+    p = tmp_path / "trunc.dt"
+    p.write_bytes(b"\x01" + (1000).to_bytes(8, "little") + b"\x00" * 10)
+    with pytest.raises(tdp.TdpError):
+        tdp.Tile.load(str(p))
+    with pytest.raises(tdp.TdpError):
+        tdp.Tile.load(str(tmp_path / "missing.dt"))
+
+
+def test_generator_shapes(tdp):
+    n, v = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=20, cube=2000.0)
+    assert (n.n_obj, v.n_obj) == (200, 10)
+    an, av = n.arrays(), v.arrays()
+    cnt = an["vox_tri_cnt"].reshape(-1, 5)
+    per_obj = np.add.reduceat(cnt, an["obj_vox_off"][:-1], axis=0)
+    assert np.all(per_obj == [257, 512, 767, 1025, 1280])   # nested icosphere LOD ladder
+    nv = np.diff(an["obj_vox_off"])
+    assert np.all(nv == 26)                                  # 1280/50 + 1 voxels
+    cntv = av["vox_tri_cnt"].reshape(-1, 5)
+    assert np.all(np.add.reduceat(cntv, av["obj_vox_off"][:-1], axis=0)[:, 4] == 4608)
+    # deterministic for a seed
+    n2, _ = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=20, cube=2000.0)
+    assert np.array_equal(n2.arrays()["tri"], an["tri"])
+    # cores inside their voxel boxes
+    vb = an["vox_box"].reshape(-1, 9)
+    assert np.all(vb[:, 6:9] >= vb[:, 0:3]) and np.all(vb[:, 6:9] <= vb[:, 3:6])
+
+
+def test_generator_bounds_cover_lod100_surface(tdp):
+    """every LOD-100 vertex of a voxel lies within the proxy bound of some LOD
+    triangle of the same voxel (what the refinement's lower bounds rely on)"""
+    n, _ = tdp.generate(seed=5, n_vessels=2, nuclei_per_vessel=3, cube=500.0)
+    a = n.arrays()
+    rec = a["tri"].reshape(-1, 12).astype(np.float64)
+    off = a["vox_tri_off"].reshape(-1, 5)
+    cnt = a["vox_tri_cnt"].reshape(-1, 5)
+
+    def point_tri(p, a, b, c):
+        # closest point by projection onto the plane, else onto the edges
+        ab, ac = b - a, c - a
+        nrm = np.cross(ab, ac)
+        q = p - np.dot(p - a, nrm) / np.dot(nrm, nrm) * nrm
+        w = np.linalg.lstsq(np.stack([ab, ac], 1), q - a, rcond=None)[0]
+        if w[0] >= 0 and w[1] >= 0 and w[0] + w[1] <= 1:
+            return np.linalg.norm(p - q)
+
+        def seg(p, x, y):
+            t = np.clip(np.dot(p - x, y - x) / np.dot(y - x, y - x), 0, 1)
+            return np.linalg.norm(p - (x + t * (y - x)))
+        return min(seg(p, a, b), seg(p, b, c), seg(p, c, a))
+
+    for v in range(0, n.n_vox, 7):
+        full = rec[off[v, 4]:off[v, 4] + cnt[v, 4]]
+        for s in range(4):
+            lod = rec[off[v, s]:off[v, s] + cnt[v, s]]
+            if len(lod) == 0:
+                continue
+            for t in full[:: max(1, len(full) // 8)]:
+                for p in (t[0:3], t[4:7], t[8:11]):
+                    gap = min(point_tri(p, l[0:3], l[4:7], l[8:11]) - l[3] for l in lod)
+                    assert gap <= 1e-3
+
+
+def test_host_octree_matches_oracle(tdp, small_tiles):
+    """candidate lists of the product's octree equal the oracle's (order
+    included) for kNN, within and a self-join"""
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    cap = 4096
+    for (t1, t2, o1, o2, self_) in ((n, v, on, ov, False), (n, None, on, on, True)):
+        for query, k, tau in (("nn", 3, 0.0), ("nn", 10, 0.0), ("within", 1, 400.0)):
+            prefix, ids = tdp.candidates(t1, t2, query=query, knn=k, within_dist=tau)
+            box = np.zeros(9, np.float32)
+            for i in range(t1.n_obj):
+                orc.lib().orc_tile_object_box(o1.h, i, orc.ptr(box))
+                ci = np.zeros(cap, np.int32); mn = np.zeros(cap, np.float32); mx = np.zeros(cap, np.float32)
+                if query == "nn":
+                    c = orc.lib().orc_query_knn(o2.h, orc.ptr(box[:3]), orc.ptr(box[3:6]),
+                                                orc.ptr(box[6:]), i if self_ else -1, k,
+                                                orc.ptr(ci), orc.ptr(mn), orc.ptr(mx), cap)
+                else:
+                    c = orc.lib().orc_query_within(o2.h, orc.ptr(box[:3]), orc.ptr(box[3:6]),
+                                                   orc.ptr(box[6:]), i if self_ else -1, tau,
+                                                   orc.ptr(ci), orc.ptr(mn), orc.ptr(mx), cap)
+                assert ids[prefix[i]:prefix[i + 1]].tolist() == ci[:c].tolist(), (query, i)
+
+
+def test_cli_without_gpu_fails_loudly(small_tiles):
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is present")
+    exe = os.path.join(ROOT, "3dpipe_amd", "bin", "tdbase")
+    pn, pv = small_tiles
+    r = subprocess.run([exe, "join", "--tile1", pn, "--tile2", pv, "-q", "nn", "-k", "3",
+                        "--lod", "20", "40", "-g"], capture_output=True, text=True)
+    assert r.returncode == 2 and "GPU unavailable" in r.stderr
+    assert "ignoring unknown option --lod" in r.stderr
+    r = subprocess.run([exe, "join", "-q", "nn"], capture_output=True, text=True)
+    assert r.returncode == 1 and "tile1" in r.stderr

@@ -1,0 +1,251 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (needs an MI355X).
+
+Bars: bit-exact for the FP32 form against the oracle's FP32 restatement (the
+reference arithmetic), bit-exact for the FP64 form against the oracle's FP64
+restatement, and the mixed form bit-identical to the FP64 form. Join result
+lists are compared in full, report order included, together with the reported
+distance ranges."""
+import numpy as np
+import pytest
+
+import orc
+from conftest import random_triangles
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev(tdp):
+    d = tdp.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def eng(tdp):
+    e = tdp.Engine(0)
+    yield e
+    e.close()
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.dtype == b.dtype and a.shape == b.shape and np.array_equal(
+        a.view(np.uint32 if a.dtype == np.float32 else np.uint64),
+        b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
+
+
+def test_tridist_ref32_bit_exact(dev):
+    rng = np.random.default_rng(1)
+    S, T = random_triangles(rng, 1 << 18)
+    g = dev.tridist(S, T, "ref32")
+    o = orc.tridist(S, T, "f32")
+    mism = np.flatnonzero(g.view(np.uint32) != o.view(np.uint32))
+    assert mism.size == 0, f"{mism.size} mismatches, first {mism[:5]}: {g[mism[:5]]} vs {o[mism[:5]]}"
+
+
+def test_tridist_fp64_bit_exact(dev):
+    rng = np.random.default_rng(2)
+    S, T = random_triangles(rng, 1 << 18, spread=50.0, offset=300.0)
+    g = dev.tridist(S, T, "fp64")
+    o = orc.tridist(S, T, "f64")
+    assert bits_equal(g, o)
+
+
+def test_tridist_large_coordinates(dev):
+    """coordinates of a 20000^3 cube, as in the full-scale workload"""
+    rng = np.random.default_rng(3)
+    S, T = random_triangles(rng, 1 << 16, spread=5.0, offset=40.0)
+    shift = rng.uniform(0, 20000, (len(S), 1)).astype(np.float32)
+    S = (S + np.tile(shift, (1, 9))).astype(np.float32)
+    T = (T + np.tile(shift, (1, 9))).astype(np.float32)
+    assert bits_equal(dev.tridist(S, T, "ref32"), orc.tridist(S, T, "f32"))
+    assert bits_equal(dev.tridist(S, T, "fp64"), orc.tridist(S, T, "f64"))
+
+
+def _voxel_pairs(tile, rng, n_pairs, lod_slot):
+    a = tile.arrays()
+    rec = a["tri"].reshape(-1, 12)
+    off = a["vox_tri_off"].reshape(-1, 5)[:, lod_slot]
+    cnt = a["vox_tri_cnt"].reshape(-1, 5)[:, lod_slot]
+    nz = np.flatnonzero(cnt > 0)
+    v1 = rng.choice(nz, n_pairs)
+    v2 = rng.choice(nz, n_pairs)
+    os_ = np.stack([off[v1], cnt[v1], off[v2], cnt[v2]], 1).astype(np.uint64)
+    return rec, os_
+
+
+@pytest.mark.parametrize("slot", [0, 2, 4])
+def test_meshdist_all_forms(dev, tdp, small_tiles, slot):
+    nuc = tdp.Tile.load(small_tiles[0])
+    rng = np.random.default_rng(10 + slot)
+    rec, os_ = _voxel_pairs(nuc, rng, 3000, slot)
+    tri9 = rec[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]]
+    hd = rec[:, [3, 7]]
+    o32 = orc.meshdist(tri9, hd, os_, "f32")
+    o64 = orc.meshdist(tri9, hd, os_, "f64")
+    assert bits_equal(dev.meshdist(rec, os_, "ref32"), o32)
+    assert bits_equal(dev.meshdist(rec, os_, "fp64"), o64)
+    assert bits_equal(dev.meshdist(rec, os_, "mixed"), o64)
+
+
+def test_meshdist_mixed_near_contact(dev):
+    """touching, crossing and coplanar triangle sets: the FP64 re-check must
+    keep the mixed result identical to FP64"""
+    rng = np.random.default_rng(5)
+    S, T = random_triangles(rng, 4096, spread=3.0, offset=2.0)
+    rec = np.zeros((2 * len(S), 12), np.float32)
+    for k, tri in enumerate((S, T)):
+        rec[k * len(S):(k + 1) * len(S), [0, 1, 2, 4, 5, 6, 8, 9, 10]] = tri
+        rec[k * len(S):(k + 1) * len(S), 3] = rng.uniform(0, 0.5, len(S))
+        rec[k * len(S):(k + 1) * len(S), 7] = rng.uniform(0, 0.5, len(S))
+    os_ = []
+    for p in range(512):
+        a = rng.integers(0, len(S) - 16)
+        b = len(S) + rng.integers(0, len(S) - 16)
+        os_.append([a, rng.integers(1, 16), b, rng.integers(1, 16)])
+    os_ = np.array(os_, np.uint64)
+    assert bits_equal(dev.meshdist(rec, os_, "mixed"), dev.meshdist(rec, os_, "fp64"))
+
+
+def test_knn_status_matches_rule(dev):
+    rng = np.random.default_rng(7)
+    N = 500
+    m = rng.integers(0, 40, N)
+    prefix = np.concatenate([[0], np.cumsum(m)]).astype(np.uint32)
+    lo = np.floor(rng.uniform(0, 20, prefix[-1]))
+    hi = lo + np.floor(rng.uniform(0, 10, prefix[-1]))
+    conf = rng.integers(0, 3, N).astype(np.int32)
+    k = 5
+    st = dev.knn_status(prefix, conf, lo, hi, k)
+    for t in range(N):
+        b, e = prefix[t], prefix[t + 1]
+        for i in range(b, e):
+            sure = sum(1 for j in range(b, e) if j != i and lo[i] >= hi[j])
+            maybe = sum(1 for j in range(b, e) if j != i and not (hi[i] <= lo[j]))
+            left = k - conf[t]
+            want = 1 if maybe < left else (2 if sure >= left else 0)
+            assert st[i] == want
+
+
+def _compare(res, ot, oo, omn, omx, exact_dist=True):
+    g = orc.group(res.target, res.obj2)
+    o = orc.group(ot, oo)
+    assert g == o, "result lists differ"
+    if exact_dist:
+        # the oracle reports in global order; compare per target in order
+        gd = orc.group(res.target, res.obj2, res.rmin, res.rmax)
+        od = orc.group(ot, oo, omn, omx)
+        assert gd == od, "reported distance ranges differ"
+
+
+@pytest.mark.parametrize("query,k,tau", [("nn", 3, 0), ("within", 1, 400.0), ("within", 1, 60.0)])
+def test_join_small_all_forms(tdp, eng, small_tiles, query, k, tau):
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    ot, oo, omn, omx, ost = orc.join(on, ov, query, k, tau, precision="f32")
+    r32 = eng.join(n, v, query=query, knn=k, within_dist=tau, precision="ref32")
+    _compare(r32, ot, oo, omn, omx)
+    assert r32.stats["tri_pairs"] == ost["tri_pairs"]
+    ot, oo, omn, omx, ost = orc.join(on, ov, query, k, tau, precision="f64")
+    r64 = eng.join(n, v, query=query, knn=k, within_dist=tau, precision="fp64")
+    _compare(r64, ot, oo, omn, omx)
+    rmx = eng.join(n, v, query=query, knn=k, within_dist=tau, precision="mixed")
+    _compare(rmx, ot, oo, omn, omx)
+    assert r64.stats["tri_pairs"] == rmx.stats["tri_pairs"] == ost["tri_pairs"]
+
+
+@pytest.mark.parametrize("lods", [(20,), (100,), (40, 20, 20), (20, 60, 100, 60), (50, 100)])
+def test_join_lod_lists(tdp, eng, small_tiles, lods):
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    ot, oo, omn, omx, _ = orc.join(on, ov, "nn", 3, 0, lods=lods, precision="f32")
+    r = eng.join(n, v, query="nn", knn=3, lods=lods, precision="ref32")
+    _compare(r, ot, oo, omn, omx)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_join_hausdorff_levels(tdp, eng, small_tiles, level):
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    ot, oo, omn, omx, _ = orc.join(on, ov, "nn", 2, 0, level=level, precision="f64")
+    r = eng.join(n, v, query="nn", knn=2, hausdorff_level=level, precision="mixed")
+    _compare(r, ot, oo, omn, omx)
+
+
+def test_self_join_knn(tdp, eng, small_tiles):
+    pn, _ = small_tiles
+    n = tdp.Tile.load(pn)
+    on = orc.OracleTile(pn)
+    ot, oo, omn, omx, _ = orc.join(on, None, "nn", 5, 0, precision="f64", max_objects1=80)
+    r = eng.join(n, None, query="nn", knn=5, precision="mixed", max_objects1=80)
+    _compare(r, ot, oo, omn, omx)
+    assert not np.any(r.target == r.obj2)
+
+
+def test_self_join_within(tdp, eng, small_tiles):
+    pn, _ = small_tiles
+    n = tdp.Tile.load(pn)
+    on = orc.OracleTile(pn)
+    ot, oo, omn, omx, _ = orc.join(on, None, "within", 1, 150.0, precision="f32")
+    r = eng.join(n, None, query="within", within_dist=150.0, precision="ref32")
+    _compare(r, ot, oo, omn, omx)
+
+
+def test_knn_k_exceeds_objects(tdp, eng, small_tiles):
+    """k >= the number of candidate objects: every candidate is reported
+    (fewer than k results), as the reference does"""
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    for k in (10, 12):
+        ot, oo, omn, omx, _ = orc.join(on, ov, "nn", k, 0, precision="f32")
+        r = eng.join(n, v, query="nn", knn=k, precision="ref32")
+        _compare(r, ot, oo, omn, omx)
+
+
+def test_within_zero_distance(tdp, eng, small_tiles):
+    """tau = 0 (the reference's intersect-by-within runs): only touching objects"""
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    ot, oo, omn, omx, _ = orc.join(on, ov, "within", 1, 0.0, precision="f64")
+    r = eng.join(n, v, query="within", within_dist=0.0, precision="mixed")
+    _compare(r, ot, oo, omn, omx)
+
+
+def test_tiny_bruteforce_exact_knn(tdp, eng, tiny_tiles):
+    """At the final LOD the reported kNN lists are the exact k nearest by
+    all-pairs triangle distance (the per-triangle bounds are conservative)."""
+    pn, pv = tiny_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    r = eng.join(n, v, query="nn", knn=2, precision="fp64")
+    an, av = n.arrays(), v.arrays()
+
+    def lod100_tris(a, o):
+        v0, v1 = a["obj_vox_off"][o], a["obj_vox_off"][o + 1]
+        rec = a["tri"].reshape(-1, 12)
+        off = a["vox_tri_off"].reshape(-1, 5)[v0:v1, 4]
+        cnt = a["vox_tri_cnt"].reshape(-1, 5)[v0:v1, 4]
+        return np.concatenate([rec[o_:o_ + c] for o_, c in zip(off, cnt)])
+
+    lists = r.lists()
+    for t in range(n.n_obj):
+        S = lod100_tris(an, t)
+        d = []
+        for o in range(v.n_obj):
+            T = lod100_tris(av, o)
+            rec = np.concatenate([S, T])
+            os_ = np.array([[0, len(S), len(S), len(T)]], np.uint64)
+            tri9 = rec[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]]
+            d.append(orc.meshdist(tri9, None, os_, "f64")[0, 0])
+        d = np.array(d)
+        kth = np.sort(d)[1]
+        got = lists.get(t, [])
+        assert len(got) >= 2
+        for o in got:
+            assert d[o] <= kth, (t, o, d[o], kth)
