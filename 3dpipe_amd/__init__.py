@@ -276,7 +276,7 @@ class Engine:
             rd = np.zeros(n, np.uint8)
             tdb.tdb_result_get(r, _ptr(t), _ptr(b), _ptr(mn), _ptr(mx), _ptr(rd))
             tm = np.zeros(7, np.float64)
-            st = np.zeros(6, np.uint64)
+            st = np.zeros(7, np.uint64)
             tdb.tdb_result_timing(r, _ptr(tm))
             tdb.tdb_result_stats(r, _ptr(st))
             nt = int(tdb.tdb_result_targets(r))
@@ -284,8 +284,8 @@ class Engine:
             tdb.tdb_result_free(r)
         timing = dict(zip(["total", "candidates", "filter", "refine", "filter_kernels",
                            "refine_kernels", "upload"], tm.tolist()))
-        stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_rechecks", "filter_voxel_pairs",
-                          "object_pairs", "rounds"], (int(x) for x in st)))
+        stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
+                          "object_pairs", "rounds", "fp32_culled"], (int(x) for x in st)))
         return JoinResult(t, b, mn, mx, rd, nt, timing, stats)
 
     def close(self):

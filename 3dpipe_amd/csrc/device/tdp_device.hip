@@ -9,6 +9,7 @@
 
 #include "../../../include/tdp.h"
 #include "tdp_kernels.hpp"
+#include "tdp_mixed.hpp"
 
 using namespace tdp;
 
@@ -32,6 +33,7 @@ struct tdp_tile {
     uint64_t *vox_tri_off = nullptr;
     uint32_t *vox_tri_cnt = nullptr;
     float4 *tri = nullptr;
+    float4 *bnd = nullptr;  // 2 per triangle: bounding sphere, plane
     size_t bytes = 0;
 };
 
@@ -52,6 +54,13 @@ struct tdp_join {
     uint32_t *v1[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr}, *vpp[2] = {nullptr, nullptr};
     void *vmin[2] = {nullptr, nullptr}, *vmax[2] = {nullptr, nullptr};
     uint32_t *keep = nullptr;
+    // mixed-precision pipeline state
+    float *mx_U = nullptr;
+    uint32_t *mx_probe = nullptr, *mx_flags = nullptr;
+    unsigned long long *mx_key = nullptr;
+    size_t cap_mx = 0;
+    uint32_t *item_vp = nullptr, *item_q = nullptr;
+    size_t cap_items = 0;
     uint64_t *pos = nullptr;
     size_t cap_vp = 0;
     int cur = 0;
@@ -86,6 +95,21 @@ struct tdp_join {
 
 static inline unsigned blocks_for(size_t n, size_t per_block) {
     return (unsigned)((n + per_block - 1) / per_block);
+}
+
+// grid for one-wave-per-item kernels (wave_global): 1-D up to 2^20 blocks,
+// 2-D beyond, so any 32-bit item count fits
+static inline dim3 wave_grid(size_t n) {
+    const size_t b = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+    const size_t x_max = (size_t)1 << 20;
+    if (b <= x_max) return dim3((unsigned)(b ? b : 1), 1, 1);
+    return dim3((unsigned)x_max, (unsigned)((b + x_max - 1) / x_max), 1);
+}
+
+// grid for grid-stride kernels
+static inline unsigned stride_grid(size_t n) {
+    const size_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)std::min<size_t>(b ? b : 1, (size_t)1 << 18);
 }
 
 template <typename T>
@@ -264,6 +288,17 @@ extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **
     up(&t->vox_tri_off, d->vox_tri_off, 5 * d->n_vox);
     up(&t->vox_tri_cnt, d->vox_tri_cnt, 5 * d->n_vox);
     up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, 2 * d->n_tri);
+    if (rc == TDP_OK && d->n_tri) {
+        t->bytes += 32 * d->n_tri;
+        k_tri_bounds<<<blocks_for(d->n_tri, kBlock), kBlock, 0, ctx->stream>>>(d->n_tri, t->tri, t->bnd);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("tile bounds: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    }
     if (rc != TDP_OK) {
         tdp_tile_free(t);
         return rc;
@@ -281,6 +316,7 @@ extern "C" int tdp_tile_free(tdp_tile *t) {
     dfree(t->vox_tri_off);
     dfree(t->vox_tri_cnt);
     dfree(t->tri);
+    dfree(t->bnd);
     delete t;
     return TDP_OK;
 }
@@ -342,26 +378,36 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
         }
     if (P == 0) return TDP_OK;
     const size_t osz = precision == TDP_PREC_REF32 ? 4 : 8;
-    float4 *dR = nullptr;
+    float4 *dR = nullptr, *dB = nullptr;
     uint64_t *dOS = nullptr;
     void *dO = nullptr;
     unsigned long long *dst = nullptr;
+    for (size_t p = 0; p < P; p++)
+        if (os[4 * p + 1] * os[4 * p + 3] > 0x7fffffffull) {
+            ctx->err = "tdp_meshdist_batch: pair " + std::to_string(p) + " has too many triangle pairs";
+            return TDP_ERR_ARG;
+        }
     int rc = dalloc(ctx, &dR, 3 * n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dB, 2 * n_tri);
     if (rc == TDP_OK) rc = dalloc(ctx, &dOS, 4 * P);
     if (rc == TDP_OK) rc = dalloc(ctx, (char **)&dO, 3 * P * osz);
     if (rc == TDP_OK) rc = dalloc(ctx, &dst, 4);
     if (rc == TDP_OK) {
         hipError_t e = hipMemcpy(dR, tri, 48 * n_tri, hipMemcpyHostToDevice);
+        if (e == hipSuccess && n_tri) {
+            k_tri_bounds<<<blocks_for(n_tri, kBlock), kBlock, 0, ctx->stream>>>(n_tri, dR, dB);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipMemcpy(dOS, os, 32 * P, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemset(dst, 0, 32);
         if (e == hipSuccess) {
-            const unsigned g = blocks_for(P, kWavesPerBlock);
+            const dim3 g = wave_grid(P);
             if (precision == TDP_PREC_REF32)
-                k_meshdist_batch<float, MODE_REF32><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, (float *)dO, dst);
+                k_meshdist_batch<float, MODE_REF32><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, dB, (float *)dO, dst);
             else if (precision == TDP_PREC_FP64)
-                k_meshdist_batch<double, MODE_FP64><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, (double *)dO, dst);
+                k_meshdist_batch<double, MODE_FP64><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, dB, (double *)dO, dst);
             else
-                k_meshdist_batch<double, MODE_MIXED><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, (double *)dO, dst);
+                k_meshdist_batch<double, MODE_MIXED><<<g, kBlock, 0, ctx->stream>>>(P, dOS, dR, dB, (double *)dO, dst);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -372,6 +418,7 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
         }
     }
     dfree(dR);
+    dfree(dB);
     dfree(dOS);
     dfree(dO);
     dfree(dst);
@@ -419,7 +466,8 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
     void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
-                    j->offs, j->keep, j->pos, j->stats};
+                    j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
+                    j->mx_key, j->item_vp, j->item_q};
     for (void *p : ptrs) dfree(p);
     for (int b = 0; b < 2; b++) {
         dfree(j->v1[b]); dfree(j->v2[b]); dfree(j->vpp[b]); dfree(j->vmin[b]); dfree(j->vmax[b]);
@@ -541,6 +589,36 @@ static int ensure_vp(tdp_join *j, size_t n) {
     return rc;
 }
 
+// per-voxel-pair state of the mixed pipeline
+static int ensure_mx(tdp_join *j, size_t n) {
+    if (n <= j->cap_mx) return TDP_OK;
+    tdp_ctx *ctx = j->ctx;
+    const size_t cap = std::max(n, j->cap_vp);
+    dfree(j->mx_U); dfree(j->mx_probe); dfree(j->mx_flags); dfree(j->mx_key);
+    j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
+    j->cap_mx = 0;
+    int rc = dalloc(ctx, &j->mx_U, 3 * cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_probe, 3 * cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_flags, cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_key, 3 * cap);
+    if (rc == TDP_OK) j->cap_mx = cap;
+    return rc;
+}
+
+// flat FP64 work list (voxel pair, triangle pair), grow-only
+static int ensure_items(tdp_join *j, size_t n) {
+    if (n <= j->cap_items) return TDP_OK;
+    tdp_ctx *ctx = j->ctx;
+    const size_t cap = std::max(n + n / 4, (size_t)1 << 16);
+    dfree(j->item_vp); dfree(j->item_q);
+    j->item_vp = nullptr; j->item_q = nullptr;
+    j->cap_items = 0;
+    int rc = dalloc(ctx, &j->item_vp, cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->item_q, cap);
+    if (rc == TDP_OK) j->cap_items = cap;
+    return rc;
+}
+
 template <typename R>
 static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     tdp_ctx *ctx = j->ctx;
@@ -551,7 +629,7 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     hipStream_t s = ctx->stream;
     TDP_CHECK(ctx, hipEventRecord(j->ev[0], s));
     if (P) {
-        k_aggregate<R><<<blocks_for(P, kWavesPerBlock), kBlock, 0, s>>>(
+        k_aggregate<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
             j->t2->obj_vox_off, j->t2->vox_box, rmin, rmax);
         TDP_LAUNCHED(ctx);
@@ -567,7 +645,7 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
                                                                           j->state, j->round);
             TDP_LAUNCHED(ctx);
         }
-        k_survivor_count<R><<<blocks_for(P, kWavesPerBlock), kBlock, 0, s>>>(
+        k_survivor_count<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
             j->t2->obj_vox_off, j->t2->vox_box, rmax, within, tau, j->state, j->round, j->count);
         TDP_LAUNCHED(ctx);
@@ -579,7 +657,7 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     if (rc != TDP_OK) return rc;
     j->cur = 0;
     if (total) {
-        k_survivor_write<R><<<blocks_for(P, kWavesPerBlock), kBlock, 0, s>>>(
+        k_survivor_write<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
             j->t2->obj_vox_off, j->t2->vox_box, rmax, within, tau, j->state, j->offs, j->v1[0],
             j->v2[0], j->vpp[0], (R *)j->vmin[0], (R *)j->vmax[0]);
@@ -625,12 +703,56 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     k_reset_acc<R, KT><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (KT *)j->acc_min,
                                                                 (KT *)j->acc_max);
     TDP_LAUNCHED(ctx);
-    k_refine<R, MODE><<<blocks_for(n, kWavesPerBlock), kBlock, 0, s>>>(
-        n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], j->t1->vox_tri_off,
-        j->t1->vox_tri_cnt, j->t1->tri, j->t2->vox_tri_off, j->t2->vox_tri_cnt, j->t2->tri,
-        j->t1->vox_obj, j->t2->vox_obj, j->t1->obj_hd, j->t2->obj_hd, slot, is_last,
-        j->prm.hausdorff_level, (KT *)j->acc_min, (KT *)j->acc_max, j->stats);
-    TDP_LAUNCHED(ctx);
+    const tdp_tile *A = j->t1, *B = j->t2;
+    if (MODE != MODE_MIXED) {
+        k_refine<R, MODE><<<wave_grid(n), kBlock, 0, s>>>(
+            n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_off,
+            A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, A->bnd, B->bnd,
+            A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last, j->prm.hausdorff_level,
+            (KT *)j->acc_min, (KT *)j->acc_max, j->stats);
+        TDP_LAUNCHED(ctx);
+    } else {
+        int rc = ensure_mx(j, n);
+        if (rc != TDP_OK) return rc;
+        const dim3 gw = wave_grid(n);
+        k_mx_bounds1<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                           A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
+                                           B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags,
+                                           j->mx_key);
+        TDP_LAUNCHED(ctx);
+        k_mx_eval<<<stride_grid(3 * n), kBlock, 0, s>>>(
+            3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+            A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
+        TDP_LAUNCHED(ctx);
+        k_mx_count<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                         A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
+                                         B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags,
+                                         j->mx_key, j->keep, j->stats);
+        TDP_LAUNCHED(ctx);
+        uint64_t items = 0;
+        rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &items);
+        if (rc != TDP_OK) return rc;
+        if (items) {
+            rc = ensure_items(j, items);
+            if (rc != TDP_OK) return rc;
+            k_mx_write<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                             A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
+                                             B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags,
+                                             j->mx_key, j->pos, j->item_vp, j->item_q);
+            TDP_LAUNCHED(ctx);
+            k_mx_eval<<<stride_grid(items), kBlock, 0, s>>>(
+                items, j->item_vp, j->item_q, 0, j->v1[c], j->v2[c], A->vox_tri_off,
+                A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key,
+                j->stats);
+            TDP_LAUNCHED(ctx);
+        }
+        k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+            n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
+            B->vox_tri_cnt, A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last,
+            j->prm.hausdorff_level, j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
+            j->stats);
+        TDP_LAUNCHED(ctx);
+    }
     TDP_CHECK(ctx, hipEventRecord(j->ev[3], s));
     k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
                                                               (const KT *)j->acc_max, rmin, rmax);
@@ -711,14 +833,15 @@ extern "C" int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin
     return TDP_OK;
 }
 
-extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out4) {
-    if (!j || !out4) return TDP_ERR_ARG;
+extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
+    if (!j || !out5) return TDP_ERR_ARG;
     unsigned long long s[4];
     TDP_CHECK(j->ctx, hipMemcpy(s, j->stats, sizeof s, hipMemcpyDeviceToHost));
-    out4[0] = s[0];
-    out4[1] = s[1];
-    out4[2] = s[2];
-    out4[3] = j->filter_vp;
+    out5[0] = s[0];
+    out5[1] = s[1];
+    out5[2] = s[2];
+    out5[3] = j->filter_vp;
+    out5[4] = s[3];
     return TDP_OK;
 }
 

@@ -22,6 +22,23 @@ __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(
 template <>
 __device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqrt(x); }
 
+// Arithmetic policies for tri_dist: Exact keeps IEEE division and square
+// root (the reference's results bit for bit); Fast uses the hardware
+// reciprocal and square root (about 1 ulp each), for FP32 estimates whose
+// error is bounded separately.
+struct Exact {
+    template <typename R>
+    static __device__ __forceinline__ R div(R a, R b) { return a / b; }
+    template <typename R>
+    static __device__ __forceinline__ R sqrt(R a) { return dsqrt<R>(a); }
+};
+struct Fast {
+    static __device__ __forceinline__ float div(float a, float b) {
+        return a * __builtin_amdgcn_rcpf(b);
+    }
+    static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
+
 #define TDP_SUB(r, a, b) { (r)[0] = (a)[0] - (b)[0]; (r)[1] = (a)[1] - (b)[1]; (r)[2] = (a)[2] - (b)[2]; }
 #define TDP_ADD(r, a, b) { (r)[0] = (a)[0] + (b)[0]; (r)[1] = (a)[1] + (b)[1]; (r)[2] = (a)[2] + (b)[2]; }
 #define TDP_ADDS(r, a, b, s) { (r)[0] = (a)[0] + (b)[0] * (s); (r)[1] = (a)[1] + (b)[1] * (s); (r)[2] = (a)[2] + (b)[2] * (s); }
@@ -31,7 +48,7 @@ __device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqr
 
 // closest points on segments P + tA and Q + uB; only the separating vector
 // VEC and the closest points X, Y are produced (SegPoints)
-template <typename R>
+template <typename R, typename Ar = Exact>
 __device__ __forceinline__ void seg_points(R VEC[3], R X[3], R Y[3], const R *P, const R *A,
                                            const R *Q, const R *B) {
     R T[3], TMP[3];
@@ -41,18 +58,18 @@ __device__ __forceinline__ void seg_points(R VEC[3], R X[3], R Y[3], const R *P,
     R t, u;
     const R denom = AA * BB - AB * AB;
     if (denom == 0) t = 0;
-    else t = (AT * BB - BT * AB) / denom;
+    else t = Ar::div(AT * BB - BT * AB, denom);
     if (BB == 0) u = 0;
-    else u = (t * AB - BT) / BB;
+    else u = Ar::div(t * AB - BT, BB);
     if (u <= 0) {
         TDP_COPY(Y, Q);
-        if (AA == 0) t = 0; else t = AT / AA;
+        if (AA == 0) t = 0; else t = Ar::div(AT, AA);
         if (t <= 0) { TDP_COPY(X, P); TDP_SUB(VEC, Q, P); }
         else if (t >= 1) { TDP_ADD(X, P, A); TDP_SUB(VEC, Q, X); }
         else { TDP_ADDS(X, P, A, t); TDP_CROSS(TMP, T, A); TDP_CROSS(VEC, A, TMP); }
     } else if (u >= 1) {
         TDP_ADD(Y, Q, B);
-        if (AA == 0) t = 0; else t = (AB + AT) / AA;
+        if (AA == 0) t = 0; else t = Ar::div(AB + AT, AA);
         if (t <= 0) { TDP_COPY(X, P); TDP_SUB(VEC, Y, P); }
         else if (t >= 1) { TDP_ADD(X, P, A); TDP_SUB(VEC, Y, X); }
         else {
@@ -79,7 +96,7 @@ __device__ __forceinline__ void seg_points(R VEC[3], R X[3], R Y[3], const R *P,
 
 // vertex-face case for the face of A against the vertices of B; returns -1
 // when it does not apply
-template <typename R>
+template <typename R, typename Ar = Exact>
 __device__ __forceinline__ R face_case(const R *A, const R Av[3][3], const R *B, bool &disjoint,
                                        bool a_first) {
     R N[3], V[3], Z[3], Bp[3];
@@ -110,12 +127,12 @@ __device__ __forceinline__ R face_case(const R *A, const R Av[3][3], const R *B,
                     TDP_CROSS(Z, N, Av[2]);
                     if (TDP_DOT(V, Z) > 0) {
                         R P[3], Q[3];
-                        const R s = Bp[point] / Nl;
+                        const R s = Ar::div(Bp[point], Nl);
                         if (a_first) { TDP_ADDS(P, Bv, N, s); TDP_COPY(Q, Bv); }
                         else { TDP_COPY(P, Bv); TDP_ADDS(Q, Bv, N, s); }
                         const R d2 = (P[0] - Q[0]) * (P[0] - Q[0]) + (P[1] - Q[1]) * (P[1] - Q[1]) +
                                      (P[2] - Q[2]) * (P[2] - Q[2]);
-                        return dsqrt<R>(d2);
+                        return Ar::sqrt(d2);
                     }
                 }
             }
@@ -132,7 +149,7 @@ __device__ __forceinline__ R face_case(const R *A, const R Av[3][3], const R *B,
 // the result is the unverified fallback (smallest edge distance when some test
 // showed the triangles disjoint, else 0), which is where FP32 and FP64 may
 // take different branches.
-template <typename R>
+template <typename R, typename Ar = Exact>
 __device__ __forceinline__ R tri_dist(const R *S, const R *T, bool *cert = nullptr) {
     R Sv[3][3], Tv[3][3];
     TDP_SUB(Sv[0], S + 3, S); TDP_SUB(Sv[1], S + 6, S + 3); TDP_SUB(Sv[2], S, S + 6);
@@ -144,7 +161,7 @@ __device__ __forceinline__ R tri_dist(const R *S, const R *T, bool *cert = nullp
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             R VEC[3], P[3], Q[3], V[3], Z[3];
-            seg_points<R>(VEC, P, Q, S + i * 3, Sv[i], T + j * 3, Tv[j]);
+            seg_points<R, Ar>(VEC, P, Q, S + i * 3, Sv[i], T + j * 3, Tv[j]);
             TDP_SUB(V, Q, P);
             const R dd = TDP_DOT(V, V);
             if (dd <= mindd) {
@@ -155,7 +172,7 @@ __device__ __forceinline__ R tri_dist(const R *S, const R *T, bool *cert = nullp
                 R b = TDP_DOT(Z, VEC);
                 if ((a <= 0) && (b >= 0)) {
                     if (cert) *cert = true;
-                    return dsqrt<R>(mindd);
+                    return Ar::sqrt(mindd);
                 }
                 const R p = TDP_DOT(V, VEC);
                 if (a < 0) a = 0;
@@ -165,12 +182,12 @@ __device__ __forceinline__ R tri_dist(const R *S, const R *T, bool *cert = nullp
         }
     }
     if (cert) *cert = true;
-    R r = face_case<R>(S, Sv, T, disjoint, true);
+    R r = face_case<R, Ar>(S, Sv, T, disjoint, true);
     if (r != (R)-1) return r;
-    r = face_case<R>(T, Tv, S, disjoint, false);
+    r = face_case<R, Ar>(T, Tv, S, disjoint, false);
     if (r != (R)-1) return r;
     if (cert) *cert = false;
-    if (disjoint) return dsqrt<R>(mindd);
+    if (disjoint) return Ar::sqrt(mindd);
     return (R)0;
 }
 

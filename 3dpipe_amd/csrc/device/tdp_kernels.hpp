@@ -16,8 +16,11 @@ enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_REPORTED = 2 };
 enum { MODE_REF32 = 0, MODE_FP64 = 1, MODE_MIXED = 2 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+// one wave per item; the grid may be 2-D (x fastest) when the item count
+// exceeds what a 1-D launch can address (2^32 work-items per dimension)
 __device__ __forceinline__ size_t wave_global() {
-    return (size_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+    return b * kWavesPerBlock + (threadIdx.x >> 6);
 }
 
 template <typename R>
@@ -310,122 +313,227 @@ __device__ __forceinline__ void pair_values(const float4 *rec1, uint64_t i1, con
     hi = d + (R)h1 + (R)h2;
 }
 
-// FP32 estimate for the mixed form: TriDist on coordinates translated to
-// the first vertex of S (each translated coordinate carries one rounding),
-// with an absolute bound e on |d32 - d64|. A result without a certificate
-// (see tri_dist) or from a near-degenerate triangle is "fragile": it gets the
-// interval [0, inf) and is always re-evaluated in FP64.
-struct F32Est {
-    float d, e;
-    bool fragile;
-};
+// ------------------------------------------------------- mixed precision
+//
+// The mixed form returns, per voxel pair, exactly the FP64 minima of
+// (d, lo, hi) = (TriDist, max(d - ph1 - ph2, 0), d + h1 + h2) over its
+// n1 x n2 triangle pairs while evaluating FP64 TriDist on few of them:
+//   1. FP32, every pair: a rigorous interval for its FP64 TriDist value from
+//      per-triangle bounding spheres (about 20 flops), giving the smallest
+//      upper bound U of each quantity;
+//   2. FP64, one pair per lane (the lane's smallest lower bound): exact
+//      values tighten U;
+//   3. FP64 on every remaining pair whose interval still reaches U,
+//      compacted through an LDS queue so that all 64 lanes work.
+// A pair that is not evaluated has a lower bound above U, so it cannot hold
+// a minimum; the result equals the all-FP64 evaluation bit for bit.
+//
+// Why the bounds hold for the reference's TriDist: every value it returns is
+// the distance between two points of the triangles, hence at most the
+// sphere upper bound and at least the true distance -- except a 0 returned
+// for separated triangles when none of its tests showed them disjoint. A
+// positive lower bound is therefore used only when the other triangle's
+// sphere lies strictly on one side of a triangle's plane, which makes
+// TriDist's face test flag the pair as disjoint (no 0 return). (An FP32
+// evaluation of TriDist itself is not used: its t = (A.T|B|^2 - B.T A.B)/denom
+// loses accuracy like 1/sin^2 of the edge angle and its branch structure is
+// discontinuous, so no forward error bound for it is rigorous.)
 
 constexpr float kU = 5.9604645e-08f;    // 2^-24
-constexpr float kBoundK = 1024.f;       // e = kBoundK * u * (L + d)
+constexpr uint32_t kMixedMinPairs = 2 * kWave;  // below this, evaluate every pair in FP64
 
-__device__ __forceinline__ F32Est f32_estimate(const float4 *rec1, uint64_t i1,
-                                               const float4 *rec2, uint64_t i2, float &ph1,
-                                               float &h1, float &ph2, float &h2) {
-    float S[9], T[9];
-    load_tri<float>(rec1, i1, S, ph1, h1);
-    load_tri<float>(rec2, i2, T, ph2, h2);
-    const float ox = S[0], oy = S[1], oz = S[2];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        S[3 * k] -= ox; S[3 * k + 1] -= oy; S[3 * k + 2] -= oz;
-        T[3 * k] -= ox; T[3 * k + 1] -= oy; T[3 * k + 2] -= oz;
+// per triangle: {centre.xyz, radius}, {unit normal.xyz, max |centre coord|}
+__global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *__restrict__ rec,
+                                                       float4 *__restrict__ bnd) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = rec[3 * i], b = rec[3 * i + 1], c = rec[3 * i + 2];
+    const float cx = ((a.x + b.x) + c.x) * (1.f / 3.f);
+    const float cy = ((a.y + b.y) + c.y) * (1.f / 3.f);
+    const float cz = ((a.z + b.z) + c.z) * (1.f / 3.f);
+    auto d = [&](float x, float y, float z) {
+        return __builtin_sqrtf((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+    };
+    float r = fmaxf(d(a.x, a.y, a.z), fmaxf(d(b.x, b.y, b.z), d(c.x, c.y, c.z)));
+    r = r * (1.f + 64.f * kU) + 1e-30f;
+    const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+    const float e2x = c.x - b.x, e2y = c.y - b.y, e2z = c.z - b.z;
+    float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const float nl = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
+    const float e = fmaxf(e1x * e1x + e1y * e1y + e1z * e1z, e2x * e2x + e2y * e2y + e2z * e2z);
+    if (nl > 1e-6f * e && nl * nl > 1e-12f) {
+        nx /= nl; ny /= nl; nz /= nl;
+    } else {  // near-degenerate: no plane test
+        nx = ny = nz = 0.f;
     }
-    float L = 0.f;
-#pragma unroll
-    for (int k = 0; k < 9; k++) L = fmaxf(L, fmaxf(fabsf(S[k]), fabsf(T[k])));
-    F32Est r;
-    bool cert = false;
-    r.d = tri_dist<float>(S, T, &cert);
-    float e1[3], e2[3], n[3];
-    TDP_SUB(e1, (S + 3), S); TDP_SUB(e2, (S + 6), (S + 3)); TDP_CROSS(n, e1, e2);
-    const float sn = TDP_DOT(n, n);
-    TDP_SUB(e1, (T + 3), T); TDP_SUB(e2, (T + 6), (T + 3)); TDP_CROSS(n, e1, e2);
-    const float tn = TDP_DOT(n, n);
-    const float L4 = (L * L) * (L * L);
-    r.fragile = !cert || !(sn > 1e-12f * L4) || !(tn > 1e-12f * L4);
-    r.e = kBoundK * kU * (L + r.d);
-    return r;
+    const float cm = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+    bnd[2 * i] = make_float4(cx, cy, cz, r);
+    bnd[2 * i + 1] = make_float4(nx, ny, nz, cm);
 }
 
-// intervals for (d, lo, hi) of one triangle pair in FP64 from its FP32 estimate
-__device__ __forceinline__ void f32_intervals(const F32Est &es, float ph1, float h1, float ph2,
-                                              float h2, float lw[3], float uw[3]) {
-    if (es.fragile) {
-        lw[0] = lw[1] = lw[2] = 0.f;
-        uw[0] = uw[1] = uw[2] = INFINITY;
-        return;
-    }
-    const float s = es.e + 8.f * kU * (es.d + ph1 + ph2 + h1 + h2);
-    lw[0] = fmaxf(es.d - s, 0.f);
-    uw[0] = es.d + s;
-    lw[1] = fmaxf(es.d - ph1 - ph2 - s, 0.f);
-    uw[1] = fmaxf(es.d - ph1 - ph2 + s, 0.f);
-    lw[2] = es.d + h1 + h2 - s;
-    uw[2] = es.d + h1 + h2 + s;
+// (d, lo, hi) intervals from an interval [dl, du] of the FP64 TriDist value
+__device__ __forceinline__ void intervals(float dl, float du, float ph1, float h1, float ph2,
+                                          float h2, float lw[3], float uw[3]) {
+    const float s = 8.f * kU * (du + ph1 + ph2 + h1 + h2);
+    lw[0] = dl;
+    uw[0] = du;
+    lw[1] = fmaxf(dl - ph1 - ph2 - s, 0.f);
+    uw[1] = fmaxf(du - ph1 - ph2 + s, 0.f);
+    lw[2] = dl + h1 + h2 - s;
+    uw[2] = du + h1 + h2 + s;
 }
 
-// Exact FP64 minima of (d, lo, hi) over the n1 x n2 triangle pairs of a
-// voxel pair, evaluating FP64 only where an FP32 interval reaches the
-// smallest FP32 upper bound. Every pair whose FP64 value could be the
-// minimum is re-evaluated, so the result equals the all-FP64 result bit for
-// bit. Per lane, the best and second-best lower bound are tracked for each
-// quantity; a lane whose second-best also qualifies re-scans its pairs.
-// Called by all 64 lanes of a wave; results are per lane (wave-min after).
-__device__ __forceinline__ void mixed_minima(const float4 *rec1, uint64_t o1,
-                                             const float4 *rec2, uint64_t o2, uint64_t n2,
-                                             uint64_t total, int lane, double &d, double &lo,
-                                             double &hi, uint64_t &rechecks) {
+// Interval of one triangle pair's FP64 TriDist value.
+//  upper: centre distance + both radii (every point pair is closer);
+//  lower: the gap between the two triangles' projections on the axis through
+//         their centres (no point pair is closer than a separating gap),
+//         used only when one triangle's vertices lie strictly on one side of
+//         the other's plane (see above), else 0.
+// All terms carry margins for FP32 rounding, including the rounded centroid's
+// distance from its triangle's plane.
+__device__ __forceinline__ void pair_interval(const float4 *r1, const float4 *b1, uint64_t i1,
+                                              const float4 *r2, const float4 *b2, uint64_t i2,
+                                              float lw[3], float uw[3]) {
+    const float4 s1 = b1[2 * i1], p1 = b1[2 * i1 + 1];
+    const float4 s2 = b2[2 * i2], p2 = b2[2 * i2 + 1];
+    const float4 a0 = r1[3 * i1], a1 = r1[3 * i1 + 1], a2 = r1[3 * i1 + 2];
+    const float4 c0 = r2[3 * i2], c1 = r2[3 * i2 + 1], c2 = r2[3 * i2 + 2];
+    const float dx = s2.x - s1.x, dy = s2.y - s1.y, dz = s2.z - s1.z;
+    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    const float rs = s1.w + s2.w;
+    const float m = 32.f * kU * (dc + rs + p1.w + p2.w) + 1e-6f;
+    const float du = dc + rs + m;
+    float dl = 0.f;
+    // vertices relative to the first centre
+    const float ax0 = a0.x - s1.x, ay0 = a0.y - s1.y, az0 = a0.z - s1.z;
+    const float ax1 = a1.x - s1.x, ay1 = a1.y - s1.y, az1 = a1.z - s1.z;
+    const float ax2 = a2.x - s1.x, ay2 = a2.y - s1.y, az2 = a2.z - s1.z;
+    const float cx0 = c0.x - s1.x, cy0 = c0.y - s1.y, cz0 = c0.z - s1.z;
+    const float cx1 = c1.x - s1.x, cy1 = c1.y - s1.y, cz1 = c1.z - s1.z;
+    const float cx2 = c2.x - s1.x, cy2 = c2.y - s1.y, cz2 = c2.z - s1.z;
+    // plane of triangle 1 against the vertices of triangle 2 (and vice versa)
+    const float t0 = p1.x * cx0 + p1.y * cy0 + p1.z * cz0;
+    const float t1 = p1.x * cx1 + p1.y * cy1 + p1.z * cz1;
+    const float t2 = p1.x * cx2 + p1.y * cy2 + p1.z * cz2;
+    const float w0 = p2.x * (ax0 - dx) + p2.y * (ay0 - dy) + p2.z * (az0 - dz);
+    const float w1 = p2.x * (ax1 - dx) + p2.y * (ay1 - dy) + p2.z * (az1 - dz);
+    const float w2 = p2.x * (ax2 - dx) + p2.y * (ay2 - dy) + p2.z * (az2 - dz);
+    const bool sep1 = fminf(t0, fminf(t1, t2)) > m || fmaxf(t0, fmaxf(t1, t2)) < -m;
+    const bool sep2 = fminf(w0, fminf(w1, w2)) > m || fmaxf(w0, fmaxf(w1, w2)) < -m;
+    if ((sep1 || sep2) && dc > 0.f) {
+        const float inv = __builtin_amdgcn_rcpf(dc);
+        const float ux = dx * inv, uy = dy * inv, uz = dz * inv;
+        const float smax = fmaxf(ux * ax0 + uy * ay0 + uz * az0,
+                                 fmaxf(ux * ax1 + uy * ay1 + uz * az1, ux * ax2 + uy * ay2 + uz * az2));
+        const float tmin = fminf(ux * cx0 + uy * cy0 + uz * cz0,
+                                 fminf(ux * cx1 + uy * cy1 + uz * cz1, ux * cx2 + uy * cy2 + uz * cz2));
+        dl = fmaxf(tmin - smax - m, 0.f);
+    }
+    intervals(dl, du, a0.w, a1.w, c0.w, c1.w, lw, uw);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the next float above a non-negative x (x itself for inf / nan)
+__device__ __forceinline__ float ulp_up(float x) {
+    if (!(x < INFINITY)) return x;
+    const uint32_t b = __float_as_uint(x);
+    return x >= 0.f ? __uint_as_float(b + 1u) : __uint_as_float(b - 1u);
+}
+
+__device__ __forceinline__ bool reaches(const float lw[3], const float U[3]) {
+    return lw[0] <= U[0] || lw[1] <= U[1] || lw[2] <= U[2];
+}
+
+// Exact FP64 minima of (d, lo, hi) for one voxel pair; called by all 64 lanes
+// of a wave (per-lane partial minima, wave-min after). `queue` is this
+// wave's 128-entry LDS buffer.
+__device__ __forceinline__ void mixed_minima(const float4 *rec1, const float4 *bnd1, uint64_t o1,
+                                             const float4 *rec2, const float4 *bnd2, uint64_t o2,
+                                             uint32_t n2, uint32_t total, int lane,
+                                             uint32_t *queue, double &d, double &lo, double &hi,
+                                             uint64_t &evals64, uint64_t &culled) {
+    float lw[3], uw[3];
+    // 1. sphere intervals: smallest upper bounds, and each lane's most
+    //    promising pair (smallest lower bound of the distance)
     float up[3] = {INFINITY, INFINITY, INFINITY};
-    float b1[3] = {INFINITY, INFINITY, INFINITY}, b2[3] = {INFINITY, INFINITY, INFINITY};
-    uint64_t i1[3] = {0, 0, 0};
-    for (uint64_t q = lane; q < total; q += kWave) {
-        const uint64_t i = q / n2, j = q - i * n2;
-        float ph1, h1, ph2, h2, lw[3], uw[3];
-        const F32Est es = f32_estimate(rec1, o1 + i, rec2, o2 + j, ph1, h1, ph2, h2);
-        f32_intervals(es, ph1, h1, ph2, h2, lw, uw);
+    float best = INFINITY;
+    uint32_t best_q = 0xffffffffu;
+    for (uint32_t q = lane; q < total; q += kWave) {
+        const uint32_t i = q / n2, j = q - i * n2;
+        pair_interval(rec1, bnd1, o1 + i, rec2, bnd2, o2 + j, lw, uw);
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
-            up[f] = fminf(up[f], uw[f]);
-            if (lw[f] < b1[f]) { b2[f] = b1[f]; b1[f] = lw[f]; i1[f] = q; }
-            else if (lw[f] < b2[f]) b2[f] = lw[f];
-        }
+        for (int f = 0; f < 3; f++) up[f] = fminf(up[f], uw[f]);
+        if (lw[0] < best) { best = lw[0]; best_q = q; }
     }
     float U[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) U[f] = wave_min(up[f]);
-    const bool rescan = (b2[0] <= U[0]) || (b2[1] <= U[1]) || (b2[2] <= U[2]);
-    if (!rescan) {
-#pragma unroll
-        for (int f = 0; f < 3; f++) {
-            if (!(b1[f] <= U[f])) continue;
-            bool seen = false;
-            for (int g = 0; g < f; g++) seen |= (b1[g] <= U[g]) && (i1[g] == i1[f]);
-            if (seen) continue;
-            const uint64_t i = i1[f] / n2, j = i1[f] - i * n2;
+    // lo >= 0: if some pair certainly has lo = 0, the minimum is exactly +0
+    const bool lo_zero = (U[1] == 0.f);
+    if (lo_zero) { lo = 0.0; U[1] = -1.f; }
+    // 2. FP64 on each lane's most promising pair
+    if (best_q != 0xffffffffu) {
+        const uint32_t i = best_q / n2, j = best_q - i * n2;
+        double a, b, c;
+        pair_values<double>(rec1, o1 + i, rec2, o2 + j, a, b, c);
+        d = rmin2(d, a); lo = rmin2(lo, b); hi = rmin2(hi, c);
+        evals64++;
+    }
+    {
+        const float Ud = wave_min((float)d), Uh = wave_min((float)hi);
+        // the FP64 values rounded to FP32 may round down; step up one ulp
+        U[0] = fminf(U[0], ulp_up(Ud));
+        U[2] = fminf(U[2], ulp_up(Uh));
+        if (!lo_zero) {
+            const float Ul = wave_min((float)lo);
+            U[1] = fminf(U[1], ulp_up(Ul));
+        }
+    }
+    // 3. FP64 on every other pair whose interval reaches U
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < total; base += kWave) {
+        const uint32_t q = base + lane;
+        bool cand = false;
+        if (q < total && q != best_q) {
+            const uint32_t i = q / n2, j = q - i * n2;
+            pair_interval(rec1, bnd1, o1 + i, rec2, bnd2, o2 + j, lw, uw);
+            cand = reaches(lw, U);
+            culled += cand ? 0 : 1;
+        }
+        const uint64_t mask = __ballot(cand);
+        if (cand) queue[cnt + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = q;
+        cnt += (uint32_t)__popcll(mask);
+        wave_sync();
+        if (cnt >= kWave) {
+            const uint32_t qq = queue[lane];
+            const uint32_t i = qq / n2, j = qq - i * n2;
             double a, b, c;
             pair_values<double>(rec1, o1 + i, rec2, o2 + j, a, b, c);
             d = rmin2(d, a); lo = rmin2(lo, b); hi = rmin2(hi, c);
-            rechecks++;
-        }
-    } else {
-        for (uint64_t q = lane; q < total; q += kWave) {
-            const uint64_t i = q / n2, j = q - i * n2;
-            float ph1, h1, ph2, h2, lw[3], uw[3];
-            const F32Est es = f32_estimate(rec1, o1 + i, rec2, o2 + j, ph1, h1, ph2, h2);
-            f32_intervals(es, ph1, h1, ph2, h2, lw, uw);
-            if (lw[0] <= U[0] || lw[1] <= U[1] || lw[2] <= U[2]) {
-                double a, b, c;
-                pair_values<double>(rec1, o1 + i, rec2, o2 + j, a, b, c);
-                d = rmin2(d, a); lo = rmin2(lo, b); hi = rmin2(hi, c);
-                rechecks++;
-            }
+            evals64++;
+            const uint32_t rest = cnt - kWave;
+            const uint32_t mv = lane < (int)rest ? queue[kWave + lane] : 0u;
+            wave_sync();
+            if (lane < (int)rest) queue[lane] = mv;
+            cnt = rest;
+            wave_sync();
         }
     }
+    if (lane < (int)cnt) {
+        const uint32_t qq = queue[lane];
+        const uint32_t i = qq / n2, j = qq - i * n2;
+        double a, b, c;
+        pair_values<double>(rec1, o1 + i, rec2, o2 + j, a, b, c);
+        d = rmin2(d, a); lo = rmin2(lo, b); hi = rmin2(hi, c);
+        evals64++;
+    }
+    wave_sync();
+    if (lo_zero) lo = 0.0;
 }
 
 // Per voxel pair: (d, lo, hi) = min over the n1 x n2 triangle pairs, folded
@@ -444,10 +552,12 @@ __global__ void __launch_bounds__(kBlock) k_refine(
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
     const float4 *__restrict__ rec1, const uint64_t *__restrict__ off2,
     const uint32_t *__restrict__ cnt2, const float4 *__restrict__ rec2,
+    const float4 *__restrict__ bnd1, const float4 *__restrict__ bnd2,
     const uint32_t *__restrict__ vobj1, const uint32_t *__restrict__ vobj2,
     const float *__restrict__ ohd1, const float *__restrict__ ohd2, int slot, int is_last,
     int level, typename Key<R>::type *__restrict__ acc_min,
     typename Key<R>::type *__restrict__ acc_max, unsigned long long *__restrict__ stats) {
+    __shared__ uint32_t queue[kWavesPerBlock][2 * kWave];
     const size_t w = wave_global();
     if (w >= n_vp) return;
     const int lane = lane_id();
@@ -468,8 +578,9 @@ __global__ void __launch_bounds__(kBlock) k_refine(
     }
     const uint32_t total = n1 * n2;
     R d = (R)__builtin_inff(), lo = (R)__builtin_inff(), hi = (R)__builtin_inff();
-    uint64_t rechecks = 0;
-    if (MODE != MODE_MIXED) {
+    uint64_t evals64 = 0, culled = 0;
+    // small voxel pairs: two FP64 passes cost no more than the bound stage
+    if (MODE != MODE_MIXED || total <= kMixedMinPairs) {
         for (uint32_t q = lane; q < total; q += kWave) {
             const uint32_t i = q / n2, j = q - i * n2;
             R a, b, c;
@@ -477,16 +588,21 @@ __global__ void __launch_bounds__(kBlock) k_refine(
             d = rmin2(d, a);
             lo = rmin2(lo, b);
             hi = rmin2(hi, c);
+            if (MODE == MODE_MIXED) evals64++;
         }
     } else {
         double dd = INFINITY, dl = INFINITY, dh = INFINITY;
-        mixed_minima(rec1, o1, rec2, o2, n2, total, lane, dd, dl, dh, rechecks);
+        mixed_minima(rec1, bnd1, o1, rec2, bnd2, o2, n2, total, lane,
+                     queue[threadIdx.x >> 6], dd, dl, dh, evals64, culled);
         d = (R)dd; lo = (R)dl; hi = (R)dh;
     }
     d = wave_min(d);
     lo = wave_min(lo);
     hi = wave_min(hi);
-    if (MODE == MODE_MIXED) rechecks = wave_sum_u64(rechecks);
+    if (MODE == MODE_MIXED) {
+        evals64 = wave_sum_u64(evals64);
+        culled = wave_sum_u64(culled);
+    }
     if (lane == 0) {
         R nmin = omin, nmax = omax;
         if (is_last) {
@@ -511,7 +627,10 @@ __global__ void __launch_bounds__(kBlock) k_refine(
         atomicMin(&acc_max[p], Key<R>::enc(nmax));
         atomicAdd(&stats[0], (unsigned long long)total);
         atomicAdd(&stats[1], 1ull);
-        if (MODE == MODE_MIXED) atomicAdd(&stats[2], (unsigned long long)rechecks);
+        if (MODE == MODE_MIXED) {
+            atomicAdd(&stats[2], (unsigned long long)evals64);
+            atomicAdd(&stats[3], (unsigned long long)culled);
+        }
     }
 }
 
@@ -684,28 +803,36 @@ __global__ void __launch_bounds__(kBlock) k_tridist_batch(size_t n, const float 
 template <typename R, int MODE>
 __global__ void __launch_bounds__(kBlock) k_meshdist_batch(size_t P, const uint64_t *__restrict__ os,
                                                            const float4 *__restrict__ rec,
+                                                           const float4 *__restrict__ bnd,
                                                            R *__restrict__ out,
                                                            unsigned long long *__restrict__ stats) {
+    __shared__ uint32_t queue[kWavesPerBlock][2 * kWave];
     const size_t w = wave_global();
     if (w >= P) return;
     const int lane = lane_id();
-    const uint64_t o1 = os[4 * w], n1 = os[4 * w + 1], o2 = os[4 * w + 2], n2 = os[4 * w + 3];
-    const uint64_t total = n1 * n2;
+    const uint64_t o1 = os[4 * w], o2 = os[4 * w + 2];
+    const uint32_t n1 = (uint32_t)os[4 * w + 1], n2 = (uint32_t)os[4 * w + 3];
+    const uint32_t total = n1 * n2;
     R d = (R)__builtin_inff(), lo = (R)__builtin_inff(), hi = (R)__builtin_inff();
     if (MODE != MODE_MIXED) {
-        for (uint64_t q = lane; q < total; q += kWave) {
-            const uint64_t i = q / n2, j = q - i * n2;
+        for (uint32_t q = lane; q < total; q += kWave) {
+            const uint32_t i = q / n2, j = q - i * n2;
             R a, b, c;
             pair_values<R>(rec, o1 + i, rec, o2 + j, a, b, c);
             d = rmin2(d, a); lo = rmin2(lo, b); hi = rmin2(hi, c);
         }
     } else {
         double dd = INFINITY, dl = INFINITY, dh = INFINITY;
-        uint64_t rechecks = 0;
-        mixed_minima(rec, o1, rec, o2, n2, total, lane, dd, dl, dh, rechecks);
+        uint64_t evals64 = 0, culled = 0;
+        mixed_minima(rec, bnd, o1, rec, bnd, o2, n2, total, lane, queue[threadIdx.x >> 6], dd,
+                     dl, dh, evals64, culled);
         d = (R)dd; lo = (R)dl; hi = (R)dh;
-        rechecks = wave_sum_u64(rechecks);
-        if (lane == 0) atomicAdd(&stats[2], (unsigned long long)rechecks);
+        evals64 = wave_sum_u64(evals64);
+        culled = wave_sum_u64(culled);
+        if (lane == 0) {
+            atomicAdd(&stats[2], (unsigned long long)evals64);
+            atomicAdd(&stats[3], (unsigned long long)culled);
+        }
     }
     d = wave_min(d);
     lo = wave_min(lo);

@@ -24,6 +24,8 @@ struct tdb_tile {
     tdb::Tile t;
     mutable std::once_flag tree_once;
     mutable std::unique_ptr<tdb::Octree> tree;
+    // engines holding a device copy of this tile (guarded by g_registry)
+    mutable std::vector<tdb_engine *> holders;
     const tdb::Octree &octree() const {
         std::call_once(tree_once, [this] {
             tree.reset(new tdb::Octree(t.obj_box.data(), t.obj_transform.data(), t.n_obj, 10));
@@ -38,13 +40,18 @@ struct tdb_engine {
     std::unordered_map<const tdb_tile *, tdp_tile *> resident;
 };
 
+// Device copies are keyed by tile; a tile that is freed drops its copies in
+// every engine that holds one, and a closing engine unregisters itself, so a
+// new tile allocated at a recycled address never meets a stale device copy.
+static std::mutex g_registry;
+
 struct tdb_result {
     uint64_t n_targets = 0;
     std::vector<int32_t> target, obj2;
     std::vector<double> rmin, rmax;
     std::vector<uint8_t> round;
     double timing[7] = {0, 0, 0, 0, 0, 0, 0};
-    uint64_t stats[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 static void set_err(char *err, size_t len, const std::string &msg) {
@@ -83,7 +90,20 @@ int tdb_tile_save(const tdb_tile *t, const char *path, char *err, size_t err_len
     return 0;
 }
 
-void tdb_tile_free(tdb_tile *t) { delete t; }
+void tdb_tile_free(tdb_tile *t) {
+    if (!t) return;
+    {
+        std::lock_guard<std::mutex> g(g_registry);
+        for (tdb_engine *e : t->holders) {
+            auto it = e->resident.find(t);
+            if (it != e->resident.end()) {
+                tdp_tile_free(it->second);
+                e->resident.erase(it);
+            }
+        }
+    }
+    delete t;
+}
 
 void tdb_tile_size(const tdb_tile *t, uint64_t *out3) {
     out3[0] = t->t.n_obj;
@@ -186,7 +206,15 @@ int tdb_engine_open(int device, tdb_engine **out) {
 
 void tdb_engine_close(tdb_engine *e) {
     if (!e) return;
-    for (auto &kv : e->resident) tdp_tile_free(kv.second);
+    {
+        std::lock_guard<std::mutex> g(g_registry);
+        for (auto &kv : e->resident) {
+            auto &h = kv.first->holders;
+            h.erase(std::remove(h.begin(), h.end(), e), h.end());
+            tdp_tile_free(kv.second);
+        }
+        e->resident.clear();
+    }
     if (e->ctx) tdp_close(e->ctx);
     delete e;
 }
@@ -199,14 +227,18 @@ int tdb_engine_device(tdb_engine *e, char *name, size_t len, int *cu) {
 }
 
 void tdb_engine_evict(tdb_engine *e, const tdb_tile *t) {
-    if (!e) return;
+    if (!e || !t) return;
+    std::lock_guard<std::mutex> g(g_registry);
     auto it = e->resident.find(t);
     if (it == e->resident.end()) return;
     tdp_tile_free(it->second);
     e->resident.erase(it);
+    auto &h = t->holders;
+    h.erase(std::remove(h.begin(), h.end(), e), h.end());
 }
 
 static int resident(tdb_engine *e, const tdb_tile *t, tdp_tile **out) {
+    std::lock_guard<std::mutex> g(g_registry);
     auto it = e->resident.find(t);
     if (it != e->resident.end()) {
         *out = it->second;
@@ -228,6 +260,7 @@ static int resident(tdb_engine *e, const tdb_tile *t, tdp_tile **out) {
         return rc;
     }
     e->resident[t] = *out;
+    t->holders.push_back(e);
     return 0;
 }
 
@@ -362,7 +395,7 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
     std::vector<uint8_t> rnd(P);
     std::vector<double> mn(P), mx(P);
     if (rc == TDP_OK) rc = tdp_join_results(j, rnd.data(), mn.data(), mx.data());
-    uint64_t st[4] = {0, 0, 0, 0};
+    uint64_t st[5] = {0, 0, 0, 0, 0};
     if (rc == TDP_OK) rc = tdp_join_stats(j, st);
     if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
     if (rc != TDP_OK) {
@@ -372,6 +405,7 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
     }
     tdp_join_destroy(j);
     for (int k = 0; k < 4; k++) res->stats[k] = st[k];
+    res->stats[6] = st[4];
     // per target: reported pairs ordered by (round, list position)
     std::vector<uint32_t> order;
     for (size_t t = 0; t < N; t++) {
@@ -409,8 +443,8 @@ void tdb_result_get(const tdb_result *r, int32_t *target, int32_t *obj2, double 
 void tdb_result_timing(const tdb_result *r, double *out7) {
     for (int k = 0; k < 7; k++) out7[k] = r->timing[k];
 }
-void tdb_result_stats(const tdb_result *r, uint64_t *out6) {
-    for (int k = 0; k < 6; k++) out6[k] = r->stats[k];
+void tdb_result_stats(const tdb_result *r, uint64_t *out7) {
+    for (int k = 0; k < 7; k++) out7[k] = r->stats[k];
 }
 void tdb_result_free(tdb_result *r) { delete r; }
 

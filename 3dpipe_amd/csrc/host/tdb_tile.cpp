@@ -371,7 +371,7 @@ Voxelized voxelize(const LodMesh &m, int voxel_tris) {
                 // barycentre: any choice is valid, this one keeps it tight
                 V3 tb = bc[T];
                 constexpr int kNear = 6;
-                int near[kNear];
+                int near[kNear] = {0};
                 double nd[kNear];
                 int nn = 0;
                 for (size_t j = 0; j < lod[v].size(); j++) {

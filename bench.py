@@ -119,7 +119,8 @@ def main():
                                f"{ves.n_obj} vessels, {args.cube:g}^3 cube, LODs 20-100",
                    "precision": args.precision, "seed": args.seed,
                    "triangle_pairs": tri_pairs, "voxel_pairs": r.stats["voxel_pairs"],
-                   "fp64_rechecks": r.stats["fp64_rechecks"],
+                   "fp32_culled_tri_pairs": r.stats["fp32_culled"],
+                   "fp64_tridist_evals": r.stats["fp64_evals"],
                    "object_pairs": r.stats["object_pairs"], "results": int(len(r.target)),
                    "timing_ms": r.timing_ms},
         "roofline": {"bound": "valu-fp32", "kernel": "k_refine", "achieved": achieved,

@@ -124,9 +124,11 @@ void tdb_result_get(const tdb_result *r, int32_t *target, int32_t *obj2, double 
  * [3] refinement, [4] filter kernels (device), [5] refine kernels (device),
  * [6] tile upload */
 void tdb_result_timing(const tdb_result *r, double *out7);
-/* [0] triangle pairs, [1] voxel pairs evaluated, [2] FP64 re-check pairs,
- * [3] voxel pairs after the filter, [4] candidate object pairs, [5] rounds */
-void tdb_result_stats(const tdb_result *r, uint64_t *out6);
+/* [0] triangle pairs covered, [1] voxel pairs evaluated, [2] FP64 TriDist
+ * evaluations (mixed form), [3] voxel pairs after the filter, [4] candidate
+ * object pairs, [5] rounds, [6] triangle pairs excluded by the FP32 bounds
+ * (mixed form) */
+void tdb_result_stats(const tdb_result *r, uint64_t *out7);
 void tdb_result_free(tdb_result *r);
 
 #ifdef __cplusplus

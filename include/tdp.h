@@ -141,9 +141,11 @@ int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pairs_out);
  * distance range when reported (or its last range). */
 int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin, double *rmax);
 
-/* counters: [0] triangle pairs evaluated, [1] voxel pairs evaluated,
- * [2] FP64 re-check triangle pairs (MIXED), [3] voxel pairs at filter */
-int tdp_join_stats(tdp_join *j, uint64_t *out4);
+/* counters: [0] triangle pairs covered (sum of n1*n2 over evaluated voxel
+ * pairs), [1] voxel pairs evaluated, [2] FP64 TriDist evaluations (MIXED),
+ * [3] voxel pairs after the filter, [4] triangle pairs excluded by the FP32
+ * bounds without any TriDist evaluation (MIXED) */
+int tdp_join_stats(tdp_join *j, uint64_t *out5);
 /* device time (ms, from HIP events) spent in the refine kernels so far */
 int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
 int tdp_join_destroy(tdp_join *j);

@@ -249,3 +249,42 @@ def test_tiny_bruteforce_exact_knn(tdp, eng, tiny_tiles):
         assert len(got) >= 2
         for o in got:
             assert d[o] <= kth, (t, o, d[o], kth)
+
+
+def test_engine_cache_survives_tile_free(tdp, eng, small_tiles):
+    """device copies are dropped when a tile is freed, so a new tile at a
+    recycled address is uploaded afresh"""
+    import gc
+    pn, pv = small_tiles
+    for _ in range(3):
+        n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+        a = eng.join(n, v, query="nn", knn=2, precision="mixed")
+        del n, v
+        gc.collect()
+        v2, n2 = tdp.Tile.load(pv), tdp.Tile.load(pn)   # opposite allocation order
+        b = eng.join(n2, v2, query="nn", knn=2, precision="mixed")
+        assert np.array_equal(a.obj2, b.obj2) and np.array_equal(a.rmin, b.rmin)
+        del n2, v2
+        gc.collect()
+
+
+@pytest.mark.parametrize("slot", [0, 1, 2, 3, 4])
+def test_meshdist_mixed_equals_fp64_real_geometry(dev, tdp, small_tiles, slot):
+    """nucleus voxels against nearby vessel voxels (the join's real work):
+    the mixed form's minima are the FP64 minima bit for bit"""
+    n, v = tdp.Tile.load(small_tiles[0]), tdp.Tile.load(small_tiles[1])
+    an, av = n.arrays(), v.arrays()
+    rec = np.concatenate([an["tri"].reshape(-1, 12), av["tri"].reshape(-1, 12)])
+    base = len(an["tri"]) // 12
+    offn = an["vox_tri_off"].reshape(-1, 5)[:, slot]; cntn = an["vox_tri_cnt"].reshape(-1, 5)[:, slot]
+    offv = av["vox_tri_off"].reshape(-1, 5)[:, slot] + base; cntv = av["vox_tri_cnt"].reshape(-1, 5)[:, slot]
+    cn = an["vox_box"].reshape(-1, 9)[:, 6:9]; cv = av["vox_box"].reshape(-1, 9)[:, 6:9]
+    rng = np.random.default_rng(40 + slot)
+    rows = []
+    for a in rng.choice(len(cn), 600, replace=False):
+        dist = np.linalg.norm(cv - cn[a], axis=1)
+        for b in np.argsort(dist)[:8]:
+            if cntn[a] and cntv[b]:
+                rows.append([offn[a], cntn[a], offv[b], cntv[b]])
+    os_ = np.array(rows, np.uint64)
+    assert bits_equal(dev.meshdist(rec, os_, "mixed"), dev.meshdist(rec, os_, "fp64"))
