@@ -33,7 +33,7 @@ struct tdp_tile {
     uint64_t *vox_tri_off = nullptr;
     uint32_t *vox_tri_cnt = nullptr;
     float4 *tri = nullptr;
-    float4 *bnd = nullptr;  // 2 per triangle: bounding sphere, plane
+    float4 *bnd = nullptr;  // kBnd per triangle: sphere, plane, hausdorff pair
     size_t bytes = 0;
 };
 
@@ -58,7 +58,11 @@ struct tdp_join {
     float *mx_U = nullptr;
     uint32_t *mx_probe = nullptr, *mx_flags = nullptr;
     unsigned long long *mx_key = nullptr;
+    uint32_t *mx_nw = nullptr;   // candidate-mask words per voxel pair
+    uint64_t *mx_woff = nullptr; // their exclusive scan
     size_t cap_mx = 0;
+    unsigned long long *mx_mask = nullptr;
+    size_t cap_mask = 0;
     uint32_t *item_vp = nullptr, *item_q = nullptr;
     size_t cap_items = 0;
     uint64_t *pos = nullptr;
@@ -109,7 +113,7 @@ static inline dim3 wave_grid(size_t n) {
 // grid for grid-stride kernels
 static inline unsigned stride_grid(size_t n) {
     const size_t b = (n + kBlock - 1) / kBlock;
-    return (unsigned)std::min<size_t>(b ? b : 1, (size_t)1 << 18);
+    return (unsigned)std::min<size_t>(b ? b : 1, (size_t)4096);
 }
 
 template <typename T>
@@ -288,7 +292,7 @@ extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **
     up(&t->vox_tri_off, d->vox_tri_off, 5 * d->n_vox);
     up(&t->vox_tri_cnt, d->vox_tri_cnt, 5 * d->n_vox);
     up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
-    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, 2 * d->n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * d->n_tri);
     if (rc == TDP_OK && d->n_tri) {
         t->bytes += 32 * d->n_tri;
         k_tri_bounds<<<blocks_for(d->n_tri, kBlock), kBlock, 0, ctx->stream>>>(d->n_tri, t->tri, t->bnd);
@@ -388,7 +392,7 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
             return TDP_ERR_ARG;
         }
     int rc = dalloc(ctx, &dR, 3 * n_tri);
-    if (rc == TDP_OK) rc = dalloc(ctx, &dB, 2 * n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &dB, kBnd * n_tri);
     if (rc == TDP_OK) rc = dalloc(ctx, &dOS, 4 * P);
     if (rc == TDP_OK) rc = dalloc(ctx, (char **)&dO, 3 * P * osz);
     if (rc == TDP_OK) rc = dalloc(ctx, &dst, 4);
@@ -467,7 +471,7 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
-                    j->mx_key, j->item_vp, j->item_q};
+                    j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q};
     for (void *p : ptrs) dfree(p);
     for (int b = 0; b < 2; b++) {
         dfree(j->v1[b]); dfree(j->v2[b]); dfree(j->vpp[b]); dfree(j->vmin[b]); dfree(j->vmax[b]);
@@ -595,13 +599,29 @@ static int ensure_mx(tdp_join *j, size_t n) {
     tdp_ctx *ctx = j->ctx;
     const size_t cap = std::max(n, j->cap_vp);
     dfree(j->mx_U); dfree(j->mx_probe); dfree(j->mx_flags); dfree(j->mx_key);
+    dfree(j->mx_nw); dfree(j->mx_woff);
     j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
+    j->mx_nw = nullptr; j->mx_woff = nullptr;
     j->cap_mx = 0;
     int rc = dalloc(ctx, &j->mx_U, 3 * cap);
     if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_probe, 3 * cap);
     if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_flags, cap);
     if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_key, 3 * cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_nw, cap);
+    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_woff, cap);
     if (rc == TDP_OK) j->cap_mx = cap;
+    return rc;
+}
+
+// candidate bit masks, grow-only
+static int ensure_mask(tdp_join *j, size_t n) {
+    if (n <= j->cap_mask) return TDP_OK;
+    const size_t cap = std::max(n + n / 4, (size_t)1 << 16);
+    dfree(j->mx_mask);
+    j->mx_mask = nullptr;
+    j->cap_mask = 0;
+    int rc = dalloc(j->ctx, &j->mx_mask, cap);
+    if (rc == TDP_OK) j->cap_mask = cap;
     return rc;
 }
 
@@ -715,19 +735,26 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         int rc = ensure_mx(j, n);
         if (rc != TDP_OK) return rc;
         const dim3 gw = wave_grid(n);
+        // quantities this round folds: d (bit 0), lo (bit 1), hi (bit 2)
+        const uint32_t need = is_last || j->prm.hausdorff_level != 2 ? 1u : 6u;
         k_mx_bounds1<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
                                            A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                           B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags,
-                                           j->mx_key);
+                                           B->bnd, slot, need, j->mx_U, j->mx_probe, j->mx_flags,
+                                           j->mx_key, j->mx_nw);
         TDP_LAUNCHED(ctx);
+        uint64_t words = 0;
+        rc = scan_exclusive<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, &words);
+        if (rc != TDP_OK) return rc;
+        rc = ensure_mask(j, words);
+        if (rc != TDP_OK) return rc;
         k_mx_eval<<<stride_grid(3 * n), kBlock, 0, s>>>(
             3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
             A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
         TDP_LAUNCHED(ctx);
-        k_mx_count<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                         A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                         B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags,
-                                         j->mx_key, j->keep, j->stats);
+        k_mx_mark<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                        A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
+                                        B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags, j->mx_key,
+                                        j->mx_woff, j->mx_mask, j->keep);
         TDP_LAUNCHED(ctx);
         uint64_t items = 0;
         rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &items);
@@ -735,10 +762,10 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         if (items) {
             rc = ensure_items(j, items);
             if (rc != TDP_OK) return rc;
-            k_mx_write<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                             A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                             B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags,
-                                             j->mx_key, j->pos, j->item_vp, j->item_q);
+            k_mx_expand<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
+                                              A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt, slot,
+                                              j->mx_flags, j->mx_woff, j->mx_mask, j->pos,
+                                              j->item_vp, j->item_q);
             TDP_LAUNCHED(ctx);
             k_mx_eval<<<stride_grid(items), kBlock, 0, s>>>(
                 items, j->item_vp, j->item_q, 0, j->v1[c], j->v2[c], A->vox_tri_off,
@@ -841,7 +868,9 @@ extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
     out5[1] = s[1];
     out5[2] = s[2];
     out5[3] = j->filter_vp;
-    out5[4] = s[3];
+    // MIXED: every triangle pair not evaluated in FP64 was excluded by its
+    // FP32 interval
+    out5[4] = j->prm.precision == TDP_PREC_MIXED && s[0] > s[2] ? s[0] - s[2] : 0;
     return TDP_OK;
 }
 

@@ -342,7 +342,18 @@ __device__ __forceinline__ void pair_values(const float4 *rec1, uint64_t i1, con
 constexpr float kU = 5.9604645e-08f;    // 2^-24
 constexpr uint32_t kMixedMinPairs = 2 * kWave;  // below this, evaluate every pair in FP64
 
-// per triangle: {centre.xyz, radius}, {unit normal.xyz, max |centre coord|}
+// Per-triangle bound record, kBnd float4 per triangle:
+//   {centre.xyz, radius}      sphere around the triangle (centre rounded to
+//                             FP32, radius measured from the rounded centre
+//                             with a relative margin);
+//   {unit normal.xyz, max |centre coord|}   normal 0 when near-degenerate;
+//   {proxy hausdorff, hausdorff, 1 if the normal is 0 else 0, 0}.
+// The normal is formed in FP64 from the (exact) FP64 edge differences and
+// rounded once, so its direction is within a few FP32 ulps of the true
+// plane's even for thin triangles (the FP64 cross product's error relative
+// to |n| stays below 1e-9 under the 1e-6 aspect cut-off).
+constexpr int kBnd = 3;
+
 __global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *__restrict__ rec,
                                                        float4 *__restrict__ bnd) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -356,19 +367,22 @@ __global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *_
     };
     float r = fmaxf(d(a.x, a.y, a.z), fmaxf(d(b.x, b.y, b.z), d(c.x, c.y, c.z)));
     r = r * (1.f + 64.f * kU) + 1e-30f;
-    const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
-    const float e2x = c.x - b.x, e2y = c.y - b.y, e2z = c.z - b.z;
-    float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-    const float nl = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
-    const float e = fmaxf(e1x * e1x + e1y * e1y + e1z * e1z, e2x * e2x + e2y * e2y + e2z * e2z);
-    if (nl > 1e-6f * e && nl * nl > 1e-12f) {
-        nx /= nl; ny /= nl; nz /= nl;
-    } else {  // near-degenerate: no plane test
-        nx = ny = nz = 0.f;
+    const double e1x = (double)b.x - a.x, e1y = (double)b.y - a.y, e1z = (double)b.z - a.z;
+    const double e2x = (double)c.x - b.x, e2y = (double)c.y - b.y, e2z = (double)c.z - b.z;
+    const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z,
+                 nz = e1x * e2y - e1y * e2x;
+    const double nl = sqrt(nx * nx + ny * ny + nz * nz);
+    const double e = fmax(e1x * e1x + e1y * e1y + e1z * e1z, e2x * e2x + e2y * e2y + e2z * e2z);
+    float fx = 0.f, fy = 0.f, fz = 0.f;  // near-degenerate: no plane test
+    if (nl > 1e-6 * e && nl * nl > 1e-12) {
+        fx = (float)(nx / nl);
+        fy = (float)(ny / nl);
+        fz = (float)(nz / nl);
     }
     const float cm = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
-    bnd[2 * i] = make_float4(cx, cy, cz, r);
-    bnd[2 * i + 1] = make_float4(nx, ny, nz, cm);
+    bnd[kBnd * i] = make_float4(cx, cy, cz, r);
+    bnd[kBnd * i + 1] = make_float4(fx, fy, fz, cm);
+    bnd[kBnd * i + 2] = make_float4(a.w, b.w, fx == 0.f && fy == 0.f && fz == 0.f ? 1.f : 0.f, 0.f);
 }
 
 // (d, lo, hi) intervals from an interval [dl, du] of the FP64 TriDist value
@@ -394,8 +408,8 @@ __device__ __forceinline__ void intervals(float dl, float du, float ph1, float h
 __device__ __forceinline__ void pair_interval(const float4 *r1, const float4 *b1, uint64_t i1,
                                               const float4 *r2, const float4 *b2, uint64_t i2,
                                               float lw[3], float uw[3]) {
-    const float4 s1 = b1[2 * i1], p1 = b1[2 * i1 + 1];
-    const float4 s2 = b2[2 * i2], p2 = b2[2 * i2 + 1];
+    const float4 s1 = b1[kBnd * i1], p1 = b1[kBnd * i1 + 1];
+    const float4 s2 = b2[kBnd * i2], p2 = b2[kBnd * i2 + 1];
     const float4 a0 = r1[3 * i1], a1 = r1[3 * i1 + 1], a2 = r1[3 * i1 + 2];
     const float4 c0 = r2[3 * i2], c1 = r2[3 * i2 + 1], c2 = r2[3 * i2 + 2];
     const float dx = s2.x - s1.x, dy = s2.y - s1.y, dz = s2.z - s1.z;
