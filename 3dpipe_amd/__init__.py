@@ -99,6 +99,7 @@ def _load():
     tdb.tdb_result_timing.argtypes = [_P, _P]
     tdb.tdb_result_stats.argtypes = [_P, _P]
     tdb.tdb_result_free.argtypes = [_P]
+    tdb.tdb_result_stages.argtypes = [_P, _P, _P, _c.c_int]
     # libtdp (kernel-level parity ops)
     tdp.tdp_open.argtypes = [_c.c_int, _c.POINTER(_P)]
     tdp.tdp_close.argtypes = [_P]
@@ -190,6 +191,10 @@ def generate(seed: int = 0, n_vessels: int = 10, nuclei_per_vessel: int = 20,
 # ------------------------------------------------------------------ join
 
 
+# refinement stages, in the order of TDP_STAGE_* (include/tdp.h)
+STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct"]
+
+
 @dataclass
 class JoinResult:
     """Reported (target, object) pairs grouped by target in report order."""
@@ -201,6 +206,9 @@ class JoinResult:
     n_targets: int
     timing_ms: dict = field(default_factory=dict)
     stats: dict = field(default_factory=dict)
+    # device ms and launches per refinement stage (STAGES order)
+    stage_ms: dict = field(default_factory=dict)
+    stage_launches: dict = field(default_factory=dict)
 
     def lists(self) -> dict:
         """{target: [obj2, ...]} in report order."""
@@ -279,6 +287,9 @@ class Engine:
             st = np.zeros(7, np.uint64)
             tdb.tdb_result_timing(r, _ptr(tm))
             tdb.tdb_result_stats(r, _ptr(st))
+            sm = np.zeros(len(STAGES), np.float64)
+            sn = np.zeros(len(STAGES), np.uint64)
+            tdb.tdb_result_stages(r, _ptr(sm), _ptr(sn), len(STAGES))
             nt = int(tdb.tdb_result_targets(r))
         finally:
             tdb.tdb_result_free(r)
@@ -286,7 +297,8 @@ class Engine:
                            "refine_kernels", "upload"], tm.tolist()))
         stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
                           "object_pairs", "rounds", "fp32_culled"], (int(x) for x in st)))
-        return JoinResult(t, b, mn, mx, rd, nt, timing, stats)
+        return JoinResult(t, b, mn, mx, rd, nt, timing, stats, dict(zip(STAGES, sm.tolist())),
+                          dict(zip(STAGES, (int(x) for x in sn))))
 
     def close(self):
         if self._h:

@@ -75,6 +75,12 @@ struct tdp_join {
     uint64_t filter_vp = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double refine_ms = 0, filter_ms = 0;
+    // per-stage timing: events recorded between the stages of a round
+    hipEvent_t sev[12] = {};
+    int sev_stage[12] = {};
+    int n_sev = 0;
+    double stage_ms[TDP_STAGE_COUNT] = {};
+    uint64_t stage_n[TDP_STAGE_COUNT] = {};
 };
 
 // ------------------------------------------------------------ helpers
@@ -478,6 +484,8 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     }
     for (auto &e : j->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : j->sev)
+        if (e) (void)hipEventDestroy(e);
     delete j;
     return TDP_OK;
 }
@@ -557,6 +565,8 @@ extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile 
     if (e == hipSuccess) e = hipMemset(j->round, 0xff, P ? P : 1);
     if (e == hipSuccess) e = hipMemset(j->state, 0, P ? P : 1);
     for (auto &ev : j->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    for (auto &ev : j->sev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e != hipSuccess) {
         ctx->err = std::string("tdp_join_create: ") + hipGetErrorString(e);
@@ -720,9 +730,19 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     const int c = j->cur;
     const uint64_t n = j->n_vp;
     TDP_CHECK(ctx, hipEventRecord(j->ev[2], s));
+    j->n_sev = 0;
+    // closes the stage that ran since the previous mark
+    auto mark = [&](int stage) -> int {
+        if (j->n_sev >= 12) return TDP_OK;
+        j->sev_stage[j->n_sev] = stage;
+        j->stage_n[stage]++;
+        TDP_CHECK(ctx, hipEventRecord(j->sev[j->n_sev++], s));
+        return TDP_OK;
+    };
     k_reset_acc<R, KT><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (KT *)j->acc_min,
                                                                 (KT *)j->acc_max);
     TDP_LAUNCHED(ctx);
+    if (mark(TDP_STAGE_RESET) != TDP_OK) return TDP_ERR_HIP;
     const tdp_tile *A = j->t1, *B = j->t2;
     if (MODE != MODE_MIXED) {
         k_refine<R, MODE><<<wave_grid(n), kBlock, 0, s>>>(
@@ -731,6 +751,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last, j->prm.hausdorff_level,
             (KT *)j->acc_min, (KT *)j->acc_max, j->stats);
         TDP_LAUNCHED(ctx);
+        if (mark(TDP_STAGE_DIRECT) != TDP_OK) return TDP_ERR_HIP;
     } else {
         int rc = ensure_mx(j, n);
         if (rc != TDP_OK) return rc;
@@ -742,23 +763,28 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                                            B->bnd, slot, need, j->mx_U, j->mx_probe, j->mx_flags,
                                            j->mx_key, j->mx_nw);
         TDP_LAUNCHED(ctx);
+        if (mark(TDP_STAGE_BOUNDS) != TDP_OK) return TDP_ERR_HIP;
         uint64_t words = 0;
         rc = scan_exclusive<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, &words);
         if (rc != TDP_OK) return rc;
+        if (mark(TDP_STAGE_SCAN) != TDP_OK) return TDP_ERR_HIP;
         rc = ensure_mask(j, words);
         if (rc != TDP_OK) return rc;
         k_mx_eval<<<stride_grid(3 * n), kBlock, 0, s>>>(
             3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
             A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
         TDP_LAUNCHED(ctx);
+        if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
         k_mx_mark<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
                                         A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
                                         B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags, j->mx_key,
                                         j->mx_woff, j->mx_mask, j->keep);
         TDP_LAUNCHED(ctx);
+        if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
         uint64_t items = 0;
         rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &items);
         if (rc != TDP_OK) return rc;
+        if (mark(TDP_STAGE_SCAN) != TDP_OK) return TDP_ERR_HIP;
         if (items) {
             rc = ensure_items(j, items);
             if (rc != TDP_OK) return rc;
@@ -767,11 +793,13 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                                               j->mx_flags, j->mx_woff, j->mx_mask, j->pos,
                                               j->item_vp, j->item_q);
             TDP_LAUNCHED(ctx);
+            if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
             k_mx_eval<<<stride_grid(items), kBlock, 0, s>>>(
                 items, j->item_vp, j->item_q, 0, j->v1[c], j->v2[c], A->vox_tri_off,
                 A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key,
                 j->stats);
             TDP_LAUNCHED(ctx);
+            if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
         }
         k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
@@ -779,6 +807,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             j->prm.hausdorff_level, j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
             j->stats);
         TDP_LAUNCHED(ctx);
+        if (mark(TDP_STAGE_FOLD) != TDP_OK) return TDP_ERR_HIP;
     }
     TDP_CHECK(ctx, hipEventRecord(j->ev[3], s));
     k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
@@ -809,6 +838,11 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     float ms = 0;
     TDP_CHECK(ctx, hipEventElapsedTime(&ms, j->ev[2], j->ev[3]));
     j->refine_ms += ms;
+    for (int k = 0; k < j->n_sev; k++) {
+        float sm = 0;
+        TDP_CHECK(ctx, hipEventElapsedTime(&sm, k ? j->sev[k - 1] : j->ev[2], j->sev[k]));
+        j->stage_ms[j->sev_stage[k]] += sm;
+    }
     j->cur = o;
     j->n_vp = kept;
     j->rounds++;
@@ -871,6 +905,15 @@ extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
     // MIXED: every triangle pair not evaluated in FP64 was excluded by its
     // FP32 interval
     out5[4] = j->prm.precision == TDP_PREC_MIXED && s[0] > s[2] ? s[0] - s[2] : 0;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_stage_ms(tdp_join *j, double *ms, uint64_t *launches, int n_stages) {
+    if (!j || n_stages < 0) return TDP_ERR_ARG;
+    for (int k = 0; k < n_stages; k++) {
+        if (ms) ms[k] = k < TDP_STAGE_COUNT ? j->stage_ms[k] : 0.0;
+        if (launches) launches[k] = k < TDP_STAGE_COUNT ? j->stage_n[k] : 0;
+    }
     return TDP_OK;
 }
 

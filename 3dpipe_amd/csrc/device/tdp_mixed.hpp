@@ -94,6 +94,11 @@ __device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t 
 //         face test shows the triangles disjoint.
 // Sines are rounded up (16 ulp inside the root for the cosine's error); a
 // triangle without a plane counts as a ball (sine 1).
+// Operation count on the separated path, as used by k_mx_mark (lower bounds
+// and the three comparisons; add/sub/mul/sqrt/rcp/min/max/compare = 1, abs
+// free): centre difference 3, dc 6, rs 1, m 5, e1 and e2 10, cross product 9,
+// st 10, guard 7, disc gap 21, du 2, s 5, lower bounds of lo and hi 7,
+// comparisons with U 3 -- 89 per triangle pair.
 __device__ __forceinline__ void disc_interval(const TriS &A, const TriS &B, float lw[3],
                                               float uw[3]) {
     const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;

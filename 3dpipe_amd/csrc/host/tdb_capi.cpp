@@ -51,6 +51,8 @@ struct tdb_result {
     std::vector<double> rmin, rmax;
     std::vector<uint8_t> round;
     double timing[7] = {0, 0, 0, 0, 0, 0, 0};
+    double stage_ms[TDP_STAGE_COUNT] = {};
+    uint64_t stage_n[TDP_STAGE_COUNT] = {};
     uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -398,6 +400,7 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
     uint64_t st[5] = {0, 0, 0, 0, 0};
     if (rc == TDP_OK) rc = tdp_join_stats(j, st);
     if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
+    if (rc == TDP_OK) tdp_join_stage_ms(j, res->stage_ms, res->stage_n, TDP_STAGE_COUNT);
     if (rc != TDP_OK) {
         e->err = tdp_last_error(e->ctx);
         tdp_join_destroy(j);
@@ -445,6 +448,12 @@ void tdb_result_timing(const tdb_result *r, double *out7) {
 }
 void tdb_result_stats(const tdb_result *r, uint64_t *out7) {
     for (int k = 0; k < 7; k++) out7[k] = r->stats[k];
+}
+void tdb_result_stages(const tdb_result *r, double *ms, uint64_t *launches, int n) {
+    for (int k = 0; k < n; k++) {
+        if (ms) ms[k] = k < TDP_STAGE_COUNT ? r->stage_ms[k] : 0.0;
+        if (launches) launches[k] = k < TDP_STAGE_COUNT ? r->stage_n[k] : 0;
+    }
 }
 void tdb_result_free(tdb_result *r) { delete r; }
 

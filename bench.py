@@ -21,9 +21,16 @@ sys.path.insert(0, ROOT)
 
 # Static FP32 operation count of one triangle pair on the canonical full path
 # of TriDist (9 edge-edge cases + 2 vertex-face cases, no early exit) plus the
-# bound update; add/sub/mul/div/sqrt/compare = 1 (pinned by tests/flops).
-F_TRI = 1138
+# bound update; add/sub/mul/div/sqrt/compare = 1. Counted by running the
+# oracle's TriDist with a counting number type (oracle/flop_count.cpp, checked
+# in tests/test_cpu.py).
+F_TRI = 997
+# Static FP32 operation count of the mixed form's per-pair disc test, the
+# part of k_mx_mark every triangle pair goes through (disc_interval() and the
+# candidate comparison; the breakdown is beside disc_interval()).
+F_MARK = 89
 PEAK_FP32_TFLOPS = 157.3
+REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct"]
 
 
 def log(msg):
@@ -104,8 +111,23 @@ def main():
     ms = 1e3 * sum(times) / len(times)
     tri_pairs = r.stats["tri_pairs"]
     refine_kernel_ms = sum(x.timing_ms["refine_kernels"] for x in results) / len(results)
-    launches = max(1, r.stats["rounds"])
+    rounds = max(1, r.stats["rounds"])
+    stage_ms = {k: sum(x.stage_ms[k] for x in results) / len(results) for k in REFINE_STAGES}
+    stage_n = r.stage_launches
+    # the refinement round (all its kernels, between two HIP events on the
+    # join stream) is the measured unit: algorithmic work = F_TRI per pair
     achieved = F_TRI * tri_pairs / (refine_kernel_ms * 1e-3) / 1e12
+    dom = max((k for k in REFINE_STAGES if k not in ("scan", "reset")), key=lambda k: stage_ms[k])
+    dom_kernel = {"mark": "k_mx_mark", "eval": "k_mx_eval", "bounds": "k_mx_bounds1",
+                  "probe": "k_mx_eval", "expand": "k_mx_expand", "fold": "k_mx_fold",
+                  "direct": "k_refine"}[dom]
+    dom_launches = max(1, stage_n.get(dom, 1))
+    dom_obj = {"name": dom_kernel, "stage": dom, "launches": dom_launches,
+               "avg_launch_ms": stage_ms[dom] / dom_launches,
+               "share_of_refine": stage_ms[dom] / refine_kernel_ms if refine_kernel_ms else None}
+    if dom == "mark":
+        a = F_MARK * tri_pairs / (stage_ms[dom] * 1e-3) / 1e12
+        dom_obj.update({"flop_per_tri_pair": F_MARK, "achieved": a, "frac": a / PEAK_FP32_TFLOPS})
     out = {
         "metric": "10-NN join time (200k nuclei); tri-pair dist/s vs FP32 peak; p50/p99 query latency",
         "value": tri_pairs / (ms * 1e-3),
@@ -123,11 +145,13 @@ def main():
                    "fp64_tridist_evals": r.stats["fp64_evals"],
                    "object_pairs": r.stats["object_pairs"], "results": int(len(r.target)),
                    "timing_ms": r.timing_ms},
-        "roofline": {"bound": "valu-fp32", "kernel": "k_refine", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu-fp32", "kernel": "refinement round (k_mx_* pipeline)"
+                     if args.precision == "mixed" else "refinement round (k_refine)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "launches": launches, "avg_launch_ms": refine_kernel_ms / launches,
-                     "flop_per_tri_pair": F_TRI},
+                     "launches": rounds, "avg_launch_ms": refine_kernel_ms / rounds,
+                     "flop_per_tri_pair": F_TRI, "dominant_kernel": dom_obj,
+                     "stage_ms": {k: v for k, v in stage_ms.items() if v > 0}},
     }
     if args.compare_fp64 and args.precision != "fp64":
         t0 = time.perf_counter()

@@ -129,6 +129,9 @@ void tdb_result_timing(const tdb_result *r, double *out7);
  * object pairs, [5] rounds, [6] triangle pairs excluded by the FP32 bounds
  * (mixed form) */
 void tdb_result_stats(const tdb_result *r, uint64_t *out7);
+/* device time (ms) and launches per refinement stage, TDP_STAGE_* order of
+ * tdp.h (n up to TDP_STAGE_COUNT) */
+void tdb_result_stages(const tdb_result *r, double *ms, uint64_t *launches, int n);
 void tdb_result_free(tdb_result *r);
 
 #ifdef __cplusplus

@@ -148,6 +148,21 @@ int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin, double *r
 int tdp_join_stats(tdp_join *j, uint64_t *out5);
 /* device time (ms, from HIP events) spent in the refine kernels so far */
 int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
+
+/* Device time per stage of the refinement rounds so far (ms, from HIP events
+ * recorded on the join's stream between the stages) and launches per stage.
+ * Stages: */
+#define TDP_STAGE_RESET 0   /* per-round accumulator reset */
+#define TDP_STAGE_BOUNDS 1  /* MIXED: FP32 upper bounds and probe choice */
+#define TDP_STAGE_SCAN 2    /* MIXED: prefix sums, including the host read-back */
+#define TDP_STAGE_PROBE 3   /* MIXED: FP64 TriDist of the probes */
+#define TDP_STAGE_MARK 4    /* MIXED: FP32 candidate test and bit masks */
+#define TDP_STAGE_EXPAND 5  /* MIXED: bit masks to the FP64 work list */
+#define TDP_STAGE_EVAL 6    /* MIXED: FP64 TriDist of the work list */
+#define TDP_STAGE_FOLD 7    /* MIXED: range fold */
+#define TDP_STAGE_DIRECT 8  /* REF32 / FP64: the one-pass distance kernel */
+#define TDP_STAGE_COUNT 9
+int tdp_join_stage_ms(tdp_join *j, double *ms, uint64_t *launches, int n_stages);
 int tdp_join_destroy(tdp_join *j);
 
 #ifdef __cplusplus

@@ -374,3 +374,51 @@ def test_cli_without_gpu_fails_loudly(small_tiles):
     assert "ignoring unknown option --lod" in r.stderr
     r = subprocess.run([exe, "join", "-q", "nn"], capture_output=True, text=True)
     assert r.returncode == 1 and "tile1" in r.stderr
+
+
+# ------------------------------------------------------- operation counts
+
+def _flops_lib():
+    import ctypes
+    path = os.path.join(ROOT, "oracle", "lib", "libflops.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/lib/libflops.so not built")
+    lib = ctypes.CDLL(path)
+    lib.flop_tridist.restype = ctypes.c_uint64
+    lib.flop_tridist.argtypes = [ctypes.c_void_p] * 4
+    return lib
+
+
+def _count(lib, S, T):
+    S = np.asarray(S, np.float64)
+    T = np.asarray(T, np.float64)
+    d = np.zeros(1)
+    early = np.zeros(1, np.int32)
+    n = lib.flop_tridist(S.ctypes.data, T.ctypes.data, d.ctypes.data, early.ctypes.data)
+    return int(n), float(d[0]), bool(early[0])
+
+
+def test_flop_count_of_the_canonical_full_path():
+    """F_TRI (the per-pair algorithmic FP32 work in the throughput figures) is
+    the counted cost of TriDist on the full path -- all 9 edge-edge cases
+    without an early return, then both vertex-face tests, the second one
+    deciding -- plus the 8 operations of the bound update."""
+    import importlib.util
+    lib = _flops_lib()
+    # a small tilted S under a large horizontal T: S's face test finds T's
+    # vertices on both sides of its plane, T's face test finds S's top vertex
+    S = [1, 1, 3, 2, 1, -1, 1, 2, -1]
+    T = [-10, -10, 5, 20, -10, 5, -10, 20, 5]
+    n, d, early = _count(lib, S, T)
+    assert d == 2.0 and not early
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.F_TRI == n + 8
+    # an early edge-edge return costs less; no random pair costs far more
+    n_edge, d_edge, early_edge = _count(lib, [0, 0, 0, 1, 0, 0, 0, 1, 0],
+                                        [0.5, -1, 2, 0.5, 1, 2, 3, 0, 2])
+    assert early_edge and n_edge < n
+    rng = np.random.default_rng(3)
+    counts = [_count(lib, rng.uniform(-5, 5, 9), rng.uniform(-5, 5, 9))[0] for _ in range(500)]
+    assert max(counts) < 1.2 * n
