@@ -758,10 +758,10 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         const dim3 gw = wave_grid(n);
         // quantities this round folds: d (bit 0), lo (bit 1), hi (bit 2)
         const uint32_t need = is_last || j->prm.hausdorff_level != 2 ? 1u : 6u;
-        k_mx_bounds1<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                           A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                           B->bnd, slot, need, j->mx_U, j->mx_probe, j->mx_flags,
-                                           j->mx_key, j->mx_nw);
+        k_mx_bounds<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                          A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
+                                          B->bnd, slot, need, j->mx_U, j->mx_probe, j->mx_flags,
+                                          j->mx_key, j->mx_nw);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_BOUNDS) != TDP_OK) return TDP_ERR_HIP;
         uint64_t words = 0;

@@ -3,14 +3,14 @@
 // formed exactly -- see the comment above pair_interval() for why the FP32
 // intervals are rigorous -- but the FP64 TriDist runs only on the pairs those
 // intervals cannot exclude:
-//   k_mx_bounds1  FP32 sphere intervals of every triangle pair: the smallest
-//                 upper bounds U and up to three probe pairs (the pairs most
-//                 likely to hold the minima of d, lo and hi);
+//   k_mx_bounds   FP32 upper bounds U of every triangle pair's quantities and
+//                 up to three probe pairs (the pairs most likely to hold the
+//                 minima of d, lo and hi);
 //   k_mx_eval     FP64 TriDist on the probes, atomically min-folded into the
-//                 voxel pair's minima (order-independent, hence exact);
-//   k_mx_mark     FP32 intervals again against U tightened by the probes:
-//                 a bit mask of the pairs that can still hold a minimum,
-//                 and their number;
+//                 voxel pair's minima (order-independent, hence exact); their
+//                 values tighten U;
+//   k_mx_mark     FP32 intervals of every triangle pair against U: a bit mask
+//                 of the pairs that can still hold a minimum, and their number;
 //   (scan)
 //   k_mx_expand   the bit masks to a flat work list;
 //   k_mx_eval     FP64 TriDist on the work list, one pair per lane;
@@ -19,19 +19,21 @@
 // register footprint, and the FP64 work runs as dense, balanced lists however
 // uneven the voxel pairs are.
 //
-// FP32 passes: one wave per voxel pair. The lanes hold the triangles of the
-// larger voxel (64 at a time, bound records in registers); the wave walks the
-// smaller voxel's triangles in a uniform loop, so their records come through
-// scalar loads and every pair costs arithmetic only. A pair is first tested
-// with its sphere interval (centre distance +- radii, about 30 flops); only
-// pairs that pass go on to the vertex-level projection interval.
+// FP32 passes: one wave per voxel pair, walking its n1 x n2 grid in tiles of
+// R rows x cw columns (R * cw <= 64 lanes, chosen per voxel pair for the
+// fewest iterations). The larger voxel's triangles are the columns: a lane
+// keeps its column's records in registers for the whole walk. With one row
+// per iteration the row triangle is wave-uniform and its records come through
+// scalar loads; with several rows each lane loads its row's records. A pair
+// is first tested with its disc interval (per-triangle data only); only pairs
+// that pass go on to the vertex-level interval.
 #pragma once
 
 #include "tdp_kernels.hpp"
 
 namespace tdp {
 
-enum : uint32_t { MX_EMPTY = 1u, MX_SMALL = 2u, MX_LOZERO = 4u };
+enum : uint32_t { MX_EMPTY = 1u, MX_SMALL = 2u };
 constexpr uint32_t kNoProbe = 0xffffffffu;
 // voxel pairs of at most this many triangle pairs go straight to FP64
 constexpr uint32_t kMxSmallPairs = 8;
@@ -58,6 +60,7 @@ __device__ __forceinline__ VpLod vp_lod(uint32_t v1, uint32_t v2, const uint64_t
 __device__ __forceinline__ uint32_t wave_item() {
     return __builtin_amdgcn_readfirstlane((uint32_t)wave_global());
 }
+
 
 // ---------------------------------------------------------------- intervals
 
@@ -101,6 +104,9 @@ __device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t 
 // comparisons with U 3 -- 89 per triangle pair.
 __device__ __forceinline__ void disc_interval(const TriS &A, const TriS &B, float lw[3],
                                               float uw[3]) {
+    // fused multiply-adds round once where two operations round twice, so the
+    // margins (per-operation error bounds) still hold
+#pragma clang fp contract(fast)
     const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
     const float rs = A.r + B.r;
@@ -126,6 +132,7 @@ __device__ __forceinline__ void disc_interval(const TriS &A, const TriS &B, floa
 // Vertex-level interval (the arithmetic of pair_interval on register data).
 __device__ __forceinline__ void vertex_interval(const TriS &A, const TriV &a, const TriS &B,
                                                 const TriV &c, float lw[3], float uw[3]) {
+#pragma clang fp contract(fast)
     const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
     const float rs = A.r + B.r;
@@ -159,30 +166,64 @@ __device__ __forceinline__ void vertex_interval(const TriS &A, const TriV &a, co
 }
 
 // ------------------------------------------------------------ pair walking
-//
-// Calls f(A, B, p, va, vc) for every triangle pair of the voxel pair: A/B
-// the tile-1/tile-2 sphere data, p the pair's place, va()/vc() the vertices
-// of A/B. The lanes hold the larger side, 64 triangles at a time (sphere data
-// and vertices in registers); the smaller side is walked in a wave-uniform
-// loop, so its records come through scalar loads. SWAP: the lanes hold
-// tile-1 triangles. Idle lanes (p.valid = false) take part in every call, so
-// f may use wave-wide operations. p.word numbers the (lane chunk, uniform
-// step) iterations 0, 1, ... in order; the pair is bit (lane) of that word.
+
+// Tiling of a voxel pair's grid: the larger side (nl triangles) is split
+// into nch column chunks of width cw, the smaller side (ns) into ngr groups
+// of R rows; iteration (chunk k, group g) is word k * ngr + g of the voxel
+// pair's candidate mask, and lane r * cw + c is its bit for row g * R + r,
+// column k * cw + c.
+struct Tiling {
+    uint32_t cw, R, nch, ngr;
+    bool swap;  // the columns are tile-1 triangles
+};
+
+__device__ __forceinline__ Tiling mx_tiling(const VpLod &g) {
+    Tiling t;
+    t.swap = g.n1 > g.n2;
+    const uint32_t nl = t.swap ? g.n1 : g.n2, ns = t.swap ? g.n2 : g.n1;
+    const uint32_t m0 = (nl + kWave - 1) / kWave;
+    uint32_t best = 0xffffffffu;
+    for (uint32_t m = m0; m < m0 + 3; m++) {
+        const uint32_t cw = (nl + m - 1) / m;
+        const uint32_t R = min((uint32_t)kWave / cw, ns);
+        const uint32_t nch = (nl + cw - 1) / cw, ngr = (ns + R - 1) / R;
+        if (nch * ngr < best) {
+            best = nch * ngr;
+            t.cw = cw;
+            t.R = R;
+            t.nch = nch;
+            t.ngr = ngr;
+        }
+    }
+    return t;
+}
+
+__device__ __forceinline__ uint32_t mx_words(const Tiling &t) { return t.nch * t.ngr; }
+
+// (i, j) of bit b of word k
+__device__ __forceinline__ void mx_pair_of(const Tiling &t, uint32_t k, uint32_t b, uint32_t &i,
+                                           uint32_t &j) {
+    const uint32_t ch = k / t.ngr, gr = k - ch * t.ngr;
+    const uint32_t r = b / t.cw, c = b - r * t.cw;
+    const uint32_t row = gr * t.R + r, col = ch * t.cw + c;
+    i = t.swap ? col : row;
+    j = t.swap ? row : col;
+}
 
 struct PairAt {
     uint32_t i, j;     // triangle indices within the two voxels
-    uint32_t word;     // iteration number: chunk * n_small + step
+    uint32_t word;     // iteration number
     bool valid;
 };
 
-__device__ __forceinline__ bool mx_swap(const VpLod &g) { return g.n1 > g.n2; }
-__device__ __forceinline__ uint32_t mx_words(const VpLod &g) {
-    const uint32_t nl = mx_swap(g) ? g.n1 : g.n2, ns = mx_swap(g) ? g.n2 : g.n1;
-    return ns * ((nl + kWave - 1) / kWave);
-}
-
-template <bool SWAP, typename F>
-__device__ __forceinline__ void walk_pairs(const VpLod &g, int lane, const float4 *__restrict__ rec1,
+// Calls f(A, B, p, va, vc) for every cell of the tiling: A/B the tile-1 /
+// tile-2 sphere data, p the place, va()/vc() the vertices of A/B. All 64
+// lanes take part in every call (p.valid = false for cells outside the grid),
+// so f may use wave-wide operations. ROWS1: one row per iteration (R == 1),
+// the row triangle is wave-uniform.
+template <bool SWAP, bool ROWS1, typename F>
+__device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
+                                           const float4 *__restrict__ rec1,
                                            const float4 *__restrict__ bnd1,
                                            const float4 *__restrict__ rec2,
                                            const float4 *__restrict__ bnd2, F &&f) {
@@ -192,35 +233,51 @@ __device__ __forceinline__ void walk_pairs(const VpLod &g, int lane, const float
     const float4 *__restrict__ bs = SWAP ? bnd2 : bnd1;
     const float4 *__restrict__ rl = SWAP ? rec1 : rec2;
     const float4 *__restrict__ rs = SWAP ? rec2 : rec1;
+    const uint32_t r = ROWS1 ? 0u : (uint32_t)lane / t.cw;
+    const uint32_t c = ROWS1 ? (uint32_t)lane : (uint32_t)lane - r * t.cw;
+    const bool in_tile = ROWS1 ? c < t.cw : r < t.R;
     uint32_t word = 0;
-    for (uint32_t lb = 0; lb < nl; lb += kWave) {
-        const uint32_t li = lb + lane;
-        const bool valid = li < nl;
-        const uint32_t lc = valid ? li : nl - 1;  // in bounds for idle lanes
-        const TriS L = load_s(bl, ol + lc);
-        const TriV LV = load_v(rl, ol + lc);
-        for (uint32_t si = 0; si < ns; si++, word++) {
-            const TriS S = load_s(bs, os + si);
+    for (uint32_t ch = 0; ch < t.nch; ch++) {
+        const uint32_t col = ch * t.cw + c;
+        const bool cv = in_tile && col < nl;
+        const uint64_t lt = ol + (cv ? col : nl - 1);  // in bounds for idle lanes
+        const TriS L = load_s(bl, lt);
+        const TriV LV = load_v(rl, lt);
+        for (uint32_t gr = 0; gr < t.ngr; gr++, word++) {
+            const uint32_t row = gr * t.R + r;
+            const bool valid = cv && row < ns;
+            const uint64_t st = os + (row < ns ? row : ns - 1);
+            const TriS S = load_s(bs, st);
             auto lane_v = [&]() { return LV; };
-            auto step_v = [&]() { return load_v(rs, os + si); };
+            auto row_v = [&]() { return load_v(rs, st); };
             if (SWAP)
-                f(L, S, PairAt{lc, si, word, valid}, lane_v, step_v);
+                f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v);
             else
-                f(S, L, PairAt{si, lc, word, valid}, step_v, lane_v);
+                f(S, L, PairAt{row, cv ? col : 0u, word, valid}, row_v, lane_v);
         }
     }
 }
 
 template <typename F>
-__device__ __forceinline__ void for_pairs(const VpLod &g, int lane, const float4 *__restrict__ rec1,
+__device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int lane,
+                                          const float4 *__restrict__ rec1,
                                           const float4 *__restrict__ bnd1,
                                           const float4 *__restrict__ rec2,
                                           const float4 *__restrict__ bnd2, F &&f) {
-    if (mx_swap(g))
-        walk_pairs<true>(g, lane, rec1, bnd1, rec2, bnd2, f);
-    else
-        walk_pairs<false>(g, lane, rec1, bnd1, rec2, bnd2, f);
+    if (t.swap) {
+        if (t.R == 1)
+            walk_tiles<true, true>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+        else
+            walk_tiles<true, false>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+    } else {
+        if (t.R == 1)
+            walk_tiles<false, true>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+        else
+            walk_tiles<false, false>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+    }
 }
+
+// ------------------------------------------------------------------ bounds
 
 // lowest q among the lanes holding the wave minimum of v (kNoProbe if none)
 __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
@@ -231,13 +288,12 @@ __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
     return c;
 }
 
-// ------------------------------------------------------------------ kernels
-//
-// `need` selects the quantities the round folds (bit 0: d, bit 1: lo, bit 2:
-// hi); the others get no probes and no candidates, and their minima are left
-// unformed.
-
-__global__ void __launch_bounds__(kBlock) k_mx_bounds1(
+// Per voxel pair: the smallest FP32 upper bound of each quantity the round
+// folds (vpU; -inf for the others) and up to three probe pairs, the pairs of
+// smallest score (centre distance less the radii, with the quantity's
+// Hausdorff terms), whose FP64 values usually are the minima. Also the flags
+// and the mask word count of the tiled walk.
+__global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
     const float4 *__restrict__ rec1, const float4 *__restrict__ bnd1,
@@ -249,37 +305,28 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds1(
     if (w >= n_vp) return;
     const int lane = lane_id();
     const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
-    const unsigned long long inf_key = dkey(INFINITY);
     if (lane < 3) {
-        mkey[3 * (size_t)w + lane] = inf_key;
+        mkey[3 * (size_t)w + lane] = dkey(INFINITY);
         probe[3 * (size_t)w + lane] = kNoProbe;
         vpU[3 * (size_t)w + lane] = -INFINITY;
     }
-    if (g.n1 == 0 || g.n2 == 0) {
+    if (g.n1 == 0 || g.n2 == 0 || g.n1 * g.n2 <= kMxSmallPairs) {
         if (lane == 0) {
-            flags[w] = MX_EMPTY;
+            flags[w] = (g.n1 == 0 || g.n2 == 0) ? MX_EMPTY : MX_SMALL;
             nwords[w] = 0;
         }
         return;
     }
     const uint32_t n2 = g.n2;
-    if (g.n1 * n2 <= kMxSmallPairs) {
-        if (lane == 0) {
-            flags[w] = MX_SMALL;
-            nwords[w] = 0;
-        }
-        return;
-    }
+    const Tiling t = mx_tiling(g);
     float up[3] = {INFINITY, INFINITY, INFINITY};
     float best[3] = {INFINITY, INFINITY, INFINITY};
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
-    for_pairs(g, lane, rec1, bnd1, rec2, bnd2,
+    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
               [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&) {
                   if (!p.valid) return;
                   float lw[3], uw[3];
                   disc_interval(A, B, lw, uw);
-                  // probe score: centre distance less the radii (not a
-                  // bound, only a guess at the closest pair)
                   const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
                   const float sc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - A.r - B.r;
                   const float key[3] = {sc, sc - A.ph - B.ph, sc + A.h + B.h};
@@ -304,11 +351,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds1(
             pq[f] = kNoProbe;
         }
     }
-    uint32_t fl = 0;
-    if (U[1] == 0.f) {  // some pair certainly has lo = +0: the minimum is +0
-        fl |= MX_LOZERO;
-        pq[1] = kNoProbe;
-    }
+    const bool lo_zero = U[1] == 0.f;  // some pair certainly has lo = +0
+    if (lo_zero) pq[1] = kNoProbe;
     if (pq[1] == pq[0]) pq[1] = kNoProbe;
     if (pq[2] == pq[0] || pq[2] == pq[1]) pq[2] = kNoProbe;
     if (lane < 3) {
@@ -316,11 +360,13 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds1(
         probe[3 * (size_t)w + lane] = lane == 0 ? pq[0] : (lane == 1 ? pq[1] : pq[2]);
     }
     if (lane == 0) {
-        flags[w] = fl;
-        nwords[w] = mx_words(g);
-        if (fl & MX_LOZERO) mkey[3 * (size_t)w + 1] = dkey(0.0);
+        flags[w] = 0;
+        nwords[w] = mx_words(t);
+        if (lo_zero) mkey[3 * (size_t)w + 1] = dkey(0.0);  // the minimum is +0
     }
 }
+
+// ------------------------------------------------------------- FP64 lists
 
 // Minima over the lanes of one voxel pair w (lanes are sorted by w; kNoProbe
 // marks an empty lane), folded with one atomic per voxel pair and quantity.
@@ -390,15 +436,20 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
     if (lane_id() == 0 && n) atomicAdd(&stats[2], (unsigned long long)n);
 }
 
-// U tightened by the exact values found so far (rounded up to FP32)
-__device__ __forceinline__ void tightened_U(const float *vpU, const unsigned long long *mkey,
-                                            uint32_t fl, size_t w, float U[3]) {
+// ------------------------------------------------------------ FP32 pass
+
+// U per quantity: the FP32 upper bound tightened by the exact values found
+// so far (rounded up to FP32); -inf for quantities the round does not fold
+// (their vpU) and for lo once it is known to be +0, its smallest value
+__device__ __forceinline__ void bounds_U(const float *vpU, const unsigned long long *mkey,
+                                         size_t w, float U[3]) {
 #pragma unroll
-    for (int f = 0; f < 3; f++) U[f] = fminf(vpU[3 * w + f], ulp_up((float)dunkey(mkey[3 * w + f])));
-    if (fl & MX_LOZERO) U[1] = -INFINITY;
+    for (int f = 0; f < 3; f++)
+        U[f] = fminf(vpU[3 * w + f], ulp_up((float)dunkey(mkey[3 * w + f])));
+    if (mkey[3 * w + 1] == dkey(0.0)) U[1] = -INFINITY;
 }
 
-// Candidate test: the sphere interval first, the vertex interval for pairs
+// Candidate test: the disc interval first, the vertex interval for pairs
 // that pass it; probes (already evaluated) are never candidates.
 template <typename VA, typename VC>
 __device__ __forceinline__ bool mx_candidate(const TriS &A, const TriS &B, const PairAt &p,
@@ -414,7 +465,7 @@ __device__ __forceinline__ bool mx_candidate(const TriS &A, const TriS &B, const
 }
 
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
-// the ballot of iteration k of the pair walk; count[w] = number of bits.
+// the ballot of iteration k of the tiled walk; count[w] = number of bits.
 // SMALL voxel pairs: every pair, no mask.
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
@@ -424,8 +475,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const float4 *__restrict__ rec2, const float4 *__restrict__ bnd2, int slot,
     const float *__restrict__ vpU, const uint32_t *__restrict__ probe,
     const uint32_t *__restrict__ flags, const unsigned long long *__restrict__ mkey,
-    const uint64_t *__restrict__ woff, unsigned long long *__restrict__ mask,
-    uint32_t *__restrict__ count) {
+    const uint64_t *__restrict__ woff,
+    unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
     const int lane = lane_id();
@@ -440,14 +491,15 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
         return;
     }
     float U[3];
-    tightened_U(vpU, mkey, fl, w, U);
+    bounds_U(vpU, mkey, w, U);
     const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
                    p2 = probe[3 * (size_t)w + 2];
-    const uint32_t n2 = g.n2, nw = mx_words(g);
+    const Tiling t = mx_tiling(g);
+    const uint32_t n2 = g.n2, nw = mx_words(t);
     unsigned long long *__restrict__ out = mask + woff[w];
     uint32_t c = 0;
     unsigned long long mine = 0;  // lane k keeps word (k mod 64) of the current group
-    for_pairs(g, lane, rec1, bnd1, rec2, bnd2,
+    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
               [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc) {
                   const bool take = mx_candidate(A, B, p, va, vc, U, p.i * n2 + p.j, p0, p1, p2);
                   const unsigned long long m = __ballot(take);
@@ -484,9 +536,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
         }
         return;
     }
-    const bool swap = mx_swap(g);
-    const uint32_t ns = swap ? g.n2 : g.n1, n2 = g.n2;
-    const uint32_t nw = mx_words(g);
+    const Tiling t = mx_tiling(g);
+    const uint32_t n2 = g.n2, nw = mx_words(t);
     const unsigned long long *__restrict__ in = mask + woff[w];
     for (uint32_t wb = 0; wb < nw; wb += kWave) {
         const uint32_t k = wb + lane;
@@ -501,12 +552,11 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
         }
         const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
         uint64_t at = pos + (incl - cnt);
-        const uint32_t chunk = k / (ns ? ns : 1), si = k - chunk * ns;
         while (m) {
             const uint32_t b = (uint32_t)__ffsll((long long)m) - 1u;
             m &= m - 1ull;
-            const uint32_t li = chunk * kWave + b;
-            const uint32_t i = swap ? li : si, j = swap ? si : li;
+            uint32_t i, j;
+            mx_pair_of(t, k, b, i, j);
             item_vp[at] = w;
             item_q[at] = i * n2 + j;
             at++;
