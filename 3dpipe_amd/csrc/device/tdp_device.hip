@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../../include/tdp.h"
@@ -21,6 +23,11 @@ struct tdp_ctx {
     std::vector<void *> scan_buf;
     std::vector<size_t> scan_cap;
     uint64_t *d_total = nullptr;
+    // block cache for join buffers: released blocks are kept and reused by
+    // later joins (no hipMalloc / hipFree per join); freed at tdp_close, or
+    // when an allocation would otherwise run out of memory
+    std::multimap<size_t, void *> pool_free;
+    std::unordered_map<void *, size_t> pool_size;
 };
 
 struct tdp_tile {
@@ -133,6 +140,62 @@ static void dfree(void *p) {
     if (p) (void)hipFree(p);
 }
 
+static size_t size_class(size_t b) {
+    if (b <= ((size_t)1 << 16)) return (size_t)1 << 16;
+    size_t p = 1;
+    while (p < b) p <<= 1;
+    const size_t step = p / 8;
+    return (b + step - 1) / step * step;
+}
+
+static void pool_release_all(tdp_ctx *ctx) {
+    for (auto &kv : ctx->pool_free) {
+        (void)hipFree(kv.second);
+        ctx->pool_size.erase(kv.second);
+    }
+    ctx->pool_free.clear();
+}
+
+// join buffers come from the context's block cache (callers release them only
+// when no queued kernel uses them: after a stream synchronisation)
+template <typename T>
+static int palloc(tdp_ctx *ctx, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    const size_t bytes = size_class(count * sizeof(T));
+    auto it = ctx->pool_free.lower_bound(bytes);
+    if (it != ctx->pool_free.end() && it->first <= bytes + bytes / 2) {
+        *p = (T *)it->second;
+        ctx->pool_free.erase(it);
+        return TDP_OK;
+    }
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e == hipErrorOutOfMemory && !ctx->pool_free.empty()) {
+        (void)hipGetLastError();
+        pool_release_all(ctx);
+        e = hipMalloc(&q, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->err = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? TDP_ERR_NOMEM : TDP_ERR_HIP;
+    }
+    ctx->pool_size[q] = bytes;
+    *p = (T *)q;
+    return TDP_OK;
+}
+
+static void pfree(tdp_ctx *ctx, void *p) {
+    if (!p) return;
+    auto it = ctx->pool_size.find(p);
+    if (it == ctx->pool_size.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    ctx->pool_free.emplace(it->second, p);
+}
+
 // exclusive scan of n counters (u32 or u64) into out (u64); *total set on host
 template <typename In>
 static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, uint64_t *total) {
@@ -229,6 +292,8 @@ extern "C" int tdp_close(tdp_ctx *ctx) {
     if (!ctx) return TDP_ERR_ARG;
     for (void *p : ctx->scan_buf) dfree(p);
     dfree(ctx->d_total);
+    pool_release_all(ctx);
+    for (auto &kv : ctx->pool_size) (void)hipFree(kv.first);  // blocks still held by joins
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TDP_OK;
@@ -474,13 +539,14 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
 
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
+    tdp_ctx *ctx = j->ctx;
     void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q};
-    for (void *p : ptrs) dfree(p);
+    for (void *p : ptrs) pfree(ctx, p);
     for (int b = 0; b < 2; b++) {
-        dfree(j->v1[b]); dfree(j->v2[b]); dfree(j->vpp[b]); dfree(j->vmin[b]); dfree(j->vmax[b]);
+        pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
     }
     for (auto &e : j->ev)
         if (e) (void)hipEventDestroy(e);
@@ -539,20 +605,20 @@ extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile 
     j->P = P;
     j->rsz = prm->precision == TDP_PREC_REF32 ? 4 : 8;
     int rc = TDP_OK;
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->prefix, N + 1);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->pair_obj2, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->pair_target, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->rmin, P * j->rsz);
-    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->rmax, P * j->rsz);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->state, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->round, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->scratch, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->confirmed, N);
-    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->acc_min, P * j->rsz);
-    if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->acc_max, P * j->rsz);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->count, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->offs, P);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->stats, 4);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->prefix, N + 1);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->pair_obj2, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->pair_target, P);
+    if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->rmin, P * j->rsz);
+    if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->rmax, P * j->rsz);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->state, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->round, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->scratch, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->confirmed, N);
+    if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->acc_min, P * j->rsz);
+    if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->acc_max, P * j->rsz);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->count, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->offs, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->stats, 4);
     if (rc != TDP_OK) {
         tdp_join_destroy(j);
         return rc;
@@ -582,23 +648,23 @@ static int ensure_vp(tdp_join *j, size_t n) {
     tdp_ctx *ctx = j->ctx;
     size_t cap = std::max(n, (size_t)1024);
     for (int b = 0; b < 2; b++) {
-        dfree(j->v1[b]); dfree(j->v2[b]); dfree(j->vpp[b]); dfree(j->vmin[b]); dfree(j->vmax[b]);
+        pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
         j->v1[b] = j->v2[b] = j->vpp[b] = nullptr;
         j->vmin[b] = j->vmax[b] = nullptr;
     }
-    dfree(j->keep); dfree(j->pos);
+    pfree(ctx, j->keep); pfree(ctx, j->pos);
     j->keep = nullptr; j->pos = nullptr;
     j->cap_vp = 0;
     int rc = TDP_OK;
     for (int b = 0; b < 2 && rc == TDP_OK; b++) {
-        rc = dalloc(ctx, &j->v1[b], cap);
-        if (rc == TDP_OK) rc = dalloc(ctx, &j->v2[b], cap);
-        if (rc == TDP_OK) rc = dalloc(ctx, &j->vpp[b], cap);
-        if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->vmin[b], cap * j->rsz);
-        if (rc == TDP_OK) rc = dalloc(ctx, (char **)&j->vmax[b], cap * j->rsz);
+        rc = palloc(ctx, &j->v1[b], cap);
+        if (rc == TDP_OK) rc = palloc(ctx, &j->v2[b], cap);
+        if (rc == TDP_OK) rc = palloc(ctx, &j->vpp[b], cap);
+        if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->vmin[b], cap * j->rsz);
+        if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->vmax[b], cap * j->rsz);
     }
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->keep, cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->pos, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->keep, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->pos, cap);
     if (rc == TDP_OK) j->cap_vp = cap;
     return rc;
 }
@@ -608,17 +674,17 @@ static int ensure_mx(tdp_join *j, size_t n) {
     if (n <= j->cap_mx) return TDP_OK;
     tdp_ctx *ctx = j->ctx;
     const size_t cap = std::max(n, j->cap_vp);
-    dfree(j->mx_U); dfree(j->mx_probe); dfree(j->mx_flags); dfree(j->mx_key);
-    dfree(j->mx_nw); dfree(j->mx_woff);
+    pfree(ctx, j->mx_U); pfree(ctx, j->mx_probe); pfree(ctx, j->mx_flags); pfree(ctx, j->mx_key);
+    pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff);
     j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
     j->mx_nw = nullptr; j->mx_woff = nullptr;
     j->cap_mx = 0;
-    int rc = dalloc(ctx, &j->mx_U, 3 * cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_probe, 3 * cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_flags, cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_key, 3 * cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_nw, cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->mx_woff, cap);
+    int rc = palloc(ctx, &j->mx_U, 3 * cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_probe, 3 * cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_flags, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_key, 3 * cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_nw, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_woff, cap);
     if (rc == TDP_OK) j->cap_mx = cap;
     return rc;
 }
@@ -626,11 +692,12 @@ static int ensure_mx(tdp_join *j, size_t n) {
 // candidate bit masks, grow-only
 static int ensure_mask(tdp_join *j, size_t n) {
     if (n <= j->cap_mask) return TDP_OK;
+    tdp_ctx *ctx = j->ctx;
     const size_t cap = std::max(n + n / 4, (size_t)1 << 16);
-    dfree(j->mx_mask);
+    pfree(ctx, j->mx_mask);
     j->mx_mask = nullptr;
     j->cap_mask = 0;
-    int rc = dalloc(j->ctx, &j->mx_mask, cap);
+    int rc = palloc(j->ctx, &j->mx_mask, cap);
     if (rc == TDP_OK) j->cap_mask = cap;
     return rc;
 }
@@ -640,11 +707,11 @@ static int ensure_items(tdp_join *j, size_t n) {
     if (n <= j->cap_items) return TDP_OK;
     tdp_ctx *ctx = j->ctx;
     const size_t cap = std::max(n + n / 4, (size_t)1 << 16);
-    dfree(j->item_vp); dfree(j->item_q);
+    pfree(ctx, j->item_vp); pfree(ctx, j->item_q);
     j->item_vp = nullptr; j->item_q = nullptr;
     j->cap_items = 0;
-    int rc = dalloc(ctx, &j->item_vp, cap);
-    if (rc == TDP_OK) rc = dalloc(ctx, &j->item_q, cap);
+    int rc = palloc(ctx, &j->item_vp, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->item_q, cap);
     if (rc == TDP_OK) j->cap_items = cap;
     return rc;
 }
@@ -742,9 +809,12 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     k_reset_acc<R, KT><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (KT *)j->acc_min,
                                                                 (KT *)j->acc_max);
     TDP_LAUNCHED(ctx);
-    if (mark(TDP_STAGE_RESET) != TDP_OK) return TDP_ERR_HIP;
     const tdp_tile *A = j->t1, *B = j->t2;
-    if (MODE != MODE_MIXED) {
+    k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_cnt,
+                                                           B->vox_tri_cnt, slot, j->stats);
+    TDP_LAUNCHED(ctx);
+    if (mark(TDP_STAGE_RESET) != TDP_OK) return TDP_ERR_HIP;
+    if constexpr (MODE != MODE_MIXED) {
         k_refine<R, MODE><<<wave_grid(n), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_off,
             A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, A->bnd, B->bnd,

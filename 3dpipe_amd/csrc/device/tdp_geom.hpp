@@ -193,9 +193,13 @@ __device__ __forceinline__ R tri_dist(const R *S, const R *T, bool *cert = nullp
 
 // voxel-pair bounds: [box min distance, core-core distance]
 // (voxel_dist_core, computing_gpu.cu:1606-1631); boxes are 9 floats
+// squared box gap s and squared core distance c of two voxel records
+// {low, high, core}; voxel_bound is (sqrt s, sqrt c). The square root being
+// monotone, min sqrt(s) over many pairs is sqrt(min s), bit for bit.
 template <typename R>
-__device__ __forceinline__ void voxel_bound(const float *a, const float *b, R &mn, R &mx) {
-    R s = 0, c = 0;
+__device__ __forceinline__ void voxel_sq(const float *a, const float *b, R &s, R &c) {
+    s = 0;
+    c = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const R t1 = (R)a[i] - (R)b[i + 3];
@@ -205,6 +209,12 @@ __device__ __forceinline__ void voxel_bound(const float *a, const float *b, R &m
         const R dc = (R)a[i + 6] - (R)b[i + 6];
         c += dc * dc;
     }
+}
+
+template <typename R>
+__device__ __forceinline__ void voxel_bound(const float *a, const float *b, R &mn, R &mx) {
+    R s, c;
+    voxel_sq<R>(a, b, s, c);
     mn = dsqrt<R>(s);
     mx = dsqrt<R>(c);
 }

@@ -71,32 +71,43 @@ template <> struct Key<double> {
 
 // object-level [min boxmin, min coredist] over all voxel pairs of each
 // candidate pair (voxel_pair_dist_kernel_for_all_aggregate_distance,
-// computing_gpu.cu:1984-2043); one wave per object pair
+// computing_gpu.cu:1984-2043); one wave per object pair. The lanes hold the
+// tile-2 voxels (records in registers), the wave walks the tile-1 voxels in a
+// uniform loop (scalar loads); squared distances are min-reduced and the
+// square roots taken once (sqrt is monotone, so the result is the same).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_aggregate(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, R *__restrict__ rmin,
     R *__restrict__ rmax) {
-    const size_t p = wave_global();
+    const uint32_t p = __builtin_amdgcn_readfirstlane((uint32_t)wave_global());
     if (p >= P) return;
     const int lane = lane_id();
     const uint32_t a0 = vo1[pair_target[p]], n1 = vo1[pair_target[p] + 1] - a0;
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
-    const uint32_t total = n1 * n2;
-    R mn = (R)__builtin_inff(), mx = (R)__builtin_inff();
-    for (uint32_t t = lane; t < total; t += kWave) {
-        const uint32_t i = t / n2, j = t - i * n2;
-        R a, b;
-        voxel_bound<R>(vb1 + 9 * (size_t)(a0 + i), vb2 + 9 * (size_t)(b0 + j), a, b);
-        mn = rmin2(mn, a);
-        mx = rmin2(mx, b);
+    R smin = (R)__builtin_inff(), cmin = (R)__builtin_inff();
+    for (uint32_t jb = 0; jb < n2; jb += kWave) {
+        const uint32_t j = jb + lane;
+        const bool valid = j < n2;
+        float bl[9];
+        const float *src = vb2 + 9 * (size_t)(b0 + (valid ? j : n2 - 1));
+#pragma unroll
+        for (int k = 0; k < 9; k++) bl[k] = src[k];
+        for (uint32_t i = 0; i < n1; i++) {
+            R s, c;
+            voxel_sq<R>(vb1 + 9 * (size_t)(a0 + i), bl, s, c);
+            if (valid) {
+                smin = rmin2(smin, s);
+                cmin = rmin2(cmin, c);
+            }
+        }
     }
-    mn = wave_min(mn);
-    mx = wave_min(mx);
+    smin = wave_min(smin);
+    cmin = wave_min(cmin);
     if (lane == 0) {
-        rmin[p] = mn;
-        rmax[p] = mx;
+        rmin[p] = dsqrt<R>(smin);
+        rmax[p] = dsqrt<R>(cmin);
     }
 }
 
@@ -571,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_refine(
     const float *__restrict__ ohd1, const float *__restrict__ ohd2, int slot, int is_last,
     int level, typename Key<R>::type *__restrict__ acc_min,
     typename Key<R>::type *__restrict__ acc_max, unsigned long long *__restrict__ stats) {
-    __shared__ uint32_t queue[kWavesPerBlock][2 * kWave];
+    static_assert(MODE != MODE_MIXED, "the mixed form runs as the k_mx_* pipeline");
     const size_t w = wave_global();
     if (w >= n_vp) return;
     const int lane = lane_id();
@@ -592,31 +603,17 @@ __global__ void __launch_bounds__(kBlock) k_refine(
     }
     const uint32_t total = n1 * n2;
     R d = (R)__builtin_inff(), lo = (R)__builtin_inff(), hi = (R)__builtin_inff();
-    uint64_t evals64 = 0, culled = 0;
-    // small voxel pairs: two FP64 passes cost no more than the bound stage
-    if (MODE != MODE_MIXED || total <= kMixedMinPairs) {
-        for (uint32_t q = lane; q < total; q += kWave) {
-            const uint32_t i = q / n2, j = q - i * n2;
-            R a, b, c;
-            pair_values<R>(rec1, o1 + i, rec2, o2 + j, a, b, c);
-            d = rmin2(d, a);
-            lo = rmin2(lo, b);
-            hi = rmin2(hi, c);
-            if (MODE == MODE_MIXED) evals64++;
-        }
-    } else {
-        double dd = INFINITY, dl = INFINITY, dh = INFINITY;
-        mixed_minima(rec1, bnd1, o1, rec2, bnd2, o2, n2, total, lane,
-                     queue[threadIdx.x >> 6], dd, dl, dh, evals64, culled);
-        d = (R)dd; lo = (R)dl; hi = (R)dh;
+    for (uint32_t q = lane; q < total; q += kWave) {
+        const uint32_t i = q / n2, j = q - i * n2;
+        R a, b, c;
+        pair_values<R>(rec1, o1 + i, rec2, o2 + j, a, b, c);
+        d = rmin2(d, a);
+        lo = rmin2(lo, b);
+        hi = rmin2(hi, c);
     }
     d = wave_min(d);
     lo = wave_min(lo);
     hi = wave_min(hi);
-    if (MODE == MODE_MIXED) {
-        evals64 = wave_sum_u64(evals64);
-        culled = wave_sum_u64(culled);
-    }
     if (lane == 0) {
         R nmin = omin, nmax = omax;
         if (is_last) {
@@ -639,12 +636,38 @@ __global__ void __launch_bounds__(kBlock) k_refine(
         vp_max[w] = nmax;
         atomicMin(&acc_min[p], Key<R>::enc(nmin));
         atomicMin(&acc_max[p], Key<R>::enc(nmax));
-        atomicAdd(&stats[0], (unsigned long long)total);
-        atomicAdd(&stats[1], 1ull);
-        if (MODE == MODE_MIXED) {
-            atomicAdd(&stats[2], (unsigned long long)evals64);
-            atomicAdd(&stats[3], (unsigned long long)culled);
+    }
+}
+
+// per round: triangle pairs covered (stats[0]) and non-empty voxel pairs
+// evaluated (stats[1]); one thread per voxel pair, one atomic per block
+__global__ void __launch_bounds__(kBlock) k_round_stats(
+    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
+    const uint32_t *__restrict__ cnt1, const uint32_t *__restrict__ cnt2, int slot,
+    unsigned long long *__restrict__ stats) {
+    __shared__ unsigned long long part[2][kWavesPerBlock];
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t pairs = 0, vps = 0;
+    if (w < n_vp && slot >= 0) {
+        const uint64_t t = (uint64_t)cnt1[5 * (size_t)vp_v1[w] + slot] * cnt2[5 * (size_t)vp_v2[w] + slot];
+        pairs = t;
+        vps = t ? 1 : 0;
+    }
+    pairs = wave_sum_u64(pairs);
+    vps = wave_sum_u64(vps);
+    if (lane_id() == 0) {
+        part[0][threadIdx.x / kWave] = pairs;
+        part[1][threadIdx.x / kWave] = vps;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int k = 0; k < kWavesPerBlock; k++) {
+            a += part[0][k];
+            b += part[1][k];
         }
+        if (a) atomicAdd(&stats[0], a);
+        if (b) atomicAdd(&stats[1], b);
     }
 }
 

@@ -200,16 +200,6 @@ __device__ __forceinline__ Tiling mx_tiling(const VpLod &g) {
 
 __device__ __forceinline__ uint32_t mx_words(const Tiling &t) { return t.nch * t.ngr; }
 
-// (i, j) of bit b of word k
-__device__ __forceinline__ void mx_pair_of(const Tiling &t, uint32_t k, uint32_t b, uint32_t &i,
-                                           uint32_t &j) {
-    const uint32_t ch = k / t.ngr, gr = k - ch * t.ngr;
-    const uint32_t r = b / t.cw, c = b - r * t.cw;
-    const uint32_t row = gr * t.R + r, col = ch * t.cw + c;
-    i = t.swap ? col : row;
-    j = t.swap ? row : col;
-}
-
 struct PairAt {
     uint32_t i, j;     // triangle indices within the two voxels
     uint32_t word;     // iteration number
@@ -539,6 +529,15 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
     const Tiling t = mx_tiling(g);
     const uint32_t n2 = g.n2, nw = mx_words(t);
     const unsigned long long *__restrict__ in = mask + woff[w];
+    // bit b of a word is row b / cw, column b % cw of its tile: a per-wave
+    // table instead of a division per bit
+    __shared__ uint16_t tab[kWavesPerBlock][kWave];
+    uint16_t *mytab = tab[threadIdx.x / kWave];
+    {
+        const uint32_t r = (uint32_t)lane / t.cw;
+        mytab[lane] = (uint16_t)((r << 8) | ((uint32_t)lane - r * t.cw));
+    }
+    wave_sync();
     for (uint32_t wb = 0; wb < nw; wb += kWave) {
         const uint32_t k = wb + lane;
         unsigned long long m = k < nw ? in[k] : 0ull;
@@ -552,20 +551,47 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
         }
         const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
         uint64_t at = pos + (incl - cnt);
+        const uint32_t ch = k / t.ngr, gr = k - ch * t.ngr;
+        const uint32_t row0 = gr * t.R, col0 = ch * t.cw;
         while (m) {
             const uint32_t b = (uint32_t)__ffsll((long long)m) - 1u;
             m &= m - 1ull;
-            uint32_t i, j;
-            mx_pair_of(t, k, b, i, j);
+            const uint32_t e = mytab[b];
+            const uint32_t row = row0 + (e >> 8), col = col0 + (e & 255u);
             item_vp[at] = w;
-            item_q[at] = i * n2 + j;
+            item_q[at] = t.swap ? col * n2 + row : row * n2 + col;
             at++;
         }
         pos += total;
     }
 }
 
-// fold the exact minima into the voxel pair's range (same rules as k_refine)
+// Minima of two keys over the lanes of one segment (lanes sorted by seg;
+// 0xffffffff marks an empty lane), folded with one atomic per segment.
+template <typename K>
+__device__ __forceinline__ void seg_min2(uint32_t seg, K a, K b, K none, K *__restrict__ ka,
+                                         K *__restrict__ kb) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t os = (uint32_t)__shfl_down((int)seg, o, kWave);
+        const K oa = __shfl_down(a, o, kWave);
+        const K ob = __shfl_down(b, o, kWave);
+        if (lane + o < kWave && os == seg) {
+            a = min(a, oa);
+            b = min(b, ob);
+        }
+    }
+    const uint32_t ps = (uint32_t)__shfl_up((int)seg, 1, kWave);
+    if ((lane == 0 || ps != seg) && seg != 0xffffffffu) {
+        if (a != none) atomicMin(&ka[seg], a);
+        if (b != none) atomicMin(&kb[seg], b);
+    }
+}
+
+// fold the exact minima into the voxel pair's range (same rules as k_refine);
+// voxel pairs of one object pair are contiguous in the list, so the object
+// accumulators take one atomic per object pair and wave
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_mx_fold(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
@@ -576,14 +602,17 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
     int level, const uint32_t *__restrict__ flags, const unsigned long long *__restrict__ mkey,
     typename Key<R>::type *__restrict__ acc_min, typename Key<R>::type *__restrict__ acc_max,
     unsigned long long *__restrict__ stats) {
+    using K = typename Key<R>::type;
+    const K none = Key<R>::enc((R)__builtin_inff());
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool counted = false;
-    uint64_t pairs = 0;
+    uint32_t seg = 0xffffffffu;
+    K kmin = none, kmax = none;
     if (w < n_vp) {
-        const uint32_t v1 = vp_v1[w], v2 = vp_v2[w], p = vp_pair[w];
+        const uint32_t v1 = vp_v1[w], v2 = vp_v2[w];
+        seg = vp_pair[w];
         const R omin = vp_min[w], omax = vp_max[w];
         if (flags[w] & MX_EMPTY) {
-            atomicMin(&acc_min[p], Key<R>::enc(omin));
+            kmin = Key<R>::enc(omin);
         } else {
             const R d = (R)dunkey(mkey[3 * w]), lo = (R)dunkey(mkey[3 * w + 1]),
                     hi = (R)dunkey(mkey[3 * w + 2]);
@@ -606,18 +635,11 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
             }
             vp_min[w] = nmin;
             vp_max[w] = nmax;
-            atomicMin(&acc_min[p], Key<R>::enc(nmin));
-            atomicMin(&acc_max[p], Key<R>::enc(nmax));
-            counted = true;
-            pairs = (uint64_t)cnt1[5 * (size_t)v1 + slot] * cnt2[5 * (size_t)v2 + slot];
+            kmin = Key<R>::enc(nmin);
+            kmax = Key<R>::enc(nmax);
         }
     }
-    const uint64_t np = wave_sum_u64(pairs);
-    const uint64_t nv = (uint64_t)__popcll(__ballot(counted));
-    if (lane_id() == 0 && nv) {
-        atomicAdd(&stats[0], (unsigned long long)np);
-        atomicAdd(&stats[1], (unsigned long long)nv);
-    }
+    seg_min2<K>(seg, kmin, kmax, none, acc_min, acc_max);
 }
 
 }  // namespace tdp

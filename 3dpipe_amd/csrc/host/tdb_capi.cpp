@@ -410,6 +410,12 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
     for (int k = 0; k < 4; k++) res->stats[k] = st[k];
     res->stats[6] = st[4];
     // per target: reported pairs ordered by (round, list position)
+    const size_t n_rep = (size_t)std::count_if(rnd.begin(), rnd.end(), [](uint8_t r) { return r != 255; });
+    res->target.reserve(n_rep);
+    res->obj2.reserve(n_rep);
+    res->rmin.reserve(n_rep);
+    res->rmax.reserve(n_rep);
+    res->round.reserve(n_rep);
     std::vector<uint32_t> order;
     for (size_t t = 0; t < N; t++) {
         order.clear();

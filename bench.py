@@ -37,7 +37,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(tiles_dir, n_targets_total, n_sub, knn, budget_s=15.0):
+def cpu_baseline(tiles_dir, n_targets_total, n_sub, query, knn, tau, budget_s=15.0):
     """The oracle (single-threaded C restatement, FP64) on the first targets
     of the same workload, grown until it has run for about budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -48,13 +48,15 @@ def cpu_baseline(tiles_dir, n_targets_total, n_sub, knn, budget_s=15.0):
     m = 20
     while True:
         t0 = time.perf_counter()
-        _, _, _, _, st = orc.join(t1, t2, "nn", knn, precision="f64", max_objects1=m)
+        _, _, _, _, st = orc.join(t1, t2, query, k=knn, tau=tau, precision="f64",
+                                  max_objects1=m)
         dt = time.perf_counter() - t0
         if dt >= budget_s / 3 or m >= n_targets_total:
             break
         m = min(n_targets_total, int(m * max(2.0, budget_s / 3 / max(dt, 1e-3))))
     return {"value": st["tri_pairs"] / dt, "unit": "tri-pair dist/s", "cores": 1, "kind": "port",
-            "sample": f"first {m} of {n_targets_total} nuclei vs all vessels, 10-NN, FP64 oracle, "
+            "sample": f"first {m} of {n_targets_total} nuclei vs all vessels, "
+                      f"{'%d-NN' % knn if query == 'nn' else 'within %g' % tau}, FP64 oracle, "
                       f"{dt:.1f} s, {st['tri_pairs']} triangle pairs"}
 
 
@@ -67,6 +69,10 @@ def main():
     ap.add_argument("--nuclei", type=int, default=200, help="nuclei per vessel")
     ap.add_argument("--cube", type=float, default=20000.0)
     ap.add_argument("--knn", type=int, default=10)
+    ap.add_argument("--query", default="nn", choices=["nn", "within"])
+    ap.add_argument("--tau", type=float, default=400.0, help="within distance")
+    ap.add_argument("--rings-min", type=int, default=12, help="vessel rings per segment, low")
+    ap.add_argument("--rings-max", type=int, default=12, help="vessel rings per segment, high")
     ap.add_argument("--precision", default="mixed", choices=["ref32", "fp64", "mixed"])
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time one plain FP64 join and check identical lists")
@@ -85,8 +91,9 @@ def main():
     tdp = importlib.import_module("3dpipe_amd")
 
     t0 = time.perf_counter()
-    nuc, ves = tdp.generate(seed=args.seed, n_vessels=args.vessels,
-                            nuclei_per_vessel=args.nuclei, cube=args.cube)
+    gen_kw = dict(seed=args.seed, n_vessels=args.vessels, nuclei_per_vessel=args.nuclei,
+                  cube=args.cube, rings_min=args.rings_min, rings_max=args.rings_max)
+    nuc, ves = tdp.generate(**gen_kw)
     log(f"generated {nuc.n_obj} nuclei ({nuc.n_tri} triangle records) and {ves.n_obj} vessels "
         f"({ves.n_tri}) in {time.perf_counter() - t0:.1f} s")
     eng = tdp.Engine(0)
@@ -94,7 +101,8 @@ def main():
     log(f"device {name}, {cus} CUs")
 
     def run(precision):
-        return eng.join(nuc, ves, query="nn", knn=args.knn, precision=precision)
+        return eng.join(nuc, ves, query=args.query, knn=args.knn, within_dist=args.tau,
+                        precision=precision)
 
     for w in range(args.warmup):
         r = run(args.precision)
@@ -137,8 +145,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == "mixed" else ("f32" if args.precision == "ref32" else "f64"),
         "data": "synthetic (seeded icosphere nuclei around tube vessels)",
-        "config": {"workload": f"{args.knn}-NN join nuclei->vessels, {nuc.n_obj} nuclei x "
-                               f"{ves.n_obj} vessels, {args.cube:g}^3 cube, LODs 20-100",
+        "config": {"workload": (f"{args.knn}-NN join" if args.query == "nn" else
+                                f"within-{args.tau:g} join") +
+                               f" nuclei->vessels, {nuc.n_obj} nuclei x {ves.n_obj} vessels "
+                               f"(rings/segment {args.rings_min}-{args.rings_max}), "
+                               f"{args.cube:g}^3 cube, LODs 20-100",
                    "precision": args.precision, "seed": args.seed,
                    "triangle_pairs": tri_pairs, "voxel_pairs": r.stats["voxel_pairs"],
                    "fp32_culled_tri_pairs": r.stats["fp32_culled"],
@@ -167,13 +178,12 @@ def main():
             # the first nuclei of the same workload (identical objects) and
             # all vessels, written for the oracle
             n_sub = min(nuc.n_obj, 2000)
-            sub, _ = tdp.generate(seed=args.seed, n_vessels=args.vessels,
-                                  nuclei_per_vessel=args.nuclei, cube=args.cube,
-                                  max_nuclei=n_sub)
+            sub, _ = tdp.generate(max_nuclei=n_sub, **gen_kw)
             sub.save(os.path.join(d, "nuclei.dt"))
             ves.save(os.path.join(d, "vessels.dt"))
             del sub
-            out["cpu_baseline"] = cpu_baseline(d, nuc.n_obj, n_sub, args.knn)
+            out["cpu_baseline"] = cpu_baseline(d, nuc.n_obj, n_sub, args.query, args.knn,
+                                               args.tau)
             out["cpu_baseline"]["vs_gpu"] = out["value"] / out["cpu_baseline"]["value"]
     eng.close()
     print(json.dumps(out), flush=True)

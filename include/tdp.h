@@ -152,7 +152,7 @@ int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
 /* Device time per stage of the refinement rounds so far (ms, from HIP events
  * recorded on the join's stream between the stages) and launches per stage.
  * Stages: */
-#define TDP_STAGE_RESET 0   /* per-round accumulator reset */
+#define TDP_STAGE_RESET 0   /* per-round bookkeeping: accumulator reset, pair counts */
 #define TDP_STAGE_BOUNDS 1  /* MIXED: FP32 upper bounds and probe choice */
 #define TDP_STAGE_SCAN 2    /* MIXED: prefix sums, including the host read-back */
 #define TDP_STAGE_PROBE 3   /* MIXED: FP64 TriDist of the probes */
