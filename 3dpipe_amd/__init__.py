@@ -48,6 +48,7 @@ class _GenParams(_c.Structure):
         ("noise", _c.c_double), ("subdiv", _c.c_int), ("place_min", _c.c_double),
         ("place_max", _c.c_double), ("prototypes", _c.c_int), ("voxel_tris", _c.c_int),
         ("threads", _c.c_int), ("max_nuclei", _c.c_int64),
+        ("nuclei_seed", _c.c_uint64), ("emit_vessels", _c.c_int),
     ]
 
 
@@ -100,6 +101,16 @@ def _load():
     tdb.tdb_result_stats.argtypes = [_P, _P]
     tdb.tdb_result_free.argtypes = [_P]
     tdb.tdb_result_stages.argtypes = [_P, _P, _P, _c.c_int]
+    tdb.tdb_tile_clone.argtypes = [_P, _c.POINTER(_P)]
+    tdb.tdb_server_defaults.argtypes = [_P]
+    tdb.tdb_server_open.argtypes = [_P, _P, _c.POINTER(_P)]
+    tdb.tdb_server_submit.argtypes = [_P, _P, _P]
+    tdb.tdb_server_submit.restype = _c.c_int64
+    tdb.tdb_server_wait.argtypes = [_P, _c.c_int64, _c.POINTER(_P), _c.POINTER(_c.c_double)]
+    tdb.tdb_server_error.argtypes = [_P]
+    tdb.tdb_server_error.restype = _c.c_char_p
+    tdb.tdb_server_stats.argtypes = [_P, _P]
+    tdb.tdb_server_close.argtypes = [_P]
     # libtdp (kernel-level parity ops)
     tdp.tdp_open.argtypes = [_c.c_int, _c.POINTER(_P)]
     tdp.tdp_close.argtypes = [_P]
@@ -136,6 +147,13 @@ class Tile:
         if tdb.tdb_tile_load(os.fsencode(path), _c.byref(h), err, 512) != 0:
             raise TdpError(err.value.decode())
         return cls(h.value)
+
+    def clone(self) -> "Tile":
+        tdb, _ = _load()
+        h = _P()
+        if tdb.tdb_tile_clone(self._h, _c.byref(h)) != 0:
+            raise TdpError("tile clone failed")
+        return Tile(h.value)
 
     def save(self, path: str) -> None:
         tdb, _ = _load()
@@ -218,6 +236,33 @@ class JoinResult:
         return out
 
 
+def _read_result(r) -> "JoinResult":
+    """Copy a tdb_result into a JoinResult and free it."""
+    tdb, _ = _load()
+    try:
+        n = int(tdb.tdb_result_count(r))
+        t = np.zeros(n, np.int32); b = np.zeros(n, np.int32)
+        mn = np.zeros(n, np.float64); mx = np.zeros(n, np.float64)
+        rd = np.zeros(n, np.uint8)
+        tdb.tdb_result_get(r, _ptr(t), _ptr(b), _ptr(mn), _ptr(mx), _ptr(rd))
+        tm = np.zeros(7, np.float64)
+        st = np.zeros(7, np.uint64)
+        tdb.tdb_result_timing(r, _ptr(tm))
+        tdb.tdb_result_stats(r, _ptr(st))
+        sm = np.zeros(len(STAGES), np.float64)
+        sn = np.zeros(len(STAGES), np.uint64)
+        tdb.tdb_result_stages(r, _ptr(sm), _ptr(sn), len(STAGES))
+        nt = int(tdb.tdb_result_targets(r))
+    finally:
+        tdb.tdb_result_free(r)
+    timing = dict(zip(["total", "candidates", "filter", "refine", "filter_kernels",
+                       "refine_kernels", "upload"], tm.tolist()))
+    stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
+                      "object_pairs", "rounds", "fp32_culled"], (int(x) for x in st)))
+    return JoinResult(t, b, mn, mx, rd, nt, timing, stats, dict(zip(STAGES, sm.tolist())),
+                      dict(zip(STAGES, (int(x) for x in sn))))
+
+
 def _opts(query, knn, within_dist, lods, hausdorff_level, max_objects1, precision, threads):
     tdb, _ = _load()
     o = _JoinOpts()
@@ -277,32 +322,77 @@ class Engine:
                           _c.byref(o), _c.byref(r))
         if rc != 0:
             raise TdpError(tdb.tdb_engine_error(self._h).decode())
-        try:
-            n = int(tdb.tdb_result_count(r))
-            t = np.zeros(n, np.int32); b = np.zeros(n, np.int32)
-            mn = np.zeros(n, np.float64); mx = np.zeros(n, np.float64)
-            rd = np.zeros(n, np.uint8)
-            tdb.tdb_result_get(r, _ptr(t), _ptr(b), _ptr(mn), _ptr(mx), _ptr(rd))
-            tm = np.zeros(7, np.float64)
-            st = np.zeros(7, np.uint64)
-            tdb.tdb_result_timing(r, _ptr(tm))
-            tdb.tdb_result_stats(r, _ptr(st))
-            sm = np.zeros(len(STAGES), np.float64)
-            sn = np.zeros(len(STAGES), np.uint64)
-            tdb.tdb_result_stages(r, _ptr(sm), _ptr(sn), len(STAGES))
-            nt = int(tdb.tdb_result_targets(r))
-        finally:
-            tdb.tdb_result_free(r)
-        timing = dict(zip(["total", "candidates", "filter", "refine", "filter_kernels",
-                           "refine_kernels", "upload"], tm.tolist()))
-        stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
-                          "object_pairs", "rounds", "fp32_culled"], (int(x) for x in st)))
-        return JoinResult(t, b, mn, mx, rd, nt, timing, stats, dict(zip(STAGES, sm.tolist())),
-                          dict(zip(STAGES, (int(x) for x in sn))))
+        return _read_result(r)
 
     def close(self):
         if self._h:
             _load()[0].tdb_engine_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ServerOpts(_c.Structure):
+    _fields_ = [("device", _c.c_int), ("hbm_budget", _c.c_uint64), ("max_batch", _c.c_int),
+                ("overlap", _c.c_int), ("precision", _c.c_int), ("threads", _c.c_int)]
+
+
+class Server:
+    """Queries joined against one device-resident tile with continuous
+    batching (tdb_server_* of include/tdbase.h). submit() takes ownership of
+    the query tile."""
+
+    def __init__(self, resident: Tile, device: int = 0, hbm_budget: int = 0,
+                 max_batch: int = 64, overlap: bool = True, precision: str = "mixed",
+                 threads: int = 0):
+        tdb, _ = _load()
+        o = _ServerOpts()
+        tdb.tdb_server_defaults(_c.byref(o))
+        o.device, o.hbm_budget, o.max_batch = int(device), int(hbm_budget), int(max_batch)
+        o.overlap, o.precision, o.threads = int(bool(overlap)), PREC[precision], int(threads)
+        self._resident = resident  # must outlive the server
+        h = _P()
+        rc = tdb.tdb_server_open(_c.byref(o), resident._h, _c.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = tdb.tdb_server_error(h).decode() if h.value else "tdb_server_open failed"
+            if h.value:
+                tdb.tdb_server_close(h)
+            self._h = None
+            raise TdpError(f"no usable GPU: {msg}")
+
+    def submit(self, tile: Tile, query: str = "nn", knn: int = 1, within_dist: float = 1000.0,
+               lods: Optional[Sequence[int]] = None, hausdorff_level: int = 2) -> int:
+        tdb, _ = _load()
+        o, keep = _opts(query, knn, within_dist, lods, hausdorff_level, None, "mixed", 0)
+        t = tdb.tdb_server_submit(self._h, tile._h, _c.byref(o))
+        if t < 0:
+            raise TdpError(tdb.tdb_server_error(self._h).decode())
+        tile._h = None  # owned by the server now
+        return int(t)
+
+    def wait(self, ticket: int) -> tuple:
+        """(JoinResult, latency ms)"""
+        tdb, _ = _load()
+        r, lat = _P(), _c.c_double(0)
+        rc = tdb.tdb_server_wait(self._h, int(ticket), _c.byref(r), _c.byref(lat))
+        if rc != 0:
+            raise TdpError(tdb.tdb_server_error(self._h).decode())
+        return _read_result(r), lat.value
+
+    def stats(self) -> dict:
+        st = np.zeros(4, np.uint64)
+        _load()[0].tdb_server_stats(self._h, _ptr(st))
+        return dict(zip(["batches", "queries", "max_batch", "peak_device_bytes"],
+                        (int(x) for x in st)))
+
+    def close(self):
+        if self._h:
+            _load()[0].tdb_server_close(self._h)
             self._h = None
 
     def __del__(self):

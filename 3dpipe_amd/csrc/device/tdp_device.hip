@@ -51,6 +51,8 @@ struct tdp_join {
     size_t N = 0, P = 0;
     size_t rsz = 4;  // sizeof range value (4 REF32, 8 otherwise)
     uint32_t *prefix = nullptr, *pair_obj2 = nullptr, *pair_target = nullptr;
+    int32_t *k_t = nullptr;  // per target: k (kNN)
+    float *tau_t = nullptr;  // per target: within distance
     void *rmin = nullptr, *rmax = nullptr;
     uint8_t *state = nullptr, *round = nullptr, *scratch = nullptr;
     int32_t *confirmed = nullptr;
@@ -303,6 +305,16 @@ extern "C" const char *tdp_last_error(const tdp_ctx *ctx) {
     return ctx ? ctx->err.c_str() : "null context";
 }
 
+extern "C" int tdp_device_memory(tdp_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!ctx) return TDP_ERR_ARG;
+    TDP_CHECK(ctx, hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    TDP_CHECK(ctx, hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return TDP_OK;
+}
+
 extern "C" int tdp_device_info(tdp_ctx *ctx, char *name, size_t name_len, int *cu_count) {
     if (!ctx) return TDP_ERR_ARG;
     hipDeviceProp_t prop;
@@ -316,11 +328,28 @@ extern "C" int tdp_device_info(tdp_ctx *ctx, char *name, size_t name_len, int *c
 
 // ------------------------------------------------------------ tiles
 
-extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **out) {
+// tri_parts: the records in n_parts host buffers (concatenated in order), or
+// nullptr for d->tri
+static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const *tri_parts,
+                       const size_t *part_n_tri, size_t n_parts, tdp_tile **out) {
     if (!ctx || !d || !out) return TDP_ERR_ARG;
     *out = nullptr;
+    if (tri_parts) {
+        size_t sum = 0;
+        for (size_t k = 0; k < n_parts; k++) {
+            if (part_n_tri[k] && !tri_parts[k]) {
+                ctx->err = "tdp_tile_upload_parts: null part";
+                return TDP_ERR_ARG;
+            }
+            sum += part_n_tri[k];
+        }
+        if (sum != d->n_tri) {
+            ctx->err = "tdp_tile_upload_parts: parts do not add up to n_tri";
+            return TDP_ERR_ARG;
+        }
+    }
     if (!d->obj_vox_off || !d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt ||
-        (d->n_tri && !d->tri) || !d->obj_hd) {
+        (d->n_tri && !d->tri && !tri_parts) || !d->obj_hd) {
         ctx->err = "tdp_tile_upload: null array";
         return TDP_ERR_ARG;
     }
@@ -362,10 +391,27 @@ extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **
     up(&t->vox_obj, vobj.data(), d->n_vox);
     up(&t->vox_tri_off, d->vox_tri_off, 5 * d->n_vox);
     up(&t->vox_tri_cnt, d->vox_tri_cnt, 5 * d->n_vox);
-    up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
+    if (!tri_parts) {
+        up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
+    } else if (rc == TDP_OK) {
+        rc = dalloc(ctx, &t->tri, 3 * d->n_tri);
+        if (rc == TDP_OK) t->bytes += 48 * d->n_tri;
+        size_t at = 0;
+        for (size_t k = 0; rc == TDP_OK && k < n_parts; k++) {
+            if (part_n_tri[k]) {
+                hipError_t e = hipMemcpy(t->tri + 3 * at, tri_parts[k], 48 * part_n_tri[k],
+                                         hipMemcpyHostToDevice);
+                if (e != hipSuccess) {
+                    ctx->err = std::string("tile upload: ") + hipGetErrorString(e);
+                    rc = TDP_ERR_HIP;
+                }
+            }
+            at += part_n_tri[k];
+        }
+    }
     if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * d->n_tri);
     if (rc == TDP_OK && d->n_tri) {
-        t->bytes += 32 * d->n_tri;
+        t->bytes += sizeof(float4) * kBnd * d->n_tri;
         k_tri_bounds<<<blocks_for(d->n_tri, kBlock), kBlock, 0, ctx->stream>>>(d->n_tri, t->tri, t->bnd);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -380,6 +426,17 @@ extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **
     }
     *out = t;
     return TDP_OK;
+}
+
+extern "C" int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, tdp_tile **out) {
+    return tile_upload(ctx, d, nullptr, nullptr, 0, out);
+}
+
+extern "C" int tdp_tile_upload_parts(tdp_ctx *ctx, const tdp_tile_desc *d,
+                                     const float *const *tri_parts, const size_t *part_n_tri,
+                                     size_t n_parts, tdp_tile **out) {
+    if (!tri_parts || !part_n_tri) return TDP_ERR_ARG;
+    return tile_upload(ctx, d, tri_parts, part_n_tri, n_parts, out);
 }
 
 extern "C" int tdp_tile_free(tdp_tile *t) {
@@ -540,7 +597,7 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
     tdp_ctx *ctx = j->ctx;
-    void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->rmin, j->rmax, j->state,
+    void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q};
@@ -556,18 +613,23 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     return TDP_OK;
 }
 
-extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
-                               const tdp_join_params *prm, size_t N, const uint32_t *prefix,
-                               const uint32_t *cand, tdp_join **out) {
+extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                                     const tdp_join_params *prm, size_t N, const uint32_t *prefix,
+                                     const uint32_t *cand, const int32_t *k_target,
+                                     const float *tau_target, tdp_join **out) {
     if (!ctx || !t1 || !t2 || !prm || !out || (N && !prefix)) return TDP_ERR_ARG;
     *out = nullptr;
     if (prm->query != TDP_QUERY_KNN && prm->query != TDP_QUERY_WITHIN) {
         ctx->err = "tdp_join_create: bad query";
         return TDP_ERR_ARG;
     }
-    if (prm->query == TDP_QUERY_KNN && prm->k < 1) {
-        ctx->err = "tdp_join_create: k must be >= 1";
-        return TDP_ERR_ARG;
+    if (prm->query == TDP_QUERY_KNN) {
+        bool ok = k_target ? true : prm->k >= 1;
+        for (size_t t = 0; k_target && t < N; t++) ok = ok && k_target[t] >= 1;
+        if (!ok) {
+            ctx->err = "tdp_join_create: k must be >= 1";
+            return TDP_ERR_ARG;
+        }
     }
     if (prm->precision < 0 || prm->precision > 2 || prm->hausdorff_level < 0 ||
         prm->hausdorff_level > 2) {
@@ -608,6 +670,8 @@ extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile 
     if (rc == TDP_OK) rc = palloc(ctx, &j->prefix, N + 1);
     if (rc == TDP_OK) rc = palloc(ctx, &j->pair_obj2, P);
     if (rc == TDP_OK) rc = palloc(ctx, &j->pair_target, P);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->k_t, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->tau_t, N);
     if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->rmin, P * j->rsz);
     if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->rmax, P * j->rsz);
     if (rc == TDP_OK) rc = palloc(ctx, &j->state, P);
@@ -626,6 +690,16 @@ extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile 
     hipError_t e = hipMemcpy(j->prefix, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess && P) e = hipMemcpy(j->pair_obj2, cand, 4 * P, hipMemcpyHostToDevice);
     if (e == hipSuccess && P) e = hipMemcpy(j->pair_target, tgt.data(), 4 * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess && N) {
+        std::vector<int32_t> kv(N);
+        std::vector<float> tv(N);
+        for (size_t t = 0; t < N; t++) {
+            kv[t] = k_target ? k_target[t] : prm->k;
+            tv[t] = tau_target ? tau_target[t] : prm->within_dist;
+        }
+        e = hipMemcpy(j->k_t, kv.data(), 4 * N, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(j->tau_t, tv.data(), 4 * N, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMemset(j->stats, 0, 32);
     if (e == hipSuccess) e = hipMemset(j->confirmed, 0, 4 * (N ? N : 1));
     if (e == hipSuccess) e = hipMemset(j->round, 0xff, P ? P : 1);
@@ -641,6 +715,12 @@ extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile 
     }
     *out = j;
     return TDP_OK;
+}
+
+extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                               const tdp_join_params *prm, size_t N, const uint32_t *prefix,
+                               const uint32_t *cand, tdp_join **out) {
+    return tdp_join_create_multi(ctx, t1, t2, prm, N, prefix, cand, nullptr, nullptr, out);
 }
 
 static int ensure_vp(tdp_join *j, size_t n) {
@@ -721,7 +801,6 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     tdp_ctx *ctx = j->ctx;
     const size_t P = j->P, N = j->N;
     const bool within = j->prm.query == TDP_QUERY_WITHIN;
-    const R tau = (R)j->prm.within_dist;
     R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
     hipStream_t s = ctx->stream;
     TDP_CHECK(ctx, hipEventRecord(j->ev[0], s));
@@ -733,18 +812,18 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
         if (!within) {
             if (N) {
                 k_filter_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
-                    N, j->prefix, rmin, rmax, j->prm.k, j->state, j->round, j->confirmed,
+                    N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed,
                     j->scratch);
                 TDP_LAUNCHED(ctx);
             }
         } else {
-            k_filter_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, rmin, rmax, tau,
-                                                                          j->state, j->round);
+            k_filter_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
+                P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round);
             TDP_LAUNCHED(ctx);
         }
         k_survivor_count<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
-            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, tau, j->state, j->round, j->count);
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, j->tau_t, j->state, j->round, j->count);
         TDP_LAUNCHED(ctx);
     }
     uint64_t total = 0;
@@ -756,7 +835,7 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     if (total) {
         k_survivor_write<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
-            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, tau, j->state, j->offs, j->v1[0],
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, j->tau_t, j->state, j->offs, j->v1[0],
             j->v2[0], j->vpp[0], (R *)j->vmin[0], (R *)j->vmax[0]);
         TDP_LAUNCHED(ctx);
     }
@@ -787,7 +866,6 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     hipStream_t s = ctx->stream;
     const size_t P = j->P, N = j->N;
     const bool within = j->prm.query == TDP_QUERY_WITHIN;
-    const R tau = (R)j->prm.within_dist;
     R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
     using KT = typename Key<R>::type;
     int slot = -1;
@@ -885,10 +963,10 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     TDP_LAUNCHED(ctx);
     if (!within) {
         k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
-            N, j->prefix, rmin, rmax, j->prm.k, j->state, j->round, j->confirmed, j->scratch, rnd);
+            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd);
     } else {
-        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, rmin, rmax, tau, j->state,
-                                                                     j->round, rnd);
+        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
+            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd);
     }
     TDP_LAUNCHED(ctx);
     k_vp_keep<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->vpp[c], (const R *)j->vmin[c],

@@ -138,13 +138,15 @@ __device__ __forceinline__ int knn_status_one(const R *rmin, const R *rmax, cons
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_filter_knn(size_t N, const uint32_t *__restrict__ prefix,
                                                        const R *__restrict__ rmin,
-                                                       const R *__restrict__ rmax, int K,
+                                                       const R *__restrict__ rmax,
+                                                       const int32_t *__restrict__ k_t,
                                                        uint8_t *__restrict__ state,
                                                        uint8_t *__restrict__ round,
                                                        int32_t *__restrict__ confirmed,
                                                        uint8_t *__restrict__ scratch) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
+    const int K = k_t[t];
     const uint32_t b = prefix[t], e = prefix[t + 1];
     const int m = (int)(e - b);
     int conf = 0;
@@ -177,11 +179,14 @@ __global__ void __launch_bounds__(kBlock) k_filter_knn(size_t N, const uint32_t 
 // min > tau dropped, max <= tau reported, else refined
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_filter_within(size_t P, const R *__restrict__ rmin,
-                                                          const R *__restrict__ rmax, R tau,
+                                                          const R *__restrict__ rmax,
+                                                          const uint32_t *__restrict__ pair_target,
+                                                          const float *__restrict__ tau_t,
                                                           uint8_t *__restrict__ state,
                                                           uint8_t *__restrict__ round) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
+    const R tau = (R)tau_t[pair_target[p]];
     uint8_t st = ST_DEAD, rd = 255;
     if (rmin[p] > tau) st = ST_DEAD;
     else if (rmax[p] <= tau) { st = ST_REPORTED; rd = 0; }
@@ -204,8 +209,8 @@ __global__ void __launch_bounds__(kBlock) k_survivor_count(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, const R *__restrict__ rmax,
-    bool within, R tau, uint8_t *__restrict__ state, uint8_t *__restrict__ round,
-    uint32_t *__restrict__ count) {
+    bool within, const float *__restrict__ tau_t, uint8_t *__restrict__ state,
+    uint8_t *__restrict__ round, uint32_t *__restrict__ count) {
     const size_t p = wave_global();
     if (p >= P) return;
     const int lane = lane_id();
@@ -217,6 +222,7 @@ __global__ void __launch_bounds__(kBlock) k_survivor_count(
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
     const uint32_t total = n1 * n2;
     const R objmax = rmax[p];
+    const R tau = within ? (R)tau_t[pair_target[p]] : (R)0;
     uint32_t cnt = 0;
     bool settled = false;
     for (uint32_t t = lane; t < total; t += kWave) {
@@ -247,7 +253,8 @@ __global__ void __launch_bounds__(kBlock) k_survivor_write(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, const R *__restrict__ rmax,
-    bool within, R tau, const uint8_t *__restrict__ state, const uint64_t *__restrict__ offs,
+    bool within, const float *__restrict__ tau_t, const uint8_t *__restrict__ state,
+    const uint64_t *__restrict__ offs,
     uint32_t *__restrict__ vp_v1, uint32_t *__restrict__ vp_v2, uint32_t *__restrict__ vp_pair,
     R *__restrict__ vp_min, R *__restrict__ vp_max) {
     const size_t p = wave_global();
@@ -258,6 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_survivor_write(
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
     const uint32_t total = n1 * n2;
     const R objmax = rmax[p];
+    const R tau = within ? (R)tau_t[pair_target[p]] : (R)0;
     uint64_t pos = offs[p];
     for (uint32_t base = 0; base < total; base += kWave) {
         const uint32_t t = base + lane;
@@ -690,13 +698,15 @@ __global__ void __launch_bounds__(kBlock) k_pair_finish(
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_decide_knn(size_t N, const uint32_t *__restrict__ prefix,
                                                        const R *__restrict__ rmin,
-                                                       const R *__restrict__ rmax, int K,
+                                                       const R *__restrict__ rmax,
+                                                       const int32_t *__restrict__ k_t,
                                                        uint8_t *__restrict__ state,
                                                        uint8_t *__restrict__ round,
                                                        int32_t *__restrict__ confirmed,
                                                        uint8_t *__restrict__ scratch, uint8_t rnd) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
+    const int K = k_t[t];
     const uint32_t b = prefix[t], e = prefix[t + 1];
     bool any = false;
     for (uint32_t i = b; i < e; i++) any |= state[i] == ST_ALIVE;
@@ -726,12 +736,15 @@ __global__ void __launch_bounds__(kBlock) k_decide_knn(size_t N, const uint32_t 
 // per-LOD within decision (WithinJoin.cpp:769-792)
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_decide_within(size_t P, const R *__restrict__ rmin,
-                                                          const R *__restrict__ rmax, R tau,
+                                                          const R *__restrict__ rmax,
+                                                          const uint32_t *__restrict__ pair_target,
+                                                          const float *__restrict__ tau_t,
                                                           uint8_t *__restrict__ state,
                                                           uint8_t *__restrict__ round, uint8_t rnd) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     if (state[p] != ST_ALIVE) return;
+    const R tau = (R)tau_t[pair_target[p]];
     if (rmax[p] <= tau) { state[p] = ST_REPORTED; round[p] = rnd; }
     else if (rmin[p] > tau) state[p] = ST_DEAD;
 }

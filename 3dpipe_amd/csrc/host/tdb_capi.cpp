@@ -15,24 +15,7 @@
 #include <omp.h>
 #endif
 
-#include "../../../include/tdbase.h"
-#include "../../../include/tdp.h"
-#include "tdb_octree.hpp"
-#include "tdb_tile.hpp"
-
-struct tdb_tile {
-    tdb::Tile t;
-    mutable std::once_flag tree_once;
-    mutable std::unique_ptr<tdb::Octree> tree;
-    // engines holding a device copy of this tile (guarded by g_registry)
-    mutable std::vector<tdb_engine *> holders;
-    const tdb::Octree &octree() const {
-        std::call_once(tree_once, [this] {
-            tree.reset(new tdb::Octree(t.obj_box.data(), t.obj_transform.data(), t.n_obj, 10));
-        });
-        return *tree;
-    }
-};
+#include "tdb_internal.hpp"
 
 struct tdb_engine {
     tdp_ctx *ctx = nullptr;
@@ -45,24 +28,95 @@ struct tdb_engine {
 // new tile allocated at a recycled address never meets a stale device copy.
 static std::mutex g_registry;
 
-struct tdb_result {
-    uint64_t n_targets = 0;
-    std::vector<int32_t> target, obj2;
-    std::vector<double> rmin, rmax;
-    std::vector<uint8_t> round;
-    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
-    double stage_ms[TDP_STAGE_COUNT] = {};
-    uint64_t stage_n[TDP_STAGE_COUNT] = {};
-    uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
-};
-
 static void set_err(char *err, size_t len, const std::string &msg) {
     if (err && len) std::snprintf(err, len, "%s", msg.c_str());
 }
 
-using Clock = std::chrono::steady_clock;
-static double ms_since(Clock::time_point a) {
-    return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+
+int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *o,
+                 const std::vector<uint32_t> &prefix, const std::vector<uint32_t> &ids,
+                 const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err) {
+    const size_t N = prefix.size() - 1, P = ids.size();
+    res->n_targets = N;
+    res->stats[4] = P;
+
+    tdp_join_params prm;
+    prm.query = o->query == TDB_QUERY_NN ? TDP_QUERY_KNN : TDP_QUERY_WITHIN;
+    prm.k = o->k;
+    prm.within_dist = o->within_dist;
+    prm.precision = o->precision;
+    prm.hausdorff_level = o->hausdorff_level;
+    tdp_join *j = nullptr;
+    int rc = tdp_join_create_multi(ctx, d1, d2, &prm, N, prefix.data(), ids.data(), k_t, tau_t, &j);
+    if (rc != TDP_OK) {
+        err = tdp_last_error(ctx);
+        return rc;
+    }
+    auto t0 = Clock::now();
+    uint64_t n_vp = 0;
+    rc = tdp_join_filter(j, &n_vp);
+    res->timing[2] = ms_since(t0);
+    t0 = Clock::now();
+    const std::vector<int> lods = tdb_lod_list(o);
+    const int highest = lods.empty() ? 0 : lods.back();
+    for (size_t li = 0; rc == TDP_OK && li < lods.size(); li++) {
+        if (n_vp == 0) break;
+        rc = tdp_join_refine(j, lods[li], lods[li] == highest, &n_vp);
+        res->stats[5]++;
+    }
+    res->timing[3] = ms_since(t0);
+    std::vector<uint8_t> rnd(P);
+    std::vector<double> mn(P), mx(P);
+    if (rc == TDP_OK) rc = tdp_join_results(j, rnd.data(), mn.data(), mx.data());
+    uint64_t st[5] = {0, 0, 0, 0, 0};
+    if (rc == TDP_OK) rc = tdp_join_stats(j, st);
+    if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
+    if (rc == TDP_OK) tdp_join_stage_ms(j, res->stage_ms, res->stage_n, TDP_STAGE_COUNT);
+    if (rc != TDP_OK) {
+        err = tdp_last_error(ctx);
+        tdp_join_destroy(j);
+        return rc;
+    }
+    tdp_join_destroy(j);
+    for (int k = 0; k < 4; k++) res->stats[k] = st[k];
+    res->stats[6] = st[4];
+    // per target: reported pairs ordered by (round, list position)
+    const size_t n_rep = (size_t)std::count_if(rnd.begin(), rnd.end(), [](uint8_t r) { return r != 255; });
+    res->target.reserve(n_rep);
+    res->obj2.reserve(n_rep);
+    res->rmin.reserve(n_rep);
+    res->rmax.reserve(n_rep);
+    res->round.reserve(n_rep);
+    std::vector<uint32_t> order;
+    for (size_t t = 0; t < N; t++) {
+        order.clear();
+        for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++)
+            if (rnd[p] != 255) order.push_back(p);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return rnd[a] < rnd[b]; });
+        for (uint32_t p : order) {
+            res->target.push_back((int32_t)t);
+            res->obj2.push_back((int32_t)ids[p]);
+            res->rmin.push_back(mn[p]);
+            res->rmax.push_back(mx[p]);
+            res->round.push_back(rnd[p]);
+        }
+    }
+    return TDP_OK;
+}
+
+tdp_tile_desc tdb_desc_of(const tdb::Tile &t) {
+    tdp_tile_desc d;
+    d.n_obj = t.n_obj;
+    d.n_vox = t.n_vox();
+    d.n_tri = t.n_tri();
+    d.obj_vox_off = t.obj_vox_off.data();
+    d.obj_hd = t.obj_hd.data();
+    d.vox_box = t.vox_box.data();
+    d.vox_tri_off = t.vox_tri_off.data();
+    d.vox_tri_cnt = t.vox_tri_cnt.data();
+    d.tri = t.tri.data();
+    return d;
 }
 
 extern "C" {
@@ -105,6 +159,14 @@ void tdb_tile_free(tdb_tile *t) {
         }
     }
     delete t;
+}
+
+int tdb_tile_clone(const tdb_tile *t, tdb_tile **out) {
+    if (!t || !out) return -1;
+    tdb_tile *c = new tdb_tile();
+    c->t = t->t;
+    *out = c;
+    return 0;
 }
 
 void tdb_tile_size(const tdb_tile *t, uint64_t *out3) {
@@ -151,6 +213,8 @@ void tdb_gen_defaults(tdb_gen_params *p) {
     p->voxel_tris = g.voxel_tris;
     p->threads = g.threads;
     p->max_nuclei = 0;
+    p->nuclei_seed = 0;
+    p->emit_vessels = 1;
 }
 
 int tdb_generate(const tdb_gen_params *p, tdb_tile **nuclei, tdb_tile **vessels) {
@@ -182,6 +246,8 @@ int tdb_generate(const tdb_gen_params *p, tdb_tile **nuclei, tdb_tile **vessels)
     g.voxel_tris = p->voxel_tris;
     g.threads = p->threads;
     g.max_nuclei = p->max_nuclei;
+    g.nuclei_seed = p->nuclei_seed;
+    g.emit_vessels = p->emit_vessels;
     std::unique_ptr<tdb_tile> n(new tdb_tile()), v(new tdb_tile());
     tdb::generate(g, n->t, v->t);
     *nuclei = n.release();
@@ -246,16 +312,7 @@ static int resident(tdb_engine *e, const tdb_tile *t, tdp_tile **out) {
         *out = it->second;
         return 0;
     }
-    tdp_tile_desc d;
-    d.n_obj = t->t.n_obj;
-    d.n_vox = t->t.n_vox();
-    d.n_tri = t->t.n_tri();
-    d.obj_vox_off = t->t.obj_vox_off.data();
-    d.obj_hd = t->t.obj_hd.data();
-    d.vox_box = t->t.vox_box.data();
-    d.vox_tri_off = t->t.vox_tri_off.data();
-    d.vox_tri_cnt = t->t.vox_tri_cnt.data();
-    d.tri = t->t.tri.data();
+    const tdp_tile_desc d = tdb_desc_of(t->t);
     int rc = tdp_tile_upload(e->ctx, &d, out);
     if (rc != TDP_OK) {
         e->err = tdp_last_error(e->ctx);
@@ -278,9 +335,11 @@ void tdb_join_defaults(tdb_join_opts *o) {
     o->threads = 0;
 }
 
+}  // extern "C"
+
 // LOD list post-processing of parse_args (query_context.h:183-188): sort,
 // then std::unique without erase, which keeps the old tail.
-static std::vector<int> lod_list(const tdb_join_opts *o) {
+std::vector<int> tdb_lod_list(const tdb_join_opts *o) {
     std::vector<int> l;
     if (o->lods && o->n_lods > 0) l.assign(o->lods, o->lods + o->n_lods);
     else
@@ -297,10 +356,9 @@ static size_t n_targets(const tdb_tile *t1, const tdb_join_opts *o) {
 }
 
 // candidate lists of every target (KNNJoin.cpp:552-573, WithinJoin.cpp:79-97)
-static void gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self,
-                           const tdb_join_opts *o, std::vector<uint32_t> &prefix,
-                           std::vector<uint32_t> &ids) {
-    const size_t N = n_targets(t1, o);
+void tdb_gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self, size_t N,
+                        const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
+                        std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids) {
     const tdb::Octree &tree = t2->octree();
     std::vector<std::vector<uint32_t>> per(N);
 #ifdef _OPENMP
@@ -311,8 +369,10 @@ static void gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self,
         std::vector<std::pair<int, tdb::Range>> c;
         const float *box = &t1->t.obj_box[6 * i];
         const float *tr = &t1->t.obj_transform[3 * i];
-        if (o->query == TDB_QUERY_NN) tree.knn(box, tr, self ? (int)i : -1, o->k, c);
-        else tree.within(box, tr, self ? (int)i : -1, o->within_dist, c);
+        if (o->query == TDB_QUERY_NN)
+            tree.knn(box, tr, self ? (int)i : -1, k_t ? k_t[i] : o->k, c);
+        else
+            tree.within(box, tr, self ? (int)i : -1, tau_t ? tau_t[i] : o->within_dist, c);
         per[i].reserve(c.size());
         for (auto &x : c) per[i].push_back((uint32_t)x.first);
     }
@@ -322,12 +382,14 @@ static void gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self,
     for (size_t i = 0; i < N; i++) std::copy(per[i].begin(), per[i].end(), ids.begin() + prefix[i]);
 }
 
+extern "C" {
+
 int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
                        uint32_t *prefix, uint32_t *ids, size_t cap) {
     if (!t1 || !o) return -1;
     const bool self = (t2 == nullptr || t2 == t1);
     std::vector<uint32_t> pf, id;
-    gen_candidates(t1, self ? t1 : t2, self, o, pf, id);
+    tdb_gen_candidates(t1, self ? t1 : t2, self, n_targets(t1, o), o, nullptr, nullptr, pf, id);
     if (prefix) std::copy(pf.begin(), pf.end(), prefix);
     if (ids) std::copy(id.begin(), id.begin() + std::min(cap, id.size()), ids);
     return (int64_t)id.size();
@@ -363,74 +425,10 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
 
     t0 = Clock::now();
     std::vector<uint32_t> prefix, ids;
-    gen_candidates(t1, T2, self, o, prefix, ids);
+    tdb_gen_candidates(t1, T2, self, n_targets(t1, o), o, nullptr, nullptr, prefix, ids);
     res->timing[1] = ms_since(t0);
-    const size_t N = prefix.size() - 1, P = ids.size();
-    res->n_targets = N;
-    res->stats[4] = P;
-
-    tdp_join_params prm;
-    prm.query = o->query == TDB_QUERY_NN ? TDP_QUERY_KNN : TDP_QUERY_WITHIN;
-    prm.k = o->k;
-    prm.within_dist = o->within_dist;
-    prm.precision = o->precision;
-    prm.hausdorff_level = o->hausdorff_level;
-    tdp_join *j = nullptr;
-    rc = tdp_join_create(e->ctx, d1, d2, &prm, N, prefix.data(), ids.data(), &j);
-    if (rc != TDP_OK) {
-        e->err = tdp_last_error(e->ctx);
-        return rc;
-    }
-    t0 = Clock::now();
-    uint64_t n_vp = 0;
-    rc = tdp_join_filter(j, &n_vp);
-    res->timing[2] = ms_since(t0);
-    t0 = Clock::now();
-    const std::vector<int> lods = lod_list(o);
-    const int highest = lods.empty() ? 0 : lods.back();
-    for (size_t li = 0; rc == TDP_OK && li < lods.size(); li++) {
-        if (n_vp == 0) break;
-        rc = tdp_join_refine(j, lods[li], lods[li] == highest, &n_vp);
-        res->stats[5]++;
-    }
-    res->timing[3] = ms_since(t0);
-    std::vector<uint8_t> rnd(P);
-    std::vector<double> mn(P), mx(P);
-    if (rc == TDP_OK) rc = tdp_join_results(j, rnd.data(), mn.data(), mx.data());
-    uint64_t st[5] = {0, 0, 0, 0, 0};
-    if (rc == TDP_OK) rc = tdp_join_stats(j, st);
-    if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
-    if (rc == TDP_OK) tdp_join_stage_ms(j, res->stage_ms, res->stage_n, TDP_STAGE_COUNT);
-    if (rc != TDP_OK) {
-        e->err = tdp_last_error(e->ctx);
-        tdp_join_destroy(j);
-        return rc;
-    }
-    tdp_join_destroy(j);
-    for (int k = 0; k < 4; k++) res->stats[k] = st[k];
-    res->stats[6] = st[4];
-    // per target: reported pairs ordered by (round, list position)
-    const size_t n_rep = (size_t)std::count_if(rnd.begin(), rnd.end(), [](uint8_t r) { return r != 255; });
-    res->target.reserve(n_rep);
-    res->obj2.reserve(n_rep);
-    res->rmin.reserve(n_rep);
-    res->rmax.reserve(n_rep);
-    res->round.reserve(n_rep);
-    std::vector<uint32_t> order;
-    for (size_t t = 0; t < N; t++) {
-        order.clear();
-        for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++)
-            if (rnd[p] != 255) order.push_back(p);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](uint32_t a, uint32_t b) { return rnd[a] < rnd[b]; });
-        for (uint32_t p : order) {
-            res->target.push_back((int32_t)t);
-            res->obj2.push_back((int32_t)ids[p]);
-            res->rmin.push_back(mn[p]);
-            res->rmax.push_back(mx[p]);
-            res->round.push_back(rnd[p]);
-        }
-    }
+    rc = tdb_run_join(e->ctx, d1, d2, o, prefix, ids, nullptr, nullptr, res.get(), e->err);
+    if (rc != TDP_OK) return rc;
     res->timing[0] = ms_since(t_start);
     *out = res.release();
     return 0;

@@ -1,0 +1,65 @@
+// Internals shared by the host join engine (tdb_capi.cpp) and the serving
+// loop (tdb_serve.cpp): the opaque types of include/tdbase.h and the join
+// driver both run.
+#pragma once
+
+#include <chrono>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/tdbase.h"
+#include "../../../include/tdp.h"
+#include "tdb_octree.hpp"
+#include "tdb_tile.hpp"
+
+struct tdb_tile {
+    tdb::Tile t;
+    mutable std::once_flag tree_once;
+    mutable std::unique_ptr<tdb::Octree> tree;
+    // engines holding a device copy of this tile (guarded by g_registry)
+    mutable std::vector<tdb_engine *> holders;
+    const tdb::Octree &octree() const {
+        std::call_once(tree_once, [this] {
+            tree.reset(new tdb::Octree(t.obj_box.data(), t.obj_transform.data(), t.n_obj, 10));
+        });
+        return *tree;
+    }
+};
+
+struct tdb_result {
+    uint64_t n_targets = 0;
+    std::vector<int32_t> target, obj2;
+    std::vector<double> rmin, rmax;
+    std::vector<uint8_t> round;
+    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
+    double stage_ms[TDP_STAGE_COUNT] = {};
+    uint64_t stage_n[TDP_STAGE_COUNT] = {};
+    uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+using Clock = std::chrono::steady_clock;
+static double ms_since(Clock::time_point a) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+}
+
+
+// the device description of a host tile (arrays stay owned by the tile)
+tdp_tile_desc tdb_desc_of(const tdb::Tile &t);
+
+// the --lods list as the reference's parser leaves it (sorted, unique without erase)
+std::vector<int> tdb_lod_list(const tdb_join_opts *o);
+
+// candidate lists of targets [0, n_targets) of t1 against t2 (octree); k_t and
+// tau_t, when given, hold each target's k / within distance
+void tdb_gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self, size_t n_targets,
+                        const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
+                        std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids);
+
+// Filter + LOD rounds on the device for the given candidates, results into
+// res (targets numbered 0..N-1, reported pairs ordered by (round, list
+// position)); timing [2..5] and stats filled. Returns a TDP_* status.
+int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *o,
+                 const std::vector<uint32_t> &prefix, const std::vector<uint32_t> &ids,
+                 const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err);

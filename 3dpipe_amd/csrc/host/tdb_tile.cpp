@@ -678,7 +678,8 @@ struct Vessel {
     LodMesh mesh;
 };
 
-Vessel make_vessel(const GenParams &p, Rng &r) {
+// with_mesh = false: the centreline only (the mesh takes no random draws)
+Vessel make_vessel(const GenParams &p, Rng &r, bool with_mesh = true) {
     Vessel V;
     const int segs = p.vessel_segments, S = p.vessel_sides;
     const int rps = p.rings_min + (p.rings_max > p.rings_min ? r.below(p.rings_max - p.rings_min + 1) : 0);
@@ -695,6 +696,7 @@ Vessel make_vessel(const GenParams &p, Rng &r) {
     }
     V.pts.push_back(start);
     for (int s = 0; s < segs; s++) V.pts.push_back(V.pts.back() + dirs[s] * p.segment_length);
+    if (!with_mesh) return V;
     const int R = segs * rps + 1;
     std::vector<V3> ctr(R), tan(R);
     for (int j = 0; j < R; j++) {
@@ -765,15 +767,16 @@ void generate(const GenParams &p, Tile &nuclei, Tile &vessels) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (long i = 0; i < (long)nvs; i++) {
         Rng r(p.seed, 1000000000ULL + (uint64_t)i);
-        vs[i] = make_vessel(p, r);
-        vvz[i] = voxelize(vs[i].mesh, p.voxel_tris);
+        vs[i] = make_vessel(p, r, p.emit_vessels != 0);
+        if (p.emit_vessels) vvz[i] = voxelize(vs[i].mesh, p.voxel_tris);
     }
-    std::vector<ObjShape> vshape(nvs);
-    for (size_t i = 0; i < nvs; i++) vshape[i] = {(uint32_t)vvz[i].nvox, total_faces(vs[i].mesh)};
+    const size_t nvs_out = p.emit_vessels ? nvs : 0;
+    std::vector<ObjShape> vshape(nvs_out);
+    for (size_t i = 0; i < nvs_out; i++) vshape[i] = {(uint32_t)vvz[i].nvox, total_faces(vs[i].mesh)};
     std::vector<uint64_t> vtri;
     layout(vessels, vshape, vtri);
 #pragma omp parallel for schedule(dynamic, 1)
-    for (long i = 0; i < (long)nvs; i++) {
+    for (long i = 0; i < (long)nvs_out; i++) {
         const LodMesh &m = vs[i].mesh;
         std::vector<float> vf(3 * m.v.size());
         for (size_t k = 0; k < m.v.size(); k++) {
@@ -809,7 +812,7 @@ void generate(const GenParams &p, Tile &nuclei, Tile &vessels) {
     std::vector<Inst> inst(nn);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)nn; i++) {
-        Rng r(p.seed, 3000000000ULL + (uint64_t)i);
+        Rng r(p.nuclei_seed ? p.nuclei_seed : p.seed, 3000000000ULL + (uint64_t)i);
         Inst &in = inst[i];
         if (p.n_vessels > 0) {
             const Vessel &V = vs[i / p.nuclei_per_vessel];

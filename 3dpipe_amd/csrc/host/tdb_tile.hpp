@@ -64,6 +64,9 @@ struct GenParams {
     int voxel_tris = 50;             // triangles per voxel at LOD 100
     int threads = 0;                 // 0 = all
     int64_t max_nuclei = 0;          // > 0: only the first max_nuclei nuclei
+    uint64_t nuclei_seed = 0;        // != 0: nucleus placements drawn from this seed (the
+                                     // vessels and the shape bank still follow `seed`)
+    int emit_vessels = 1;            // 0: leave the vessel tile empty (placements only)
 };
 
 // Nuclei around vessels (n_vessels > 0) or nuclei uniform in the cube.

@@ -60,6 +60,85 @@ def cpu_baseline(tiles_dir, n_targets_total, n_sub, query, knn, tau, budget_s=15
                       f"{dt:.1f} s, {st['tri_pairs']} triangle pairs"}
 
 
+def serve_bench(tdp, vessels, gen_kw, n_queries, rate, seed, log_fn):
+    """Config-5 style serving: `vessels` resident; n_queries fresh nucleus
+    tiles of log-uniform size 100..50000 (seeded), each kNN with k in
+    {1, 5, 10, 20} or within with tau U(50, 500), arriving as a Poisson
+    process of `rate` queries/s. Run twice on the same schedule: the batched
+    mixed server and one-query-at-a-time plain FP64. Returns latency
+    percentiles, throughput and whether every query's results agree."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(100), np.log(50000), n_queries)).astype(int)
+    kinds = []
+    for q in range(n_queries):
+        if rng.random() < 0.5:
+            kinds.append(("nn", int(rng.choice([1, 5, 10, 20])), 0.0))
+        else:
+            kinds.append(("within", 1, float(rng.uniform(50, 500))))
+    gaps = rng.exponential(1.0 / rate, n_queries)
+    t0 = time.perf_counter()
+    nv = gen_kw["n_vessels"]
+    tiles = []
+    for q in range(n_queries):
+        kw = dict(gen_kw)
+        kw.update(nuclei_per_vessel=max(1, -(-int(sizes[q]) // nv)), max_nuclei=int(sizes[q]),
+                  nuclei_seed=10_000 + q, emit_vessels=0)
+        tiles.append(tdp.generate(**kw)[0])
+    n_nuclei = int(sum(t.n_obj for t in tiles))
+    log_fn(f"serving: {n_queries} query tiles ({n_nuclei} nuclei) generated in "
+           f"{time.perf_counter() - t0:.1f} s")
+
+    def run(precision, max_batch, overlap):
+        srv = tdp.Server(vessels, max_batch=max_batch, overlap=overlap, precision=precision)
+        copies = [t.clone() for t in tiles]  # each server owns what it is given
+        tickets = [None] * n_queries
+        start = time.perf_counter()
+
+        def feed():
+            t = start
+            for q in range(n_queries):
+                t += gaps[q]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[q] = srv.submit(copies[q], query=kinds[q][0], knn=kinds[q][1],
+                                        within_dist=kinds[q][2])
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        res, lat = [], []
+        for q in range(n_queries):
+            r, l = srv.wait(tickets[q])
+            res.append(r)
+            lat.append(l)
+        makespan = time.perf_counter() - start
+        st = srv.stats()
+        srv.close()
+        lat = np.array(lat)
+        return res, {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "mean_ms": float(lat.mean()), "queries_per_s": n_queries / makespan,
+                     "nuclei_per_s": n_nuclei / makespan, "batches": st["batches"],
+                     "max_batch": st["max_batch"], "peak_device_gb": st["peak_device_bytes"] / 1e9}
+
+    res_b, batched = run("mixed", 64, True)
+    log_fn(f"serving batched mixed: p50 {batched['p50_ms']:.1f} ms, p99 {batched['p99_ms']:.1f} ms, "
+           f"{batched['queries_per_s']:.1f} q/s, {batched['batches']} batches")
+    res_s, single = run("fp64", 1, False)
+    log_fn(f"serving one-at-a-time FP64: p50 {single['p50_ms']:.1f} ms, p99 {single['p99_ms']:.1f} ms, "
+           f"{single['queries_per_s']:.1f} q/s")
+    same = all(np.array_equal(a.target, b.target) and np.array_equal(a.obj2, b.obj2)
+               and np.array_equal(a.rmin, b.rmin) and np.array_equal(a.rmax, b.rmax)
+               for a, b in zip(res_b, res_s))
+    return {"queries": n_queries, "nuclei": n_nuclei, "arrival_rate_per_s": rate,
+            "sizes": "log-uniform 100..50000 nuclei", "mix": "kNN k in {1,5,10,20} / within tau U(50,500)",
+            "batched_mixed": batched, "one_at_a_time_fp64": single,
+            "p99_speedup": single["p99_ms"] / batched["p99_ms"],
+            "throughput_speedup": batched["queries_per_s"] / single["queries_per_s"],
+            "identical_results": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +157,9 @@ def main():
                     help="also time one plain FP64 join and check identical lists")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--serve-queries", type=int, default=24,
+                    help="serving run (resident vessels, Poisson arrivals) after the join bench; 0: off")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="query arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the join shards by "
@@ -186,6 +268,10 @@ def main():
                                                args.tau)
             out["cpu_baseline"]["vs_gpu"] = out["value"] / out["cpu_baseline"]["value"]
     eng.close()
+    if args.serve_queries > 0:
+        del nuc, results, r
+        out["serving"] = serve_bench(tdp, ves, gen_kw, args.serve_queries, args.serve_rate,
+                                     args.seed + 100, log)
     print(json.dumps(out), flush=True)
 
 

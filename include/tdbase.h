@@ -33,6 +33,8 @@ int tdb_tile_load(const char *path, tdb_tile **out, char *err, size_t err_len);
  * (src/storage/tile_IO.cpp:79-172). */
 int tdb_tile_save(const tdb_tile *t, const char *path, char *err, size_t err_len);
 void tdb_tile_free(tdb_tile *t);
+/* an independent copy of a tile */
+int tdb_tile_clone(const tdb_tile *t, tdb_tile **out);
 /* n_obj, n_vox, n_tri */
 void tdb_tile_size(const tdb_tile *t, uint64_t *out3);
 /* read-only views of the tile arrays (valid while the tile lives):
@@ -67,6 +69,9 @@ typedef struct {
     int voxel_tris;
     int threads;
     int64_t max_nuclei;       /* > 0: only the first max_nuclei nuclei (same objects) */
+    uint64_t nuclei_seed;     /* != 0: nucleus placements from this seed (vessels and the
+                                 nucleus shape bank still follow seed) */
+    int emit_vessels;         /* 0: the vessel tile is left empty */
 } tdb_gen_params;
 
 void tdb_gen_defaults(tdb_gen_params *p);
@@ -133,6 +138,44 @@ void tdb_result_stats(const tdb_result *r, uint64_t *out7);
  * tdp.h (n up to TDP_STAGE_COUNT) */
 void tdb_result_stages(const tdb_result *r, double *ms, uint64_t *launches, int n);
 void tdb_result_free(tdb_result *r);
+
+/* ---- serving: queries against one resident tile, continuously batched ----
+ *
+ * No counterpart in the reference (which runs one join per process). A
+ * server keeps `resident` (tile 2 of every query) on the device and joins
+ * each submitted query tile (tile 1) against it. Queries waiting when the
+ * device frees up are merged into one join -- those sharing the query type,
+ * LOD list and Hausdorff level, as many as fit the device-memory budget; k and
+ * the within distance may differ per query -- and the next batch is prepared
+ * (tile merge, octree candidates, upload) on a second stream while one runs.
+ * A query's result is the same alone and in any batch. */
+typedef struct tdb_server tdb_server;
+typedef struct {
+    int device;
+    uint64_t hbm_budget;  /* device bytes the server may hold, resident tile included
+                             (0 = the resident tile + 90% of the free memory) */
+    int max_batch;        /* most queries in one join (1 = one at a time) */
+    int overlap;          /* 1: prepare the next batch while one runs */
+    int precision;        /* TDP_PREC_* of tdp.h */
+    int threads;          /* host threads for candidates (0 = all) */
+} tdb_server_opts;
+
+void tdb_server_defaults(tdb_server_opts *o);
+/* resident must outlive the server */
+int tdb_server_open(const tdb_server_opts *o, const tdb_tile *resident, tdb_server **out);
+/* Queue a query: the server takes ownership of `query` (it frees it). Uses
+ * o->query, k, within_dist, lods, hausdorff_level. Returns a ticket >= 0, or
+ * a negative status. */
+int64_t tdb_server_submit(tdb_server *s, tdb_tile *query, const tdb_join_opts *o);
+/* Block until the ticket's query is done; *out as tdb_join's (free with
+ * tdb_result_free), *latency_ms = submit to completion. */
+int tdb_server_wait(tdb_server *s, int64_t ticket, tdb_result **out, double *latency_ms);
+const char *tdb_server_error(const tdb_server *s);
+/* [0] batches run, [1] queries done, [2] most queries in a batch,
+ * [3] peak device bytes held (resident tile included) */
+void tdb_server_stats(tdb_server *s, uint64_t *out4);
+/* finishes the queued queries, then stops */
+void tdb_server_close(tdb_server *s);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,8 @@ int tdp_close(tdp_ctx *ctx);
 const char *tdp_last_error(const tdp_ctx *ctx);
 /* device name and compute-unit count (for reports) */
 int tdp_device_info(tdp_ctx *ctx, char *name, size_t name_len, int *cu_count);
+/* free and total device memory in bytes (for memory budgets) */
+int tdp_device_memory(tdp_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Device-resident tile. Triangles are 48-byte records
  * {v0.xyz, proxy_hausdorff, v1.xyz, hausdorff, v2.xyz, 0}; voxel boxes are
@@ -71,6 +73,12 @@ typedef struct {
 } tdp_tile_desc;
 
 int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *desc, tdp_tile **out);
+/* The same with the triangle records in n_parts host buffers, concatenated
+ * in order (desc->tri is ignored): tiles assembled from several sources, such
+ * as a batch of query tiles, go up without a host-side copy. No counterpart
+ * in the reference. */
+int tdp_tile_upload_parts(tdp_ctx *ctx, const tdp_tile_desc *desc, const float *const *tri_parts,
+                          const size_t *part_n_tri, size_t n_parts, tdp_tile **out);
 int tdp_tile_free(tdp_tile *tile);
 size_t tdp_tile_bytes(const tdp_tile *tile);
 
@@ -119,6 +127,17 @@ typedef struct {
 int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
                     const tdp_join_params *params, size_t n_targets, const uint32_t *cand_prefix,
                     const uint32_t *cand_obj, tdp_join **out);
+
+/* The same with a k (kNN) and a within distance per target (host arrays of
+ * n_targets; NULL = params->k / params->within_dist for every target), so
+ * that queries with different parameters share one join: each target's
+ * lists and ranges are what a join of that target alone would give. No
+ * counterpart in the reference, where a join has one k / tau (its callers
+ * are serving loops that batch concurrent queries). */
+int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                          const tdp_join_params *params, size_t n_targets,
+                          const uint32_t *cand_prefix, const uint32_t *cand_obj,
+                          const int32_t *k_target, const float *tau_target, tdp_join **out);
 
 /* Filter stage on the device: object-level voxel bounds, the filter-time kNN
  * status / within decisions and compaction of surviving voxel pairs.

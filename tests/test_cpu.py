@@ -248,6 +248,22 @@ def test_device_calls_fail_loudly_without_gpu(tdp):
         tdp.Engine(0)
     with pytest.raises(tdp.TdpError):
         tdp.Device(0)
+    _, v = tdp.generate(seed=0, n_vessels=2, nuclei_per_vessel=1)
+    with pytest.raises(tdp.TdpError):
+        tdp.Server(v)
+
+
+def test_generator_nuclei_seed_keeps_vessels(tdp):
+    """queries for a resident vessel tile: other nucleus placements, the same
+    vessels; emit_vessels=0 leaves the vessel tile empty"""
+    n0, v0 = tdp.generate(seed=5, n_vessels=3, nuclei_per_vessel=4)
+    n1, v1 = tdp.generate(seed=5, n_vessels=3, nuclei_per_vessel=4, nuclei_seed=77)
+    _, v2 = tdp.generate(seed=5, n_vessels=3, nuclei_per_vessel=4, nuclei_seed=77,
+                         emit_vessels=0)
+    a0, a1 = v0.arrays(), v1.arrays()
+    assert all(np.array_equal(a0[k], a1[k]) for k in a0)
+    assert not np.array_equal(n0.arrays()["obj_box"], n1.arrays()["obj_box"])
+    assert v2.n_obj == 0 and n1.n_obj == 12
 
 
 # ------------------------------------------------------- RAW format, generator
