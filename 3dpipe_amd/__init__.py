@@ -102,6 +102,9 @@ def _load():
     tdb.tdb_result_free.argtypes = [_P]
     tdb.tdb_result_stages.argtypes = [_P, _P, _P, _c.c_int]
     tdb.tdb_tile_clone.argtypes = [_P, _c.POINTER(_P)]
+    tdb.tdb_engine_candidates.argtypes = [_P, _P, _P, _P, _P, _P, _c.c_size_t,
+                                          _c.POINTER(_c.c_uint64)]
+    tdb.tdb_engine_candidates.restype = _c.c_int64
     tdb.tdb_server_defaults.argtypes = [_P]
     tdb.tdb_server_open.argtypes = [_P, _P, _c.POINTER(_P)]
     tdb.tdb_server_submit.argtypes = [_P, _P, _P]
@@ -309,6 +312,27 @@ class Engine:
 
     def evict(self, tile: Tile) -> None:
         _load()[0].tdb_engine_evict(self._h, tile._h)
+
+    def candidates(self, tile1: Tile, tile2: Optional[Tile] = None, query: str = "nn",
+                   knn: int = 1, within_dist: float = 1000.0,
+                   max_objects1: Optional[int] = None):
+        """Candidate lists as the join builds them (on the device for kNN
+        against at most 1024 objects): (prefix, ids, targets left to the
+        host octree)."""
+        tdb, _ = _load()
+        o, _keep = _opts(query, knn, within_dist, None, 2, max_objects1, "mixed", 0)
+        n = min(tile1.n_obj, max_objects1) if max_objects1 else tile1.n_obj
+        prefix = np.zeros(n + 1, np.uint32)
+        fb = _c.c_uint64(0)
+        t2 = tile2._h if tile2 is not None else None
+        total = tdb.tdb_engine_candidates(self._h, tile1._h, t2, _c.byref(o), _ptr(prefix), None,
+                                          0, _c.byref(fb))
+        if total < 0:
+            raise TdpError(tdb.tdb_engine_error(self._h).decode())
+        ids = np.zeros(max(int(total), 1), np.uint32)
+        tdb.tdb_engine_candidates(self._h, tile1._h, t2, _c.byref(o), _ptr(prefix), _ptr(ids),
+                                  int(total), _c.byref(fb))
+        return prefix, ids[: int(total)], int(fb.value)
 
     def join(self, tile1: Tile, tile2: Optional[Tile] = None, query: str = "nn", knn: int = 1,
              within_dist: float = 1000.0, lods: Optional[Sequence[int]] = None,

@@ -12,6 +12,7 @@
 #include "../../../include/tdp.h"
 #include "tdp_kernels.hpp"
 #include "tdp_mixed.hpp"
+#include "tdp_cand.hpp"
 
 using namespace tdp;
 
@@ -41,6 +42,7 @@ struct tdp_tile {
     uint32_t *vox_tri_cnt = nullptr;
     float4 *tri = nullptr;
     float4 *bnd = nullptr;  // kBnd per triangle: sphere, plane, hausdorff pair
+    float *obj_box = nullptr, *obj_tr = nullptr;  // for candidate queries (optional)
     size_t bytes = 0;
 };
 
@@ -391,6 +393,10 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
     up(&t->vox_obj, vobj.data(), d->n_vox);
     up(&t->vox_tri_off, d->vox_tri_off, 5 * d->n_vox);
     up(&t->vox_tri_cnt, d->vox_tri_cnt, 5 * d->n_vox);
+    if (d->obj_box && d->obj_transform) {
+        up(&t->obj_box, d->obj_box, 6 * d->n_obj);
+        up(&t->obj_tr, d->obj_transform, 3 * d->n_obj);
+    }
     if (!tri_parts) {
         up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
     } else if (rc == TDP_OK) {
@@ -449,6 +455,8 @@ extern "C" int tdp_tile_free(tdp_tile *t) {
     dfree(t->vox_tri_cnt);
     dfree(t->tri);
     dfree(t->bnd);
+    dfree(t->obj_box);
+    dfree(t->obj_tr);
     delete t;
     return TDP_OK;
 }
@@ -593,6 +601,81 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
 }
 
 // ------------------------------------------------------------ join
+
+extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
+                                  int self, const int32_t *k_target, uint32_t *prefix,
+                                  uint32_t *ids, size_t ids_cap, uint64_t *n_ids,
+                                  uint8_t *fallback) {
+    if (!ctx || !t1 || !t2 || !n_ids || (N && (!k_target || !prefix || !fallback)))
+        return TDP_ERR_ARG;
+    *n_ids = 0;
+    if (!t1->obj_box || !t2->obj_box || !t1->obj_tr || !t2->obj_tr) {
+        ctx->err = "tdp_knn_candidates: tiles uploaded without object boxes";
+        return TDP_ERR_ARG;
+    }
+    if (t2->n_obj > (size_t)kCandMaxObj || N > t1->n_obj) {
+        ctx->err = "tdp_knn_candidates: tile 2 has more than 1024 objects";
+        return TDP_ERR_ARG;
+    }
+    prefix[0] = 0;
+    if (N == 0) return TDP_OK;
+    const uint32_t cap = kCandMaxCap;
+    int32_t *dk = nullptr;
+    uint32_t *dc = nullptr, *dl = nullptr, *dids = nullptr;
+    uint64_t *doff = nullptr;
+    uint8_t *df = nullptr;
+    int rc = palloc(ctx, &dk, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &dc, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &dl, N * (size_t)cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &df, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &doff, N);
+    hipError_t e = hipSuccess;
+    uint64_t total = 0;
+    if (rc == TDP_OK) {
+        e = hipMemcpyAsync(dk, k_target, 4 * N, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) {
+            k_knn_candidates<<<wave_grid(N), kBlock, 0, ctx->stream>>>(
+                N, t1->obj_box, t1->obj_tr, (uint32_t)t2->n_obj, t2->obj_box, t2->obj_tr, self, dk,
+                cap, dc, dl, df);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_knn_candidates: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    }
+    if (rc == TDP_OK) rc = scan_exclusive<uint32_t>(ctx, dc, N, doff, &total);
+    *n_ids = total;
+    if (rc == TDP_OK && total > ids_cap) {
+        ctx->err = "tdp_knn_candidates: ids array too small";
+        rc = TDP_ERR_ARG;
+    }
+    if (rc == TDP_OK) rc = palloc(ctx, &dids, total);
+    if (rc == TDP_OK) {
+        k_cand_compact<<<wave_grid(N), kBlock, 0, ctx->stream>>>(N, cap, dc, doff, dl, dids);
+        e = hipGetLastError();
+        if (e == hipSuccess && total)
+            e = hipMemcpyAsync(ids, dids, 4 * total, hipMemcpyDeviceToHost, ctx->stream);
+        std::vector<uint32_t> cnt(N);
+        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), dc, 4 * N, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(fallback, df, N, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_knn_candidates: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        } else {
+            for (size_t t = 0; t < N; t++) prefix[t + 1] = prefix[t] + cnt[t];
+        }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    pfree(ctx, dk);
+    pfree(ctx, dc);
+    pfree(ctx, dl);
+    pfree(ctx, df);
+    pfree(ctx, doff);
+    pfree(ctx, dids);
+    return rc;
+}
 
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;

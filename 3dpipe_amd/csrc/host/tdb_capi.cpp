@@ -116,7 +116,78 @@ tdp_tile_desc tdb_desc_of(const tdb::Tile &t) {
     d.vox_tri_off = t.vox_tri_off.data();
     d.vox_tri_cnt = t.vox_tri_cnt.data();
     d.tri = t.tri.data();
+    d.obj_box = t.obj_box.data();
+    d.obj_transform = t.obj_transform.data();
     return d;
+}
+
+// Candidates with the device where it applies (kNN, tile 2 of at most 1024
+// objects, both tiles resident): the device lists, and the octree for the
+// targets the device leaves to it. Elsewhere the octree for all targets.
+int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
+                           const tdb_tile *t1, const tdb_tile *t2, bool self, size_t N,
+                           const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
+                           std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids,
+                           uint64_t *n_fallback, std::string &err) {
+    if (n_fallback) *n_fallback = 0;
+    if (!ctx || !d1 || !d2 || o->query != TDB_QUERY_NN || t2->t.n_obj > 1024 || N == 0) {
+        tdb_gen_candidates(t1, t2, self, N, o, k_t, tau_t, prefix, ids);
+        return TDP_OK;
+    }
+    std::vector<int32_t> kv(N);
+    for (size_t t = 0; t < N; t++) kv[t] = k_t ? k_t[t] : o->k;
+    std::vector<uint32_t> dpre(N + 1);
+    std::vector<uint8_t> fb(N);
+    // room for about 16 per target; asked again with the exact count when short
+    size_t room = 16 * N + 1024;
+    std::unique_ptr<uint32_t[]> dids(new uint32_t[room]);
+    uint64_t n_ids = 0;
+    int rc = tdp_knn_candidates(ctx, d1, N, d2, self ? 1 : 0, kv.data(), dpre.data(), dids.get(),
+                                room, &n_ids, fb.data());
+    if (rc == TDP_ERR_ARG && n_ids > room) {
+        room = n_ids;
+        dids.reset(new uint32_t[room]);
+        rc = tdp_knn_candidates(ctx, d1, N, d2, self ? 1 : 0, kv.data(), dpre.data(), dids.get(),
+                                room, &n_ids, fb.data());
+    }
+    if (rc != TDP_OK) {
+        err = tdp_last_error(ctx);
+        return rc;
+    }
+    std::vector<long> redo;
+    for (size_t t = 0; t < N; t++)
+        if (fb[t]) redo.push_back((long)t);
+    if (n_fallback) *n_fallback = redo.size();
+    std::vector<std::vector<uint32_t>> host(redo.size());
+    const tdb::Octree &tree = t2->octree();
+#ifdef _OPENMP
+    const int nt = o->threads > 0 ? o->threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+#endif
+    for (long r = 0; r < (long)redo.size(); r++) {
+        const long i = redo[r];
+        std::vector<std::pair<int, tdb::Range>> c;
+        tree.knn(&t1->t.obj_box[6 * i], &t1->t.obj_transform[3 * i], self ? (int)i : -1, kv[i], c);
+        for (auto &x : c) host[r].push_back((uint32_t)x.first);
+    }
+    if (redo.empty()) {
+        prefix.swap(dpre);
+        ids.assign(dids.get(), dids.get() + n_ids);
+        return TDP_OK;
+    }
+    prefix.assign(N + 1, 0);
+    size_t r = 0;
+    for (size_t t = 0; t < N; t++) {
+        const size_t n = fb[t] ? host[r++].size() : dpre[t + 1] - dpre[t];
+        prefix[t + 1] = prefix[t] + (uint32_t)n;
+    }
+    ids.resize(prefix[N]);
+    r = 0;
+    for (size_t t = 0; t < N; t++) {
+        if (fb[t]) std::copy(host[r].begin(), host[r].end(), ids.begin() + prefix[t]), r++;
+        else std::copy(dids.get() + dpre[t], dids.get() + dpre[t + 1], ids.begin() + prefix[t]);
+    }
+    return TDP_OK;
 }
 
 extern "C" {
@@ -395,6 +466,23 @@ int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_op
     return (int64_t)id.size();
 }
 
+int64_t tdb_engine_candidates(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
+                              const tdb_join_opts *o, uint32_t *prefix, uint32_t *ids, size_t cap,
+                              uint64_t *n_fallback) {
+    if (!e || !t1 || !o || !e->ctx) return -1;
+    const bool self = (t2 == nullptr || t2 == t1);
+    const tdb_tile *T2 = self ? t1 : t2;
+    tdp_tile *d1 = nullptr, *d2 = nullptr;
+    if (resident(e, t1, &d1) != 0 || resident(e, T2, &d2) != 0) return -1;
+    std::vector<uint32_t> pf, id;
+    if (tdb_gen_candidates_dev(e->ctx, d1, d2, t1, T2, self, n_targets(t1, o), o, nullptr, nullptr,
+                               pf, id, n_fallback, e->err) != TDP_OK)
+        return -1;
+    if (prefix) std::copy(pf.begin(), pf.end(), prefix);
+    if (ids) std::copy(id.begin(), id.begin() + std::min(cap, id.size()), ids);
+    return (int64_t)id.size();
+}
+
 int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
              tdb_result **out) {
     if (!e || !t1 || !o || !out) return -1;
@@ -425,7 +513,9 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
 
     t0 = Clock::now();
     std::vector<uint32_t> prefix, ids;
-    tdb_gen_candidates(t1, T2, self, n_targets(t1, o), o, nullptr, nullptr, prefix, ids);
+    rc = tdb_gen_candidates_dev(e->ctx, d1, d2, t1, T2, self, n_targets(t1, o), o, nullptr, nullptr,
+                                prefix, ids, nullptr, e->err);
+    if (rc != TDP_OK) return rc;
     res->timing[1] = ms_since(t0);
     rc = tdb_run_join(e->ctx, d1, d2, o, prefix, ids, nullptr, nullptr, res.get(), e->err);
     if (rc != TDP_OK) return rc;

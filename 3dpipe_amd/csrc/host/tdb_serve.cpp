@@ -181,8 +181,6 @@ void tdb_server::preparer() {
         }
         tdb_join_opts o = b->qs[0].o;
         o.threads = opts.threads;
-        tdb_gen_candidates(&b->merged, resident, false, N, &o, b->k_t.data(), b->tau_t.data(),
-                           b->prefix, b->ids);
         tdp_tile_desc d = tdb_desc_of(b->merged.t);
         std::vector<const float *> parts;
         std::vector<size_t> part_n;
@@ -195,6 +193,10 @@ void tdb_server::preparer() {
         if (tdp_tile_upload_parts(prep_ctx, &d, parts.data(), part_n.data(), parts.size(),
                                   &b->dev) != TDP_OK)
             b->err = tdp_last_error(prep_ctx);
+        else if (tdb_gen_candidates_dev(prep_ctx, b->dev, res_dev, &b->merged, resident, false, N, &o,
+                                        b->k_t.data(), b->tau_t.data(), b->prefix, b->ids, nullptr,
+                                        b->err) != TDP_OK && b->err.empty())
+            b->err = "candidate generation failed";
         {
             std::lock_guard<std::mutex> lk(mu);
             ready.push_back(std::move(b));

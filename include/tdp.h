@@ -70,6 +70,8 @@ typedef struct {
     const uint64_t *vox_tri_off; /* 5 per voxel */
     const uint32_t *vox_tri_cnt; /* 5 per voxel */
     const float *tri;            /* 12 per triangle */
+    const float *obj_box;        /* 6 per object {low, high}, or NULL */
+    const float *obj_transform;  /* 3 per object (the max-distance proxy point), or NULL */
 } tdp_tile_desc;
 
 int tdp_tile_upload(tdp_ctx *ctx, const tdp_tile_desc *desc, tdp_tile **out);
@@ -109,6 +111,23 @@ int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri, size_t n_t
 int tdp_knn_status(tdp_ctx *ctx, size_t n_targets, const uint32_t *prefix,
                    const int32_t *confirmed, const double *rmin, const double *rmax, int k,
                    int32_t *status);
+
+/* kNN candidate lists on the device, equal to the octree's query_knn_fast
+ * lists (octree.cpp:215-291 with add_candidate :120-154) including their
+ * order: per target t of t1 (t < n_targets), the objects o of t2 (o != t when
+ * self) with box distance < M or among the k_t smallest transform distances,
+ * M being the k_t-th smallest, ordered by transform distance. Where that
+ * order or the k-th place is decided by equal distances -- which the octree
+ * breaks by its visiting order -- or a list is longer than 256, fallback[t]
+ * = 1 and the target's list is left empty for the host octree. Both tiles
+ * need obj_box and obj_transform; t2 may hold at most 1024 objects
+ * (TDP_ERR_ARG otherwise). Host arrays: k_target[n_targets],
+ * prefix[n_targets + 1], ids[ids_cap], fallback[n_targets]; *n_ids = the
+ * number of ids (if it exceeds ids_cap, nothing is written to ids and the
+ * call returns TDP_ERR_ARG: call again with room for *n_ids). */
+int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t n_targets, const tdp_tile *t2,
+                       int self, const int32_t *k_target, uint32_t *prefix, uint32_t *ids,
+                       size_t ids_cap, uint64_t *n_ids, uint8_t *fallback);
 
 /* ---- device-resident join (filter + LOD refinement) ---- */
 

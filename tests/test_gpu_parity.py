@@ -288,3 +288,40 @@ def test_meshdist_mixed_equals_fp64_real_geometry(dev, tdp, small_tiles, slot):
                 rows.append([offn[a], cntn[a], offv[b], cntv[b]])
     os_ = np.array(rows, np.uint64)
     assert bits_equal(dev.meshdist(rec, os_, "mixed"), dev.meshdist(rec, os_, "fp64"))
+
+
+# ------------------------------------------------------ device candidates
+
+def _dup_first_object(src, dst):
+    """a RAW tile with object 0 appended again (a second object with the same
+    boxes and transform point: equal distances to every target)"""
+    import struct
+    b = open(src, "rb").read()
+    pos = 1
+    (dsz,) = struct.unpack_from("<Q", b, pos)
+    pos += 8 + dsz
+    (nv,) = struct.unpack_from("<Q", b, pos)
+    end = pos + 8 + 40 + nv * (36 + 80) + 12
+    open(dst, "wb").write(b + b[1:end])
+
+
+@pytest.mark.parametrize("k", [1, 3, 10])
+def test_device_knn_candidates_equal_octree(tdp, eng, small_tiles, tmp_path, k):
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    hp, hi = tdp.candidates(n, v, "nn", k)
+    dp, di, fb = eng.candidates(n, v, "nn", k)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+    # self-join (the target itself excluded)
+    hp, hi = tdp.candidates(n, None, "nn", k)
+    dp, di, fb = eng.candidates(n, None, "nn", k)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+    # equal distances: object 0 duplicated -> those targets go to the octree
+    dup = str(tmp_path / "vdup.dt")
+    _dup_first_object(pv, dup)
+    v2 = tdp.Tile.load(dup)
+    assert v2.n_obj == v.n_obj + 1
+    hp, hi = tdp.candidates(n, v2, "nn", k)
+    dp, di, fb = eng.candidates(n, v2, "nn", k)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+    assert fb > 0
