@@ -52,6 +52,11 @@ class _GenParams(_c.Structure):
     ]
 
 
+class _Tuning(_c.Structure):
+    _fields_ = [("small_pairs", _c.c_int), ("max_rows", _c.c_int), ("drain", _c.c_int),
+                ("drain_blocks", _c.c_int), ("eval_blocks", _c.c_int)]
+
+
 class _JoinOpts(_c.Structure):
     _fields_ = [
         ("query", _c.c_int), ("k", _c.c_int), ("within_dist", _c.c_float),
@@ -102,6 +107,9 @@ def _load():
     tdb.tdb_result_free.argtypes = [_P]
     tdb.tdb_result_stages.argtypes = [_P, _P, _P, _c.c_int]
     tdb.tdb_tile_clone.argtypes = [_P, _c.POINTER(_P)]
+    tdb.tdb_engine_get_tuning.argtypes = [_P, _P]
+    tdb.tdb_engine_autotune.argtypes = [_P, _P, _P, _P, _c.c_size_t, _P, _P, _P, _c.c_int]
+    tdb.tdb_engine_set_tuning.argtypes = [_P, _P]
     tdb.tdb_engine_candidates.argtypes = [_P, _P, _P, _P, _P, _P, _c.c_size_t,
                                           _c.POINTER(_c.c_uint64)]
     tdb.tdb_engine_candidates.restype = _c.c_int64
@@ -312,6 +320,44 @@ class Engine:
 
     def evict(self, tile: Tile) -> None:
         _load()[0].tdb_engine_evict(self._h, tile._h)
+
+    def tuning(self) -> dict:
+        """The kernel tuning (tdp_tuning of include/tdp.h)."""
+        tdb, _ = _load()
+        t = _Tuning()
+        if tdb.tdb_engine_get_tuning(self._h, _c.byref(t)) != 0:
+            raise TdpError(tdb.tdb_engine_error(self._h).decode())
+        return {name: getattr(t, name) for name, _ in _Tuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        tdb, _ = _load()
+        t = _Tuning()
+        tdb.tdb_engine_get_tuning(self._h, _c.byref(t))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise TypeError(f"unknown tuning parameter {k}")
+            setattr(t, k, int(v))
+        if tdb.tdb_engine_set_tuning(self._h, _c.byref(t)) != 0:
+            raise TdpError(tdb.tdb_engine_error(self._h).decode())
+
+    def autotune(self, tile1: Tile, tile2: Optional[Tile] = None, query: str = "nn",
+                 knn: int = 1, within_dist: float = 1000.0, sample_targets: int = 0,
+                 precision: str = "mixed") -> dict:
+        """Time a sample of the join under each candidate tuning and keep the
+        fastest (see tdb_engine_autotune). Returns {"best": tuning,
+        "tried": [(tuning, ms), ...]}."""
+        tdb, _ = _load()
+        o, _keep = _opts(query, knn, within_dist, None, 2, None, precision, 0)
+        best = _Tuning()
+        tried = (_Tuning * 32)()
+        ms = np.zeros(32, np.float64)
+        n = tdb.tdb_engine_autotune(self._h, tile1._h, tile2._h if tile2 is not None else None,
+                                    _c.byref(o), int(sample_targets), _c.byref(best), tried,
+                                    _ptr(ms), 32)
+        if n < 0:
+            raise TdpError(tdb.tdb_engine_error(self._h).decode())
+        as_dict = lambda t: {name: getattr(t, name) for name, _ in _Tuning._fields_}
+        return {"best": as_dict(best), "tried": [(as_dict(tried[i]), float(ms[i])) for i in range(n)]}
 
     def candidates(self, tile1: Tile, tile2: Optional[Tile] = None, query: str = "nn",
                    knn: int = 1, within_dist: float = 1000.0,

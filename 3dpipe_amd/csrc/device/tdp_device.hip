@@ -18,7 +18,9 @@ using namespace tdp;
 
 struct tdp_ctx {
     int device = 0;
+    int cu_count = 256;
     hipStream_t stream = nullptr;
+    tdp_tuning tune{(int)tdp::kMxSmallPairs, 64, 0, 0, 4096};
     std::string err;
     // grow-only scratch for scans
     std::vector<void *> scan_buf;
@@ -84,6 +86,8 @@ struct tdp_join {
     bool filtered = false;
     unsigned long long *stats = nullptr;
     uint64_t filter_vp = 0;
+    tdp_tuning tune{};
+    uint32_t *work = nullptr;  // work counter of the persistent stage
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double refine_ms = 0, filter_ms = 0;
     // per-stage timing: events recorded between the stages of a round
@@ -128,9 +132,9 @@ static inline dim3 wave_grid(size_t n) {
 }
 
 // grid for grid-stride kernels
-static inline unsigned stride_grid(size_t n) {
+static inline unsigned stride_grid(size_t n, int cap = 4096) {
     const size_t b = (n + kBlock - 1) / kBlock;
-    return (unsigned)std::min<size_t>(b ? b : 1, (size_t)4096);
+    return (unsigned)std::min<size_t>(b ? b : 1, (size_t)std::max(cap, 1));
 }
 
 template <typename T>
@@ -283,12 +287,32 @@ extern "C" int tdp_open(int device, tdp_ctx **out) {
     }
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
         ctx->err = std::string("hip init: ") + hipGetErrorString(e);
         *out = ctx;
         return TDP_ERR_HIP;
     }
+    ctx->cu_count = prop.multiProcessorCount;
     *out = ctx;
+    return TDP_OK;
+}
+
+extern "C" int tdp_get_tuning(tdp_ctx *ctx, tdp_tuning *out) {
+    if (!ctx || !out) return TDP_ERR_ARG;
+    *out = ctx->tune;
+    return TDP_OK;
+}
+
+extern "C" int tdp_set_tuning(tdp_ctx *ctx, const tdp_tuning *t) {
+    if (!ctx || !t) return TDP_ERR_ARG;
+    if (t->small_pairs < 0 || t->max_rows < 1 || t->max_rows > kWave || t->drain_blocks < 0 ||
+        t->eval_blocks < 1) {
+        ctx->err = "tdp_set_tuning: value out of range";
+        return TDP_ERR_ARG;
+    }
+    ctx->tune = *t;
     return TDP_OK;
 }
 
@@ -680,7 +704,7 @@ extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, co
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
     tdp_ctx *ctx = j->ctx;
-    void *ptrs[] = {j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
+    void *ptrs[] = {j->work, j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q};
@@ -749,6 +773,7 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
     j->N = N;
     j->P = P;
     j->rsz = prm->precision == TDP_PREC_REF32 ? 4 : 8;
+    j->tune = ctx->tune;
     int rc = TDP_OK;
     if (rc == TDP_OK) rc = palloc(ctx, &j->prefix, N + 1);
     if (rc == TDP_OK) rc = palloc(ctx, &j->pair_obj2, P);
@@ -766,6 +791,7 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
     if (rc == TDP_OK) rc = palloc(ctx, &j->count, P);
     if (rc == TDP_OK) rc = palloc(ctx, &j->offs, P);
     if (rc == TDP_OK) rc = palloc(ctx, &j->stats, 4);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->work, 4);
     if (rc != TDP_OK) {
         tdp_join_destroy(j);
         return rc;
@@ -991,8 +1017,9 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         const uint32_t need = is_last || j->prm.hausdorff_level != 2 ? 1u : 6u;
         k_mx_bounds<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
                                           A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                          B->bnd, slot, need, j->mx_U, j->mx_probe, j->mx_flags,
-                                          j->mx_key, j->mx_nw);
+                                          B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
+                                          (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
+                                          j->mx_flags, j->mx_key, j->mx_nw);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_BOUNDS) != TDP_OK) return TDP_ERR_HIP;
         uint64_t words = 0;
@@ -1001,36 +1028,50 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         if (mark(TDP_STAGE_SCAN) != TDP_OK) return TDP_ERR_HIP;
         rc = ensure_mask(j, words);
         if (rc != TDP_OK) return rc;
-        k_mx_eval<<<stride_grid(3 * n), kBlock, 0, s>>>(
+        k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
             3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
             A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
         k_mx_mark<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
                                         A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                        B->bnd, slot, j->mx_U, j->mx_probe, j->mx_flags, j->mx_key,
-                                        j->mx_woff, j->mx_mask, j->keep);
+                                        B->bnd, slot, (uint32_t)j->tune.max_rows, j->mx_U,
+                                        j->mx_probe, j->mx_flags, j->mx_key, j->mx_woff,
+                                        j->mx_mask, j->keep);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
-        uint64_t items = 0;
-        rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &items);
-        if (rc != TDP_OK) return rc;
-        if (mark(TDP_STAGE_SCAN) != TDP_OK) return TDP_ERR_HIP;
-        if (items) {
-            rc = ensure_items(j, items);
-            if (rc != TDP_OK) return rc;
-            k_mx_expand<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
-                                              A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt, slot,
-                                              j->mx_flags, j->mx_woff, j->mx_mask, j->pos,
-                                              j->item_vp, j->item_q);
-            TDP_LAUNCHED(ctx);
-            if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
-            k_mx_eval<<<stride_grid(items), kBlock, 0, s>>>(
-                items, j->item_vp, j->item_q, 0, j->v1[c], j->v2[c], A->vox_tri_off,
-                A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key,
-                j->stats);
+        if (j->tune.drain) {
+            // persistent FP64 stage: no work list, no host read-back
+            const int nb = j->tune.drain_blocks ? j->tune.drain_blocks : 4 * ctx->cu_count;
+            k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
+                                             A->vox_tri_cnt, A->tri, B->vox_tri_off,
+                                             B->vox_tri_cnt, B->tri, slot,
+                                             (uint32_t)j->tune.max_rows, j->mx_flags, j->mx_woff,
+                                             j->mx_mask, j->mx_key, j->stats);
             TDP_LAUNCHED(ctx);
             if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
+        } else {
+            uint64_t items = 0;
+            rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &items);
+            if (rc != TDP_OK) return rc;
+            if (mark(TDP_STAGE_SCAN) != TDP_OK) return TDP_ERR_HIP;
+            if (items) {
+                rc = ensure_items(j, items);
+                if (rc != TDP_OK) return rc;
+                k_mx_expand<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
+                                                  A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
+                                                  slot, (uint32_t)j->tune.max_rows, j->mx_flags,
+                                                  j->mx_woff, j->mx_mask, j->pos, j->item_vp,
+                                                  j->item_q);
+                TDP_LAUNCHED(ctx);
+                if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
+                k_mx_eval<<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(
+                    items, j->item_vp, j->item_q, 0, j->v1[c], j->v2[c], A->vox_tri_off,
+                    A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot,
+                    j->mx_key, j->stats);
+                TDP_LAUNCHED(ctx);
+                if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
+            }
         }
         k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,

@@ -35,7 +35,7 @@ namespace tdp {
 
 enum : uint32_t { MX_EMPTY = 1u, MX_SMALL = 2u };
 constexpr uint32_t kNoProbe = 0xffffffffu;
-// voxel pairs of at most this many triangle pairs go straight to FP64
+// voxel pairs of at most this many triangle pairs go straight to FP64 (default)
 constexpr uint32_t kMxSmallPairs = 8;
 
 struct VpLod {
@@ -177,7 +177,7 @@ struct Tiling {
     bool swap;  // the columns are tile-1 triangles
 };
 
-__device__ __forceinline__ Tiling mx_tiling(const VpLod &g) {
+__device__ __forceinline__ Tiling mx_tiling(const VpLod &g, uint32_t max_rows) {
     Tiling t;
     t.swap = g.n1 > g.n2;
     const uint32_t nl = t.swap ? g.n1 : g.n2, ns = t.swap ? g.n2 : g.n1;
@@ -185,7 +185,7 @@ __device__ __forceinline__ Tiling mx_tiling(const VpLod &g) {
     uint32_t best = 0xffffffffu;
     for (uint32_t m = m0; m < m0 + 3; m++) {
         const uint32_t cw = (nl + m - 1) / m;
-        const uint32_t R = min((uint32_t)kWave / cw, ns);
+        const uint32_t R = min(min((uint32_t)kWave / cw, ns), max(max_rows, 1u));
         const uint32_t nch = (nl + cw - 1) / cw, ngr = (ns + R - 1) / R;
         if (nch * ngr < best) {
             best = nch * ngr;
@@ -289,7 +289,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     const float4 *__restrict__ rec1, const float4 *__restrict__ bnd1,
     const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2,
     const float4 *__restrict__ rec2, const float4 *__restrict__ bnd2, int slot, uint32_t need,
-    float *__restrict__ vpU, uint32_t *__restrict__ probe, uint32_t *__restrict__ flags,
+    uint32_t small_pairs, uint32_t max_rows, float *__restrict__ vpU,
+    uint32_t *__restrict__ probe, uint32_t *__restrict__ flags,
     unsigned long long *__restrict__ mkey, uint32_t *__restrict__ nwords) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
         probe[3 * (size_t)w + lane] = kNoProbe;
         vpU[3 * (size_t)w + lane] = -INFINITY;
     }
-    if (g.n1 == 0 || g.n2 == 0 || g.n1 * g.n2 <= kMxSmallPairs) {
+    if (g.n1 == 0 || g.n2 == 0 || g.n1 * g.n2 <= small_pairs) {
         if (lane == 0) {
             flags[w] = (g.n1 == 0 || g.n2 == 0) ? MX_EMPTY : MX_SMALL;
             nwords[w] = 0;
@@ -308,7 +309,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
         return;
     }
     const uint32_t n2 = g.n2;
-    const Tiling t = mx_tiling(g);
+    const Tiling t = mx_tiling(g, max_rows);
     float up[3] = {INFINITY, INFINITY, INFINITY};
     float best[3] = {INFINITY, INFINITY, INFINITY};
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const float4 *__restrict__ rec1, const float4 *__restrict__ bnd1,
     const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2,
     const float4 *__restrict__ rec2, const float4 *__restrict__ bnd2, int slot,
-    const float *__restrict__ vpU, const uint32_t *__restrict__ probe,
+    uint32_t max_rows, const float *__restrict__ vpU, const uint32_t *__restrict__ probe,
     const uint32_t *__restrict__ flags, const unsigned long long *__restrict__ mkey,
     const uint64_t *__restrict__ woff,
     unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
@@ -484,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     bounds_U(vpU, mkey, w, U);
     const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
                    p2 = probe[3 * (size_t)w + 2];
-    const Tiling t = mx_tiling(g);
+    const Tiling t = mx_tiling(g, max_rows);
     const uint32_t n2 = g.n2, nw = mx_words(t);
     unsigned long long *__restrict__ out = mask + woff[w];
     uint32_t c = 0;
@@ -508,7 +509,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
     const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, int slot,
-    const uint32_t *__restrict__ flags, const uint64_t *__restrict__ woff,
+    uint32_t max_rows, const uint32_t *__restrict__ flags, const uint64_t *__restrict__ woff,
     const unsigned long long *__restrict__ mask, const uint64_t *__restrict__ offs,
     uint32_t *__restrict__ item_vp, uint32_t *__restrict__ item_q) {
     const uint32_t w = wave_item();
@@ -526,7 +527,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
         }
         return;
     }
-    const Tiling t = mx_tiling(g);
+    const Tiling t = mx_tiling(g, max_rows);
     const uint32_t n2 = g.n2, nw = mx_words(t);
     const unsigned long long *__restrict__ in = mask + woff[w];
     // bit b of a word is row b / cw, column b % cw of its tile: a per-wave
@@ -564,6 +565,158 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
         }
         pos += total;
     }
+}
+
+
+// ------------------------------------------------------ persistent drain
+
+// queue of (voxel pair, triangle pair) items per wave
+constexpr uint32_t kDrainQueue = 2 * kWave;
+
+// Persistent FP64 stage: every wave takes every G-th voxel pair (G waves in
+// the grid) until none are left, turns each one's
+// candidate bit masks (or, for SMALL voxel pairs, every pair) into items in a
+// wave-local LDS queue, and evaluates the queue 64 items at a time -- full
+// lanes however few candidates a voxel pair has, and no global work list.
+// Minima fold as in k_mx_eval (the queue keeps voxel-pair order).
+__global__ void __launch_bounds__(kBlock) k_mx_drain(
+    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
+    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
+    const float4 *__restrict__ rec1, const uint64_t *__restrict__ off2,
+    const uint32_t *__restrict__ cnt2, const float4 *__restrict__ rec2, int slot,
+    uint32_t max_rows, const uint32_t *__restrict__ flags, const uint64_t *__restrict__ woff,
+    const unsigned long long *__restrict__ mask, unsigned long long *__restrict__ mkey,
+    unsigned long long *__restrict__ stats) {
+    __shared__ uint32_t qw[kWavesPerBlock][kDrainQueue], qq[kWavesPerBlock][kDrainQueue];
+    const int lane = lane_id();
+    const int wv = threadIdx.x / kWave;
+    uint32_t *mw = qw[wv], *mq = qq[wv];
+    uint32_t qn = 0;      // items in the queue (wave-uniform)
+    uint64_t evals = 0;
+    const unsigned long long inf_key = dkey(INFINITY);
+
+    // append the lanes' items whose `take` is set (at most 64)
+    auto push = [&](bool take, uint32_t w, uint32_t q) {
+        const uint64_t m = __ballot(take);
+        if (take) {
+            const uint32_t at = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            mw[at] = w;
+            mq[at] = q;
+        }
+        qn += (uint32_t)__popcll(m);
+    };
+    // the voxel pair being queued: SMALL pairs go as they are; otherwise its
+    // mask words are taken 64 at a time (one per lane) and their set bits
+    // popped one per lane and round
+    __shared__ uint16_t tabs[kWavesPerBlock][kWave];
+    uint16_t *tab = tabs[wv];
+    uint32_t cur = kNoProbe, n2 = 1, total = 0, k = 0, nw = 0, wb = 0;
+    uint32_t row0 = 0, col0 = 0;   // this lane's word: first row and column of its tile
+    unsigned long long m = 0;      // this lane's word: bits not yet queued
+    bool small = false, done = false;
+    Tiling t{};
+    const unsigned long long *in = nullptr;
+    // voxel pairs interleaved over the waves of the grid (a shared atomic
+    // counter would serialise on one address)
+    uint32_t wnext = __builtin_amdgcn_readfirstlane((uint32_t)wave_global());
+    const uint32_t wstride = gridDim.x * kWavesPerBlock;
+    auto load_group = [&]() {
+        const uint32_t kw = wb + lane;
+        m = kw < nw ? in[kw] : 0ull;
+        const uint32_t ch = kw / t.ngr, gr = kw - ch * t.ngr;
+        row0 = gr * t.R;
+        col0 = ch * t.cw;
+    };
+    for (;;) {
+        // fill the queue to 64 items or until the work is gone
+        while (qn < (uint32_t)kWave && !done) {
+            if (cur == kNoProbe) {
+                const uint32_t w = wnext;
+                wnext += wstride;
+                if (w >= n_vp) {
+                    done = true;
+                    break;
+                }
+                const uint32_t fl = flags[w];
+                if (fl & MX_EMPTY) continue;
+                const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+                cur = w;
+                n2 = g.n2;
+                small = (fl & MX_SMALL) != 0;
+                if (small) {
+                    total = g.n1 * g.n2;
+                    k = 0;
+                } else {
+                    t = mx_tiling(g, max_rows);
+                    nw = mx_words(t);
+                    in = mask + woff[w];
+                    wb = 0;
+                    wave_sync();
+                    const uint32_t r = (uint32_t)lane / t.cw;
+                    tab[lane] = (uint16_t)((r << 8) | ((uint32_t)lane - r * t.cw));
+                    wave_sync();
+                    load_group();
+                }
+            }
+            if (small) {
+                push(k + lane < total, cur, k + lane);
+                k += kWave;
+                if (k >= total) cur = kNoProbe;
+            } else if (!__any(m != 0ull)) {
+                wb += kWave;
+                if (wb >= nw) cur = kNoProbe;
+                else load_group();
+            } else {
+                const bool has = m != 0ull;
+                uint32_t q = 0;
+                if (has) {
+                    const uint32_t b = (uint32_t)__ffsll((long long)m) - 1u;
+                    m &= m - 1ull;
+                    const uint32_t e = tab[b];
+                    const uint32_t row = row0 + (e >> 8), col = col0 + (e & 255u);
+                    q = t.swap ? col * n2 + row : row * n2 + col;
+                }
+                push(has, cur, q);
+            }
+        }
+        if (qn == 0) break;
+        // evaluate the first (at most 64) queued items, then shift the rest down
+        const uint32_t cnt = min(qn, (uint32_t)kWave);
+        wave_sync();
+        uint32_t w = kNoProbe;
+        unsigned long long a = inf_key, b = inf_key, c = inf_key;
+        if ((uint32_t)lane < cnt) {
+            w = mw[lane];
+            const uint32_t q = mq[lane];
+            const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+            const uint32_t i = q / g.n2, j = q - i * g.n2;
+            double d, lo, hi;
+            pair_values<double>(rec1, g.o1 + i, rec2, g.o2 + j, d, lo, hi);
+            a = dkey(d);
+            b = dkey(lo);
+            c = dkey(hi);
+        }
+        seg_min_fold(w, a, b, c, mkey);
+        evals += cnt;
+        const uint32_t rest = qn - cnt;
+        uint32_t mw2 = 0, mq2 = 0;
+        if ((uint32_t)lane < rest) {
+            mw2 = mw[cnt + lane];
+            mq2 = mq[cnt + lane];
+        }
+        wave_sync();
+        if ((uint32_t)lane < rest) {
+            mw[lane] = mw2;
+            mq[lane] = mq2;
+        }
+        qn = rest;
+    }
+    if (lane == 0 && evals) atomicAdd(&stats[2], (unsigned long long)evals);
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Minima of two keys over the lanes of one segment (lanes sorted by seg;

@@ -376,6 +376,18 @@ void tdb_engine_evict(tdb_engine *e, const tdb_tile *t) {
     h.erase(std::remove(h.begin(), h.end(), e), h.end());
 }
 
+int tdb_engine_get_tuning(tdb_engine *e, tdp_tuning *out) {
+    if (!e || !e->ctx) return TDP_ERR_ARG;
+    return tdp_get_tuning(e->ctx, out);
+}
+
+int tdb_engine_set_tuning(tdb_engine *e, const tdp_tuning *t) {
+    if (!e || !e->ctx) return TDP_ERR_ARG;
+    const int rc = tdp_set_tuning(e->ctx, t);
+    if (rc != TDP_OK) e->err = tdp_last_error(e->ctx);
+    return rc;
+}
+
 static int resident(tdb_engine *e, const tdb_tile *t, tdp_tile **out) {
     std::lock_guard<std::mutex> g(g_registry);
     auto it = e->resident.find(t);
@@ -481,6 +493,55 @@ int64_t tdb_engine_candidates(tdb_engine *e, const tdb_tile *t1, const tdb_tile 
     if (prefix) std::copy(pf.begin(), pf.end(), prefix);
     if (ids) std::copy(id.begin(), id.begin() + std::min(cap, id.size()), ids);
     return (int64_t)id.size();
+}
+
+int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
+                        const tdb_join_opts *o, size_t sample, tdp_tuning *best,
+                        tdp_tuning *tried, double *ms, int cap) {
+    if (!e || !e->ctx || !t1 || !o) return -1;
+    tdp_tuning base;
+    if (tdp_get_tuning(e->ctx, &base) != TDP_OK) return -1;
+    std::vector<tdp_tuning> cand;
+    for (int drain : {0, 1})
+        for (int rows : {64, 4})
+            for (int small : {8, 64}) {
+                tdp_tuning t = base;
+                t.drain = drain;
+                t.max_rows = rows;
+                t.small_pairs = small;
+                t.drain_blocks = drain ? 8 * 256 : 0;
+                t.eval_blocks = 4096;
+                cand.push_back(t);
+            }
+    tdb_join_opts so = *o;
+    if (sample && (so.max_objects1 == 0 || so.max_objects1 > sample)) so.max_objects1 = sample;
+    double best_ms = 1e300;
+    tdp_tuning pick = base;
+    int n = 0;
+    for (const tdp_tuning &t : cand) {
+        if (tdp_set_tuning(e->ctx, &t) != TDP_OK) continue;
+        tdb_result *r = nullptr;
+        double dev = 1e300;
+        // twice: the first run also warms the buffer cache
+        for (int rep = 0; rep < 2; rep++) {
+            if (tdb_join(e, t1, t2, &so, &r) != 0) {
+                tdp_set_tuning(e->ctx, &base);
+                return -1;
+            }
+            dev = std::min(dev, r->timing[4] + r->timing[5]);
+            tdb_result_free(r);
+        }
+        if (tried && n < cap) tried[n] = t;
+        if (ms && n < cap) ms[n] = dev;
+        n++;
+        if (dev < best_ms) {
+            best_ms = dev;
+            pick = t;
+        }
+    }
+    tdp_set_tuning(e->ctx, &pick);
+    if (best) *best = pick;
+    return n;
 }
 
 int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,

@@ -160,6 +160,9 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=24,
                     help="serving run (resident vessels, Poisson arrivals) after the join bench; 0: off")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="query arrivals per second")
+    ap.add_argument("--tune", default="", help="kernel tuning, e.g. drain=0,max_rows=4")
+    ap.add_argument("--autotune", type=int, default=4000,
+                    help="autotune on this many targets before timing (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the join shards by "
@@ -179,6 +182,16 @@ def main():
     log(f"generated {nuc.n_obj} nuclei ({nuc.n_tri} triangle records) and {ves.n_obj} vessels "
         f"({ves.n_tri}) in {time.perf_counter() - t0:.1f} s")
     eng = tdp.Engine(0)
+    tuned = None
+    if args.tune:
+        eng.set_tuning(**{k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))})
+    elif args.autotune:
+        t0 = time.perf_counter()
+        tuned = eng.autotune(nuc, ves, query=args.query, knn=args.knn, within_dist=args.tau,
+                             sample_targets=args.autotune, precision=args.precision)
+        tuned["seconds"] = time.perf_counter() - t0
+        log(f"autotune ({args.autotune} targets, {len(tuned['tried'])} tunings, "
+            f"{tuned['seconds']:.1f} s): {tuned['best']}")
     name, cus = eng.device()
     log(f"device {name}, {cus} CUs")
 
@@ -237,6 +250,7 @@ def main():
                    "fp32_culled_tri_pairs": r.stats["fp32_culled"],
                    "fp64_tridist_evals": r.stats["fp64_evals"],
                    "object_pairs": r.stats["object_pairs"], "results": int(len(r.target)),
+                   "tuning": eng.tuning(), "autotune": tuned,
                    "timing_ms": r.timing_ms},
         "roofline": {"bound": "valu-fp32", "kernel": "refinement round (k_mx_* pipeline)"
                      if args.precision == "mixed" else "refinement round (k_refine)",

@@ -14,6 +14,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "tdp.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -88,6 +90,10 @@ const char *tdb_engine_error(const tdb_engine *e);
 int tdb_engine_device(tdb_engine *e, char *name, size_t len, int *cu_count);
 /* drop the device copy of a tile (it is uploaded again on next use) */
 void tdb_engine_evict(tdb_engine *e, const tdb_tile *t);
+/* the engine's kernel tuning (tdp_tuning of tdp.h; performance only) */
+int tdb_engine_get_tuning(tdb_engine *e, tdp_tuning *out);
+int tdb_engine_set_tuning(tdb_engine *e, const tdp_tuning *t);
+
 
 #define TDB_QUERY_NN 0
 #define TDB_QUERY_WITHIN 1
@@ -112,6 +118,17 @@ void tdb_join_defaults(tdb_join_opts *o);
  * (SpatialJoin.cpp:435-484). */
 int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
              tdb_result **out);
+
+/* Autotuner: run the join of the first sample_targets targets (all if 0)
+ * under each tuning of a fixed candidate set -- FP64 stage as a work list or
+ * persistent, rows per wave iteration, the small-pair threshold, the FP64
+ * grid -- timing the device work of filter and refinement, and keep the
+ * fastest on the engine (results never depend on the tuning). tried/ms
+ * (optional, room for cap entries): each candidate in the order tried and
+ * its time. Returns the number of candidates tried, or < 0. */
+int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
+                        const tdb_join_opts *o, size_t sample_targets, tdp_tuning *best,
+                        tdp_tuning *tried, double *ms, int cap);
 
 /* candidate lists alone (octree query_knn_fast / query_within per target):
  * prefix[n_targets+1] and ids; returns the number of ids (written up to cap) */

@@ -129,6 +129,21 @@ int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t n_targets, const
                        int self, const int32_t *k_target, uint32_t *prefix, uint32_t *ids,
                        size_t ids_cap, uint64_t *n_ids, uint8_t *fallback);
 
+/* ---- tuning (performance only: results never depend on it) ---- */
+typedef struct {
+    int small_pairs;  /* MIXED: voxel pairs of at most this many triangle pairs go to FP64
+                         without the FP32 passes */
+    int max_rows;     /* MIXED: most rows of a voxel pair's grid per wave iteration of the
+                         FP32 passes (1..64; the tile is rows x columns <= 64 lanes) */
+    int drain;        /* MIXED: 1 = persistent FP64 stage (candidates queued in LDS per wave,
+                         voxel pairs taken from a work counter); 0 = a flat work list */
+    int drain_blocks; /* blocks of the persistent stage (0 = 4 per compute unit) */
+    int eval_blocks;  /* grid of the work-list FP64 stage (grid-stride) */
+} tdp_tuning;
+/* the context's tuning (defaults at tdp_open); joins created afterwards use it */
+int tdp_get_tuning(tdp_ctx *ctx, tdp_tuning *out);
+int tdp_set_tuning(tdp_ctx *ctx, const tdp_tuning *t);
+
 /* ---- device-resident join (filter + LOD refinement) ---- */
 
 typedef struct {

@@ -325,3 +325,40 @@ def test_device_knn_candidates_equal_octree(tdp, eng, small_tiles, tmp_path, k):
     dp, di, fb = eng.candidates(n, v2, "nn", k)
     assert np.array_equal(hp, dp) and np.array_equal(hi, di)
     assert fb > 0
+
+
+# ------------------------------------------------------------- tuning
+
+@pytest.mark.parametrize("tune", [dict(drain=0), dict(drain=1, drain_blocks=3),
+                                  dict(small_pairs=0, max_rows=1), dict(small_pairs=200, max_rows=3)])
+def test_tuning_never_changes_results(tdp, small_tiles, tune):
+    """every tuning gives the FP64 form's lists and ranges bit for bit"""
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    e = tdp.Engine(0)
+    want = e.join(n, v, query="nn", knn=3, precision="fp64")
+    e.set_tuning(**tune)
+    assert all(e.tuning()[k] == x for k, x in tune.items())
+    for q, k, tau in (("nn", 3, 0.0), ("within", 1, 300.0)):
+        ref = e.join(n, v, query=q, knn=k, within_dist=tau, precision="fp64")
+        got = e.join(n, v, query=q, knn=k, within_dist=tau, precision="mixed")
+        assert np.array_equal(got.target, ref.target) and np.array_equal(got.obj2, ref.obj2)
+        assert bits_equal(got.rmin, ref.rmin) and bits_equal(got.rmax, ref.rmax)
+    assert want.stats["tri_pairs"] > 0
+    e.close()
+
+
+def test_autotune_keeps_results(tdp, small_tiles):
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    e = tdp.Engine(0)
+    want = e.join(n, v, query="nn", knn=3, precision="fp64")
+    out = e.autotune(n, v, query="nn", knn=3, sample_targets=100)
+    assert len(out["tried"]) == 8 and all(ms > 0 for _, ms in out["tried"])
+    assert e.tuning() == out["best"]
+    assert min(ms for _, ms in out["tried"]) == dict(
+        (tuple(sorted(t.items())), ms) for t, ms in out["tried"])[tuple(sorted(out["best"].items()))]
+    got = e.join(n, v, query="nn", knn=3, precision="mixed")
+    assert np.array_equal(got.target, want.target) and np.array_equal(got.obj2, want.obj2)
+    assert bits_equal(got.rmin, want.rmin) and bits_equal(got.rmax, want.rmax)
+    e.close()
