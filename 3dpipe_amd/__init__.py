@@ -54,7 +54,7 @@ class _GenParams(_c.Structure):
 
 class _Tuning(_c.Structure):
     _fields_ = [("small_pairs", _c.c_int), ("max_rows", _c.c_int), ("drain", _c.c_int),
-                ("drain_blocks", _c.c_int), ("eval_blocks", _c.c_int)]
+                ("drain_blocks", _c.c_int), ("eval_blocks", _c.c_int), ("graph", _c.c_int)]
 
 
 class _JoinOpts(_c.Structure):
@@ -221,7 +221,8 @@ def generate(seed: int = 0, n_vessels: int = 10, nuclei_per_vessel: int = 20,
 
 
 # refinement stages, in the order of TDP_STAGE_* (include/tdp.h)
-STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct"]
+STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",
+          "graph"]
 
 
 @dataclass

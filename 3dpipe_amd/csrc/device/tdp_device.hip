@@ -20,7 +20,7 @@ struct tdp_ctx {
     int device = 0;
     int cu_count = 256;
     hipStream_t stream = nullptr;
-    tdp_tuning tune{(int)tdp::kMxSmallPairs, 64, 0, 0, 4096};
+    tdp_tuning tune{(int)tdp::kMxSmallPairs, 64, 0, 0, 4096, 0};
     std::string err;
     // grow-only scratch for scans
     std::vector<void *> scan_buf;
@@ -88,6 +88,12 @@ struct tdp_join {
     uint64_t filter_vp = 0;
     tdp_tuning tune{};
     uint32_t *work = nullptr;  // work counter of the persistent stage
+    // graph-captured rounds: device totals, pinned read-back, the executable
+    unsigned long long *words5 = nullptr;  // per-slot mask-word bound
+    uint64_t *d_words = nullptr, *d_kept = nullptr;
+    uint64_t *h_kept = nullptr;             // pinned
+    hipGraphExec_t gexec = nullptr;
+    bool planned = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double refine_ms = 0, filter_ms = 0;
     // per-stage timing: events recorded between the stages of a round
@@ -204,23 +210,17 @@ static void pfree(tdp_ctx *ctx, void *p) {
     ctx->pool_free.emplace(it->second, p);
 }
 
-// exclusive scan of n counters (u32 or u64) into out (u64); *total set on host
-template <typename In>
-static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, uint64_t *total) {
-    if (n == 0) {
-        *total = 0;
-        return TDP_OK;
-    }
-    // level sizes
+// block-sum buffers of a scan of n elements (grow-only; allocating here, not
+// during a graph capture)
+static int scan_reserve(tdp_ctx *ctx, size_t n, std::vector<size_t> *levels = nullptr) {
     std::vector<size_t> sizes;
-    size_t m = n;
+    size_t m = n ? n : 1;
     while (true) {
         size_t b = (m + 1023) / 1024;
         sizes.push_back(b);
         if (b == 1) break;
         m = b;
     }
-    // block-sum buffers: for each level a sums array and an offsets array
     if (ctx->scan_buf.size() < 2 * sizes.size()) {
         ctx->scan_buf.resize(2 * sizes.size(), nullptr);
         ctx->scan_cap.resize(2 * sizes.size(), 0);
@@ -236,7 +236,23 @@ static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, u
             }
         }
     if (!ctx->d_total) TDP_CHECK(ctx, hipMalloc((void **)&ctx->d_total, sizeof(uint64_t)));
-    // level 0
+    if (levels) *levels = sizes;
+    return TDP_OK;
+}
+
+// exclusive scan of n counters (u32 or u64) into out (u64) and the total into
+// *total_dev, kernels only (no host synchronisation: usable in a graph
+// capture once scan_reserve has run for n)
+template <typename In>
+static int scan_launch(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, uint64_t *total_dev) {
+    std::vector<size_t> sizes;
+    int rc = scan_reserve(ctx, n, &sizes);
+    if (rc != TDP_OK) return rc;
+    if (n == 0) {
+        k_scan_total<In><<<1, 1, 0, ctx->stream>>>(in, out, 0, total_dev);
+        TDP_LAUNCHED(ctx);
+        return TDP_OK;
+    }
     uint64_t *sums0 = (uint64_t *)ctx->scan_buf[0];
     k_scan_local<In><<<(unsigned)sizes[0], kBlock, 0, ctx->stream>>>(in, n, out, sums0);
     TDP_LAUNCHED(ctx);
@@ -262,8 +278,22 @@ static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, u
         k_scan_add<<<blocks_for(n, kBlock), kBlock, 0, ctx->stream>>>(out, n, offs0);
         TDP_LAUNCHED(ctx);
     }
-    k_scan_total<In><<<1, 1, 0, ctx->stream>>>(in, out, n, ctx->d_total);
+    k_scan_total<In><<<1, 1, 0, ctx->stream>>>(in, out, n, total_dev);
     TDP_LAUNCHED(ctx);
+    return TDP_OK;
+}
+
+// exclusive scan of n counters (u32 or u64) into out (u64); *total set on host
+template <typename In>
+static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, uint64_t *total) {
+    if (n == 0) {
+        *total = 0;
+        return TDP_OK;
+    }
+    int rc = scan_reserve(ctx, n);
+    if (rc != TDP_OK) return rc;
+    rc = scan_launch<In>(ctx, in, n, out, ctx->d_total);
+    if (rc != TDP_OK) return rc;
     TDP_CHECK(ctx, hipMemcpyAsync(total, ctx->d_total, sizeof(uint64_t), hipMemcpyDeviceToHost,
                                   ctx->stream));
     TDP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -716,6 +746,11 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : j->sev)
         if (e) (void)hipEventDestroy(e);
+    pfree(ctx, j->words5);
+    pfree(ctx, j->d_words);
+    pfree(ctx, j->d_kept);
+    if (j->h_kept) (void)hipHostFree(j->h_kept);
+    if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
     delete j;
     return TDP_OK;
 }
@@ -1122,6 +1157,173 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     return TDP_OK;
 }
 
+// The mixed round with no host synchronisation inside: mask capacity planned
+// once from the filter's list (every later list is a subset), the FP64 stage
+// persistent (no work-list size to read back), totals kept on the device --
+// so the whole round is captured into a hipGraph and launched as one; the
+// kept count comes back through pinned memory at the end.
+static int plan_graph_rounds(tdp_join *j) {
+    tdp_ctx *ctx = j->ctx;
+    hipStream_t s = ctx->stream;
+    const uint64_t n = j->n_vp;
+    int rc = palloc(ctx, &j->words5, 5);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->d_words, 1);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->d_kept, 1);
+    if (rc != TDP_OK) return rc;
+    if (!j->h_kept) TDP_CHECK(ctx, hipHostMalloc((void **)&j->h_kept, sizeof(uint64_t)));
+    k_zero_u32<<<1, 64, 0, s>>>((uint32_t *)j->words5, 10);
+    TDP_LAUNCHED(ctx);
+    const tdp_tile *A = j->t1, *B = j->t2;
+    if (n) {
+        k_words_bound<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+            n, j->v1[j->cur], j->v2[j->cur], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off,
+            B->vox_tri_cnt, (uint32_t)j->tune.max_rows, j->words5);
+        TDP_LAUNCHED(ctx);
+    }
+    unsigned long long w5[5];
+    TDP_CHECK(ctx, hipMemcpyAsync(w5, j->words5, sizeof w5, hipMemcpyDeviceToHost, s));
+    TDP_CHECK(ctx, hipStreamSynchronize(s));
+    uint64_t most = 0;
+    for (int k = 0; k < 5; k++) most = std::max<uint64_t>(most, w5[k]);
+    rc = ensure_mask(j, most);
+    if (rc == TDP_OK) rc = ensure_mx(j, n);
+    if (rc == TDP_OK) rc = scan_reserve(ctx, n);
+    if (rc == TDP_OK) j->planned = true;
+    return rc;
+}
+
+static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
+    using R = double;
+    using KT = Key<R>::type;
+    if (j->n_vp == 0) return join_refine_t<double, MODE_MIXED>(j, lod, is_last, pairs_out);
+    tdp_ctx *ctx = j->ctx;
+    hipStream_t s = ctx->stream;
+    if (!j->planned) {
+        int rc = plan_graph_rounds(j);
+        if (rc != TDP_OK) return rc;
+    }
+    const size_t P = j->P, N = j->N;
+    const bool within = j->prm.query == TDP_QUERY_WITHIN;
+    R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
+    int slot = -1;
+    for (int k = 0; k < 5; k++)
+        if (lod == 20 * (k + 1)) slot = k;
+    const uint8_t rnd = (uint8_t)std::min(j->rounds + 1, 254);
+    const int c = j->cur, o = 1 - c;
+    const uint64_t n = j->n_vp;
+    const tdp_tile *A = j->t1, *B = j->t2;
+    const dim3 gw = wave_grid(n);
+    const uint32_t need = is_last || j->prm.hausdorff_level != 2 ? 1u : 6u;
+    const int nb = j->tune.drain_blocks ? j->tune.drain_blocks : 4 * ctx->cu_count;
+
+    TDP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = TDP_OK;
+    auto launched = [&]() {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess && rc == TDP_OK) {
+            ctx->err = std::string("captured launch: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    };
+    k_reset_acc<R, KT><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (KT *)j->acc_min,
+                                                                (KT *)j->acc_max);
+    launched();
+    k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_cnt,
+                                                           B->vox_tri_cnt, slot, j->stats);
+    launched();
+    k_mx_bounds<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                      A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
+                                      B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
+                                      (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
+                                      j->mx_flags, j->mx_key, j->mx_nw);
+    launched();
+    if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, j->d_words);
+    k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
+        3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+        A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
+    launched();
+    k_mx_mark<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                    A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri, B->bnd,
+                                    slot, (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
+                                    j->mx_flags, j->mx_key, j->mx_woff, j->mx_mask, j->keep);
+    launched();
+    k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
+                                     A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot,
+                                     (uint32_t)j->tune.max_rows, j->mx_flags, j->mx_woff,
+                                     j->mx_mask, j->mx_key, j->stats);
+    launched();
+    k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+        n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
+        B->vox_tri_cnt, A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last,
+        j->prm.hausdorff_level, j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
+        j->stats);
+    launched();
+    k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
+                                                              (const KT *)j->acc_max, rmin, rmax);
+    launched();
+    if (!within)
+        k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
+            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd);
+    else
+        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
+            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd);
+    launched();
+    k_vp_keep<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->vpp[c], (const R *)j->vmin[c],
+                                                          j->state, rmax, j->keep);
+    launched();
+    if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->keep, n, j->pos, j->d_kept);
+    k_vp_scatter<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+        n, j->keep, j->pos, j->v1[c], j->v2[c], j->vpp[c], (const R *)j->vmin[c],
+        (const R *)j->vmax[c], j->v1[o], j->v2[o], j->vpp[o], (R *)j->vmin[o], (R *)j->vmax[o]);
+    launched();
+    hipError_t e = hipMemcpyAsync(j->h_kept, j->d_kept, sizeof(uint64_t), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess && rc == TDP_OK) {
+        ctx->err = std::string("captured copy: ") + hipGetErrorString(e);
+        rc = TDP_ERR_HIP;
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess && rc == TDP_OK) {
+        ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        rc = TDP_ERR_HIP;
+    }
+    if (rc != TDP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    // reuse the executable when the round's graph has the same shape
+    bool ok = false;
+    if (j->gexec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res;
+        ok = hipGraphExecUpdate(j->gexec, graph, &bad, &res) == hipSuccess &&
+             res == hipGraphExecUpdateSuccess;
+        (void)hipGetLastError();
+        if (!ok) {
+            (void)hipGraphExecDestroy(j->gexec);
+            j->gexec = nullptr;
+        }
+    }
+    if (!ok) e = hipGraphInstantiate(&j->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    TDP_CHECK(ctx, e);
+    TDP_CHECK(ctx, hipEventRecord(j->ev[2], s));
+    TDP_CHECK(ctx, hipGraphLaunch(j->gexec, s));
+    TDP_CHECK(ctx, hipEventRecord(j->ev[3], s));
+    TDP_CHECK(ctx, hipStreamSynchronize(s));
+    float ms = 0;
+    TDP_CHECK(ctx, hipEventElapsedTime(&ms, j->ev[2], j->ev[3]));
+    j->refine_ms += ms;
+    j->stage_ms[TDP_STAGE_GRAPH] += ms;
+    j->stage_n[TDP_STAGE_GRAPH]++;
+    const uint64_t kept = *j->h_kept;
+    j->cur = o;
+    j->n_vp = kept;
+    j->rounds++;
+    if (pairs_out) *pairs_out = kept;
+    return TDP_OK;
+}
+
 extern "C" int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
     if (!j) return TDP_ERR_ARG;
     if (!j->filtered) {
@@ -1135,7 +1337,9 @@ extern "C" int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pair
     switch (j->prm.precision) {
         case TDP_PREC_REF32: return join_refine_t<float, MODE_REF32>(j, lod, is_last, pairs_out);
         case TDP_PREC_FP64: return join_refine_t<double, MODE_FP64>(j, lod, is_last, pairs_out);
-        default: return join_refine_t<double, MODE_MIXED>(j, lod, is_last, pairs_out);
+        default:
+            if (j->tune.graph) return join_refine_graph(j, lod, is_last, pairs_out);
+            return join_refine_t<double, MODE_MIXED>(j, lod, is_last, pairs_out);
     }
 }
 

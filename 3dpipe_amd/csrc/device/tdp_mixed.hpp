@@ -714,6 +714,34 @@ __global__ void __launch_bounds__(kBlock) k_mx_drain(
     if (lane == 0 && evals) atomicAdd(&stats[2], (unsigned long long)evals);
 }
 
+// per LOD slot: the mask words every voxel pair of the list would need --
+// a bound for any later round, whose list is a subset (one atomic per block
+// and slot)
+__global__ void __launch_bounds__(kBlock) k_words_bound(
+    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
+    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
+    const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, uint32_t max_rows,
+    unsigned long long *__restrict__ words5) {
+    __shared__ unsigned long long part[5][kWavesPerBlock];
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        uint64_t n = 0;
+        if (w < n_vp) {
+            const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, s);
+            if (g.n1 && g.n2) n = mx_words(mx_tiling(g, max_rows));
+        }
+        n = wave_sum_u64(n);
+        if (lane_id() == 0) part[s][threadIdx.x / kWave] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long a = 0;
+        for (int k = 0; k < kWavesPerBlock; k++) a += part[threadIdx.x][k];
+        if (a) atomicAdd(&words5[threadIdx.x], a);
+    }
+}
+
 __global__ void k_zero_u32(uint32_t *__restrict__ p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;

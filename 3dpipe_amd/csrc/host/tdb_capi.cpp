@@ -502,14 +502,16 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
     tdp_tuning base;
     if (tdp_get_tuning(e->ctx, &base) != TDP_OK) return -1;
     std::vector<tdp_tuning> cand;
-    for (int drain : {0, 1})
+    // FP64 stage: work list (0), persistent (1), persistent in a captured round (2)
+    for (int form : {0, 1, 2})
         for (int rows : {64, 4})
             for (int small : {8, 64}) {
                 tdp_tuning t = base;
-                t.drain = drain;
+                t.drain = form >= 1;
+                t.graph = form == 2;
                 t.max_rows = rows;
                 t.small_pairs = small;
-                t.drain_blocks = drain ? 8 * 256 : 0;
+                t.drain_blocks = form ? 8 * 256 : 0;
                 t.eval_blocks = 4096;
                 cand.push_back(t);
             }
@@ -528,7 +530,8 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
                 tdp_set_tuning(e->ctx, &base);
                 return -1;
             }
-            dev = std::min(dev, r->timing[4] + r->timing[5]);
+            // wall time of filter and refinement: host synchronisation counts
+            dev = std::min(dev, r->timing[2] + r->timing[3]);
             tdb_result_free(r);
         }
         if (tried && n < cap) tried[n] = t;

@@ -30,7 +30,8 @@ F_TRI = 997
 # candidate comparison; the breakdown is beside disc_interval()).
 F_MARK = 89
 PEAK_FP32_TFLOPS = 157.3
-REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct"]
+REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",
+                 "graph"]
 
 
 def log(msg):
@@ -221,9 +222,9 @@ def main():
     # join stream) is the measured unit: algorithmic work = F_TRI per pair
     achieved = F_TRI * tri_pairs / (refine_kernel_ms * 1e-3) / 1e12
     dom = max((k for k in REFINE_STAGES if k not in ("scan", "reset")), key=lambda k: stage_ms[k])
-    dom_kernel = {"mark": "k_mx_mark", "eval": "k_mx_eval", "bounds": "k_mx_bounds1",
+    dom_kernel = {"mark": "k_mx_mark", "eval": "k_mx_eval", "bounds": "k_mx_bounds",
                   "probe": "k_mx_eval", "expand": "k_mx_expand", "fold": "k_mx_fold",
-                  "direct": "k_refine"}[dom]
+                  "direct": "k_refine", "graph": "captured round"}[dom]
     dom_launches = max(1, stage_n.get(dom, 1))
     dom_obj = {"name": dom_kernel, "stage": dom, "launches": dom_launches,
                "avg_launch_ms": stage_ms[dom] / dom_launches,

@@ -120,10 +120,11 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
              tdb_result **out);
 
 /* Autotuner: run the join of the first sample_targets targets (all if 0)
- * under each tuning of a fixed candidate set -- FP64 stage as a work list or
- * persistent, rows per wave iteration, the small-pair threshold, the FP64
- * grid -- timing the device work of filter and refinement, and keep the
- * fastest on the engine (results never depend on the tuning). tried/ms
+ * under each tuning of a fixed candidate set -- FP64 stage as a work list,
+ * persistent, or persistent in a graph-captured round; rows per wave
+ * iteration; the small-pair threshold -- timing filter and refinement on the
+ * host clock (so host synchronisation counts), and keep the fastest on the
+ * engine (results never depend on the tuning). tried/ms
  * (optional, room for cap entries): each candidate in the order tried and
  * its time. Returns the number of candidates tried, or < 0. */
 int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,

@@ -139,6 +139,8 @@ typedef struct {
                          voxel pairs taken from a work counter); 0 = a flat work list */
     int drain_blocks; /* blocks of the persistent stage (0 = 4 per compute unit) */
     int eval_blocks;  /* grid of the work-list FP64 stage (grid-stride) */
+    int graph;        /* MIXED: 1 = each LOD round is captured in a hipGraph and launched as one
+                         (implies the persistent FP64 stage; one host synchronisation per round) */
 } tdp_tuning;
 /* the context's tuning (defaults at tdp_open); joins created afterwards use it */
 int tdp_get_tuning(tdp_ctx *ctx, tdp_tuning *out);
@@ -214,7 +216,8 @@ int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
 #define TDP_STAGE_EVAL 6    /* MIXED: FP64 TriDist of the work list */
 #define TDP_STAGE_FOLD 7    /* MIXED: range fold */
 #define TDP_STAGE_DIRECT 8  /* REF32 / FP64: the one-pass distance kernel */
-#define TDP_STAGE_COUNT 9
+#define TDP_STAGE_GRAPH 9   /* MIXED with graph tuning: the whole captured round */
+#define TDP_STAGE_COUNT 10
 int tdp_join_stage_ms(tdp_join *j, double *ms, uint64_t *launches, int n_stages);
 int tdp_join_destroy(tdp_join *j);
 

@@ -330,7 +330,9 @@ def test_device_knn_candidates_equal_octree(tdp, eng, small_tiles, tmp_path, k):
 # ------------------------------------------------------------- tuning
 
 @pytest.mark.parametrize("tune", [dict(drain=0), dict(drain=1, drain_blocks=3),
-                                  dict(small_pairs=0, max_rows=1), dict(small_pairs=200, max_rows=3)])
+                                  dict(small_pairs=0, max_rows=1), dict(small_pairs=200, max_rows=3),
+                                  dict(graph=1, drain=1), dict(graph=1, drain=1, max_rows=2,
+                                                               drain_blocks=7)])
 def test_tuning_never_changes_results(tdp, small_tiles, tune):
     """every tuning gives the FP64 form's lists and ranges bit for bit"""
     pn, pv = small_tiles
@@ -354,7 +356,7 @@ def test_autotune_keeps_results(tdp, small_tiles):
     e = tdp.Engine(0)
     want = e.join(n, v, query="nn", knn=3, precision="fp64")
     out = e.autotune(n, v, query="nn", knn=3, sample_targets=100)
-    assert len(out["tried"]) == 8 and all(ms > 0 for _, ms in out["tried"])
+    assert len(out["tried"]) == 12 and all(ms > 0 for _, ms in out["tried"])
     assert e.tuning() == out["best"]
     assert min(ms for _, ms in out["tried"]) == dict(
         (tuple(sorted(t.items())), ms) for t, ms in out["tried"])[tuple(sorted(out["best"].items()))]
