@@ -122,6 +122,15 @@ def _load():
     tdb.tdb_server_error.restype = _c.c_char_p
     tdb.tdb_server_stats.argtypes = [_P, _P]
     tdb.tdb_server_close.argtypes = [_P]
+    tdb.tdb_source_generate.argtypes = [_c.POINTER(_GenParams), _c.POINTER(_P)]
+    tdb.tdb_source_tile.argtypes = [_P, _c.POINTER(_P)]
+    tdb.tdb_source_objects.argtypes = [_P]
+    tdb.tdb_source_objects.restype = u64
+    tdb.tdb_source_decode.argtypes = [_P, _P, sz, _c.POINTER(_P)]
+    tdb.tdb_source_free.argtypes = [_P]
+    tdb.tdb_stream_defaults.argtypes = [_P]
+    tdb.tdb_stream_join.argtypes = [_P, _P, _P, _c.POINTER(_JoinOpts), _P, _c.POINTER(_P)]
+    tdb.tdb_result_stream.argtypes = [_P, _P]
     # libtdp (kernel-level parity ops)
     tdp.tdp_open.argtypes = [_c.c_int, _c.POINTER(_P)]
     tdp.tdp_close.argtypes = [_P]
@@ -199,10 +208,7 @@ class Tile:
             pass
 
 
-def generate(seed: int = 0, n_vessels: int = 10, nuclei_per_vessel: int = 20,
-             cube: float = 2000.0, **kw) -> tuple:
-    """Seeded synthetic (nuclei, vessels) tiles; keyword names follow
-    tdb_gen_params in include/tdbase.h."""
+def _gen_params(seed, n_vessels, nuclei_per_vessel, cube, kw) -> "_GenParams":
     tdb, _ = _load()
     p = _GenParams()
     tdb.tdb_gen_defaults(_c.byref(p))
@@ -211,11 +217,75 @@ def generate(seed: int = 0, n_vessels: int = 10, nuclei_per_vessel: int = 20,
         if not hasattr(p, k):
             raise TypeError(f"unknown generator parameter {k}")
         setattr(p, k, v)
+    return p
+
+
+def generate(seed: int = 0, n_vessels: int = 10, nuclei_per_vessel: int = 20,
+             cube: float = 2000.0, **kw) -> tuple:
+    """Seeded synthetic (nuclei, vessels) tiles; keyword names follow
+    tdb_gen_params in include/tdbase.h."""
+    tdb, _ = _load()
+    p = _gen_params(seed, n_vessels, nuclei_per_vessel, cube, kw)
     n, v = _P(), _P()
     if tdb.tdb_generate(_c.byref(p), _c.byref(n), _c.byref(v)) != 0:
         raise TdpError("invalid generator parameters")
     return Tile(n.value), Tile(v.value)
 
+
+class Source:
+    """Objects decoded on demand for the streaming join (tdb_source_* of
+    include/tdbase.h): the generator's nuclei, or a tile's objects."""
+
+    def __init__(self, handle, keep=None):
+        self._h = _P(handle)
+        self._keep = keep  # a tile the source reads from
+        self.n_obj = int(_load()[0].tdb_source_objects(self._h))
+
+    @classmethod
+    def generate(cls, seed: int = 0, n_vessels: int = 0, nuclei_per_vessel: int = 1000,
+                 cube: float = 2000.0, **kw) -> "Source":
+        tdb, _ = _load()
+        p = _gen_params(seed, n_vessels, nuclei_per_vessel, cube, kw)
+        h = _P()
+        if tdb.tdb_source_generate(_c.byref(p), _c.byref(h)) != 0:
+            raise TdpError("invalid generator parameters")
+        return cls(h.value)
+
+    @classmethod
+    def of_tile(cls, tile: Tile) -> "Source":
+        tdb, _ = _load()
+        h = _P()
+        if tdb.tdb_source_tile(tile._h, _c.byref(h)) != 0:
+            raise TdpError("tdb_source_tile failed")
+        return cls(h.value, keep=tile)
+
+    def decode(self, objects) -> Tile:
+        """The given objects as a new tile, in that order."""
+        tdb, _ = _load()
+        ids = np.ascontiguousarray(objects, np.uint64)
+        h = _P()
+        if tdb.tdb_source_decode(self._h, _ptr(ids), ids.size, _c.byref(h)) != 0:
+            raise TdpError("tdb_source_decode: object out of range")
+        return Tile(h.value)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _libs["tdbase"].tdb_source_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class _StreamOpts(_c.Structure):
+    _fields_ = [("hbm_budget", _c.c_uint64), ("tile_share", _c.c_double),
+                ("max_chunk_targets", _c.c_uint64), ("decode_threads", _c.c_int),
+                ("overlap", _c.c_int)]
+
+
+# streaming figures of a result (tdb_result_stream)
+STREAM = ["chunks", "h2d_bytes", "decode_ms", "copy_ms", "join_ms", "wall_ms", "overlap",
+          "peak_device_bytes", "index_ms", "candidates_ms", "objects_decoded", "splits"]
 
 # ------------------------------------------------------------------ join
 
@@ -239,6 +309,8 @@ class JoinResult:
     # device ms and launches per refinement stage (STAGES order)
     stage_ms: dict = field(default_factory=dict)
     stage_launches: dict = field(default_factory=dict)
+    # streaming figures (STREAM names; streamed joins only)
+    stream: dict = field(default_factory=dict)
 
     def lists(self) -> dict:
         """{target: [obj2, ...]} in report order."""
@@ -264,6 +336,8 @@ def _read_result(r) -> "JoinResult":
         sm = np.zeros(len(STAGES), np.float64)
         sn = np.zeros(len(STAGES), np.uint64)
         tdb.tdb_result_stages(r, _ptr(sm), _ptr(sn), len(STAGES))
+        sf = np.zeros(len(STREAM), np.float64)
+        tdb.tdb_result_stream(r, _ptr(sf))
         nt = int(tdb.tdb_result_targets(r))
     finally:
         tdb.tdb_result_free(r)
@@ -272,7 +346,8 @@ def _read_result(r) -> "JoinResult":
     stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
                       "object_pairs", "rounds", "fp32_culled"], (int(x) for x in st)))
     return JoinResult(t, b, mn, mx, rd, nt, timing, stats, dict(zip(STAGES, sm.tolist())),
-                      dict(zip(STAGES, (int(x) for x in sn))))
+                      dict(zip(STAGES, (int(x) for x in sn))),
+                      dict(zip(STREAM, sf.tolist())) if sf[0] > 0 else {})
 
 
 def _opts(query, knn, within_dist, lods, hausdorff_level, max_objects1, precision, threads):
@@ -391,6 +466,31 @@ class Engine:
         r = _P()
         rc = tdb.tdb_join(self._h, tile1._h, tile2._h if tile2 is not None else None,
                           _c.byref(o), _c.byref(r))
+        if rc != 0:
+            raise TdpError(tdb.tdb_engine_error(self._h).decode())
+        return _read_result(r)
+
+    def stream_join(self, src1: Source, src2: Optional[Source] = None, query: str = "nn",
+                    knn: int = 1, within_dist: float = 1000.0,
+                    lods: Optional[Sequence[int]] = None, hausdorff_level: int = 2,
+                    max_objects1: Optional[int] = None, precision: str = "mixed",
+                    threads: int = 0, hbm_budget: int = 0, tile_share: float = 0.5,
+                    max_chunk_targets: int = 0, decode_threads: int = 0,
+                    overlap: bool = True) -> JoinResult:
+        """Chunked join within a device budget (tdb_stream_join); the result
+        equals join() on the same objects, with the streaming figures in
+        .stream."""
+        tdb, _ = _load()
+        o, keep = _opts(query, knn, within_dist, lods, hausdorff_level, max_objects1, precision,
+                        threads)
+        so = _StreamOpts()
+        tdb.tdb_stream_defaults(_c.byref(so))
+        so.hbm_budget, so.tile_share = int(hbm_budget), float(tile_share)
+        so.max_chunk_targets, so.decode_threads = int(max_chunk_targets), int(decode_threads)
+        so.overlap = 1 if overlap else 0
+        r = _P()
+        rc = tdb.tdb_stream_join(self._h, src1._h, src2._h if src2 is not None else None,
+                                 _c.byref(o), _c.byref(so), _c.byref(r))
         if rc != 0:
             raise TdpError(tdb.tdb_engine_error(self._h).decode())
         return _read_result(r)

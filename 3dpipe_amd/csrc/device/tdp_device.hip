@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -31,6 +32,11 @@ struct tdp_ctx {
     // when an allocation would otherwise run out of memory
     std::multimap<size_t, void *> pool_free;
     std::unordered_map<void *, size_t> pool_size;
+    // every device allocation: its size, the bytes held, the cap on them and
+    // the most held (a tile may be freed from another thread: mem_mu)
+    std::mutex mem_mu;
+    std::unordered_map<void *, size_t> dsize;
+    uint64_t mem_limit = 0, mem_used = 0, mem_peak = 0;
 };
 
 struct tdp_tile {
@@ -46,6 +52,7 @@ struct tdp_tile {
     float4 *bnd = nullptr;  // kBnd per triangle: sphere, plane, hausdorff pair
     float *obj_box = nullptr, *obj_tr = nullptr;  // for candidate queries (optional)
     size_t bytes = 0;
+    size_t cap_obj = 0, cap_vox = 0, cap_tri = 0;  // reserved tiles: capacity
 };
 
 struct tdp_join {
@@ -143,16 +150,63 @@ static inline unsigned stride_grid(size_t n, int cap = 4096) {
     return (unsigned)std::min<size_t>(b ? b : 1, (size_t)std::max(cap, 1));
 }
 
+static void pool_release_all(tdp_ctx *ctx);
+
+// Every device allocation of the library: the context's cap on the bytes it
+// holds, its high-water mark, and one release of the block cache before an
+// allocation is refused.
+static int dev_alloc(tdp_ctx *ctx, void **p, size_t bytes) {
+    *p = nullptr;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        bool over;
+        {
+            std::lock_guard<std::mutex> g(ctx->mem_mu);
+            over = ctx->mem_limit && ctx->mem_used + bytes > ctx->mem_limit;
+        }
+        hipError_t e = hipSuccess;
+        if (!over) {
+            e = hipMalloc(p, bytes);
+            if (e == hipSuccess) {
+                std::lock_guard<std::mutex> g(ctx->mem_mu);
+                ctx->dsize[*p] = bytes;
+                ctx->mem_used += bytes;
+                ctx->mem_peak = std::max(ctx->mem_peak, ctx->mem_used);
+                return TDP_OK;
+            }
+            (void)hipGetLastError();
+        }
+        if (attempt == 0 && !ctx->pool_free.empty() && (over || e == hipErrorOutOfMemory)) {
+            pool_release_all(ctx);
+            continue;
+        }
+        if (over)
+            ctx->err = "device memory cap: " + std::to_string(bytes) + " more bytes over " +
+                       std::to_string(ctx->mem_limit) + " (" + std::to_string(ctx->mem_used) +
+                       " held)";
+        else
+            ctx->err = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
+        return over || e == hipErrorOutOfMemory ? TDP_ERR_NOMEM : TDP_ERR_HIP;
+    }
+    return TDP_ERR_NOMEM;
+}
+
+static void dev_free(tdp_ctx *ctx, void *p) {
+    if (!p) return;
+    (void)hipFree(p);
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    auto it = ctx->dsize.find(p);
+    if (it != ctx->dsize.end()) {
+        ctx->mem_used -= it->second;
+        ctx->dsize.erase(it);
+    }
+}
+
 template <typename T>
 static int dalloc(tdp_ctx *ctx, T **p, size_t count) {
-    *p = nullptr;
     if (count == 0) count = 1;
-    TDP_CHECK(ctx, hipMalloc((void **)p, count * sizeof(T)));
-    return TDP_OK;
+    return dev_alloc(ctx, (void **)p, count * sizeof(T));
 }
-static void dfree(void *p) {
-    if (p) (void)hipFree(p);
-}
+static void dfree(tdp_ctx *ctx, void *p) { dev_free(ctx, p); }
 
 static size_t size_class(size_t b) {
     if (b <= ((size_t)1 << 16)) return (size_t)1 << 16;
@@ -164,7 +218,7 @@ static size_t size_class(size_t b) {
 
 static void pool_release_all(tdp_ctx *ctx) {
     for (auto &kv : ctx->pool_free) {
-        (void)hipFree(kv.second);
+        dev_free(ctx, kv.second);
         ctx->pool_size.erase(kv.second);
     }
     ctx->pool_free.clear();
@@ -184,17 +238,8 @@ static int palloc(tdp_ctx *ctx, T **p, size_t count) {
         return TDP_OK;
     }
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e == hipErrorOutOfMemory && !ctx->pool_free.empty()) {
-        (void)hipGetLastError();
-        pool_release_all(ctx);
-        e = hipMalloc(&q, bytes);
-    }
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        ctx->err = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
-        return e == hipErrorOutOfMemory ? TDP_ERR_NOMEM : TDP_ERR_HIP;
-    }
+    const int rc = dev_alloc(ctx, &q, bytes);
+    if (rc != TDP_OK) return rc;
     ctx->pool_size[q] = bytes;
     *p = (T *)q;
     return TDP_OK;
@@ -204,7 +249,7 @@ static void pfree(tdp_ctx *ctx, void *p) {
     if (!p) return;
     auto it = ctx->pool_size.find(p);
     if (it == ctx->pool_size.end()) {
-        (void)hipFree(p);
+        dev_free(ctx, p);
         return;
     }
     ctx->pool_free.emplace(it->second, p);
@@ -229,13 +274,18 @@ static int scan_reserve(tdp_ctx *ctx, size_t n, std::vector<size_t> *levels = nu
         for (int k = 0; k < 2; k++) {
             size_t idx = 2 * l + k;
             if (ctx->scan_cap[idx] < sizes[l]) {
-                dfree(ctx->scan_buf[idx]);
+                dfree(ctx, ctx->scan_buf[idx]);
                 ctx->scan_buf[idx] = nullptr;
-                TDP_CHECK(ctx, hipMalloc(&ctx->scan_buf[idx], sizes[l] * sizeof(uint64_t)));
+                ctx->scan_cap[idx] = 0;
+                const int rc = dev_alloc(ctx, &ctx->scan_buf[idx], sizes[l] * sizeof(uint64_t));
+                if (rc != TDP_OK) return rc;
                 ctx->scan_cap[idx] = sizes[l];
             }
         }
-    if (!ctx->d_total) TDP_CHECK(ctx, hipMalloc((void **)&ctx->d_total, sizeof(uint64_t)));
+    if (!ctx->d_total) {
+        const int rc = dalloc(ctx, &ctx->d_total, 1);
+        if (rc != TDP_OK) return rc;
+    }
     if (levels) *levels = sizes;
     return TDP_OK;
 }
@@ -348,10 +398,10 @@ extern "C" int tdp_set_tuning(tdp_ctx *ctx, const tdp_tuning *t) {
 
 extern "C" int tdp_close(tdp_ctx *ctx) {
     if (!ctx) return TDP_ERR_ARG;
-    for (void *p : ctx->scan_buf) dfree(p);
-    dfree(ctx->d_total);
+    for (void *p : ctx->scan_buf) dfree(ctx, p);
+    dfree(ctx, ctx->d_total);
     pool_release_all(ctx);
-    for (auto &kv : ctx->pool_size) (void)hipFree(kv.first);  // blocks still held by joins
+    for (auto &kv : ctx->pool_size) dev_free(ctx, kv.first);  // blocks still held by joins
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return TDP_OK;
@@ -499,18 +549,142 @@ extern "C" int tdp_tile_upload_parts(tdp_ctx *ctx, const tdp_tile_desc *d,
     return tile_upload(ctx, d, tri_parts, part_n_tri, n_parts, out);
 }
 
+// ------------------------------------------------------------ streaming fill
+
+extern "C" int tdp_set_memory_limit(tdp_ctx *ctx, uint64_t bytes) {
+    if (!ctx) return TDP_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    ctx->mem_limit = bytes;
+    return TDP_OK;
+}
+
+extern "C" int tdp_memory_usage(tdp_ctx *ctx, uint64_t *used, uint64_t *peak, int reset_peak) {
+    if (!ctx) return TDP_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mem_mu);
+    if (used) *used = ctx->mem_used;
+    if (peak) *peak = ctx->mem_peak;
+    if (reset_peak) ctx->mem_peak = ctx->mem_used;
+    return TDP_OK;
+}
+
+extern "C" int tdp_host_alloc(tdp_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return TDP_ERR_ARG;
+    *out = nullptr;
+    TDP_CHECK(ctx, hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return TDP_OK;
+}
+
+extern "C" int tdp_host_free(tdp_ctx *ctx, void *p) {
+    if (!ctx) return TDP_ERR_ARG;
+    if (p) TDP_CHECK(ctx, hipHostFree(p));
+    return TDP_OK;
+}
+
+extern "C" int tdp_sync(tdp_ctx *ctx) {
+    if (!ctx) return TDP_ERR_ARG;
+    TDP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TDP_OK;
+}
+
+extern "C" int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t n_tri,
+                                tdp_tile **out) {
+    if (!ctx || !out || !n_obj || !n_vox || !n_tri) return TDP_ERR_ARG;
+    *out = nullptr;
+    tdp_tile *t = new tdp_tile();
+    t->ctx = ctx;
+    t->cap_obj = n_obj;
+    t->cap_vox = n_vox;
+    t->cap_tri = n_tri;
+    int rc = dalloc(ctx, &t->obj_vox_off, n_obj + 1);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->obj_hd, 10 * n_obj);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_box, 9 * n_vox);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_obj, n_vox);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_tri_off, 5 * n_vox);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_tri_cnt, 5 * n_vox);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->tri, 3 * n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * n_tri);
+    if (rc != TDP_OK) {
+        tdp_tile_free(t);
+        return rc;
+    }
+    t->bytes = (n_obj + 1) * 4 + n_obj * 40 + n_vox * (36 + 4 + 40 + 20) + n_tri * 48 * (1 + kBnd);
+    *out = t;
+    return TDP_OK;
+}
+
+extern "C" int tdp_tile_fill_async(tdp_tile *t, const tdp_tile_desc *d) {
+    if (!t || !d) return TDP_ERR_ARG;
+    tdp_ctx *ctx = t->ctx;
+    if (!t->cap_obj) {
+        ctx->err = "tdp_tile_fill_async: the tile was not made by tdp_tile_reserve";
+        return TDP_ERR_STATE;
+    }
+    if (d->n_obj > t->cap_obj || d->n_vox > t->cap_vox || d->n_tri > t->cap_tri) {
+        ctx->err = "tdp_tile_fill_async: sizes beyond the reserved capacity";
+        return TDP_ERR_ARG;
+    }
+    if (!d->obj_vox_off || !d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt || !d->obj_hd) {
+        ctx->err = "tdp_tile_fill_async: null array";
+        return TDP_ERR_ARG;
+    }
+    if (d->obj_vox_off[0] != 0 || d->obj_vox_off[d->n_obj] != d->n_vox) {
+        ctx->err = "tdp_tile_fill_async: obj_vox_off does not span the voxels";
+        return TDP_ERR_ARG;
+    }
+    for (size_t o = 0; o < d->n_obj; o++)
+        if (d->obj_vox_off[o + 1] < d->obj_vox_off[o]) {
+            ctx->err = "tdp_tile_fill_async: obj_vox_off decreases";
+            return TDP_ERR_ARG;
+        }
+    for (size_t v = 0; v < 5 * d->n_vox; v++)
+        if (d->vox_tri_off[v] + d->vox_tri_cnt[v] > d->n_tri) {
+            ctx->err = "tdp_tile_fill_async: voxel triangles outside the record array";
+            return TDP_ERR_ARG;
+        }
+    hipStream_t s = ctx->stream;
+    auto cp = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+    };
+    TDP_CHECK(ctx, cp(t->obj_vox_off, d->obj_vox_off, 4 * (d->n_obj + 1)));
+    TDP_CHECK(ctx, cp(t->obj_hd, d->obj_hd, 40 * d->n_obj));
+    TDP_CHECK(ctx, cp(t->vox_box, d->vox_box, 36 * d->n_vox));
+    TDP_CHECK(ctx, cp(t->vox_tri_off, d->vox_tri_off, 40 * d->n_vox));
+    TDP_CHECK(ctx, cp(t->vox_tri_cnt, d->vox_tri_cnt, 20 * d->n_vox));
+    if (d->n_obj) {
+        k_vox_owner<<<blocks_for(d->n_obj, kBlock), kBlock, 0, s>>>(d->n_obj, t->obj_vox_off,
+                                                                     t->vox_obj);
+        TDP_LAUNCHED(ctx);
+    }
+    t->n_obj = d->n_obj;
+    t->n_vox = d->n_vox;
+    t->n_tri = d->n_tri;
+    if (d->tri) return tdp_tile_fill_tri_async(t, 0, d->n_tri, d->tri);
+    return TDP_OK;
+}
+
+extern "C" int tdp_tile_fill_tri_async(tdp_tile *t, size_t first, size_t n, const float *records) {
+    if (!t || (n && !records)) return TDP_ERR_ARG;
+    tdp_ctx *ctx = t->ctx;
+    if (!t->cap_obj || first > t->n_tri || n > t->n_tri - first) {
+        ctx->err = "tdp_tile_fill_tri_async: records outside the tile being filled";
+        return TDP_ERR_ARG;
+    }
+    if (!n) return TDP_OK;
+    hipStream_t s = ctx->stream;
+    TDP_CHECK(ctx, hipMemcpyAsync(t->tri + 3 * first, records, 48 * n, hipMemcpyHostToDevice, s));
+    k_tri_bounds<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, t->tri + 3 * first,
+                                                          t->bnd + kBnd * first);
+    TDP_LAUNCHED(ctx);
+    return TDP_OK;
+}
+
 extern "C" int tdp_tile_free(tdp_tile *t) {
     if (!t) return TDP_ERR_ARG;
-    dfree(t->obj_vox_off);
-    dfree(t->obj_hd);
-    dfree(t->vox_box);
-    dfree(t->vox_obj);
-    dfree(t->vox_tri_off);
-    dfree(t->vox_tri_cnt);
-    dfree(t->tri);
-    dfree(t->bnd);
-    dfree(t->obj_box);
-    dfree(t->obj_tr);
+    tdp_ctx *ctx = t->ctx;
+    for (void *p : {(void *)t->obj_vox_off, (void *)t->obj_hd, (void *)t->vox_box,
+                    (void *)t->vox_obj, (void *)t->vox_tri_off, (void *)t->vox_tri_cnt,
+                    (void *)t->tri, (void *)t->bnd, (void *)t->obj_box, (void *)t->obj_tr})
+        dfree(ctx, p);
     delete t;
     return TDP_OK;
 }
@@ -552,9 +726,9 @@ extern "C" int tdp_tridist_batch(tdp_ctx *ctx, int precision, const float *S, co
             rc = TDP_ERR_HIP;
         }
     }
-    dfree(dS);
-    dfree(dT);
-    dfree(dO);
+    dfree(ctx, dS);
+    dfree(ctx, dT);
+    dfree(ctx, dO);
     return rc;
 }
 
@@ -611,11 +785,11 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
             rc = TDP_ERR_HIP;
         }
     }
-    dfree(dR);
-    dfree(dB);
-    dfree(dOS);
-    dfree(dO);
-    dfree(dst);
+    dfree(ctx, dR);
+    dfree(ctx, dB);
+    dfree(ctx, dOS);
+    dfree(ctx, dO);
+    dfree(ctx, dst);
     return rc;
 }
 
@@ -650,7 +824,7 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
             rc = TDP_ERR_HIP;
         }
     }
-    dfree(dp); dfree(dc); dfree(ds); dfree(dmn); dfree(dmx);
+    dfree(ctx, dp); dfree(ctx, dc); dfree(ctx, ds); dfree(ctx, dmn); dfree(ctx, dmx);
     return rc;
 }
 

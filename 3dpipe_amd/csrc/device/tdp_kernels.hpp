@@ -373,6 +373,15 @@ constexpr uint32_t kMixedMinPairs = 2 * kWave;  // below this, evaluate every pa
 // to |n| stays below 1e-9 under the 1e-6 aspect cut-off).
 constexpr int kBnd = 3;
 
+// owning object of every voxel, from the per-object voxel offsets
+__global__ void __launch_bounds__(kBlock) k_vox_owner(size_t n_obj,
+                                                      const uint32_t *__restrict__ obj_vox_off,
+                                                      uint32_t *__restrict__ vox_obj) {
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_obj) return;
+    for (uint32_t v = obj_vox_off[o]; v < obj_vox_off[o + 1]; v++) vox_obj[v] = (uint32_t)o;
+}
+
 __global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *__restrict__ rec,
                                                        float4 *__restrict__ bnd) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

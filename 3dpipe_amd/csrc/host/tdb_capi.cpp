@@ -17,12 +17,6 @@
 
 #include "tdb_internal.hpp"
 
-struct tdb_engine {
-    tdp_ctx *ctx = nullptr;
-    std::string err;
-    std::unordered_map<const tdb_tile *, tdp_tile *> resident;
-};
-
 // Device copies are keyed by tile; a tile that is freed drops its copies in
 // every engine that holds one, and a closing engine unregisters itself, so a
 // new tile allocated at a recycled address never meets a stale device copy.
@@ -331,6 +325,7 @@ int tdb_generate(const tdb_gen_params *p, tdb_tile **nuclei, tdb_tile **vessels)
 int tdb_engine_open(int device, tdb_engine **out) {
     if (!out) return -1;
     tdb_engine *e = new tdb_engine();
+    e->device = device;
     int rc = tdp_open(device, &e->ctx);
     if (rc != TDP_OK) {
         e->err = e->ctx ? tdp_last_error(e->ctx) : "tdp_open failed";

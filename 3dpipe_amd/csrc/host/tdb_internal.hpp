@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../../include/tdbase.h"
@@ -28,6 +29,13 @@ struct tdb_tile {
     }
 };
 
+struct tdb_engine {
+    int device = 0;
+    tdp_ctx *ctx = nullptr;
+    std::string err;
+    std::unordered_map<const tdb_tile *, tdp_tile *> resident;
+};
+
 struct tdb_result {
     uint64_t n_targets = 0;
     std::vector<int32_t> target, obj2;
@@ -37,6 +45,7 @@ struct tdb_result {
     double stage_ms[TDP_STAGE_COUNT] = {};
     uint64_t stage_n[TDP_STAGE_COUNT] = {};
     uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
+    double stream[12] = {};  // streaming figures (tdb_result_stream)
 };
 
 using Clock = std::chrono::steady_clock;

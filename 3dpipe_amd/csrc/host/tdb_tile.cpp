@@ -252,7 +252,22 @@ struct Voxelized {
     std::vector<int> core_face;              // per voxel: LOD-100 face whose barycentre is the core
     std::vector<int> face_vox[kLods];        // per slot, per face: voxel
     std::vector<double> hd[kLods];           // per slot, per face: proxy, hausdorff
+    std::vector<int> order[kLods];           // per slot: faces by voxel, then face index
+    std::vector<uint32_t> vcnt[kLods];       // per slot, per voxel: faces
 };
+
+// the record order of an object: slot -> voxel -> face
+void set_order(Voxelized &vz) {
+    for (int s = 0; s < kLods; s++) {
+        const size_t nf = vz.face_vox[s].size();
+        vz.vcnt[s].assign(vz.nvox, 0);
+        for (size_t i = 0; i < nf; i++) vz.vcnt[s][vz.face_vox[s][i]]++;
+        std::vector<uint32_t> at(vz.nvox + 1, 0);
+        for (int v = 0; v < vz.nvox; v++) at[v + 1] = at[v] + vz.vcnt[s][v];
+        vz.order[s].assign(nf, 0);
+        for (size_t i = 0; i < nf; i++) vz.order[s][at[vz.face_vox[s][i]]++] = (int)i;
+    }
+}
 
 V3 bary(const LodMesh &m, const std::array<int, 3> &f) {
     return (m.v[f[0]] + m.v[f[1]] + m.v[f[2]]) * (1.0 / 3.0);
@@ -427,6 +442,7 @@ Voxelized voxelize(const LodMesh &m, int voxel_tris) {
             vz.hd[s][2 * i + 1] = h;
         }
     }
+    set_order(vz);
     return vz;
 }
 
@@ -451,63 +467,33 @@ size_t total_faces(const LodMesh &m) {
     return n;
 }
 
-// Build the object's records from float vertices and a voxel partition;
-// bounds in `vz` are multiplied by `scale` and widened by a margin that covers
-// rounding the vertices to float.
-void emit_object(const LodMesh &m, const std::vector<float> &vf, const Voxelized &vz,
-                 double scale, const ObjSink &out) {
+// Voxel boxes from the LOD-100 faces, cores from the core face's vertices,
+// and the max-distance proxy point: the LOD-100 vertex nearest the box centre.
+void emit_boxes(const LodMesh &m, const std::vector<float> &vf, const Voxelized &vz,
+                float *vox_box, float *transform) {
     const int nv = vz.nvox;
-    float maxc = 0;
-    for (float c : vf) maxc = std::max(maxc, std::fabs(c));
-    const double margin = 8.0 * ulpf(maxc) + 1e-6 * scale;
     auto P = [&](int i, int d) { return vf[3 * i + d]; };
-    // voxel boxes from LOD-100 faces, cores from the core face's vertices
     for (int v = 0; v < nv; v++)
         for (int d = 0; d < 3; d++) {
-            out.vox_box[9 * v + d] = INFINITY;
-            out.vox_box[9 * v + 3 + d] = -INFINITY;
+            vox_box[9 * v + d] = INFINITY;
+            vox_box[9 * v + 3 + d] = -INFINITY;
         }
     for (size_t i = 0; i < m.f[4].size(); i++) {
         int v = vz.face_vox[4][i];
         for (int k = 0; k < 3; k++)
             for (int d = 0; d < 3; d++) {
                 float c = P(m.f[4][i][k], d);
-                out.vox_box[9 * v + d] = std::min(out.vox_box[9 * v + d], c);
-                out.vox_box[9 * v + 3 + d] = std::max(out.vox_box[9 * v + 3 + d], c);
+                vox_box[9 * v + d] = std::min(vox_box[9 * v + d], c);
+                vox_box[9 * v + 3 + d] = std::max(vox_box[9 * v + 3 + d], c);
             }
     }
     for (int v = 0; v < nv; v++) {
         const auto &cf = m.f[4][vz.core_face[v]];
         for (int d = 0; d < 3; d++) {
             double s = ((double)P(cf[0], d) + (double)P(cf[1], d) + (double)P(cf[2], d)) / 3.0;
-            out.vox_box[9 * v + 6 + d] = (float)s;
+            vox_box[9 * v + 6 + d] = (float)s;
         }
     }
-    size_t w = 0;
-    for (int s = 0; s < kLods; s++) {
-        out.hd[2 * s] = 0;
-        out.hd[2 * s + 1] = 0;
-        std::vector<std::vector<int>> by(nv);
-        for (size_t i = 0; i < m.f[s].size(); i++) by[vz.face_vox[s][i]].push_back((int)i);
-        for (int v = 0; v < nv; v++) {
-            out.cnt[5 * v + s] = (uint32_t)by[v].size();
-            for (int i : by[v]) {
-                const auto &f = m.f[s][i];
-                float ph = 0, h = 0;
-                if (s < 4) {
-                    ph = (float)(vz.hd[s][2 * i] * scale + margin);
-                    h = (float)(vz.hd[s][2 * i + 1] * scale + margin);
-                }
-                float r[12] = {P(f[0], 0), P(f[0], 1), P(f[0], 2), ph, P(f[1], 0), P(f[1], 1),
-                               P(f[1], 2), h,          P(f[2], 0), P(f[2], 1), P(f[2], 2), 0.f};
-                std::memcpy(out.rec + 12 * w, r, sizeof r);
-                w++;
-                out.hd[2 * s] = std::max(out.hd[2 * s], h);           // object hausdorff
-                out.hd[2 * s + 1] = std::max(out.hd[2 * s + 1], ph);  // object proxy
-            }
-        }
-    }
-    // max-distance proxy point: the LOD-100 vertex nearest the box centre
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     std::vector<uint8_t> used(vf.size() / 3, 0);
     for (auto &f : m.f[4])
@@ -527,7 +513,45 @@ void emit_object(const LodMesh &m, const std::vector<float> &vf, const Voxelized
         float d2 = dx * dx + dy * dy + dz * dz;
         if (d2 < best) { best = d2; bi = (int)i; }
     }
-    for (int d = 0; d < 3; d++) out.transform[d] = P(bi, d);
+    for (int d = 0; d < 3; d++) transform[d] = P(bi, d);
+}
+
+// Triangle records (slot -> voxel -> face), per-voxel counts and the object
+// Hausdorff pairs; bounds in `vz` are multiplied by `scale` and widened by a
+// margin that covers rounding the vertices to float.
+void emit_records(const LodMesh &m, const std::vector<float> &vf, const Voxelized &vz,
+                  double scale, uint32_t *cnt, float *rec, float *hd) {
+    float maxc = 0;
+    for (float c : vf) maxc = std::max(maxc, std::fabs(c));
+    const double margin = 8.0 * ulpf(maxc) + 1e-6 * scale;
+    auto P = [&](int i, int d) { return vf[3 * i + d]; };
+    size_t w = 0;
+    for (int s = 0; s < kLods; s++) {
+        hd[2 * s] = 0;
+        hd[2 * s + 1] = 0;
+        for (int v = 0; v < vz.nvox; v++) cnt[5 * v + s] = vz.vcnt[s][v];
+        for (int i : vz.order[s]) {
+            const auto &f = m.f[s][i];
+            float ph = 0, h = 0;
+            if (s < 4) {
+                ph = (float)(vz.hd[s][2 * i] * scale + margin);
+                h = (float)(vz.hd[s][2 * i + 1] * scale + margin);
+            }
+            float r[12] = {P(f[0], 0), P(f[0], 1), P(f[0], 2), ph, P(f[1], 0), P(f[1], 1),
+                           P(f[1], 2), h,          P(f[2], 0), P(f[2], 1), P(f[2], 2), 0.f};
+            std::memcpy(rec + 12 * w, r, sizeof r);
+            w++;
+            hd[2 * s] = std::max(hd[2 * s], h);           // object hausdorff
+            hd[2 * s + 1] = std::max(hd[2 * s + 1], ph);  // object proxy
+        }
+    }
+}
+
+// Build the object's arrays from float vertices and a voxel partition.
+void emit_object(const LodMesh &m, const std::vector<float> &vf, const Voxelized &vz,
+                 double scale, const ObjSink &out) {
+    emit_boxes(m, vf, vz, out.vox_box, out.transform);
+    emit_records(m, vf, vz, scale, out.cnt, out.rec, out.hd);
 }
 
 // Per-object layout inside a tile: voxel and triangle counts are known
@@ -753,6 +777,79 @@ void random_rotation(Rng &r, double R[9]) {
     R[6] = 2 * (x * z - y * w);     R[7] = 2 * (y * z + x * w);     R[8] = 1 - 2 * (x * x + y * y);
 }
 
+// nucleus instances: placement, shape, size and rotation, drawn per nucleus
+struct NucleusInst {
+    V3 center;
+    int proto;
+    double radius;
+    double R[9];
+};
+
+struct NucleusBank {
+    std::vector<LodMesh> protos;
+    std::vector<Voxelized> pvz;
+    std::vector<NucleusInst> inst;
+};
+
+size_t nucleus_count(const GenParams &p) {
+    size_t nn = p.n_vessels > 0 ? (size_t)p.n_vessels * p.nuclei_per_vessel
+                                : (size_t)p.nuclei_per_vessel;
+    if (p.max_nuclei > 0 && (size_t)p.max_nuclei < nn) nn = (size_t)p.max_nuclei;
+    return nn;
+}
+
+// prototypes and instances (vs: the vessels' centrelines, when nuclei follow vessels)
+void build_nuclei(const GenParams &p, const std::vector<Vessel> &vs, NucleusBank &b) {
+    IcoTopo topo = make_ico(p.subdiv);
+    const int np = std::max(1, p.prototypes);
+    b.protos.assign(np, LodMesh());
+    b.pvz.assign(np, Voxelized());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < np; k++) {
+        Rng r(p.seed, 2000000000ULL + (uint64_t)k);
+        b.protos[k] = make_nucleus_proto(topo, p.noise, r);
+        b.pvz[k] = voxelize(b.protos[k], p.voxel_tris);
+    }
+    const size_t nn = nucleus_count(p);
+    b.inst.assign(nn, NucleusInst());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)nn; i++) {
+        Rng r(p.nuclei_seed ? p.nuclei_seed : p.seed, 3000000000ULL + (uint64_t)i);
+        NucleusInst &in = b.inst[i];
+        if (p.n_vessels > 0) {
+            const Vessel &V = vs[i / p.nuclei_per_vessel];
+            int segs = (int)V.pts.size() - 1;
+            double a = r.uni(0, segs);
+            int s = std::min((int)a, segs - 1);
+            V3 t = unit(V.pts[s + 1] - V.pts[s]);
+            V3 c = V.pts[s] + (V.pts[s + 1] - V.pts[s]) * (a - s);
+            V3 w = random_unit(r);
+            w = w - t * dot(w, t);
+            if (norm(w) < 1e-9) w = any_perp(t);
+            in.center = c + unit(w) * r.uni(p.place_min, p.place_max);
+        } else {
+            in.center = V3{r.uni(0, p.cube), r.uni(0, p.cube), r.uni(0, p.cube)};
+        }
+        in.proto = r.below(np);
+        in.radius = r.uni(p.nucleus_r_min, p.nucleus_r_max);
+        random_rotation(r, in.R);
+    }
+}
+
+// float vertices of nucleus i
+void nucleus_vertices(const NucleusBank &b, size_t i, std::vector<float> &vf) {
+    const NucleusInst &in = b.inst[i];
+    const LodMesh &m = b.protos[in.proto];
+    vf.resize(3 * m.v.size());
+    for (size_t q = 0; q < m.v.size(); q++) {
+        V3 x = m.v[q] * in.radius;
+        const double *R = in.R;
+        vf[3 * q] = (float)(R[0] * x.x + R[1] * x.y + R[2] * x.z + in.center.x);
+        vf[3 * q + 1] = (float)(R[3] * x.x + R[4] * x.y + R[5] * x.z + in.center.y);
+        vf[3 * q + 2] = (float)(R[6] * x.x + R[7] * x.y + R[8] * x.z + in.center.z);
+    }
+}
+
 }  // namespace
 
 void generate(const GenParams &p, Tile &nuclei, Tile &vessels) {
@@ -788,74 +885,125 @@ void generate(const GenParams &p, Tile &nuclei, Tile &vessels) {
         finish_object(vessels, vtri, i);
     }
     vessels.finalize_boxes();
-    // nucleus prototypes
-    IcoTopo topo = make_ico(p.subdiv);
-    const int np = std::max(1, p.prototypes);
-    std::vector<LodMesh> protos(np);
-    std::vector<Voxelized> pvz(np);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int k = 0; k < np; k++) {
-        Rng r(p.seed, 2000000000ULL + (uint64_t)k);
-        protos[k] = make_nucleus_proto(topo, p.noise, r);
-        pvz[k] = voxelize(protos[k], p.voxel_tris);
-    }
-    // nucleus instances: draw placement, shape, size and rotation first
-    size_t nn = p.n_vessels > 0 ? (size_t)p.n_vessels * p.nuclei_per_vessel
-                                : (size_t)p.nuclei_per_vessel;
-    if (p.max_nuclei > 0 && (size_t)p.max_nuclei < nn) nn = (size_t)p.max_nuclei;
-    struct Inst {
-        V3 center;
-        int proto;
-        double radius;
-        double R[9];
-    };
-    std::vector<Inst> inst(nn);
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < (long)nn; i++) {
-        Rng r(p.nuclei_seed ? p.nuclei_seed : p.seed, 3000000000ULL + (uint64_t)i);
-        Inst &in = inst[i];
-        if (p.n_vessels > 0) {
-            const Vessel &V = vs[i / p.nuclei_per_vessel];
-            int segs = (int)V.pts.size() - 1;
-            double a = r.uni(0, segs);
-            int s = std::min((int)a, segs - 1);
-            V3 t = unit(V.pts[s + 1] - V.pts[s]);
-            V3 c = V.pts[s] + (V.pts[s + 1] - V.pts[s]) * (a - s);
-            V3 w = random_unit(r);
-            w = w - t * dot(w, t);
-            if (norm(w) < 1e-9) w = any_perp(t);
-            in.center = c + unit(w) * r.uni(p.place_min, p.place_max);
-        } else {
-            in.center = V3{r.uni(0, p.cube), r.uni(0, p.cube), r.uni(0, p.cube)};
-        }
-        in.proto = r.below(np);
-        in.radius = r.uni(p.nucleus_r_min, p.nucleus_r_max);
-        random_rotation(r, in.R);
-    }
+    // nuclei: instances of the prototype bank, written in place
+    NucleusBank bank;
+    build_nuclei(p, vs, bank);
+    const size_t nn = bank.inst.size();
     std::vector<ObjShape> nshape(nn);
-    for (size_t i = 0; i < nn; i++)
-        nshape[i] = {(uint32_t)pvz[inst[i].proto].nvox, total_faces(protos[inst[i].proto])};
+    for (size_t i = 0; i < nn; i++) {
+        const int k = bank.inst[i].proto;
+        nshape[i] = {(uint32_t)bank.pvz[k].nvox, total_faces(bank.protos[k])};
+    }
     std::vector<uint64_t> ntri;
     layout(nuclei, nshape, ntri);
 #pragma omp parallel for schedule(dynamic, 64)
     for (long i = 0; i < (long)nn; i++) {
-        const Inst &in = inst[i];
-        const LodMesh &m = protos[in.proto];
-        std::vector<float> vf(3 * m.v.size());
-        for (size_t q = 0; q < m.v.size(); q++) {
-            V3 x = m.v[q] * in.radius;
-            const double *R = in.R;
-            vf[3 * q] = (float)(R[0] * x.x + R[1] * x.y + R[2] * x.z + in.center.x);
-            vf[3 * q + 1] = (float)(R[3] * x.x + R[4] * x.y + R[5] * x.z + in.center.y);
-            vf[3 * q + 2] = (float)(R[6] * x.x + R[7] * x.y + R[8] * x.z + in.center.z);
-        }
-        emit_object(m, vf, pvz[in.proto], in.radius, sink_for(nuclei, ntri, i));
+        const int k = bank.inst[i].proto;
+        std::vector<float> vf;
+        nucleus_vertices(bank, i, vf);
+        emit_object(bank.protos[k], vf, bank.pvz[k], bank.inst[i].radius,
+                    sink_for(nuclei, ntri, i));
         finish_object(nuclei, ntri, i);
     }
     nuclei.finalize_boxes();
 #ifdef _OPENMP
     omp_set_num_threads(saved);
 #endif
+}
+
+// ------------------------------------------------------------ object sources
+
+namespace {
+
+// the generator's nuclei, decoded one object at a time
+class NucleusSource final : public ObjectSource {
+  public:
+    explicit NucleusSource(const GenParams &p) {
+        std::vector<Vessel> vs;
+        if (p.n_vessels > 0) {
+            vs.resize(p.n_vessels);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (long i = 0; i < (long)p.n_vessels; i++) {
+                Rng r(p.seed, 1000000000ULL + (uint64_t)i);
+                vs[i] = make_vessel(p, r, false);
+            }
+        }
+        build_nuclei(p, vs, b_);
+    }
+    size_t n_obj() const override { return b_.inst.size(); }
+    uint32_t n_vox(size_t o) const override { return (uint32_t)b_.pvz[b_.inst[o].proto].nvox; }
+    uint64_t n_tri(size_t o) const override { return total_faces(b_.protos[b_.inst[o].proto]); }
+    void meta(size_t o, float box[6], float tr[3]) const override {
+        const int k = b_.inst[o].proto;
+        std::vector<float> vf, vb(9 * (size_t)b_.pvz[k].nvox);
+        nucleus_vertices(b_, o, vf);
+        emit_boxes(b_.protos[k], vf, b_.pvz[k], vb.data(), tr);
+        for (int d = 0; d < 3; d++) {
+            box[d] = INFINITY;
+            box[3 + d] = -INFINITY;
+        }
+        for (int v = 0; v < b_.pvz[k].nvox; v++)
+            for (int d = 0; d < 3; d++) {
+                box[d] = std::min(box[d], vb[9 * v + d]);
+                box[3 + d] = std::max(box[3 + d], vb[9 * v + 3 + d]);
+            }
+    }
+    void decode(size_t o, const ObjArrays &out) const override {
+        const int k = b_.inst[o].proto;
+        std::vector<float> vf;
+        float tr[3];
+        nucleus_vertices(b_, o, vf);
+        emit_boxes(b_.protos[k], vf, b_.pvz[k], out.vox_box, tr);
+        emit_records(b_.protos[k], vf, b_.pvz[k], b_.inst[o].radius, out.cnt, out.rec, out.hd);
+    }
+
+  private:
+    NucleusBank b_;
+};
+
+// an in-memory tile's objects (records copied out in slot -> voxel order)
+class TileSource final : public ObjectSource {
+  public:
+    explicit TileSource(const Tile &t) : t_(t) {}
+    size_t n_obj() const override { return t_.n_obj; }
+    uint32_t n_vox(size_t o) const override { return t_.obj_vox_off[o + 1] - t_.obj_vox_off[o]; }
+    uint64_t n_tri(size_t o) const override {
+        uint64_t n = 0;
+        for (uint32_t v = t_.obj_vox_off[o]; v < t_.obj_vox_off[o + 1]; v++)
+            for (int s = 0; s < kLods; s++) n += t_.vox_tri_cnt[5 * v + s];
+        return n;
+    }
+    void meta(size_t o, float box[6], float tr[3]) const override {
+        std::memcpy(box, &t_.obj_box[6 * o], 6 * sizeof(float));
+        std::memcpy(tr, &t_.obj_transform[3 * o], 3 * sizeof(float));
+    }
+    void decode(size_t o, const ObjArrays &out) const override {
+        const uint32_t v0 = t_.obj_vox_off[o], nv = t_.obj_vox_off[o + 1] - v0;
+        std::memcpy(out.vox_box, &t_.vox_box[9 * (size_t)v0], 36 * (size_t)nv);
+        std::memcpy(out.cnt, &t_.vox_tri_cnt[5 * (size_t)v0], 20 * (size_t)nv);
+        std::memcpy(out.hd, &t_.obj_hd[10 * o], 40);
+        size_t w = 0;
+        for (int s = 0; s < kLods; s++)
+            for (uint32_t v = 0; v < nv; v++) {
+                const size_t n = t_.vox_tri_cnt[5 * (size_t)(v0 + v) + s];
+                std::memcpy(out.rec + 12 * w, &t_.tri[12 * t_.vox_tri_off[5 * (size_t)(v0 + v) + s]],
+                            48 * n);
+                w += n;
+            }
+    }
+
+  private:
+    const Tile &t_;
+};
+
+}  // namespace
+
+std::unique_ptr<ObjectSource> nucleus_source(const GenParams &p) {
+    return std::unique_ptr<ObjectSource>(new NucleusSource(p));
+}
+
+std::unique_ptr<ObjectSource> tile_source(const Tile &t) {
+    return std::unique_ptr<ObjectSource>(new TileSource(t));
 }
 
 }  // namespace tdb

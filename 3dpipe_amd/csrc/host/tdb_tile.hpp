@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -71,5 +72,33 @@ struct GenParams {
 
 // Nuclei around vessels (n_vessels > 0) or nuclei uniform in the cube.
 void generate(const GenParams &p, Tile &nuclei, Tile &vessels);
+
+// One object's arrays, as a tile holds them: 9 floats per voxel box, 5 counts
+// per voxel ([voxel][slot]), 10 Hausdorff values, and the triangle records in
+// slot -> voxel order.
+struct ObjArrays {
+    float *vox_box;
+    uint32_t *cnt;
+    float *hd;
+    float *rec;
+};
+
+// Objects produced on demand -- the streaming join's "decode" step: any
+// object's arrays, and its box and max-distance proxy point, without the
+// rest of the tile in memory. Calls may run concurrently.
+class ObjectSource {
+  public:
+    virtual ~ObjectSource() = default;
+    virtual size_t n_obj() const = 0;
+    virtual uint32_t n_vox(size_t o) const = 0;
+    virtual uint64_t n_tri(size_t o) const = 0;
+    virtual void meta(size_t o, float box[6], float transform[3]) const = 0;
+    virtual void decode(size_t o, const ObjArrays &out) const = 0;
+};
+
+// the generator's nuclei (equal, object for object, to generate()'s nuclei tile)
+std::unique_ptr<ObjectSource> nucleus_source(const GenParams &p);
+// the objects of an in-memory tile (which must outlive the source)
+std::unique_ptr<ObjectSource> tile_source(const Tile &t);
 
 }  // namespace tdb

@@ -61,6 +61,45 @@ def cpu_baseline(tiles_dir, n_targets_total, n_sub, query, knn, tau, budget_s=15
                       f"{dt:.1f} s, {st['tri_pairs']} triangle pairs"}
 
 
+def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn):
+    """Chunked streaming 10-NN self-join of n_nuclei nuclei (radii 6-14,
+    uniform in the cube) within a device budget: host decode into pinned
+    buffers, copies and joins overlapped (tdb_stream_join)."""
+    t0 = time.perf_counter()
+    src = tdp.Source.generate(seed=seed, n_vessels=0, nuclei_per_vessel=n_nuclei, cube=cube,
+                              nucleus_r_min=6.0, nucleus_r_max=14.0)
+    eng = tdp.Engine(0)
+    if tuning:
+        eng.set_tuning(**tuning)
+    budget = int(budget_gib * (1 << 30))
+    r = eng.stream_join(src, None, query="nn", knn=knn, precision="mixed", hbm_budget=budget)
+    wall = time.perf_counter() - t0
+    eng.close()
+    st = r.stream
+    h2d = st["h2d_bytes"]
+    log_fn(f"stream: {n_nuclei} nuclei, {st['chunks']:.0f} chunks, {wall:.1f} s, "
+           f"copy {h2d / 1e9:.1f} GB at {h2d / (st['copy_ms'] * 1e6):.1f} GB/s, "
+           f"overlap {st['overlap']:.2f}, peak {st['peak_device_bytes'] / 2**30:.1f} GiB")
+    return {
+        "workload": f"{knn}-NN self-join of {n_nuclei} nuclei (radii 6-14) in a {cube:g}^3 cube, "
+                    f"LODs 20-100, mixed precision, device budget {budget_gib:g} GiB",
+        "seed": seed, "wall_s": wall, "setup_s": wall - st["wall_ms"] / 1e3,
+        "chunks": int(st["chunks"]), "objects_decoded": int(st["objects_decoded"]),
+        "h2d_bytes": int(h2d),
+        "pcie_gbps": h2d / (st["copy_ms"] * 1e6) if st["copy_ms"] else None,
+        "end_to_end_gbps": h2d / (st["wall_ms"] * 1e6) if st["wall_ms"] else None,
+        "decode_gbps": h2d / (st["decode_ms"] * 1e6) if st["decode_ms"] else None,
+        "stage_ms": {"index": st["index_ms"], "candidates": st["candidates_ms"],
+                     "decode": st["decode_ms"], "copy": st["copy_ms"], "join": st["join_ms"],
+                     "pipeline_wall": st["wall_ms"]},
+        "overlap_fraction": st["overlap"],
+        "peak_device_bytes": int(st["peak_device_bytes"]),
+        "budget_bytes": budget, "chunk_splits": int(st["splits"]),
+        "triangle_pairs": r.stats["tri_pairs"], "results": int(len(r.target)),
+        "tri_pair_dist_per_s": r.stats["tri_pairs"] / wall,
+    }
+
+
 def serve_bench(tdp, vessels, gen_kw, n_queries, rate, seed, log_fn):
     """Config-5 style serving: `vessels` resident; n_queries fresh nucleus
     tiles of log-uniform size 100..50000 (seeded), each kNN with k in
@@ -164,6 +203,10 @@ def main():
     ap.add_argument("--tune", default="", help="kernel tuning, e.g. drain=0,max_rows=4")
     ap.add_argument("--autotune", type=int, default=4000,
                     help="autotune on this many targets before timing (0: off)")
+    ap.add_argument("--stream-nuclei", type=int, default=0,
+                    help="streaming self-join of this many nuclei after the join bench; 0: off")
+    ap.add_argument("--stream-cube", type=float, default=50000.0)
+    ap.add_argument("--stream-budget-gib", type=float, default=24.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the join shards by "
@@ -282,7 +325,12 @@ def main():
             out["cpu_baseline"] = cpu_baseline(d, nuc.n_obj, n_sub, args.query, args.knn,
                                                args.tau)
             out["cpu_baseline"]["vs_gpu"] = out["value"] / out["cpu_baseline"]["value"]
+    tuning_used = eng.tuning()
     eng.close()
+    if args.stream_nuclei > 0:
+        out["streaming"] = stream_bench(tdp, args.stream_nuclei, args.stream_cube,
+                                        args.stream_budget_gib, args.knn, args.seed,
+                                        tuning_used, log)
     if args.serve_queries > 0:
         del nuc, results, r
         out["serving"] = serve_bench(tdp, ves, gen_kw, args.serve_queries, args.serve_rate,

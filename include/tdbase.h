@@ -164,6 +164,53 @@ void tdb_result_stats(const tdb_result *r, uint64_t *out7);
 void tdb_result_stages(const tdb_result *r, double *ms, uint64_t *launches, int n);
 void tdb_result_free(tdb_result *r);
 
+/* ---- streaming join: object sets larger than the device budget ----
+ *
+ * A source produces any object's arrays on demand (the LOD decode of the
+ * reference, HiMesh_Wrapper::decode_to, himesh_wrapper.cpp:137-161, done per
+ * chunk instead of per tile). tdb_stream_join joins chunks of targets --
+ * taken in Morton order of their box centres, each with every object its
+ * candidate lists name -- within a device-memory budget, decoding chunk i+2
+ * on host threads into pinned buffers while chunk i+1 is copied to the device
+ * on one stream and chunk i is joined on another. Replaces the reference's
+ * chunked pipeline (nearest_neighbor_chunking / within_chunking,
+ * KNNJoin.cpp:1524-1811, WithinJoin.cpp:644-813, with the packing producer
+ * KNNJoin.cpp:1586-1617 and SpatialJoin.cpp:169-248). The result equals
+ * tdb_join's on the same objects. */
+typedef struct tdb_source tdb_source;
+/* the generator's nuclei (the vessel tile is not produced), object for object
+ * equal to tdb_generate's nuclei tile */
+int tdb_source_generate(const tdb_gen_params *p, tdb_source **out);
+/* the objects of a tile (which must outlive the source) */
+int tdb_source_tile(const tdb_tile *t, tdb_source **out);
+uint64_t tdb_source_objects(const tdb_source *s);
+/* objects[0..n) decoded into a new tile, in that order */
+int tdb_source_decode(const tdb_source *s, const uint64_t *objects, size_t n, tdb_tile **out);
+void tdb_source_free(tdb_source *s);
+
+typedef struct {
+    uint64_t hbm_budget;        /* device bytes the join may hold: both tile slots and the
+                                   join buffers (0 = 90% of the free memory) */
+    double tile_share;          /* share of the budget for the two tile slots, in (0, 1) */
+    uint64_t max_chunk_targets; /* most targets per chunk (0 = as many as fit a slot) */
+    int decode_threads;         /* host threads decoding (0 = all) */
+    int overlap;                /* 1: the decode, copy and join stages overlap; 0: in turn */
+} tdb_stream_opts;
+
+void tdb_stream_defaults(tdb_stream_opts *o);
+/* Join source 1's objects (targets: the first max_objects1) against source
+ * 2's (NULL = self-join). */
+int tdb_stream_join(tdb_engine *e, const tdb_source *s1, const tdb_source *s2,
+                    const tdb_join_opts *o, const tdb_stream_opts *so, tdb_result **out);
+/* figures of a streamed result: [0] chunks, [1] bytes copied to the device,
+ * [2] decode ms (host, summed over chunks), [3] copy ms (submit to done),
+ * [4] join ms, [5] pipeline wall ms, [6] overlap fraction ((sum of the three
+ * stages - wall) / (sum - slowest stage): 1 = the wall equals the slowest
+ * stage, 0 = no overlap), [7] peak device bytes held, [8] index ms (object
+ * boxes), [9] candidate ms, [10] objects decoded, [11] chunk joins split to
+ * fit the budget */
+void tdb_result_stream(const tdb_result *r, double *out12);
+
 /* ---- serving: queries against one resident tile, continuously batched ----
  *
  * No counterpart in the reference (which runs one join per process). A

@@ -84,6 +84,39 @@ int tdp_tile_upload_parts(tdp_ctx *ctx, const tdp_tile_desc *desc, const float *
 int tdp_tile_free(tdp_tile *tile);
 size_t tdp_tile_bytes(const tdp_tile *tile);
 
+/* ---- streaming: a device-memory cap, pinned staging, tiles refilled in place ----
+ *
+ * Replaces the reference's chunked streaming (the producer/consumer of
+ * KNNJoin.cpp:1586-1617 packing into pinned slots, SpatialJoin.cpp:169-248,
+ * and the per-chunk cudaMalloc + H2D + kernel + D2H of
+ * computing_gpu.cu:6540-6658 on refine_streams): a tile reserved once per
+ * pipeline slot is refilled for each chunk from pinned host buffers by
+ * copies queued on the filling context's stream, while joins run on another
+ * context. */
+/* Cap on the device bytes ctx holds (tiles it uploaded or reserved, join
+ * buffers, caches); an allocation beyond it first releases the cache, then
+ * fails with TDP_ERR_NOMEM. 0 = no cap (the default). */
+int tdp_set_memory_limit(tdp_ctx *ctx, uint64_t bytes);
+/* bytes held now and the most held (since open, or the last reset) */
+int tdp_memory_usage(tdp_ctx *ctx, uint64_t *used, uint64_t *peak, int reset_peak);
+/* pinned host memory for the asynchronous fills */
+int tdp_host_alloc(tdp_ctx *ctx, size_t bytes, void **out);
+int tdp_host_free(tdp_ctx *ctx, void *p);
+/* wait until the work queued on ctx's stream (fills) has finished */
+int tdp_sync(tdp_ctx *ctx);
+/* an empty device tile with room for n_obj objects, n_vox voxels and n_tri
+ * triangle records */
+int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t n_tri, tdp_tile **out);
+/* Refill a reserved tile (sizes within its room). The arrays of desc must be
+ * pinned (tdp_host_alloc) and stay untouched until tdp_sync: the copies, the
+ * voxel owners and the triangle bounds are queued on the stream of the
+ * context that reserved the tile and the call returns at once. desc->tri may
+ * be NULL: the records then come in pieces through tdp_tile_fill_tri_async.
+ * obj_box / obj_transform are not kept (no device candidate queries). */
+int tdp_tile_fill_async(tdp_tile *tile, const tdp_tile_desc *desc);
+/* records [first, first + n) of the tile being filled, from pinned memory */
+int tdp_tile_fill_tri_async(tdp_tile *tile, size_t first, size_t n, const float *records);
+
 /* Triangle-triangle distance for n independent pairs; S and T are 9 floats
  * per triangle (host memory), out is n values (host) in the given precision:
  * REF32 -> float* (bit-identical to the reference's TriDist), FP64 and MIXED

@@ -266,6 +266,32 @@ def test_generator_nuclei_seed_keeps_vessels(tdp):
     assert v2.n_obj == 0 and n1.n_obj == 12
 
 
+@pytest.mark.parametrize("kw", [dict(seed=0, n_vessels=10, nuclei_per_vessel=20, cube=2000.0),
+                                dict(seed=3, n_vessels=0, nuclei_per_vessel=300, cube=5000.0,
+                                     nucleus_r_min=6.0, nucleus_r_max=14.0)])
+def test_source_decodes_the_generated_objects(tdp, kw):
+    """the streaming join's decode: every object of a generator source, in
+    any order, byte for byte the generated tile's object; a tile source gives
+    the tile's own objects back"""
+    n, _ = tdp.generate(**kw)
+    src = tdp.Source.generate(**kw)
+    assert src.n_obj == n.n_obj
+    a = n.arrays()
+    b = src.decode(np.arange(n.n_obj)).arrays()
+    assert all(np.array_equal(a[k], b[k]) for k in a)
+    perm = np.random.default_rng(1).permutation(n.n_obj)[: n.n_obj // 3]
+    d = src.decode(perm).arrays()
+    for k, w in (("obj_box", 6), ("obj_transform", 3), ("obj_hd", 10)):
+        assert np.array_equal(a[k].reshape(-1, w)[perm], d[k].reshape(-1, w))
+    off = a["obj_vox_off"]
+    vb = np.concatenate([a["vox_box"].reshape(-1, 9)[off[o]:off[o + 1]] for o in perm])
+    assert np.array_equal(vb, d["vox_box"].reshape(-1, 9))
+    t = tdp.Source.of_tile(n).decode(perm).arrays()
+    assert all(np.array_equal(t[k], d[k]) for k in d)
+    with pytest.raises(tdp.TdpError):
+        src.decode([n.n_obj])
+
+
 # ------------------------------------------------------- RAW format, generator
 
 def test_raw_roundtrip_and_oracle_reads_same_bytes(tdp, small_tiles, tmp_path):
