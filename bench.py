@@ -30,6 +30,15 @@ F_TRI = 997
 # candidate comparison; the breakdown is beside disc_interval()).
 F_MARK = 89
 PEAK_FP32_TFLOPS = 157.3
+# HBM bytes per launch on the default workload (seed 1, 200k nuclei x 1000
+# vessels, 10-NN, mixed form, default tuning), from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs of this script, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the
+# factors at which the counters read a plain 1 GiB device copy,
+# --calibrate-copy 1024, as 1 GiB): one refinement round, and k_mx_mark.
+ROUND_TRAFFIC_BYTES = 26.19e9
+MARK_TRAFFIC_BYTES = 11.09e9
+DEFAULT_TUNING = {"small_pairs": 8, "max_rows": 64, "drain": 0, "drain_blocks": 0,
+                  "eval_blocks": 4096, "graph": 0}
 REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",
                  "graph"]
 
@@ -203,8 +212,13 @@ def main():
     ap.add_argument("--tune", default="", help="kernel tuning, e.g. drain=0,max_rows=4")
     ap.add_argument("--autotune", type=int, default=4000,
                     help="autotune on this many targets before timing (0: off)")
-    ap.add_argument("--stream-nuclei", type=int, default=0,
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a plain device copy of this many MiB three times (the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--stream-nuclei", type=int, default=2000000,
                     help="streaming self-join of this many nuclei after the join bench; 0: off")
+    ap.add_argument("--only-stream", type=int, default=0,
+                    help="run only the streaming self-join and print its figures")
     ap.add_argument("--stream-cube", type=float, default=50000.0)
     ap.add_argument("--stream-budget-gib", type=float, default=24.0)
     args = ap.parse_args()
@@ -214,10 +228,27 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy > 0:
+        import torch
+        n = args.calibrate_copy << 20
+        x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}),
+              flush=True)
+        return
+
     import importlib
     import tempfile
     import numpy as np
     tdp = importlib.import_module("3dpipe_amd")
+    if args.only_stream:
+        print(json.dumps(stream_bench(tdp, args.stream_nuclei, args.stream_cube,
+                                      args.stream_budget_gib, args.knn, args.seed, None, log)),
+              flush=True)
+        return
 
     t0 = time.perf_counter()
     gen_kw = dict(seed=args.seed, n_vessels=args.vessels, nuclei_per_vessel=args.nuclei,
@@ -272,9 +303,15 @@ def main():
     dom_obj = {"name": dom_kernel, "stage": dom, "launches": dom_launches,
                "avg_launch_ms": stage_ms[dom] / dom_launches,
                "share_of_refine": stage_ms[dom] / refine_kernel_ms if refine_kernel_ms else None}
+    # the measured traffic applies to the workload it was measured on
+    default_workload = (args.seed == 1 and args.vessels == 1000 and args.nuclei == 200 and
+                        args.cube == 20000.0 and args.query == "nn" and args.knn == 10 and
+                        args.rings_min == args.rings_max == 12 and args.precision == "mixed" and
+                        eng.tuning() == DEFAULT_TUNING)
     if dom == "mark":
         a = F_MARK * tri_pairs / (stage_ms[dom] * 1e-3) / 1e12
-        dom_obj.update({"flop_per_tri_pair": F_MARK, "achieved": a, "frac": a / PEAK_FP32_TFLOPS})
+        dom_obj.update({"flop_per_tri_pair": F_MARK, "achieved": a, "frac": a / PEAK_FP32_TFLOPS,
+                        "traffic": MARK_TRAFFIC_BYTES if default_workload else None})
     out = {
         "metric": "10-NN join time (200k nuclei); tri-pair dist/s vs FP32 peak; p50/p99 query latency",
         "value": tri_pairs / (ms * 1e-3),
@@ -299,7 +336,8 @@ def main():
         "roofline": {"bound": "valu-fp32", "kernel": "refinement round (k_mx_* pipeline)"
                      if args.precision == "mixed" else "refinement round (k_refine)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": ROUND_TRAFFIC_BYTES if default_workload else None,
                      "launches": rounds, "avg_launch_ms": refine_kernel_ms / rounds,
                      "flop_per_tri_pair": F_TRI, "dominant_kernel": dom_obj,
                      "stage_ms": {k: v for k, v in stage_ms.items() if v > 0}},
