@@ -496,8 +496,14 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
     if (!e || !e->ctx || !t1 || !o) return -1;
     tdp_tuning base;
     if (tdp_get_tuning(e->ctx, &base) != TDP_OK) return -1;
-    std::vector<tdp_tuning> cand;
-    // FP64 stage: work list (0), persistent (1), persistent in a captured round (2)
+    // the engine's tuning first, then a fixed grid -- FP64 stage: work list
+    // (0), persistent (1), persistent in a captured round (2)
+    std::vector<tdp_tuning> cand{base};
+    auto same = [](const tdp_tuning &a, const tdp_tuning &b) {
+        return a.small_pairs == b.small_pairs && a.max_rows == b.max_rows && a.drain == b.drain &&
+               a.drain_blocks == b.drain_blocks && a.eval_blocks == b.eval_blocks &&
+               a.graph == b.graph;
+    };
     for (int form : {0, 1, 2})
         for (int rows : {64, 4})
             for (int small : {8, 64}) {
@@ -508,11 +514,11 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
                 t.small_pairs = small;
                 t.drain_blocks = form ? 8 * 256 : 0;
                 t.eval_blocks = 4096;
-                cand.push_back(t);
+                if (!same(t, base)) cand.push_back(t);
             }
     tdb_join_opts so = *o;
     if (sample && (so.max_objects1 == 0 || so.max_objects1 > sample)) so.max_objects1 = sample;
-    double best_ms = 1e300;
+    double best_ms = 1e300, base_ms = 1e300;
     tdp_tuning pick = base;
     int n = 0;
     for (const tdp_tuning &t : cand) {
@@ -531,12 +537,16 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
         }
         if (tried && n < cap) tried[n] = t;
         if (ms && n < cap) ms[n] = dev;
+        if (n == 0) base_ms = dev;
         n++;
         if (dev < best_ms) {
             best_ms = dev;
             pick = t;
         }
     }
+    // the engine's tuning stays unless another is clearly faster (timing
+    // noise on the sample is a few percent)
+    if (base_ms <= best_ms * 1.03) pick = base;
     tdp_set_tuning(e->ctx, &pick);
     if (best) *best = pick;
     return n;

@@ -124,7 +124,8 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
  * persistent, or persistent in a graph-captured round; rows per wave
  * iteration; the small-pair threshold -- timing filter and refinement on the
  * host clock (so host synchronisation counts), and keep the fastest on the
- * engine (results never depend on the tuning). tried/ms
+ * engine -- the engine's own tuning, tried first, stays unless another is
+ * more than 3% faster (results never depend on the tuning). tried/ms
  * (optional, room for cap entries): each candidate in the order tried and
  * its time. Returns the number of candidates tried, or < 0. */
 int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,

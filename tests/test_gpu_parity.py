@@ -356,10 +356,13 @@ def test_autotune_keeps_results(tdp, small_tiles):
     e = tdp.Engine(0)
     want = e.join(n, v, query="nn", knn=3, precision="fp64")
     out = e.autotune(n, v, query="nn", knn=3, sample_targets=100)
-    assert len(out["tried"]) == 12 and all(ms > 0 for _, ms in out["tried"])
+    assert 12 <= len(out["tried"]) <= 13 and all(ms > 0 for _, ms in out["tried"])
     assert e.tuning() == out["best"]
-    assert min(ms for _, ms in out["tried"]) == dict(
-        (tuple(sorted(t.items())), ms) for t, ms in out["tried"])[tuple(sorted(out["best"].items()))]
+    times = dict((tuple(sorted(t.items())), ms) for t, ms in out["tried"])
+    best_ms = times[tuple(sorted(out["best"].items()))]
+    fastest = min(ms for _, ms in out["tried"])
+    # the engine's own tuning (tried first) is kept unless beaten by more than 3%
+    assert best_ms == fastest or (out["best"] == out["tried"][0][0] and best_ms <= fastest * 1.03)
     got = e.join(n, v, query="nn", knn=3, precision="mixed")
     assert np.array_equal(got.target, want.target) and np.array_equal(got.obj2, want.obj2)
     assert bits_equal(got.rmin, want.rmin) and bits_equal(got.rmax, want.rmax)
