@@ -28,7 +28,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 PREC = {"ref32": 0, "fp64": 1, "mixed": 2}
-QUERY = {"nn": 0, "within": 1}
+QUERY = {"nn": 0, "within": 1, "intersect": 2}
 
 _c = ctypes
 _P = _c.c_void_p
@@ -355,7 +355,7 @@ def _opts(query, knn, within_dist, lods, hausdorff_level, max_objects1, precisio
     o = _JoinOpts()
     tdb.tdb_join_defaults(_c.byref(o))
     if query not in QUERY:
-        raise ValueError("query must be 'nn' or 'within'")
+        raise ValueError("query must be 'nn', 'within' or 'intersect'")
     if precision not in PREC:
         raise ValueError("precision must be ref32, fp64 or mixed")
     o.query = QUERY[query]

@@ -935,7 +935,8 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
                                      const float *tau_target, tdp_join **out) {
     if (!ctx || !t1 || !t2 || !prm || !out || (N && !prefix)) return TDP_ERR_ARG;
     *out = nullptr;
-    if (prm->query != TDP_QUERY_KNN && prm->query != TDP_QUERY_WITHIN) {
+    if (prm->query != TDP_QUERY_KNN && prm->query != TDP_QUERY_WITHIN &&
+        prm->query != TDP_QUERY_INTERSECT) {
         ctx->err = "tdp_join_create: bad query";
         return TDP_ERR_ARG;
     }
@@ -1118,16 +1119,22 @@ template <typename R>
 static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     tdp_ctx *ctx = j->ctx;
     const size_t P = j->P, N = j->N;
-    const bool within = j->prm.query == TDP_QUERY_WITHIN;
+    const int mode = j->prm.query == TDP_QUERY_WITHIN      ? FILTER_WITHIN
+                     : j->prm.query == TDP_QUERY_INTERSECT ? FILTER_INTERSECT
+                                                           : FILTER_KNN;
     R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
     hipStream_t s = ctx->stream;
     TDP_CHECK(ctx, hipEventRecord(j->ev[0], s));
-    if (P) {
+    if (P && mode == FILTER_INTERSECT) {
+        // no object-level ranges: every voxel pair whose boxes overlap
+        k_state_alive<<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, j->round);
+        TDP_LAUNCHED(ctx);
+    } else if (P) {
         k_aggregate<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
             j->t2->obj_vox_off, j->t2->vox_box, rmin, rmax);
         TDP_LAUNCHED(ctx);
-        if (!within) {
+        if (mode == FILTER_KNN) {
             if (N) {
                 k_filter_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
                     N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed,
@@ -1139,9 +1146,11 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
                 P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round);
             TDP_LAUNCHED(ctx);
         }
+    }
+    if (P) {
         k_survivor_count<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
-            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, j->tau_t, j->state, j->round, j->count);
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, mode, j->tau_t, j->state, j->round, j->count);
         TDP_LAUNCHED(ctx);
     }
     uint64_t total = 0;
@@ -1153,7 +1162,7 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     if (total) {
         k_survivor_write<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
-            j->t2->obj_vox_off, j->t2->vox_box, rmax, within, j->tau_t, j->state, j->offs, j->v1[0],
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, mode, j->tau_t, j->state, j->offs, j->v1[0],
             j->v2[0], j->vpp[0], (R *)j->vmin[0], (R *)j->vmax[0]);
         TDP_LAUNCHED(ctx);
     }
@@ -1178,12 +1187,42 @@ extern "C" int tdp_join_filter(tdp_join *j, uint64_t *pairs_out) {
     return j->rsz == 4 ? join_filter_t<float>(j, pairs_out) : join_filter_t<double>(j, pairs_out);
 }
 
+// the level the folds take: the Hausdorff level, with the intersection
+// join's rule or-ed in
+static int fold_level(const tdp_join *j) {
+    return j->prm.hausdorff_level | (j->prm.query == TDP_QUERY_INTERSECT ? kFoldIntersect : 0);
+}
+
+// quantities a mixed round folds: d (bit 0), lo (bit 1), hi (bit 2)
+static uint32_t fold_need(const tdp_join *j, int is_last) {
+    const int lv = j->prm.hausdorff_level;
+    if (j->prm.query == TDP_QUERY_INTERSECT) return 4u | (lv == 2 ? 2u : 0u) | (lv == 1 ? 1u : 0u);
+    return is_last || lv != 2 ? 1u : 6u;
+}
+
+// the per-round decisions of the join's query
+template <typename R>
+static void launch_decide(tdp_join *j, uint8_t rnd) {
+    hipStream_t s = j->ctx->stream;
+    const size_t P = j->P, N = j->N;
+    R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
+    if (j->prm.query == TDP_QUERY_KNN)
+        k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
+            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd);
+    else if (j->prm.query == TDP_QUERY_WITHIN)
+        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
+            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd);
+    else
+        k_decide_intersect<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
+            P, rmin, rmax, j->prm.hausdorff_level, j->state, j->round, rnd);
+}
+
 template <typename R, int MODE>
 static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
     tdp_ctx *ctx = j->ctx;
     hipStream_t s = ctx->stream;
     const size_t P = j->P, N = j->N;
-    const bool within = j->prm.query == TDP_QUERY_WITHIN;
+    const int evict = j->prm.query != TDP_QUERY_INTERSECT;
     R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
     using KT = typename Key<R>::type;
     int slot = -1;
@@ -1214,7 +1253,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         k_refine<R, MODE><<<wave_grid(n), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_off,
             A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, A->bnd, B->bnd,
-            A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last, j->prm.hausdorff_level,
+            A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last, fold_level(j),
             (KT *)j->acc_min, (KT *)j->acc_max, j->stats);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_DIRECT) != TDP_OK) return TDP_ERR_HIP;
@@ -1223,7 +1262,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         if (rc != TDP_OK) return rc;
         const dim3 gw = wave_grid(n);
         // quantities this round folds: d (bit 0), lo (bit 1), hi (bit 2)
-        const uint32_t need = is_last || j->prm.hausdorff_level != 2 ? 1u : 6u;
+        const uint32_t need = fold_need(j, is_last);
         k_mx_bounds<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
                                           A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
                                           B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
@@ -1285,7 +1324,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
             B->vox_tri_cnt, A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last,
-            j->prm.hausdorff_level, j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
+            fold_level(j), j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
             j->stats);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_FOLD) != TDP_OK) return TDP_ERR_HIP;
@@ -1294,16 +1333,10 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
     k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
                                                               (const KT *)j->acc_max, rmin, rmax);
     TDP_LAUNCHED(ctx);
-    if (!within) {
-        k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
-            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd);
-    } else {
-        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
-            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd);
-    }
+    launch_decide<R>(j, rnd);
     TDP_LAUNCHED(ctx);
     k_vp_keep<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->vpp[c], (const R *)j->vmin[c],
-                                                          j->state, rmax, j->keep);
+                                                          j->state, rmax, evict, j->keep);
     TDP_LAUNCHED(ctx);
     uint64_t kept = 0;
     int rc = scan_exclusive<uint32_t>(ctx, j->keep, n, j->pos, &kept);
@@ -1376,8 +1409,8 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
         int rc = plan_graph_rounds(j);
         if (rc != TDP_OK) return rc;
     }
-    const size_t P = j->P, N = j->N;
-    const bool within = j->prm.query == TDP_QUERY_WITHIN;
+    const size_t P = j->P;
+    const int evict = j->prm.query != TDP_QUERY_INTERSECT;
     R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
     int slot = -1;
     for (int k = 0; k < 5; k++)
@@ -1387,7 +1420,7 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     const uint64_t n = j->n_vp;
     const tdp_tile *A = j->t1, *B = j->t2;
     const dim3 gw = wave_grid(n);
-    const uint32_t need = is_last || j->prm.hausdorff_level != 2 ? 1u : 6u;
+    const uint32_t need = fold_need(j, is_last);
     const int nb = j->tune.drain_blocks ? j->tune.drain_blocks : 4 * ctx->cu_count;
 
     TDP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1429,21 +1462,16 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
         n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
         B->vox_tri_cnt, A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last,
-        j->prm.hausdorff_level, j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
+        fold_level(j), j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
         j->stats);
     launched();
     k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
                                                               (const KT *)j->acc_max, rmin, rmax);
     launched();
-    if (!within)
-        k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
-            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd);
-    else
-        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
-            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd);
+    launch_decide<R>(j, rnd);
     launched();
     k_vp_keep<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->vpp[c], (const R *)j->vmin[c],
-                                                          j->state, rmax, j->keep);
+                                                          j->state, rmax, evict, j->keep);
     launched();
     if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->keep, n, j->pos, j->d_kept);
     k_vp_scatter<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(

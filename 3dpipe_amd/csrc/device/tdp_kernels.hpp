@@ -13,6 +13,12 @@ constexpr int kBlock = 256;                 // 4 waves
 constexpr int kWavesPerBlock = kBlock / kWave;
 
 enum : uint8_t { ST_DEAD = 0, ST_ALIVE = 1, ST_REPORTED = 2 };
+
+// filter predicate of a join: kNN, within, intersect
+enum : int { FILTER_KNN = 0, FILTER_WITHIN = 1, FILTER_INTERSECT = 2 };
+// or-ed into the Hausdorff level given to the folds: the intersection join's
+// per-round rule instead of the kNN / within range fold
+constexpr int kFoldIntersect = 8;
 enum { MODE_REF32 = 0, MODE_FP64 = 1, MODE_MIXED = 2 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -196,10 +202,19 @@ __global__ void __launch_bounds__(kBlock) k_filter_within(size_t P, const R *__r
 }
 
 // voxel pairs kept for refinement: boxmin <= object max (kNN), and also
-// boxmin <= tau (within) (computing_gpu.cu:2604-2812)
+// boxmin <= tau (within) (computing_gpu.cu:2604-2812); for the intersection
+// join, voxel boxes that overlap with positive volume (aab::intersect,
+// himesh_box.cpp:95-99, as mbb_intersect uses it, IntersectionJoin.cpp:36)
+__device__ __forceinline__ bool boxes_overlap(const float *a, const float *b) {
+    return !(b[0] >= a[3] || b[3] <= a[0] || b[1] >= a[4] || b[4] <= a[1] || b[2] >= a[5] ||
+             b[5] <= a[2]);
+}
+
 template <typename R>
-__device__ __forceinline__ bool survives(R mn, R objmax, bool within, R tau) {
-    return within ? (mn <= objmax && mn <= tau) : (mn <= objmax);
+__device__ __forceinline__ bool survives(const float *a, const float *b, R mn, R objmax, int mode,
+                                         R tau) {
+    if (mode == FILTER_INTERSECT) return boxes_overlap(a, b);
+    return mode == FILTER_WITHIN ? (mn <= objmax && mn <= tau) : (mn <= objmax);
 }
 
 // per live pair: number of surviving voxel pairs; for within, a surviving
@@ -209,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_survivor_count(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, const R *__restrict__ rmax,
-    bool within, const float *__restrict__ tau_t, uint8_t *__restrict__ state,
+    int mode, const float *__restrict__ tau_t, uint8_t *__restrict__ state,
     uint8_t *__restrict__ round, uint32_t *__restrict__ count) {
     const size_t p = wave_global();
     if (p >= P) return;
@@ -221,15 +236,17 @@ __global__ void __launch_bounds__(kBlock) k_survivor_count(
     const uint32_t a0 = vo1[pair_target[p]], n1 = vo1[pair_target[p] + 1] - a0;
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
     const uint32_t total = n1 * n2;
-    const R objmax = rmax[p];
+    const bool within = mode == FILTER_WITHIN;
+    const R objmax = mode == FILTER_INTERSECT ? (R)0 : rmax[p];
     const R tau = within ? (R)tau_t[pair_target[p]] : (R)0;
     uint32_t cnt = 0;
     bool settled = false;
     for (uint32_t t = lane; t < total; t += kWave) {
         const uint32_t i = t / n2, j = t - i * n2;
         R mn, mx;
-        voxel_bound<R>(vb1 + 9 * (size_t)(a0 + i), vb2 + 9 * (size_t)(b0 + j), mn, mx);
-        if (survives<R>(mn, objmax, within, tau)) {
+        const float *va = vb1 + 9 * (size_t)(a0 + i), *vb = vb2 + 9 * (size_t)(b0 + j);
+        voxel_bound<R>(va, vb, mn, mx);
+        if (survives<R>(va, vb, mn, objmax, mode, tau)) {
             cnt++;
             if (within && mx <= tau) settled = true;
         }
@@ -253,7 +270,7 @@ __global__ void __launch_bounds__(kBlock) k_survivor_write(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, const R *__restrict__ rmax,
-    bool within, const float *__restrict__ tau_t, const uint8_t *__restrict__ state,
+    int mode, const float *__restrict__ tau_t, const uint8_t *__restrict__ state,
     const uint64_t *__restrict__ offs,
     uint32_t *__restrict__ vp_v1, uint32_t *__restrict__ vp_v2, uint32_t *__restrict__ vp_pair,
     R *__restrict__ vp_min, R *__restrict__ vp_max) {
@@ -264,8 +281,8 @@ __global__ void __launch_bounds__(kBlock) k_survivor_write(
     const uint32_t a0 = vo1[pair_target[p]], n1 = vo1[pair_target[p] + 1] - a0;
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
     const uint32_t total = n1 * n2;
-    const R objmax = rmax[p];
-    const R tau = within ? (R)tau_t[pair_target[p]] : (R)0;
+    const R objmax = mode == FILTER_INTERSECT ? (R)0 : rmax[p];
+    const R tau = mode == FILTER_WITHIN ? (R)tau_t[pair_target[p]] : (R)0;
     uint64_t pos = offs[p];
     for (uint32_t base = 0; base < total; base += kWave) {
         const uint32_t t = base + lane;
@@ -275,8 +292,9 @@ __global__ void __launch_bounds__(kBlock) k_survivor_write(
         if (t < total) {
             i = t / n2;
             j = t - i * n2;
-            voxel_bound<R>(vb1 + 9 * (size_t)(a0 + i), vb2 + 9 * (size_t)(b0 + j), mn, mx);
-            keep = survives<R>(mn, objmax, within, tau);
+            const float *va = vb1 + 9 * (size_t)(a0 + i), *vb = vb2 + 9 * (size_t)(b0 + j);
+            voxel_bound<R>(va, vb, mn, mx);
+            keep = survives<R>(va, vb, mn, objmax, mode, tau);
         }
         const uint64_t mask = __ballot(keep);
         if (keep) {
@@ -578,6 +596,62 @@ __device__ __forceinline__ void mixed_minima(const float4 *rec1, const float4 *b
     if (lo_zero) lo = 0.0;
 }
 
+// The range a voxel pair carries after a round, from this round's minima
+// (d, lo, hi) and its previous range:
+//   last LOD        -> [d, d]
+//   hausdorff lvl 2 -> [min(min, lo), min(max, hi)]
+//   hausdorff lvl 1 -> [min(min, d - H1 - H2), min(max, d + P1 + P2)] (object-level H, P)
+//   hausdorff lvl 0 -> [min, min(max, d)]
+// (KNNJoin.cpp:1663-1700, WithinJoin.cpp:596-632). With kFoldIntersect the
+// intersection join's per-round values instead (IntersectionJoin.cpp:100-115,
+// computing_gpu.cu:1441-1512): max = hi of this round (0 exactly when a
+// triangle pair at distance 0 has no Hausdorff slack: an intersection), min =
+// the value whose positivity rules the pair out -- lo (level 2),
+// d - P1 - P2 (level 1), +inf (level 0: never ruled out).
+// object-level Hausdorff (hd) and proxy (pr) values of a voxel pair's two
+// objects at a LOD slot, subtracted / added one at a time as the reference
+// does (KNNJoin.cpp:1681-1683, IntersectionJoin.cpp:104-105)
+template <typename R>
+struct ObjSlack {
+    R hd1 = 0, hd2 = 0, pr1 = 0, pr2 = 0;
+};
+
+template <typename R>
+__device__ __forceinline__ ObjSlack<R> obj_slack(const float *__restrict__ ohd1,
+                                                 const float *__restrict__ ohd2, uint32_t o1,
+                                                 uint32_t o2, int slot) {
+    ObjSlack<R> x;
+    x.hd1 = (R)ohd1[10 * (size_t)o1 + 2 * slot];
+    x.hd2 = (R)ohd2[10 * (size_t)o2 + 2 * slot];
+    x.pr1 = (R)ohd1[10 * (size_t)o1 + 2 * slot + 1];
+    x.pr2 = (R)ohd2[10 * (size_t)o2 + 2 * slot + 1];
+    return x;
+}
+
+template <typename R>
+__device__ __forceinline__ void fold_range(int is_last, int level, R d, R lo, R hi,
+                                           const ObjSlack<R> &x, R &nmin, R &nmax) {
+    if (level & kFoldIntersect) {
+        const int lv = level & 3;
+        nmax = hi;
+        nmin = lv == 2 ? lo : lv == 1 ? d - x.pr1 - x.pr2 : (R)__builtin_inff();
+        return;
+    }
+    if (is_last) {
+        nmin = d;
+        nmax = d;
+    } else if (level == 2) {
+        if (lo < nmin) nmin = lo;
+        if (hi < nmax) nmax = hi;
+    } else if (level == 1) {
+        const R a = d - x.hd1 - x.hd2, b = d + x.pr1 + x.pr2;
+        if (a < nmin) nmin = a;
+        if (b < nmax) nmax = b;
+    } else {
+        if (d < nmax) nmax = d;
+    }
+}
+
 // Per voxel pair: (d, lo, hi) = min over the n1 x n2 triangle pairs, folded
 // into the voxel pair's range and min-accumulated per object pair.
 // Replaces TriDist_cuda_op_shm_flat_fused (computing_gpu.cu:1069-1152) plus
@@ -615,7 +689,8 @@ __global__ void __launch_bounds__(kBlock) k_refine(
     R omin = vp_min[w], omax = vp_max[w];
     using KT = typename Key<R>::type;
     if (n1 == 0 || n2 == 0) {
-        if (lane == 0) atomicMin(&acc_min[p], Key<R>::enc(omin));
+        // an empty voxel keeps its range (the intersection join: no values)
+        if (lane == 0 && !(level & kFoldIntersect)) atomicMin(&acc_min[p], Key<R>::enc(omin));
         return;
     }
     const uint32_t total = n1 * n2;
@@ -633,22 +708,9 @@ __global__ void __launch_bounds__(kBlock) k_refine(
     hi = wave_min(hi);
     if (lane == 0) {
         R nmin = omin, nmax = omax;
-        if (is_last) {
-            nmin = d;
-            nmax = d;
-        } else if (level == 2) {
-            if (lo < nmin) nmin = lo;
-            if (hi < nmax) nmax = hi;
-        } else if (level == 1) {
-            const R a = d - (R)ohd1[10 * (size_t)vobj1[v1] + 2 * slot] -
-                        (R)ohd2[10 * (size_t)vobj2[v2] + 2 * slot];
-            const R b = d + (R)ohd1[10 * (size_t)vobj1[v1] + 2 * slot + 1] +
-                        (R)ohd2[10 * (size_t)vobj2[v2] + 2 * slot + 1];
-            if (a < nmin) nmin = a;
-            if (b < nmax) nmax = b;
-        } else {
-            if (d < nmax) nmax = d;
-        }
+        ObjSlack<R> x;
+        if ((level & 3) == 1) x = obj_slack<R>(ohd1, ohd2, vobj1[v1], vobj2[v2], slot);
+        fold_range<R>(is_last, level, d, lo, hi, x, nmin, nmax);
         vp_min[w] = nmin;
         vp_max[w] = nmax;
         atomicMin(&acc_min[p], Key<R>::enc(nmin));
@@ -742,6 +804,38 @@ __global__ void __launch_bounds__(kBlock) k_decide_knn(size_t N, const uint32_t 
     confirmed[t] = c;
 }
 
+// per-LOD intersection decision (IntersectionJoin.cpp:96-130): a voxel pair
+// with hi = 0 reports the object pair (its distance range [0, 0]); else, at
+// Hausdorff levels 1 and 2, the pair is ruled out when every voxel pair's
+// lower value is positive; otherwise it stays for the next LOD
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_decide_intersect(size_t P, R *__restrict__ rmin,
+                                                             R *__restrict__ rmax, int level,
+                                                             uint8_t *__restrict__ state,
+                                                             uint8_t *__restrict__ round,
+                                                             uint8_t rnd) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    if (state[p] != ST_ALIVE) return;
+    if (rmax[p] == (R)0) {
+        state[p] = ST_REPORTED;
+        round[p] = rnd;
+        rmin[p] = 0;
+        rmax[p] = 0;
+    } else if (level >= 1 && rmin[p] > (R)0) {
+        state[p] = ST_DEAD;
+    }
+}
+
+// every object pair of the intersection join enters the filter undecided
+__global__ void __launch_bounds__(kBlock) k_state_alive(size_t P, uint8_t *__restrict__ state,
+                                                        uint8_t *__restrict__ round) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    state[p] = ST_ALIVE;
+    round[p] = 255;
+}
+
 // per-LOD within decision (WithinJoin.cpp:769-792)
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_decide_within(size_t P, const R *__restrict__ rmin,
@@ -759,17 +853,18 @@ __global__ void __launch_bounds__(kBlock) k_decide_within(size_t P, const R *__r
 }
 
 // a voxel pair survives the round if its object pair is still refined and
-// it is not evicted (mindist > the object's min of maxima)
+// (evict) it is not evicted (mindist > the object's min of maxima); the
+// intersection join evicts no voxel pair of an undecided object pair
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_vp_keep(uint64_t n_vp, const uint32_t *__restrict__ vp_pair,
                                                     const R *__restrict__ vp_min,
                                                     const uint8_t *__restrict__ state,
-                                                    const R *__restrict__ rmax,
+                                                    const R *__restrict__ rmax, int evict,
                                                     uint32_t *__restrict__ keep) {
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n_vp) return;
     const uint32_t p = vp_pair[w];
-    keep[w] = (state[p] == ST_ALIVE && !(vp_min[w] > rmax[p])) ? 1u : 0u;
+    keep[w] = (state[p] == ST_ALIVE && !(evict && vp_min[w] > rmax[p])) ? 1u : 0u;
 }
 
 template <typename R>

@@ -793,27 +793,15 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
         seg = vp_pair[w];
         const R omin = vp_min[w], omax = vp_max[w];
         if (flags[w] & MX_EMPTY) {
-            kmin = Key<R>::enc(omin);
+            // an empty voxel keeps its range (the intersection join: no values)
+            if (!(level & kFoldIntersect)) kmin = Key<R>::enc(omin);
         } else {
             const R d = (R)dunkey(mkey[3 * w]), lo = (R)dunkey(mkey[3 * w + 1]),
                     hi = (R)dunkey(mkey[3 * w + 2]);
             R nmin = omin, nmax = omax;
-            if (is_last) {
-                nmin = d;
-                nmax = d;
-            } else if (level == 2) {
-                if (lo < nmin) nmin = lo;
-                if (hi < nmax) nmax = hi;
-            } else if (level == 1) {
-                const R a = d - (R)ohd1[10 * (size_t)vobj1[v1] + 2 * slot] -
-                            (R)ohd2[10 * (size_t)vobj2[v2] + 2 * slot];
-                const R b = d + (R)ohd1[10 * (size_t)vobj1[v1] + 2 * slot + 1] +
-                            (R)ohd2[10 * (size_t)vobj2[v2] + 2 * slot + 1];
-                if (a < nmin) nmin = a;
-                if (b < nmax) nmax = b;
-            } else {
-                if (d < nmax) nmax = d;
-            }
+            ObjSlack<R> x;
+            if ((level & 3) == 1) x = obj_slack<R>(ohd1, ohd2, vobj1[v1], vobj2[v2], slot);
+            fold_range<R>(is_last, level, d, lo, hi, x, nmin, nmax);
             vp_min[w] = nmin;
             vp_max[w] = nmax;
             kmin = Key<R>::enc(nmin);

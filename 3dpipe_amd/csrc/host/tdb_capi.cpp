@@ -35,7 +35,9 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     res->stats[4] = P;
 
     tdp_join_params prm;
-    prm.query = o->query == TDB_QUERY_NN ? TDP_QUERY_KNN : TDP_QUERY_WITHIN;
+    prm.query = o->query == TDB_QUERY_NN       ? TDP_QUERY_KNN
+                : o->query == TDB_QUERY_WITHIN ? TDP_QUERY_WITHIN
+                                               : TDP_QUERY_INTERSECT;
     prm.k = o->k;
     prm.within_dist = o->within_dist;
     prm.precision = o->precision;
@@ -447,6 +449,12 @@ void tdb_gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self, size_
         std::vector<std::pair<int, tdb::Range>> c;
         const float *box = &t1->t.obj_box[6 * i];
         const float *tr = &t1->t.obj_transform[3 * i];
+        if (o->query == TDB_QUERY_INTERSECT) {
+            std::vector<int> ids;
+            tree.intersect(box, self ? (int)i : -1, ids);
+            per[i].assign(ids.begin(), ids.end());
+            continue;
+        }
         if (o->query == TDB_QUERY_NN)
             tree.knn(box, tr, self ? (int)i : -1, k_t ? k_t[i] : o->k, c);
         else
@@ -560,8 +568,9 @@ int tdb_join(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2, const tdb_jo
         e->err = "engine has no device: " + e->err;
         return TDP_ERR_HIP;
     }
-    if (o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN) {
-        e->err = "query must be nn or within";
+    if (o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN &&
+        o->query != TDB_QUERY_INTERSECT) {
+        e->err = "query must be nn, within or intersect";
         return TDP_ERR_ARG;
     }
     if (o->query == TDB_QUERY_NN && o->k < 1) {

@@ -1,5 +1,7 @@
 #include "tdb_octree.hpp"
 
+#include <algorithm>
+
 #include <cmath>
 #include <functional>
 #include <limits>
@@ -193,6 +195,28 @@ void Octree::within_rec(const Node *n, const float *qbox, const float *qt, int s
 void Octree::within(const float *qbox, const float *qt, int self, float tau,
                     std::vector<std::pair<int, Range>> &out) const {
     within_rec(root_, qbox, qt, self, tau, out);
+}
+
+bool box_overlap(const float *a, const float *b) {
+    return !(b[0] >= a[3] || b[3] <= a[0] || b[1] >= a[4] || b[4] <= a[1] || b[2] >= a[5] ||
+             b[5] <= a[2]);
+}
+
+void Octree::intersect_rec(const Node *n, const float *qbox, int self, std::vector<int> &out) const {
+    if (!box_overlap(n->box, qbox)) return;
+    if (n->leaf) {
+        for (int o : n->objs)
+            if (o != self && box_overlap(boxes_ + 6 * o, qbox)) out.push_back(o);
+    } else {
+        for (const Node *c : n->child) intersect_rec(c, qbox, self, out);
+    }
+}
+
+void Octree::intersect(const float *qbox, int self, std::vector<int> &out) const {
+    out.clear();
+    if (root_) intersect_rec(root_, qbox, self, out);
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
 }
 
 }  // namespace tdb

@@ -35,6 +35,10 @@ public:
     // all objects whose box distance is <= tau, in depth-first order
     void within(const float *qbox, const float *qtransform, int self, float tau,
                 std::vector<std::pair<int, Range>> &out) const;
+    // all objects whose box overlaps qbox with positive volume, ascending
+    // (query_intersect, octree.cpp:332-349, then the sort and de-duplication
+    // of mbb_intersect, IntersectionJoin.cpp:20-33)
+    void intersect(const float *qbox, int self, std::vector<int> &out) const;
 
 private:
     struct Node;
@@ -44,10 +48,13 @@ private:
     void add(Node *n, int id);
     void within_rec(const Node *n, const float *qbox, const float *qt, int self, float tau,
                     std::vector<std::pair<int, Range>> &out) const;
+    void intersect_rec(const Node *n, const float *qbox, int self, std::vector<int> &out) const;
 };
 
 // aab::distance (src/spatial/himesh_box.cpp:149-168): a is the box the
 // method is called on, b its argument
 Range box_distance(const float *a, const float *b);
+// aab::intersect (src/spatial/himesh_box.cpp:95-99): overlap with positive volume
+bool box_overlap(const float *a, const float *b);
 
 }  // namespace tdb

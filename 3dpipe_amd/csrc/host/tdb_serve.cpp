@@ -318,7 +318,8 @@ int tdb_server_open(const tdb_server_opts *o, const tdb_tile *resident, tdb_serv
 
 int64_t tdb_server_submit(tdb_server *s, tdb_tile *query, const tdb_join_opts *o) {
     if (!s || !query || !o || !s->run_ctx) return TDP_ERR_ARG;
-    if ((o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN) ||
+    if ((o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN &&
+         o->query != TDB_QUERY_INTERSECT) ||
         (o->query == TDB_QUERY_NN && o->k < 1)) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = "tdb_server_submit: bad query options";

@@ -400,8 +400,9 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         e->err = "engine has no device: " + e->err;
         return TDP_ERR_HIP;
     }
-    if (o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN) {
-        e->err = "query must be nn or within";
+    if (o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN &&
+        o->query != TDB_QUERY_INTERSECT) {
+        e->err = "query must be nn, within or intersect";
         return TDP_ERR_ARG;
     }
     if (o->query == TDB_QUERY_NN && o->k < 1) {

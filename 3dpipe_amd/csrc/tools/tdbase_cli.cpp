@@ -120,7 +120,7 @@ int cmd_join(int argc, char **argv) {
     Parsed p = parse(argc, argv, specs);
     if (p.has("help")) {
         std::printf(
-            "tdbase join --tile1 PATH [--tile2 PATH] -q nn|within [-k K] [-w TAU]\n"
+            "tdbase join --tile1 PATH [--tile2 PATH] -q nn|within|intersect [-k K] [-w TAU]\n"
             "            [-l LOD]... [--hausdorff_level 0|1|2] [--max_objects1 N]\n"
             "            [--print_result] [--print_distance] [--precision ref32|fp64|mixed]\n"
             "            [--device D] [-t THREADS]\n");
@@ -132,17 +132,13 @@ int cmd_join(int argc, char **argv) {
         return 1;
     }
     const std::string q = p.str("query", "");
-    if (q == "intersect") {
-        std::fprintf(stderr, "query type intersect is not part of this build\n");
-        return 1;
-    }
-    if (q != "nn" && q != "within") {
+    if (q != "nn" && q != "within" && q != "intersect") {
         std::printf("error query type: %s\n", q.c_str());
         return 0;
     }
     tdb_join_opts o;
     tdb_join_defaults(&o);
-    o.query = q == "nn" ? TDB_QUERY_NN : TDB_QUERY_WITHIN;
+    o.query = q == "nn" ? TDB_QUERY_NN : q == "within" ? TDB_QUERY_WITHIN : TDB_QUERY_INTERSECT;
     o.k = to_int(p.str("knn", "1"), "--knn");
     o.within_dist = (float)to_double(p.str("within_dist", "1000"), "--within_dist");
     o.hausdorff_level = to_int(p.str("hausdorff_level", "2"), "--hausdorff_level");

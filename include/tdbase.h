@@ -97,10 +97,11 @@ int tdb_engine_set_tuning(tdb_engine *e, const tdp_tuning *t);
 
 #define TDB_QUERY_NN 0
 #define TDB_QUERY_WITHIN 1
+#define TDB_QUERY_INTERSECT 2  /* SpatialJoin::intersect (IntersectionJoin.cpp:68-161) */
 
 /* the `tdbase join` option set (query_context.h:141-192) */
 typedef struct {
-    int query;               /* TDB_QUERY_NN or TDB_QUERY_WITHIN */
+    int query;               /* TDB_QUERY_NN, TDB_QUERY_WITHIN or TDB_QUERY_INTERSECT */
     int k;                   /* --knn */
     float within_dist;       /* --within_dist */
     const int *lods;         /* --lods; NULL or n_lods == 0 -> 20 40 60 80 100 */

@@ -36,6 +36,7 @@ extern "C" {
 
 #define TDP_QUERY_KNN 0
 #define TDP_QUERY_WITHIN 1
+#define TDP_QUERY_INTERSECT 2  /* object pairs whose meshes touch (distance 0) */
 
 typedef struct tdp_ctx tdp_ctx;
 typedef struct tdp_tile tdp_tile;
@@ -181,8 +182,17 @@ int tdp_set_tuning(tdp_ctx *ctx, const tdp_tuning *t);
 
 /* ---- device-resident join (filter + LOD refinement) ---- */
 
+/* The intersection join (TDP_QUERY_INTERSECT) follows SpatialJoin::intersect
+ * (IntersectionJoin.cpp:68-161) with the per-voxel-pair values of TriInt_cuda
+ * (computing_gpu.cu:1441-1512): candidates are the objects whose boxes
+ * overlap the target's (sorted by index), voxel pairs those whose boxes
+ * overlap; in each LOD round an object pair is reported (range [0, 0]) when
+ * a voxel pair's min of d + h1 + h2 is 0, and, at Hausdorff levels 2 / 1,
+ * ruled out when every voxel pair's min of max(d - ph1 - ph2, 0) / d minus
+ * both objects' proxy Hausdorff is positive. Pairs left after the last LOD
+ * are not reported. */
 typedef struct {
-    int query;            /* TDP_QUERY_KNN or TDP_QUERY_WITHIN */
+    int query;            /* TDP_QUERY_KNN, TDP_QUERY_WITHIN or TDP_QUERY_INTERSECT */
     int k;                /* kNN */
     float within_dist;    /* within */
     int precision;        /* TDP_PREC_* */

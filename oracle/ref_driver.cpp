@@ -2,7 +2,8 @@
 // where they lie under the reference tree by oracle/Makefile into
 // oracle/_ref/libtdref.so. Test infrastructure only: it exists so the C
 // restatement in tdp_oracle.c can be checked against the reference itself
-// (TriDist/MeshDist: src/geometry/computing_cpu.cpp:403-481; aab::distance:
+// (TriDist/MeshDist: src/geometry/computing_cpu.cpp:403-481; MeshInt:
+// :482-523; TriInt: src/geometry/TriInt.cpp:400; aab::distance:
 // src/spatial/himesh_box.cpp:149-168; octree: src/index/octree.cpp:33-365).
 #include <cstdint>
 #include <vector>
@@ -33,6 +34,21 @@ void ref_meshdist(const float *tri, const float *hd, const uint64_t *os, size_t 
         out[3 * p] = r.distance;
         out[3 * p + 1] = r.min_dist;
         out[3 * p + 2] = r.max_dist;
+    }
+}
+
+void ref_triint(const float *S, const float *T, size_t n, uint8_t *out) {
+    for (size_t i = 0; i < n; i++) out[i] = TriInt(S + 9 * i, T + 9 * i) ? 1 : 0;
+}
+
+// MeshInt per voxel pair (os as for ref_meshdist): intersected, and the
+// first intersecting pair it found
+void ref_meshint(const float *tri, const float *hd, const uint64_t *os, size_t P, uint8_t *out) {
+    for (size_t p = 0; p < P; p++) {
+        const uint64_t *q = os + 4 * p;
+        result_container r = MeshInt(tri + 9 * q[0], tri + 9 * q[2], q[1], q[3],
+                                     hd ? hd + 2 * q[0] : nullptr, hd ? hd + 2 * q[2] : nullptr);
+        out[p] = r.intersected ? 1 : 0;
     }
 }
 
@@ -89,6 +105,19 @@ size_t ref_query_knn(void *h, const float *ql, const float *qh, const float *qt,
     float minmax = DBL_MAX;
     s->tree->query_knn_fast(qp, t, c, minmax, k);
     return emit(c, ids, mn, mx, cap);
+}
+
+// query_intersect ids in the order the octree returns them (duplicates kept)
+size_t ref_query_intersect(void *h, const float *ql, const float *qh, int self, int *ids,
+                           size_t cap) {
+    BoxSet *s = (BoxSet *)h;
+    weighted_aab q;
+    q.set_box(ql[0], ql[1], ql[2], qh[0], qh[1], qh[2]);
+    weighted_aab *qp = self >= 0 ? s->boxes[self] : &q;
+    std::vector<int> c;
+    s->tree->query_intersect(qp, c);
+    for (size_t i = 0; i < c.size() && i < cap; i++) ids[i] = c[i];
+    return c.size();
 }
 
 size_t ref_query_within(void *h, const float *ql, const float *qh, const float *qt, int self,

@@ -13,10 +13,11 @@
  *                 add_candidate                   src/index/octree.cpp:120-154
  *                 query_knn_fast                  src/index/octree.cpp:215-291
  *                 query_within                    src/index/octree.cpp:293-330
+ *                 query_intersect                 src/index/octree.cpp:332-349
  *                 build_octree (leaf 10)          src/storage/tile.cpp:76-85, tile_IO.cpp:47
  *   - the RAW tile reader                         src/storage/tile_IO.cpp:13-49,
  *                                                 src/spatial/himesh_wrapper.cpp:17-85,137-161
- *   - the kNN / within join                       (oracle_join.inc)
+ *   - the kNN / within / intersect join           (oracle_join.inc)
  *
  * Two arithmetic forms: REAL=float reproduces the reference's FP32 operation
  * order (build with -ffp-contract=off, IEEE division and sqrt), REAL=double is
@@ -421,6 +422,47 @@ static void octree_within(octnode *n, const orc_tile *t, const float *ql, const 
     }
 }
 
+/* query_intersect: objects whose boxes overlap the query box (strictly),
+ * collected depth-first; mbb_intersect (IntersectionJoin.cpp:20-33) then
+ * sorts the ids and skips duplicates */
+static void octree_intersect_rec(octnode *n, const orc_tile *t, const float *ql, const float *qh,
+                                 int self, int **ids, size_t *cnt, size_t *cap) {
+    if (!box_intersect(n->low, n->high, ql, qh)) return;
+    if (n->is_leaf) {
+        for (size_t i = 0; i < n->nobj; i++) {
+            int id = n->objs[i];
+            if (id == self) continue;
+            const orc_object *o = &t->obj[id];
+            if (!box_intersect(o->box_low, o->box_high, ql, qh)) continue;
+            if (*cnt == *cap) {
+                *cap = *cap ? 2 * *cap : 16;
+                *ids = (int *)realloc(*ids, *cap * sizeof(int));
+            }
+            (*ids)[(*cnt)++] = id;
+        }
+    } else {
+        for (int c = 0; c < 8; c++) octree_intersect_rec(n->child[c], t, ql, qh, self, ids, cnt, cap);
+    }
+}
+
+static int cmp_int(const void *a, const void *b) {
+    const int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* sorted, duplicates removed; returns the count (the caller frees *ids) */
+static size_t octree_intersect(octnode *root, const orc_tile *t, const float *ql, const float *qh,
+                               int self, int **ids) {
+    size_t cnt = 0, cap = 0;
+    *ids = NULL;
+    octree_intersect_rec(root, t, ql, qh, self, ids, &cnt, &cap);
+    if (cnt > 1) qsort(*ids, cnt, sizeof(int), cmp_int);
+    size_t w = 0;
+    for (size_t i = 0; i < cnt; i++)
+        if (w == 0 || (*ids)[w - 1] != (*ids)[i]) (*ids)[w++] = (*ids)[i];
+    return w;
+}
+
 static void tile_build_octree(orc_tile *t, int leaf) {
     t->tree = oct_new(t->space_low, t->space_high, 0, leaf);
     for (size_t i = 0; i < t->n; i++) oct_add(t->tree, t, (int)i);
@@ -632,6 +674,14 @@ size_t orc_query_within(const orc_tile *t, const float *ql, const float *qh, con
     cand_free(&c);
     return n;
 }
+size_t orc_query_intersect(const orc_tile *t, const float *ql, const float *qh, int self,
+                           int *ids, size_t cap) {
+    int *c = NULL;
+    size_t n = octree_intersect(t->tree, t, ql, qh, self, &c);
+    for (size_t i = 0; i < n && i < cap; i++) ids[i] = c[i];
+    free(c);
+    return n;
+}
 /* box / transform of object i of a loaded tile (6 + 3 floats) */
 void orc_tile_object_box(const orc_tile *t, size_t i, float *out9) {
     memcpy(out9, t->obj[i].box_low, 12);
@@ -657,16 +707,23 @@ int orc_lod_list(const int *in, int n, int *out) {
     return n;
 }
 
-/* precision: 0 = FP32 reference arithmetic, 1 = FP64 */
-orc_result *orc_join(const orc_tile *t1, const orc_tile *t2, int is_knn, int k, float tau,
+/* query: 1 = kNN, 0 = within, 2 = intersect;
+ * precision: 0 = FP32 reference arithmetic, 1 = FP64 */
+orc_result *orc_join(const orc_tile *t1, const orc_tile *t2, int query, int k, float tau,
                      const int *lods_in, int nlods, int level, int precision,
                      uint64_t max_objects1) {
-    if (!t1 || !t2 || (is_knn && k < 1)) return NULL;
+    const int is_knn = query == 1;
+    if (!t1 || !t2 || (is_knn && k < 1) || query < 0 || query > 2) return NULL;
     int *lods = (int *)malloc((nlods ? nlods : 1) * sizeof(int));
     orc_lod_list(lods_in, nlods, lods);
-    orc_result *r = precision == 0
-                        ? join_f(t1, t2, is_knn, k, tau, lods, nlods, level, (size_t)max_objects1)
-                        : join_d(t1, t2, is_knn, k, tau, lods, nlods, level, (size_t)max_objects1);
+    orc_result *r;
+    if (query == 2)
+        r = precision == 0 ? join_intersect_f(t1, t2, lods, nlods, level, (size_t)max_objects1)
+                           : join_intersect_d(t1, t2, lods, nlods, level, (size_t)max_objects1);
+    else
+        r = precision == 0
+                ? join_f(t1, t2, is_knn, k, tau, lods, nlods, level, (size_t)max_objects1)
+                : join_d(t1, t2, is_knn, k, tau, lods, nlods, level, (size_t)max_objects1);
     free(lods);
     return r;
 }

@@ -47,6 +47,8 @@ def lib():
             f.restype = SZ
         L.orc_query_within.argtypes = [P, P, P, P, c.c_int, c.c_float, P, P, P, SZ]
         L.orc_query_within.restype = SZ
+        L.orc_query_intersect.argtypes = [P, P, P, c.c_int, P, SZ]
+        L.orc_query_intersect.restype = SZ
         L.orc_lod_list.argtypes = [P, c.c_int, P]
         L.orc_join.argtypes = [P, P, c.c_int, c.c_int, c.c_float, P, c.c_int, c.c_int, c.c_int,
                                c.c_uint64]
@@ -81,6 +83,10 @@ def ref():
         L.ref_query_knn.restype = SZ
         L.ref_query_within.argtypes = [P, P, P, P, c.c_int, c.c_float, P, P, P, SZ]
         L.ref_query_within.restype = SZ
+        L.ref_query_intersect.argtypes = [P, P, P, c.c_int, P, SZ]
+        L.ref_query_intersect.restype = SZ
+        L.ref_triint.argtypes = [P, P, SZ, P]
+        L.ref_meshint.argtypes = [P, P, P, SZ, P]
         _ref = L
     return _ref
 
@@ -144,7 +150,8 @@ def join(t1, t2, query="nn", k=1, tau=1000.0, lods=(20, 40, 60, 80, 100), level=
     """Returns (target, obj2, min, max) arrays in report order and stats."""
     L = lib()
     lods = np.array(lods, np.int32)
-    r = L.orc_join(t1.h, (t2 or t1).h, 1 if query == "nn" else 0, int(k), float(tau), ptr(lods),
+    qcode = {"nn": 1, "within": 0, "intersect": 2}[query]
+    r = L.orc_join(t1.h, (t2 or t1).h, qcode, int(k), float(tau), ptr(lods),
                    len(lods), int(level), 0 if precision == "f32" else 1,
                    int(max_objects1 or 2**63 - 1))
     if not r:

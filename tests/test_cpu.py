@@ -90,7 +90,102 @@ def test_oracle_octree_matches_reference_build(seed):
                   orc.ptr(mn), orc.ptr(mx), cap)
             out.append((ids[:c].tolist(), mn[:c].tolist(), mx[:c].tolist()))
         assert out[0] == out[1], i
+        # query_intersect + mbb_intersect's sort and de-duplication; and the
+        # strict-overlap brute force it amounts to
+        ids = np.zeros(cap, np.int32)
+        c = orc.lib().orc_query_intersect(ot, orc.ptr(low[i]), orc.ptr(high[i]), i, orc.ptr(ids), cap)
+        mine = ids[:c].tolist()
+        c = ref.ref_query_intersect(rt, orc.ptr(low[i]), orc.ptr(high[i]), i, orc.ptr(ids), cap)
+        assert mine == sorted(set(ids[:c].tolist())), i
+        brute = [o for o in range(n) if o != i and
+                 not np.any((low[o] >= high[i]) | (high[o] <= low[i]))]
+        assert mine == brute, i
     ref.ref_boxset_free(rt)
+
+
+def _lod_voxel(a, v, slot):
+    """triangles (9 floats) and (proxy, hausdorff) of voxel v at a LOD slot"""
+    off, cnt = int(a["vox_tri_off"][5 * v + slot]), int(a["vox_tri_cnt"][5 * v + slot])
+    rec = a["tri"].reshape(-1, 12)[off:off + cnt]
+    return (np.ascontiguousarray(rec[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]]),
+            np.ascontiguousarray(rec[:, [3, 7]]))
+
+
+def _overlap(b1, b2):
+    return not np.any((b2[:3] >= b1[3:6]) | (b2[3:6] <= b1[:3]))
+
+
+def _intersect_bruteforce(ta, tb, self, ref, slot=4):
+    """object pairs with overlapping boxes holding a voxel pair (boxes
+    overlapping) that the reference's MeshInt finds intersecting at the LOD
+    slot; and the subset where the reference's TriInt finds a crossing pair"""
+    A, B = ta.arrays(), tb.arrays()
+    meshint, triint = set(), set()
+    for i in range(ta.n_obj):
+        for o in range(tb.n_obj):
+            if self and o == i:
+                continue
+            if not _overlap(A["obj_box"][6 * i:6 * i + 6], B["obj_box"][6 * o:6 * o + 6]):
+                continue
+            for v1 in range(A["obj_vox_off"][i], A["obj_vox_off"][i + 1]):
+                for v2 in range(B["obj_vox_off"][o], B["obj_vox_off"][o + 1]):
+                    if not _overlap(A["vox_box"][9 * v1:9 * v1 + 6], B["vox_box"][9 * v2:9 * v2 + 6]):
+                        continue
+                    t1, h1 = _lod_voxel(A, v1, slot)
+                    t2, h2 = _lod_voxel(B, v2, slot)
+                    if len(t1) == 0 or len(t2) == 0:
+                        continue
+                    tri = np.ascontiguousarray(np.concatenate([t1, t2]), np.float32)
+                    hd = np.ascontiguousarray(np.concatenate([h1, h2]), np.float32)
+                    os_ = np.array([0, len(t1), len(t1), len(t2)], np.uint64)
+                    hit = np.zeros(1, np.uint8)
+                    ref.ref_meshint(orc.ptr(tri), orc.ptr(hd), orc.ptr(os_), 1, orc.ptr(hit))
+                    if hit[0]:
+                        meshint.add((i, o))
+                    S = np.ascontiguousarray(np.repeat(t1, len(t2), axis=0))
+                    T = np.ascontiguousarray(np.tile(t2, (len(t1), 1)))
+                    ti = np.zeros(len(S), np.uint8)
+                    ref.ref_triint(orc.ptr(S), orc.ptr(T), len(S), orc.ptr(ti))
+                    if ti.any():
+                        triint.add((i, o))
+    return meshint, triint
+
+
+def test_oracle_intersect_join_pinned_by_reference_meshint(tdp, tiny_tiles):
+    """-q intersect: at LOD 100 alone, the oracle's result is exactly the
+    object pairs the reference's own MeshInt finds intersecting in a voxel
+    pair with overlapping boxes, and it holds every pair the reference's
+    TriInt (an exact triangle-triangle test) finds crossing. Over all LODs it
+    is the same set at Hausdorff level 0 (never ruled out early) and a subset
+    at levels 1 and 2 (whose lower bounds may rule out pairs early)."""
+    ref = orc.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    pn, pv = tiny_tiles
+    ta, tb = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    o1, o2 = orc.OracleTile(pn), orc.OracleTile(pv)
+    meshint, triint = _intersect_bruteforce(ta, tb, False, ref)
+    assert meshint and triint <= meshint
+    for prec in ("f32", "f64"):
+        t, o, mn, mx, _ = orc.join(o1, o2, "intersect", lods=(100,), level=2, precision=prec)
+        assert set(zip(t.tolist(), o.tolist())) == meshint, prec
+        assert np.all(mn == 0) and np.all(mx == 0)
+        for level in (0, 1, 2):
+            t, o, *_ = orc.join(o1, o2, "intersect", level=level, precision=prec)
+            got = set(zip(t.tolist(), o.tolist()))
+            assert len(got) == len(t)
+            assert (got == meshint) if level == 0 else (got <= meshint), (prec, level)
+    # self-join of overlapping nuclei: symmetric, self excluded
+    n, _ = tdp.generate(seed=5, n_vessels=0, nuclei_per_vessel=60, cube=150.0, subdiv=2,
+                        prototypes=8, voxel_tris=20)
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        n.save(os.path.join(d, "n.dt"))
+        on = orc.OracleTile(os.path.join(d, "n.dt"))
+        meshint, _ = _intersect_bruteforce(n, n, True, ref)
+        t, o, *_ = orc.join(on, None, "intersect", lods=(100,), level=2, precision="f32")
+        got = set(zip(t.tolist(), o.tolist()))
+        assert got == meshint and got == {(b, a) for a, b in got} and len(got) > 10
 
 
 def test_oracle_golden_fixtures():

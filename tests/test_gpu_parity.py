@@ -367,3 +367,75 @@ def test_autotune_keeps_results(tdp, small_tiles):
     assert np.array_equal(got.target, want.target) and np.array_equal(got.obj2, want.obj2)
     assert bits_equal(got.rmin, want.rmin) and bits_equal(got.rmax, want.rmax)
     e.close()
+
+
+# ---------------------------------------------------------- intersection join
+
+@pytest.fixture(scope="module")
+def touching_tiles(tdp, tmp_path_factory):
+    """nuclei 10-60 from their vessels' centrelines (many touch a vessel) and
+    densely packed nuclei (many touch each other), as RAW files"""
+    d = tmp_path_factory.mktemp("touch")
+    n, v = tdp.generate(seed=2, n_vessels=10, nuclei_per_vessel=20, cube=2000.0,
+                        place_min=10.0, place_max=60.0)
+    dense, _ = tdp.generate(seed=5, n_vessels=0, nuclei_per_vessel=300, cube=400.0)
+    paths = [str(d / f) for f in ("nuclei.dt", "vessels.dt", "dense.dt")]
+    for t, p in zip((n, v, dense), paths):
+        t.save(p)
+    return paths
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_intersect_join_all_forms(tdp, eng, touching_tiles, level):
+    pn, pv, _ = touching_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    m = 120
+    ot, oo, omn, omx, ost = orc.join(on, ov, "intersect", level=level, precision="f32",
+                                     max_objects1=m)
+    assert len(ot) > 10
+    r32 = eng.join(n, v, query="intersect", hausdorff_level=level, precision="ref32",
+                   max_objects1=m)
+    _compare(r32, ot, oo, omn, omx)
+    assert r32.stats["tri_pairs"] == ost["tri_pairs"]
+    ot, oo, omn, omx, ost = orc.join(on, ov, "intersect", level=level, precision="f64",
+                                     max_objects1=m)
+    for prec in ("fp64", "mixed"):
+        r = eng.join(n, v, query="intersect", hausdorff_level=level, precision=prec,
+                     max_objects1=m)
+        _compare(r, ot, oo, omn, omx)
+        assert r.stats["tri_pairs"] == ost["tri_pairs"], prec
+        assert np.all(r.rmin == 0) and np.all(r.rmax == 0) and np.all(r.round >= 1)
+
+
+@pytest.mark.parametrize("lods", [(40, 100), (20,), (100, 60, 60)])
+def test_intersect_self_join(tdp, eng, touching_tiles, lods):
+    _, _, pd = touching_tiles
+    t = tdp.Tile.load(pd)
+    ot_ = orc.OracleTile(pd)
+    ot, oo, omn, omx, _ = orc.join(ot_, None, "intersect", lods=lods, precision="f64")
+    for prec in ("fp64", "mixed"):
+        r = eng.join(t, None, query="intersect", lods=lods, precision=prec)
+        _compare(r, ot, oo, omn, omx)
+        assert not np.any(r.target == r.obj2)
+    if lods == (40, 100):
+        assert len(ot) > 10
+
+
+def test_intersect_captured_rounds_and_stream(tdp, touching_tiles):
+    """the graph-captured round and the streaming join give the same pairs"""
+    pn, pv, pd = touching_tiles
+    n, v, dense = tdp.Tile.load(pn), tdp.Tile.load(pv), tdp.Tile.load(pd)
+    e = tdp.Engine(0)
+    want = e.join(n, v, query="intersect", precision="fp64")
+    want_self = e.join(dense, None, query="intersect", precision="mixed")
+    e.set_tuning(graph=1, drain=1)
+    got = e.join(n, v, query="intersect", precision="mixed")
+    assert np.array_equal(got.target, want.target) and np.array_equal(got.obj2, want.obj2)
+    assert np.array_equal(got.round, want.round)
+    s = e.stream_join(tdp.Source.of_tile(dense), None, query="intersect", precision="mixed",
+                      hbm_budget=48 << 20)
+    e.close()
+    assert s.stream["chunks"] > 1
+    assert np.array_equal(s.target, want_self.target) and np.array_equal(s.obj2, want_self.obj2)
+    assert np.array_equal(s.round, want_self.round)
