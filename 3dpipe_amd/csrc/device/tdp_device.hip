@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -1552,22 +1553,17 @@ extern "C" int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin
     if (P == 0) return TDP_OK;
     if (report_round)
         TDP_CHECK(ctx, hipMemcpy(report_round, j->round, P, hipMemcpyDeviceToHost));
-    if (rmin || rmax) {
-        std::vector<char> a(P * j->rsz), b(P * j->rsz);
-        TDP_CHECK(ctx, hipMemcpy(a.data(), j->rmin, P * j->rsz, hipMemcpyDeviceToHost));
-        TDP_CHECK(ctx, hipMemcpy(b.data(), j->rmax, P * j->rsz, hipMemcpyDeviceToHost));
-        for (size_t p = 0; p < P; p++) {
-            double x, y;
-            if (j->rsz == 4) {
-                x = ((const float *)a.data())[p];
-                y = ((const float *)b.data())[p];
-            } else {
-                x = ((const double *)a.data())[p];
-                y = ((const double *)b.data())[p];
-            }
-            if (rmin) rmin[p] = x;
-            if (rmax) rmax[p] = y;
-        }
+    if (j->rsz == 8) {  // straight into the caller's arrays
+        if (rmin) TDP_CHECK(ctx, hipMemcpy(rmin, j->rmin, P * 8, hipMemcpyDeviceToHost));
+        if (rmax) TDP_CHECK(ctx, hipMemcpy(rmax, j->rmax, P * 8, hipMemcpyDeviceToHost));
+        return TDP_OK;
+    }
+    std::unique_ptr<float[]> a(new float[P]);
+    for (int k = 0; k < 2; k++) {
+        double *dst = k ? rmax : rmin;
+        if (!dst) continue;
+        TDP_CHECK(ctx, hipMemcpy(a.get(), k ? j->rmax : j->rmin, P * 4, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) dst[p] = a[p];
     }
     return TDP_OK;
 }

@@ -61,9 +61,9 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
         res->stats[5]++;
     }
     res->timing[3] = ms_since(t0);
-    std::vector<uint8_t> rnd(P);
-    std::vector<double> mn(P), mx(P);
-    if (rc == TDP_OK) rc = tdp_join_results(j, rnd.data(), mn.data(), mx.data());
+    std::unique_ptr<uint8_t[]> rnd(new uint8_t[P ? P : 1]);
+    std::unique_ptr<double[]> mn(new double[P ? P : 1]), mx(new double[P ? P : 1]);
+    if (rc == TDP_OK) rc = tdp_join_results(j, rnd.get(), mn.get(), mx.get());
     uint64_t st[5] = {0, 0, 0, 0, 0};
     if (rc == TDP_OK) rc = tdp_join_stats(j, st);
     if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
@@ -76,26 +76,44 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     tdp_join_destroy(j);
     for (int k = 0; k < 4; k++) res->stats[k] = st[k];
     res->stats[6] = st[4];
-    // per target: reported pairs ordered by (round, list position)
-    const size_t n_rep = (size_t)std::count_if(rnd.begin(), rnd.end(), [](uint8_t r) { return r != 255; });
-    res->target.reserve(n_rep);
-    res->obj2.reserve(n_rep);
-    res->rmin.reserve(n_rep);
-    res->rmax.reserve(n_rep);
-    res->round.reserve(n_rep);
-    std::vector<uint32_t> order;
-    for (size_t t = 0; t < N; t++) {
-        order.clear();
-        for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++)
-            if (rnd[p] != 255) order.push_back(p);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](uint32_t a, uint32_t b) { return rnd[a] < rnd[b]; });
-        for (uint32_t p : order) {
-            res->target.push_back((int32_t)t);
-            res->obj2.push_back((int32_t)ids[p]);
-            res->rmin.push_back(mn[p]);
-            res->rmax.push_back(mx[p]);
-            res->round.push_back(rnd[p]);
+    // per target: reported pairs ordered by (round, list position); targets
+    // in parallel, each written at its offset
+    std::vector<uint64_t> at(N + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < (long)N; t++) {
+        uint64_t c = 0;
+        for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++) c += rnd[p] != 255;
+        at[t + 1] = c;
+    }
+    for (size_t t = 0; t < N; t++) at[t + 1] += at[t];
+    const size_t n_rep = at[N];
+    res->target.resize(n_rep);
+    res->obj2.resize(n_rep);
+    res->rmin.resize(n_rep);
+    res->rmax.resize(n_rep);
+    res->round.resize(n_rep);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long t = 0; t < (long)N; t++) {
+        uint64_t w = at[t];
+        if (w == at[t + 1]) continue;
+        // rounds are few: one pass per round present, list order inside each
+        uint8_t r = 255, next = 0;
+        for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++) r = std::min(r, rnd[p]);
+        while (r != 255) {
+            next = 255;
+            for (uint32_t p = prefix[t]; p < prefix[t + 1]; p++) {
+                if (rnd[p] == r) {
+                    res->target[w] = (int32_t)t;
+                    res->obj2[w] = (int32_t)ids[p];
+                    res->rmin[w] = mn[p];
+                    res->rmax[w] = mx[p];
+                    res->round[w] = r;
+                    w++;
+                } else if (rnd[p] > r && rnd[p] < next) {
+                    next = rnd[p];
+                }
+            }
+            r = next;
         }
     }
     return TDP_OK;
