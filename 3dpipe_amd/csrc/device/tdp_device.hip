@@ -8,6 +8,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -1201,6 +1202,18 @@ static uint32_t fold_need(const tdp_join *j, int is_last) {
     return is_last || lv != 2 ? 1u : 6u;
 }
 
+// calls f(std::integral_constant<uint32_t, NEED>) for the round's quantity set
+template <typename F>
+static void with_need(uint32_t need, F &&f) {
+    switch (need) {
+        case 1: f(std::integral_constant<uint32_t, 1>{}); break;
+        case 4: f(std::integral_constant<uint32_t, 4>{}); break;
+        case 5: f(std::integral_constant<uint32_t, 5>{}); break;
+        case 6: f(std::integral_constant<uint32_t, 6>{}); break;
+        default: f(std::integral_constant<uint32_t, 7>{}); break;
+    }
+}
+
 // the per-round decisions of the join's query
 template <typename R>
 static void launch_decide(tdp_join *j, uint8_t rnd) {
@@ -1264,11 +1277,13 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         const dim3 gw = wave_grid(n);
         // quantities this round folds: d (bit 0), lo (bit 1), hi (bit 2)
         const uint32_t need = fold_need(j, is_last);
-        k_mx_bounds<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                          A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                          B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
-                                          (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
-                                          j->mx_flags, j->mx_key, j->mx_nw);
+        with_need(need, [&](auto nd) {
+            k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
+                n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd,
+                B->vox_tri_off, B->vox_tri_cnt, B->tri, B->bnd, slot, need,
+                (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
+                j->mx_flags, j->mx_key, j->mx_nw);
+        });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_BOUNDS) != TDP_OK) return TDP_ERR_HIP;
         uint64_t words = 0;
@@ -1282,11 +1297,12 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
-        k_mx_mark<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                        A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                        B->bnd, slot, (uint32_t)j->tune.max_rows, j->mx_U,
-                                        j->mx_probe, j->mx_flags, j->mx_key, j->mx_woff,
-                                        j->mx_mask, j->keep);
+        with_need(need, [&](auto nd) {
+            k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
+                n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd,
+                B->vox_tri_off, B->vox_tri_cnt, B->tri, B->bnd, slot, (uint32_t)j->tune.max_rows,
+                j->mx_U, j->mx_probe, j->mx_flags, j->mx_key, j->mx_woff, j->mx_mask, j->keep);
+        });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
         if (j->tune.drain) {
@@ -1439,21 +1455,24 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_cnt,
                                                            B->vox_tri_cnt, slot, j->stats);
     launched();
-    k_mx_bounds<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                      A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri,
-                                      B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
-                                      (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
-                                      j->mx_flags, j->mx_key, j->mx_nw);
+    with_need(need, [&](auto nd) {
+        k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
+            n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd, B->vox_tri_off,
+            B->vox_tri_cnt, B->tri, B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
+            (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe, j->mx_flags, j->mx_key, j->mx_nw);
+    });
     launched();
     if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, j->d_words);
     k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
         3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
         A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
     launched();
-    k_mx_mark<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                    A->tri, A->bnd, B->vox_tri_off, B->vox_tri_cnt, B->tri, B->bnd,
-                                    slot, (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
-                                    j->mx_flags, j->mx_key, j->mx_woff, j->mx_mask, j->keep);
+    with_need(need, [&](auto nd) {
+        k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
+            n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd, B->vox_tri_off,
+            B->vox_tri_cnt, B->tri, B->bnd, slot, (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
+            j->mx_flags, j->mx_key, j->mx_woff, j->mx_mask, j->keep);
+    });
     launched();
     k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
                                      A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot,

@@ -278,11 +278,21 @@ __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
     return c;
 }
 
+// the quantity tests a round needs: NEED (bit 0 d, 1 lo, 2 hi) is a launch
+// constant, so the arithmetic of the others drops out
+template <uint32_t NEED>
+__device__ __forceinline__ bool reaches_n(const float lw[3], const float U[3]) {
+    return ((NEED & 1u) && lw[0] <= U[0]) || ((NEED & 2u) && lw[1] <= U[1]) ||
+           ((NEED & 4u) && lw[2] <= U[2]);
+}
+
 // Per voxel pair: the smallest FP32 upper bound of each quantity the round
 // folds (vpU; -inf for the others) and up to three probe pairs, the pairs of
 // smallest score (centre distance less the radii, with the quantity's
 // Hausdorff terms), whose FP64 values usually are the minima. Also the flags
-// and the mask word count of the tiled walk.
+// and the mask word count of the tiled walk. NEED as for reaches_n (and
+// `need` the same bits at run time).
+template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
@@ -324,6 +334,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                   const uint32_t q = p.i * n2 + p.j;
 #pragma unroll
                   for (int f = 0; f < 3; f++) {
+                      if (!((NEED >> f) & 1u)) continue;
                       up[f] = fminf(up[f], uw[f]);
                       if (key[f] < best[f]) {
                           best[f] = key[f];
@@ -335,9 +346,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint32_t pq[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        U[f] = wave_min(up[f]);
-        pq[f] = wave_argmin_q(best[f], bq[f]);
-        if (!((need >> f) & 1u)) {
+        if (((NEED & need) >> f) & 1u) {
+            U[f] = wave_min(up[f]);
+            pq[f] = wave_argmin_q(best[f], bq[f]);
+        } else {
             U[f] = -INFINITY;
             pq[f] = kNoProbe;
         }
@@ -442,22 +454,24 @@ __device__ __forceinline__ void bounds_U(const float *vpU, const unsigned long l
 
 // Candidate test: the disc interval first, the vertex interval for pairs
 // that pass it; probes (already evaluated) are never candidates.
-template <typename VA, typename VC>
+template <uint32_t NEED, typename VA, typename VC>
 __device__ __forceinline__ bool mx_candidate(const TriS &A, const TriS &B, const PairAt &p,
                                              VA &&va, VC &&vc, const float U[3], uint32_t q,
                                              uint32_t p0, uint32_t p1, uint32_t p2) {
     if (!p.valid) return false;
     float lw[3], uw[3];
     disc_interval(A, B, lw, uw);
-    if (!reaches(lw, U)) return false;
+    if (!reaches_n<NEED>(lw, U)) return false;
     if (q == p0 || q == p1 || q == p2) return false;
     vertex_interval(A, va(), B, vc(), lw, uw);
-    return reaches(lw, U);
+    return reaches_n<NEED>(lw, U);
 }
 
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
 // the ballot of iteration k of the tiled walk; count[w] = number of bits.
-// SMALL voxel pairs: every pair, no mask.
+// SMALL voxel pairs: every pair, no mask. NEED: the quantities of the round
+// (U is -inf for the others).
+template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
@@ -492,7 +506,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     unsigned long long mine = 0;  // lane k keeps word (k mod 64) of the current group
     for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
               [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc) {
-                  const bool take = mx_candidate(A, B, p, va, vc, U, p.i * n2 + p.j, p0, p1, p2);
+                  const bool take =
+                      mx_candidate<NEED>(A, B, p, va, vc, U, p.i * n2 + p.j, p0, p1, p2);
                   const unsigned long long m = __ballot(take);
                   c += (uint32_t)__popcll(m);
                   const uint32_t k = p.word & (kWave - 1);
