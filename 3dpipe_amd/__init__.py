@@ -122,6 +122,7 @@ def _load():
     tdb.tdb_server_error.restype = _c.c_char_p
     tdb.tdb_server_stats.argtypes = [_P, _P]
     tdb.tdb_server_close.argtypes = [_P]
+    tdb.tdb_server_stage_ms.argtypes = [_P, _P]
     tdb.tdb_source_generate.argtypes = [_c.POINTER(_GenParams), _c.POINTER(_P)]
     tdb.tdb_source_tile.argtypes = [_P, _c.POINTER(_P)]
     tdb.tdb_source_objects.argtypes = [_P]
@@ -556,10 +557,15 @@ class Server:
         return _read_result(r), lat.value
 
     def stats(self) -> dict:
+        tdb = _load()[0]
         st = np.zeros(4, np.uint64)
-        _load()[0].tdb_server_stats(self._h, _ptr(st))
-        return dict(zip(["batches", "queries", "max_batch", "peak_device_bytes"],
-                        (int(x) for x in st)))
+        tdb.tdb_server_stats(self._h, _ptr(st))
+        ms = np.zeros(4, np.float64)
+        tdb.tdb_server_stage_ms(self._h, _ptr(ms))
+        out = dict(zip(["batches", "queries", "max_batch", "peak_device_bytes"],
+                       (int(x) for x in st)))
+        out["stage_ms"] = dict(zip(["merge", "upload", "candidates", "join"], ms.tolist()))
+        return out
 
     def close(self):
         if self._h:

@@ -39,6 +39,10 @@ struct tdp_ctx {
     std::mutex mem_mu;
     std::unordered_map<void *, size_t> dsize;
     uint64_t mem_limit = 0, mem_used = 0, mem_peak = 0;
+    // points in the stream's queue the host can wait for (tdp_mark)
+    static constexpr int kMarks = 16;
+    hipEvent_t marks[kMarks] = {};
+    uint64_t n_marks = 0;
 };
 
 struct tdp_tile {
@@ -400,6 +404,8 @@ extern "C" int tdp_set_tuning(tdp_ctx *ctx, const tdp_tuning *t) {
 
 extern "C" int tdp_close(tdp_ctx *ctx) {
     if (!ctx) return TDP_ERR_ARG;
+    for (auto &e : ctx->marks)
+        if (e) (void)hipEventDestroy(e);
     for (void *p : ctx->scan_buf) dfree(ctx, p);
     dfree(ctx, ctx->d_total);
     pool_release_all(ctx);
@@ -588,6 +594,23 @@ extern "C" int tdp_sync(tdp_ctx *ctx) {
     return TDP_OK;
 }
 
+extern "C" int tdp_mark(tdp_ctx *ctx, uint64_t *ticket) {
+    if (!ctx || !ticket) return TDP_ERR_ARG;
+    hipEvent_t &e = ctx->marks[ctx->n_marks % tdp_ctx::kMarks];
+    if (!e) TDP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    TDP_CHECK(ctx, hipEventRecord(e, ctx->stream));
+    *ticket = ctx->n_marks++;
+    return TDP_OK;
+}
+
+extern "C" int tdp_wait_mark(tdp_ctx *ctx, uint64_t ticket) {
+    if (!ctx || ticket >= ctx->n_marks) return TDP_ERR_ARG;
+    // a slot recorded again since marks a later point: waiting for it covers
+    // this ticket too
+    TDP_CHECK(ctx, hipEventSynchronize(ctx->marks[ticket % tdp_ctx::kMarks]));
+    return TDP_OK;
+}
+
 extern "C" int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t n_tri,
                                 tdp_tile **out) {
     if (!ctx || !out || !n_obj || !n_vox || !n_tri) return TDP_ERR_ARG;
@@ -605,11 +628,13 @@ extern "C" int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t
     if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_tri_cnt, 5 * n_vox);
     if (rc == TDP_OK) rc = dalloc(ctx, &t->tri, 3 * n_tri);
     if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->obj_box, 6 * n_obj);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->obj_tr, 3 * n_obj);
     if (rc != TDP_OK) {
         tdp_tile_free(t);
         return rc;
     }
-    t->bytes = (n_obj + 1) * 4 + n_obj * 40 + n_vox * (36 + 4 + 40 + 20) + n_tri * 48 * (1 + kBnd);
+    t->bytes = (n_obj + 1) * 4 + n_obj * 76 + n_vox * (36 + 4 + 40 + 20) + n_tri * 48 * (1 + kBnd);
     *out = t;
     return TDP_OK;
 }
@@ -652,6 +677,10 @@ extern "C" int tdp_tile_fill_async(tdp_tile *t, const tdp_tile_desc *d) {
     TDP_CHECK(ctx, cp(t->vox_box, d->vox_box, 36 * d->n_vox));
     TDP_CHECK(ctx, cp(t->vox_tri_off, d->vox_tri_off, 40 * d->n_vox));
     TDP_CHECK(ctx, cp(t->vox_tri_cnt, d->vox_tri_cnt, 20 * d->n_vox));
+    if (d->obj_box && d->obj_transform) {
+        TDP_CHECK(ctx, cp(t->obj_box, d->obj_box, 24 * d->n_obj));
+        TDP_CHECK(ctx, cp(t->obj_tr, d->obj_transform, 12 * d->n_obj));
+    }
     if (d->n_obj) {
         k_vox_owner<<<blocks_for(d->n_obj, kBlock), kBlock, 0, s>>>(d->n_obj, t->obj_vox_off,
                                                                      t->vox_obj);

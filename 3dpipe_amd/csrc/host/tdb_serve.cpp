@@ -15,6 +15,16 @@
 // kernels), and batch n+1 starts as soon as batch n ends. Each target's join
 // depends only on that target and the resident tile, so a query's results
 // are the same alone and in any mix.
+//
+// Uploads go into two device tiles reserved once (grown when a batch needs
+// more) that the preparer and the executor hand back and forth, through
+// pinned staging: the object and voxel arrays in one pinned block, the
+// triangle records in chunks copied by host threads into two pinned buffers
+// that alternate while the previous chunk's DMA runs.
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include <cstring>
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
@@ -38,6 +48,7 @@ struct Batch {
     std::vector<Query> qs;
     tdb_tile merged;
     tdp_tile *dev = nullptr;
+    int slot = -1;  // the device tile slot dev lives in
     std::vector<uint32_t> prefix, ids;
     std::vector<int32_t> k_t;
     std::vector<float> tau_t;
@@ -124,11 +135,119 @@ struct tdb_server {
     bool stop = false, prep_exit = false;
     std::string err;
     uint64_t n_batches = 0, n_queries = 0, max_batch_seen = 0, peak_bytes = 0;
+    // device tile slots (guarded by mu: busy) and the preparer's pinned staging
+    static constexpr size_t kChunkTri = (size_t)1 << 20;  // records per staging chunk
+    tdp_tile *slot[2] = {nullptr, nullptr};
+    size_t slot_cap[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    bool slot_busy[2] = {false, false};
+    char *meta_pin = nullptr;
+    size_t meta_cap = 0;
+    float *stage[2] = {nullptr, nullptr};
+    int upload(Batch &b);
+    // host time of the stages (ms, summed over batches): merge, upload,
+    // candidates (preparer); join (executor)
+    double t_merge = 0, t_upload = 0, t_cand = 0, t_join = 0;
     std::thread prep, exec;
 
     void preparer();
     void executor();
 };
+
+// the merged tile into a free slot through pinned staging (preparer thread)
+int tdb_server::upload(Batch &b) {
+    const tdb::Tile &m = b.merged.t;
+    const size_t no = m.n_obj, nv = m.n_vox();
+    size_t nt = 0;
+    for (auto &q : b.qs) nt += q.tile->t.n_tri();
+    int s = -1;
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return !slot_busy[0] || !slot_busy[1]; });
+        s = slot_busy[0] ? 1 : 0;
+        slot_busy[s] = true;
+    }
+    b.slot = s;
+    int rc = TDP_OK;
+    size_t *cap = slot_cap[s];
+    if (!slot[s] || cap[0] < no || cap[1] < nv || cap[2] < nt) {
+        if (slot[s]) tdp_tile_free(slot[s]);
+        slot[s] = nullptr;
+        cap[0] = std::max(cap[0], std::max<size_t>(no, 1));
+        cap[1] = std::max(cap[1], std::max<size_t>(nv, 1));
+        cap[2] = std::max(cap[2], std::max<size_t>(nt, 1));
+        rc = tdp_tile_reserve(prep_ctx, cap[0], cap[1], cap[2], &slot[s]);
+        if (rc != TDP_OK) {
+            cap[0] = cap[1] = cap[2] = 0;
+            return rc;
+        }
+    }
+    if (!stage[0]) {
+        for (int k = 0; k < 2 && rc == TDP_OK; k++)
+            rc = tdp_host_alloc(prep_ctx, 48 * kChunkTri, (void **)&stage[k]);
+        if (rc != TDP_OK) return rc;
+    }
+    // object and voxel arrays into one pinned block
+    const size_t sz[8] = {4 * (no + 1), 40 * no, 36 * nv, 40 * nv, 20 * nv, 24 * no, 12 * no, 0};
+    size_t need = 0, at[8];
+    for (int k = 0; k < 7; k++) {
+        at[k] = need;
+        need += (sz[k] + 63) / 64 * 64;
+    }
+    if (meta_cap < need) {
+        if (meta_pin) tdp_host_free(prep_ctx, meta_pin);
+        meta_pin = nullptr;
+        meta_cap = 0;
+        rc = tdp_host_alloc(prep_ctx, need, (void **)&meta_pin);
+        if (rc != TDP_OK) return rc;
+        meta_cap = need;
+    }
+    const void *src[7] = {m.obj_vox_off.data(), m.obj_hd.data(),      m.vox_box.data(),
+                          m.vox_tri_off.data(), m.vox_tri_cnt.data(), m.obj_box.data(),
+                          m.obj_transform.data()};
+    for (int k = 0; k < 7; k++)
+        if (sz[k]) std::memcpy(meta_pin + at[k], src[k], sz[k]);
+    tdp_tile_desc d{};
+    d.n_obj = no;
+    d.n_vox = nv;
+    d.n_tri = nt;
+    d.obj_vox_off = (const uint32_t *)(meta_pin + at[0]);
+    d.obj_hd = (const float *)(meta_pin + at[1]);
+    d.vox_box = (const float *)(meta_pin + at[2]);
+    d.vox_tri_off = (const uint64_t *)(meta_pin + at[3]);
+    d.vox_tri_cnt = (const uint32_t *)(meta_pin + at[4]);
+    d.obj_box = (const float *)(meta_pin + at[5]);
+    d.obj_transform = (const float *)(meta_pin + at[6]);
+    rc = tdp_tile_fill_async(slot[s], &d);
+    // triangle records: host threads copy chunk k while chunk k-1 is on the wire
+    uint64_t mk[2] = {0, 0};
+    size_t k = 0, first = 0;
+#ifdef _OPENMP
+    const int nth = opts.threads > 0 ? opts.threads : std::min(16, omp_get_max_threads());
+#endif
+    for (size_t qi = 0; rc == TDP_OK && qi < b.qs.size(); qi++) {
+        const tdb::Tile &t = b.qs[qi].tile->t;
+        for (size_t off = 0; rc == TDP_OK && off < t.n_tri(); off += kChunkTri, k++) {
+            const size_t n = std::min(kChunkTri, t.n_tri() - off);
+            float *buf = stage[k % 2];
+            if (k >= 2) rc = tdp_wait_mark(prep_ctx, mk[k % 2]);
+            if (rc != TDP_OK) break;
+            const float *from = t.tri.data() + 12 * off;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nth) schedule(static)
+#endif
+            for (long p = 0; p < 16; p++) {
+                const size_t a = n * p / 16, e = n * (p + 1) / 16;
+                if (e > a) std::memcpy(buf + 12 * a, from + 12 * a, 48 * (e - a));
+            }
+            rc = tdp_tile_fill_tri_async(slot[s], first, n, buf);
+            if (rc == TDP_OK) rc = tdp_mark(prep_ctx, &mk[k % 2]);
+            first += n;
+        }
+    }
+    if (rc == TDP_OK) rc = tdp_sync(prep_ctx);
+    if (rc == TDP_OK) b.dev = slot[s];
+    return rc;
+}
 
 void tdb_server::preparer() {
     for (;;) {
@@ -168,6 +287,7 @@ void tdb_server::preparer() {
             peak_bytes = std::max(peak_bytes, res_bytes + in_use);
         }
         // merge, candidates, upload: outside the lock, beside the running join
+        auto t0 = Clock::now();
         merge_tiles(b->qs, b->merged.t, b->first);
         const size_t N = b->merged.t.n_obj;
         b->k_t.resize(N);
@@ -181,24 +301,26 @@ void tdb_server::preparer() {
         }
         tdb_join_opts o = b->qs[0].o;
         o.threads = opts.threads;
-        tdp_tile_desc d = tdb_desc_of(b->merged.t);
-        std::vector<const float *> parts;
-        std::vector<size_t> part_n;
-        for (auto &q : b->qs) {
-            parts.push_back(q.tile->t.tri.data());
-            part_n.push_back(q.tile->t.n_tri());
-        }
-        d.n_tri = 0;
-        for (size_t n : part_n) d.n_tri += n;
-        if (tdp_tile_upload_parts(prep_ctx, &d, parts.data(), part_n.data(), parts.size(),
-                                  &b->dev) != TDP_OK)
+        const double ms_merge = ms_since(t0);
+        t0 = Clock::now();
+        double ms_upload = 0;
+        if (upload(*b) != TDP_OK) {
             b->err = tdp_last_error(prep_ctx);
-        else if (tdb_gen_candidates_dev(prep_ctx, b->dev, res_dev, &b->merged, resident, false, N, &o,
-                                        b->k_t.data(), b->tau_t.data(), b->prefix, b->ids, nullptr,
-                                        b->err) != TDP_OK && b->err.empty())
-            b->err = "candidate generation failed";
+        } else {
+            ms_upload = ms_since(t0);
+            t0 = Clock::now();
+            if (tdb_gen_candidates_dev(prep_ctx, b->dev, res_dev, &b->merged, resident, false, N,
+                                       &o, b->k_t.data(), b->tau_t.data(), b->prefix, b->ids,
+                                       nullptr, b->err) != TDP_OK &&
+                b->err.empty())
+                b->err = "candidate generation failed";
+        }
+        const double ms_cand = ms_since(t0);
         {
             std::lock_guard<std::mutex> lk(mu);
+            t_merge += ms_merge;
+            t_upload += ms_upload;
+            t_cand += ms_cand;
             ready.push_back(std::move(b));
         }
         cv.notify_all();
@@ -225,11 +347,19 @@ void tdb_server::executor() {
             o.n_lods = (int)b->qs[0].lods.size();
             o.precision = opts.precision;
             o.max_objects1 = 0;
+            const auto t0 = Clock::now();
             rc = tdb_run_join(run_ctx, b->dev, res_dev, &o, b->prefix, b->ids, b->k_t.data(),
                               b->tau_t.data(), &all, msg);
+            const double ms = ms_since(t0);
+            std::lock_guard<std::mutex> lk(mu);
+            t_join += ms;
         }
-        if (b->dev) tdp_tile_free(b->dev);
         b->dev = nullptr;
+        if (b->slot >= 0) {
+            std::lock_guard<std::mutex> lk(mu);
+            slot_busy[b->slot] = false;
+        }
+        cv.notify_all();
         // per query results and latencies
         const auto now = Clock::now();
         std::vector<std::pair<int64_t, Done>> out;
@@ -378,6 +508,14 @@ void tdb_server_stats(tdb_server *s, uint64_t *out4) {
     out4[3] = s->peak_bytes;
 }
 
+void tdb_server_stage_ms(tdb_server *s, double *out4) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    out4[0] = s->t_merge;
+    out4[1] = s->t_upload;
+    out4[2] = s->t_cand;
+    out4[3] = s->t_join;
+}
+
 void tdb_server_close(tdb_server *s) {
     if (!s) return;
     {
@@ -389,6 +527,11 @@ void tdb_server_close(tdb_server *s) {
     if (s->exec.joinable()) s->exec.join();
     for (auto &kv : s->done) delete kv.second.res;
     for (auto &q : s->pending) tdb_tile_free(q.tile);
+    for (int k = 0; k < 2; k++) {
+        if (s->slot[k]) tdp_tile_free(s->slot[k]);
+        if (s->stage[k]) tdp_host_free(s->prep_ctx, s->stage[k]);
+    }
+    if (s->meta_pin) tdp_host_free(s->prep_ctx, s->meta_pin);
     if (s->res_dev) tdp_tile_free(s->res_dev);
     if (s->prep_ctx) tdp_close(s->prep_ctx);
     if (s->run_ctx) tdp_close(s->run_ctx);

@@ -109,13 +109,14 @@ def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn):
     }
 
 
-def serve_bench(tdp, vessels, gen_kw, n_queries, rate, seed, log_fn):
+def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
     """Config-5 style serving: `vessels` resident; n_queries fresh nucleus
     tiles of log-uniform size 100..50000 (seeded), each kNN with k in
     {1, 5, 10, 20} or within with tau U(50, 500), arriving as a Poisson
-    process of `rate` queries/s. Run twice on the same schedule: the batched
-    mixed server and one-query-at-a-time plain FP64. Returns latency
-    percentiles, throughput and whether every query's results agree."""
+    process (unit-rate gaps scaled to each of `rates` queries/s). At each rate
+    run twice on the same schedule: the batched mixed server and
+    one-query-at-a-time plain FP64. Returns, per rate, latency percentiles,
+    throughput and whether every query's results agree."""
     import threading
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -126,7 +127,7 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rate, seed, log_fn):
             kinds.append(("nn", int(rng.choice([1, 5, 10, 20])), 0.0))
         else:
             kinds.append(("within", 1, float(rng.uniform(50, 500))))
-    gaps = rng.exponential(1.0 / rate, n_queries)
+    unit_gaps = rng.exponential(1.0, n_queries)
     t0 = time.perf_counter()
     nv = gen_kw["n_vessels"]
     tiles = []
@@ -139,7 +140,7 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rate, seed, log_fn):
     log_fn(f"serving: {n_queries} query tiles ({n_nuclei} nuclei) generated in "
            f"{time.perf_counter() - t0:.1f} s")
 
-    def run(precision, max_batch, overlap):
+    def run(precision, max_batch, overlap, gaps):
         srv = tdp.Server(vessels, max_batch=max_batch, overlap=overlap, precision=precision)
         copies = [t.clone() for t in tiles]  # each server owns what it is given
         tickets = [None] * n_queries
@@ -169,23 +170,31 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rate, seed, log_fn):
         return res, {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                      "mean_ms": float(lat.mean()), "queries_per_s": n_queries / makespan,
                      "nuclei_per_s": n_nuclei / makespan, "batches": st["batches"],
-                     "max_batch": st["max_batch"], "peak_device_gb": st["peak_device_bytes"] / 1e9}
+                     "max_batch": st["max_batch"], "peak_device_gb": st["peak_device_bytes"] / 1e9,
+                     "host_stage_ms": st["stage_ms"], "makespan_s": makespan}
 
-    res_b, batched = run("mixed", 64, True)
-    log_fn(f"serving batched mixed: p50 {batched['p50_ms']:.1f} ms, p99 {batched['p99_ms']:.1f} ms, "
-           f"{batched['queries_per_s']:.1f} q/s, {batched['batches']} batches")
-    res_s, single = run("fp64", 1, False)
-    log_fn(f"serving one-at-a-time FP64: p50 {single['p50_ms']:.1f} ms, p99 {single['p99_ms']:.1f} ms, "
-           f"{single['queries_per_s']:.1f} q/s")
-    same = all(np.array_equal(a.target, b.target) and np.array_equal(a.obj2, b.obj2)
-               and np.array_equal(a.rmin, b.rmin) and np.array_equal(a.rmax, b.rmax)
-               for a, b in zip(res_b, res_s))
-    return {"queries": n_queries, "nuclei": n_nuclei, "arrival_rate_per_s": rate,
-            "sizes": "log-uniform 100..50000 nuclei", "mix": "kNN k in {1,5,10,20} / within tau U(50,500)",
-            "batched_mixed": batched, "one_at_a_time_fp64": single,
+    out = {"queries": n_queries, "nuclei": n_nuclei,
+           "sizes": "log-uniform 100..50000 nuclei",
+           "mix": "kNN k in {1,5,10,20} / within tau U(50,500)", "by_rate": []}
+    for rate in rates:
+        gaps = unit_gaps / rate
+        res_b, batched = run("mixed", 64, True, gaps)
+        log_fn(f"serving {rate:g} q/s, batched mixed: p50 {batched['p50_ms']:.1f} ms, p99 "
+               f"{batched['p99_ms']:.1f} ms, {batched['queries_per_s']:.1f} q/s, "
+               f"{batched['batches']} batches")
+        res_s, single = run("fp64", 1, False, gaps)
+        log_fn(f"serving {rate:g} q/s, one-at-a-time FP64: p50 {single['p50_ms']:.1f} ms, p99 "
+               f"{single['p99_ms']:.1f} ms, {single['queries_per_s']:.1f} q/s")
+        same = all(np.array_equal(a.target, b.target) and np.array_equal(a.obj2, b.obj2)
+                   and np.array_equal(a.rmin, b.rmin) and np.array_equal(a.rmax, b.rmax)
+                   for a, b in zip(res_b, res_s))
+        out["by_rate"].append({
+            "arrival_rate_per_s": rate, "batched_mixed": batched, "one_at_a_time_fp64": single,
+            "p50_speedup": single["p50_ms"] / batched["p50_ms"],
             "p99_speedup": single["p99_ms"] / batched["p99_ms"],
             "throughput_speedup": batched["queries_per_s"] / single["queries_per_s"],
-            "identical_results": bool(same)}
+            "identical_results": bool(same)})
+    return out
 
 
 def main():
@@ -208,7 +217,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-queries", type=int, default=24,
                     help="serving run (resident vessels, Poisson arrivals) after the join bench; 0: off")
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="query arrivals per second")
+    ap.add_argument("--serve-rates", default="8,32",
+                    help="query arrival rates (per second) of the serving runs")
     ap.add_argument("--tune", default="", help="kernel tuning, e.g. drain=0,max_rows=4")
     ap.add_argument("--autotune", type=int, default=4000,
                     help="autotune on this many targets before timing (0: off)")
@@ -219,6 +229,8 @@ def main():
                     help="streaming self-join of this many nuclei after the join bench; 0: off")
     ap.add_argument("--only-stream", type=int, default=0,
                     help="run only the streaming self-join and print its figures")
+    ap.add_argument("--only-serve", type=int, default=0,
+                    help="run only the serving benchmark and print its figures")
     ap.add_argument("--stream-cube", type=float, default=50000.0)
     ap.add_argument("--stream-budget-gib", type=float, default=24.0)
     args = ap.parse_args()
@@ -244,6 +256,14 @@ def main():
     import tempfile
     import numpy as np
     tdp = importlib.import_module("3dpipe_amd")
+    if args.only_serve:
+        gen_kw = dict(seed=args.seed, n_vessels=args.vessels, nuclei_per_vessel=args.nuclei,
+                      cube=args.cube, rings_min=args.rings_min, rings_max=args.rings_max)
+        _, ves = tdp.generate(**dict(gen_kw, max_nuclei=1))
+        print(json.dumps(serve_bench(tdp, ves, gen_kw, args.serve_queries,
+                                     [float(x) for x in args.serve_rates.split(",")],
+                                     args.seed + 100, log)), flush=True)
+        return
     if args.only_stream:
         print(json.dumps(stream_bench(tdp, args.stream_nuclei, args.stream_cube,
                                       args.stream_budget_gib, args.knn, args.seed, None, log)),
@@ -371,7 +391,8 @@ def main():
                                         tuning_used, log)
     if args.serve_queries > 0:
         del nuc, results, r
-        out["serving"] = serve_bench(tdp, ves, gen_kw, args.serve_queries, args.serve_rate,
+        out["serving"] = serve_bench(tdp, ves, gen_kw, args.serve_queries,
+                                     [float(x) for x in args.serve_rates.split(",")],
                                      args.seed + 100, log)
     print(json.dumps(out), flush=True)
 

@@ -248,6 +248,9 @@ const char *tdb_server_error(const tdb_server *s);
 /* [0] batches run, [1] queries done, [2] most queries in a batch,
  * [3] peak device bytes held (resident tile included) */
 void tdb_server_stats(tdb_server *s, uint64_t *out4);
+/* host time (ms, summed over batches) of [0] tile merge, [1] upload, [2]
+ * candidates (the preparer) and [3] joins (the executor) */
+void tdb_server_stage_ms(tdb_server *s, double *out4);
 /* finishes the queued queries, then stops */
 void tdb_server_close(tdb_server *s);
 

@@ -105,6 +105,11 @@ int tdp_host_alloc(tdp_ctx *ctx, size_t bytes, void **out);
 int tdp_host_free(tdp_ctx *ctx, void *p);
 /* wait until the work queued on ctx's stream (fills) has finished */
 int tdp_sync(tdp_ctx *ctx);
+/* a point in ctx's stream (after the work queued so far): *ticket names it;
+ * tdp_wait_mark blocks until the stream has passed it (for reusing a pinned
+ * buffer whose copy was queued before the mark) */
+int tdp_mark(tdp_ctx *ctx, uint64_t *ticket);
+int tdp_wait_mark(tdp_ctx *ctx, uint64_t ticket);
 /* an empty device tile with room for n_obj objects, n_vox voxels and n_tri
  * triangle records */
 int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t n_tri, tdp_tile **out);
@@ -113,7 +118,7 @@ int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t n_tri, tdp
  * voxel owners and the triangle bounds are queued on the stream of the
  * context that reserved the tile and the call returns at once. desc->tri may
  * be NULL: the records then come in pieces through tdp_tile_fill_tri_async.
- * obj_box / obj_transform are not kept (no device candidate queries). */
+ * obj_box / obj_transform are copied when given (for device candidates). */
 int tdp_tile_fill_async(tdp_tile *tile, const tdp_tile_desc *desc);
 /* records [first, first + n) of the tile being filled, from pinned memory */
 int tdp_tile_fill_tri_async(tdp_tile *tile, size_t first, size_t n, const float *records);
