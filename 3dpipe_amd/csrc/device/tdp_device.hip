@@ -1355,8 +1355,8 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                 k_mx_expand<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
                                                   A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
                                                   slot, (uint32_t)j->tune.max_rows, j->mx_flags,
-                                                  j->mx_woff, j->mx_mask, j->pos, j->item_vp,
-                                                  j->item_q);
+                                                  j->mx_woff, j->mx_mask, j->keep, j->pos,
+                                                  j->item_vp, j->item_q);
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
                 k_mx_eval<<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(

@@ -525,13 +525,14 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
     const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, int slot,
     uint32_t max_rows, const uint32_t *__restrict__ flags, const uint64_t *__restrict__ woff,
-    const unsigned long long *__restrict__ mask, const uint64_t *__restrict__ offs,
-    uint32_t *__restrict__ item_vp, uint32_t *__restrict__ item_q) {
+    const unsigned long long *__restrict__ mask, const uint32_t *__restrict__ count,
+    const uint64_t *__restrict__ offs, uint32_t *__restrict__ item_vp,
+    uint32_t *__restrict__ item_q) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
+    if (count[w] == 0) return;  // no candidate (empty voxels among them)
     const int lane = lane_id();
     const uint32_t fl = flags[w];
-    if (fl & MX_EMPTY) return;
     const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
     uint64_t pos = offs[w];
     if (fl & MX_SMALL) {

@@ -110,7 +110,7 @@ def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn):
 
 
 def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
-    """Config-5 style serving: `vessels` resident; n_queries fresh nucleus
+    """Serving benchmark: `vessels` resident; n_queries fresh nucleus
     tiles of log-uniform size 100..50000 (seeded), each kNN with k in
     {1, 5, 10, 20} or within with tau U(50, 500), arriving as a Poisson
     process (unit-rate gaps scaled to each of `rates` queries/s). At each rate

@@ -25,7 +25,7 @@ def tdp():
 
 @pytest.fixture(scope="session")
 def small_tiles(tdp, tmp_path_factory):
-    """Config-1 style tiles: 10 vessels x 20 nuclei in a 2000^3 cube, seed 0,
+    """Small tiles: 10 vessels x 20 nuclei in a 2000^3 cube, seed 0,
     written as RAW .dt files so that the oracle reads the same bytes."""
     d = tmp_path_factory.mktemp("tiles")
     n, v = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=20, cube=2000.0)
