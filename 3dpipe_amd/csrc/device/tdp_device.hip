@@ -39,6 +39,8 @@ struct tdp_ctx {
     std::mutex mem_mu;
     std::unordered_map<void *, size_t> dsize;
     uint64_t mem_limit = 0, mem_used = 0, mem_peak = 0;
+    // timing events of finished joins, reused by later ones
+    std::vector<hipEvent_t> ev_pool;
     // points in the stream's queue the host can wait for (tdp_mark)
     static constexpr int kMarks = 16;
     hipEvent_t marks[kMarks] = {};
@@ -406,6 +408,7 @@ extern "C" int tdp_close(tdp_ctx *ctx) {
     if (!ctx) return TDP_ERR_ARG;
     for (auto &e : ctx->marks)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (void *p : ctx->scan_buf) dfree(ctx, p);
     dfree(ctx, ctx->d_total);
     pool_release_all(ctx);
@@ -948,9 +951,9 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
         pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
     }
     for (auto &e : j->ev)
-        if (e) (void)hipEventDestroy(e);
+        if (e) ctx->ev_pool.push_back(e);
     for (auto &e : j->sev)
-        if (e) (void)hipEventDestroy(e);
+        if (e) ctx->ev_pool.push_back(e);
     pfree(ctx, j->words5);
     pfree(ctx, j->d_words);
     pfree(ctx, j->d_kept);
@@ -993,14 +996,11 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
         ctx->err = "tdp_join_create: too many candidate pairs for 32-bit pair ids";
         return TDP_ERR_ARG;
     }
-    std::vector<uint32_t> tgt(P);
-    for (size_t t = 0; t < N; t++) {
+    for (size_t t = 0; t < N; t++)
         if (prefix[t] > prefix[t + 1]) {
             ctx->err = "tdp_join_create: cand_prefix not monotone";
             return TDP_ERR_ARG;
         }
-        for (uint32_t i = prefix[t]; i < prefix[t + 1]; i++) tgt[i] = (uint32_t)t;
-    }
     for (size_t i = 0; i < P; i++)
         if (cand[i] >= t2->n_obj) {
             ctx->err = "tdp_join_create: candidate id outside tile 2";
@@ -1039,7 +1039,11 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
     }
     hipError_t e = hipMemcpy(j->prefix, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess && P) e = hipMemcpy(j->pair_obj2, cand, 4 * P, hipMemcpyHostToDevice);
-    if (e == hipSuccess && P) e = hipMemcpy(j->pair_target, tgt.data(), 4 * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess && N) {
+        k_pair_targets<<<blocks_for(N, kBlock), kBlock, 0, ctx->stream>>>(N, j->prefix,
+                                                                        j->pair_target);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess && N) {
         std::vector<int32_t> kv(N);
         std::vector<float> tv(N);
@@ -1054,10 +1058,17 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
     if (e == hipSuccess) e = hipMemset(j->confirmed, 0, 4 * (N ? N : 1));
     if (e == hipSuccess) e = hipMemset(j->round, 0xff, P ? P : 1);
     if (e == hipSuccess) e = hipMemset(j->state, 0, P ? P : 1);
-    for (auto &ev : j->ev)
-        if (e == hipSuccess) e = hipEventCreate(&ev);
-    for (auto &ev : j->sev)
-        if (e == hipSuccess) e = hipEventCreate(&ev);
+    auto take_event = [&](hipEvent_t &ev) {
+        if (e != hipSuccess) return;
+        if (!ctx->ev_pool.empty()) {
+            ev = ctx->ev_pool.back();
+            ctx->ev_pool.pop_back();
+        } else {
+            e = hipEventCreate(&ev);
+        }
+    };
+    for (auto &ev : j->ev) take_event(ev);
+    for (auto &ev : j->sev) take_event(ev);
     if (e != hipSuccess) {
         ctx->err = std::string("tdp_join_create: ") + hipGetErrorString(e);
         tdp_join_destroy(j);

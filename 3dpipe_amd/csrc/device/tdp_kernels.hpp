@@ -827,6 +827,14 @@ __global__ void __launch_bounds__(kBlock) k_decide_intersect(size_t P, R *__rest
     }
 }
 
+// the target of every candidate pair, from the per-target list offsets
+__global__ void __launch_bounds__(kBlock) k_pair_targets(size_t N, const uint32_t *__restrict__ prefix,
+                                                         uint32_t *__restrict__ pair_target) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    for (uint32_t i = prefix[t]; i < prefix[t + 1]; i++) pair_target[i] = (uint32_t)t;
+}
+
 // every object pair of the intersection join enters the filter undecided
 __global__ void __launch_bounds__(kBlock) k_state_alive(size_t P, uint8_t *__restrict__ state,
                                                         uint8_t *__restrict__ round) {

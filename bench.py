@@ -28,7 +28,7 @@ F_TRI = 997
 # Static FP32 operation count of the mixed form's per-pair disc test, the
 # part of k_mx_mark every triangle pair goes through (disc_interval() and the
 # candidate comparison; the breakdown is beside disc_interval()).
-F_MARK = 89
+F_MARK = 68
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch on the default workload (seed 1, 200k nuclei x 1000
 # vessels, 10-NN, mixed form, default tuning), from rocprofv3 --pmc FETCH_SIZE
