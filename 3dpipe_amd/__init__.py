@@ -103,6 +103,8 @@ def _load():
     tdb.tdb_result_targets.restype = u64
     tdb.tdb_result_get.argtypes = [_P, _P, _P, _P, _P, _P]
     tdb.tdb_result_timing.argtypes = [_P, _P]
+    tdb.tdb_result_timing_n.argtypes = [_P, _P, _c.c_int]
+    tdb.tdb_result_timing_n.restype = _c.c_int
     tdb.tdb_result_stats.argtypes = [_P, _P]
     tdb.tdb_result_free.argtypes = [_P]
     tdb.tdb_result_stages.argtypes = [_P, _P, _P, _c.c_int]
@@ -330,9 +332,9 @@ def _read_result(r) -> "JoinResult":
         mn = np.zeros(n, np.float64); mx = np.zeros(n, np.float64)
         rd = np.zeros(n, np.uint8)
         tdb.tdb_result_get(r, _ptr(t), _ptr(b), _ptr(mn), _ptr(mx), _ptr(rd))
-        tm = np.zeros(7, np.float64)
+        tm = np.zeros(10, np.float64)
         st = np.zeros(7, np.uint64)
-        tdb.tdb_result_timing(r, _ptr(tm))
+        tdb.tdb_result_timing_n(r, _ptr(tm), 10)
         tdb.tdb_result_stats(r, _ptr(st))
         sm = np.zeros(len(STAGES), np.float64)
         sn = np.zeros(len(STAGES), np.uint64)
@@ -343,7 +345,7 @@ def _read_result(r) -> "JoinResult":
     finally:
         tdb.tdb_result_free(r)
     timing = dict(zip(["total", "candidates", "filter", "refine", "filter_kernels",
-                       "refine_kernels", "upload"], tm.tolist()))
+                       "refine_kernels", "upload", "setup", "download", "assembly"], tm.tolist()))
     stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
                       "object_pairs", "rounds", "fp32_culled"], (int(x) for x in st)))
     return JoinResult(t, b, mn, mx, rd, nt, timing, stats, dict(zip(STAGES, sm.tolist())),

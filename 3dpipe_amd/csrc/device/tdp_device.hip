@@ -41,6 +41,13 @@ struct tdp_ctx {
     uint64_t mem_limit = 0, mem_used = 0, mem_peak = 0;
     // timing events of finished joins, reused by later ones
     std::vector<hipEvent_t> ev_pool;
+    // pinned host blocks given back through tdp_host_free, kept for later
+    // tdp_host_alloc calls (pinning costs milliseconds per call), at most
+    // kHostCache bytes; freed at tdp_close (under mem_mu)
+    static constexpr size_t kHostCache = size_t(1) << 30;
+    std::multimap<size_t, void *> host_free;
+    std::unordered_map<void *, size_t> host_size;
+    size_t host_cached = 0;
     // points in the stream's queue the host can wait for (tdp_mark)
     static constexpr int kMarks = 16;
     hipEvent_t marks[kMarks] = {};
@@ -409,6 +416,7 @@ extern "C" int tdp_close(tdp_ctx *ctx) {
     for (auto &e : ctx->marks)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto &kv : ctx->host_free) (void)hipHostFree(kv.second);
     for (void *p : ctx->scan_buf) dfree(ctx, p);
     dfree(ctx, ctx->d_total);
     pool_release_all(ctx);
@@ -581,13 +589,44 @@ extern "C" int tdp_memory_usage(tdp_ctx *ctx, uint64_t *used, uint64_t *peak, in
 extern "C" int tdp_host_alloc(tdp_ctx *ctx, size_t bytes, void **out) {
     if (!ctx || !out) return TDP_ERR_ARG;
     *out = nullptr;
-    TDP_CHECK(ctx, hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    bytes = bytes ? bytes : 1;
+    {
+        // a cached block of at least the size and at most twice it
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        auto it = ctx->host_free.lower_bound(bytes);
+        if (it != ctx->host_free.end() && it->first / 2 <= bytes) {
+            *out = it->second;
+            ctx->host_cached -= it->first;
+            ctx->host_free.erase(it);
+            return TDP_OK;
+        }
+    }
+    void *p = nullptr;
+    TDP_CHECK(ctx, hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    std::lock_guard<std::mutex> lk(ctx->mem_mu);
+    ctx->host_size[p] = bytes;
+    *out = p;
     return TDP_OK;
 }
 
 extern "C" int tdp_host_free(tdp_ctx *ctx, void *p) {
     if (!ctx) return TDP_ERR_ARG;
-    if (p) TDP_CHECK(ctx, hipHostFree(p));
+    if (!p) return TDP_OK;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        auto it = ctx->host_size.find(p);
+        if (it == ctx->host_size.end()) {
+            ctx->err = "tdp_host_free: not a block of this context";
+            return TDP_ERR_ARG;
+        }
+        if (ctx->host_cached + it->second <= tdp_ctx::kHostCache) {
+            ctx->host_free.emplace(it->second, p);
+            ctx->host_cached += it->second;
+            return TDP_OK;
+        }
+        ctx->host_size.erase(it);
+    }
+    TDP_CHECK(ctx, hipHostFree(p));
     return TDP_OK;
 }
 
@@ -1363,11 +1402,11 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             if (items) {
                 rc = ensure_items(j, items);
                 if (rc != TDP_OK) return rc;
-                k_mx_expand<<<gw, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
-                                                  A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
-                                                  slot, (uint32_t)j->tune.max_rows, j->mx_flags,
-                                                  j->mx_woff, j->mx_mask, j->keep, j->pos,
-                                                  j->item_vp, j->item_q);
+                // one wave per kWave voxel pairs
+                k_mx_expand<<<wave_grid((n + kWave - 1) / kWave), kBlock, 0, s>>>(
+                    n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off,
+                    B->vox_tri_cnt, slot, (uint32_t)j->tune.max_rows, j->mx_flags, j->mx_woff,
+                    j->mx_mask, j->keep, j->pos, j->item_vp, j->item_q);
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
                 k_mx_eval<<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(

@@ -77,10 +77,12 @@ template <> struct Key<double> {
 
 // object-level [min boxmin, min coredist] over all voxel pairs of each
 // candidate pair (voxel_pair_dist_kernel_for_all_aggregate_distance,
-// computing_gpu.cu:1984-2043); one wave per object pair. The lanes hold the
-// tile-2 voxels (records in registers), the wave walks the tile-1 voxels in a
-// uniform loop (scalar loads); squared distances are min-reduced and the
-// square roots taken once (sqrt is monotone, so the result is the same).
+// computing_gpu.cu:1984-2043); one wave per object pair. The wave walks the
+// n1 x n2 voxel grid in tiles of R rows x cw columns (R * cw <= 64, the fewest
+// iterations over 1-3 column chunks): lanes keep their column voxel (of the
+// object with more voxels) in registers and load their row voxel per
+// iteration; squared distances are min-reduced and the square roots taken once
+// (sqrt is monotone, and the min does not depend on the order).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_aggregate(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
@@ -89,23 +91,54 @@ __global__ void __launch_bounds__(kBlock) k_aggregate(
     R *__restrict__ rmax) {
     const uint32_t p = __builtin_amdgcn_readfirstlane((uint32_t)wave_global());
     if (p >= P) return;
-    const int lane = lane_id();
+    const uint32_t lane = (uint32_t)lane_id();
     const uint32_t a0 = vo1[pair_target[p]], n1 = vo1[pair_target[p] + 1] - a0;
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
     R smin = (R)__builtin_inff(), cmin = (R)__builtin_inff();
-    for (uint32_t jb = 0; jb < n2; jb += kWave) {
-        const uint32_t j = jb + lane;
-        const bool valid = j < n2;
-        float bl[9];
-        const float *src = vb2 + 9 * (size_t)(b0 + (valid ? j : n2 - 1));
+    const bool sw = n1 > n2;  // columns: tile-1 voxels
+    const uint32_t nl = sw ? n1 : n2, ns = sw ? n2 : n1;
+    if (ns > 0) {
+        uint32_t cw = kWave, rows = 1, nch = 0, ngr = 0, best = 0xffffffffu;
+        const uint32_t m0 = (nl + kWave - 1) / kWave;
+        for (uint32_t m = m0; m < m0 + 3; m++) {
+            const uint32_t w = (nl + m - 1) / m;
+            const uint32_t r = min((uint32_t)kWave / w, ns);
+            const uint32_t c = (nl + w - 1) / w, g = (ns + r - 1) / r;
+            if (c * g < best) {
+                best = c * g;
+                cw = w;
+                rows = r;
+                nch = c;
+                ngr = g;
+            }
+        }
+        const float *__restrict__ lb = sw ? vb1 + 9 * (size_t)a0 : vb2 + 9 * (size_t)b0;
+        const float *__restrict__ sb = sw ? vb2 + 9 * (size_t)b0 : vb1 + 9 * (size_t)a0;
+        const uint32_t r = lane / cw, c = lane - r * cw;
+        const bool in_tile = r < rows;
+        for (uint32_t ch = 0; ch < nch; ch++) {
+            const uint32_t col = ch * cw + c;
+            const bool cv = in_tile && col < nl;
+            float lv[9];
+            const float *src = lb + 9 * (size_t)(cv ? col : nl - 1);
 #pragma unroll
-        for (int k = 0; k < 9; k++) bl[k] = src[k];
-        for (uint32_t i = 0; i < n1; i++) {
-            R s, c;
-            voxel_sq<R>(vb1 + 9 * (size_t)(a0 + i), bl, s, c);
-            if (valid) {
-                smin = rmin2(smin, s);
-                cmin = rmin2(cmin, c);
+            for (int k = 0; k < 9; k++) lv[k] = src[k];
+            for (uint32_t gr = 0; gr < ngr; gr++) {
+                const uint32_t row = gr * rows + r;
+                const float *rv = sb + 9 * (size_t)(row < ns ? row : ns - 1);
+                float sv[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) sv[k] = rv[k];
+                R s, cd;
+                // the tile-1 voxel is always the first operand
+                if (sw)
+                    voxel_sq<R>(lv, sv, s, cd);
+                else
+                    voxel_sq<R>(sv, lv, s, cd);
+                if (cv && row < ns) {
+                    smin = rmin2(smin, s);
+                    cmin = rmin2(cmin, cd);
+                }
             }
         }
     }

@@ -520,7 +520,12 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     if (lane == 0) count[w] = c;
 }
 
-// Bit masks (and SMALL voxel pairs) to the flat work list at offs[w].
+// Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One wave
+// per kWave consecutive voxel pairs: lane i loads the figures of voxel pair
+// w0 + i and works out its tiling (all 64 at once), then the wave expands the
+// voxel pairs that have items one after another, reading each one's figures
+// from its lane into scalar registers. (One wave per voxel pair spent most of
+// its instructions on the per-pair set-up: the tiling's divisions.)
 __global__ void __launch_bounds__(kBlock) k_mx_expand(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
@@ -529,58 +534,123 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
     const unsigned long long *__restrict__ mask, const uint32_t *__restrict__ count,
     const uint64_t *__restrict__ offs, uint32_t *__restrict__ item_vp,
     uint32_t *__restrict__ item_q) {
-    const uint32_t w = wave_item();
-    if (w >= n_vp) return;
-    if (count[w] == 0) return;  // no candidate (empty voxels among them)
+    const uint64_t w0 = (uint64_t)wave_item() * kWave;
+    if (w0 >= n_vp) return;
     const int lane = lane_id();
-    const uint32_t fl = flags[w];
-    const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
-    uint64_t pos = offs[w];
-    if (fl & MX_SMALL) {
-        const uint32_t total = g.n1 * g.n2;
-        for (uint32_t q = lane; q < total; q += kWave) {
-            item_vp[pos + q] = w;
-            item_q[pos + q] = q;
+    const uint64_t wl = w0 + (uint64_t)lane;
+    // lane's voxel pair: items (SMALL: all n1 * n2 pairs) or tiling
+    uint32_t c = 0, l_small = 0, l_n2 = 0, l_cw = 1, l_R = 1, l_ngr = 1, l_nw = 0, l_swap = 0;
+    uint64_t l_pos = 0, l_wo = 0;
+    if (wl < n_vp) {
+        c = count[wl];
+        if (c) {  // no items: no candidate, or empty voxels among them
+            const uint32_t fl = flags[wl];
+            const VpLod g = vp_lod(vp_v1[wl], vp_v2[wl], off1, cnt1, off2, cnt2, slot);
+            l_n2 = g.n2;
+            l_pos = offs[wl];
+            if (fl & MX_SMALL) {
+                l_small = g.n1 * g.n2;
+            } else {
+                l_wo = woff[wl];
+                const Tiling t = mx_tiling(g, max_rows);
+                l_cw = t.cw;
+                l_R = t.R;
+                l_ngr = t.ngr;
+                l_nw = mx_words(t);
+                l_swap = t.swap;
+            }
         }
-        return;
     }
-    const Tiling t = mx_tiling(g, max_rows);
-    const uint32_t n2 = g.n2, nw = mx_words(t);
-    const unsigned long long *__restrict__ in = mask + woff[w];
-    // bit b of a word is row b / cw, column b % cw of its tile: a per-wave
-    // table instead of a division per bit
-    __shared__ uint16_t tab[kWavesPerBlock][kWave];
-    uint16_t *mytab = tab[threadIdx.x / kWave];
-    {
-        const uint32_t r = (uint32_t)lane / t.cw;
-        mytab[lane] = (uint16_t)((r << 8) | ((uint32_t)lane - r * t.cw));
-    }
-    wave_sync();
-    for (uint32_t wb = 0; wb < nw; wb += kWave) {
-        const uint32_t k = wb + lane;
-        unsigned long long m = k < nw ? in[k] : 0ull;
-        const uint32_t cnt = (uint32_t)__popcll(m);
-        // exclusive prefix of cnt over the lanes
-        uint32_t incl = cnt;
+    // Per group of 64 mask words: lane k holds word k, its inclusive prefix
+    // of set bits and its tile origin (in LDS); then lane i of each pass
+    // takes set bit number base + i of the group -- its word by binary search
+    // over the prefixes, its bit by halving the word -- so consecutive lanes
+    // write consecutive items.
+    __shared__ unsigned long long s_word[kWavesPerBlock][kWave];
+    __shared__ uint32_t s_incl[kWavesPerBlock][kWave];
+    __shared__ uint32_t s_row0[kWavesPerBlock][kWave], s_col0[kWavesPerBlock][kWave];
+    // bit b of a word is row b / cw, column b % cw of its tile
+    __shared__ uint16_t s_tab[kWavesPerBlock][kWave];
+    const int wv = threadIdx.x / kWave;
+    unsigned long long *const sw = s_word[wv];
+    uint32_t *const si = s_incl[wv];
+    uint32_t *const sr = s_row0[wv];
+    uint32_t *const sc = s_col0[wv];
+    uint16_t *const tab = s_tab[wv];
+    auto rl = [](uint32_t v, int L) { return (uint32_t)__builtin_amdgcn_readlane((int)v, L); };
+    auto rl64 = [&](uint64_t v, int L) {
+        return (uint64_t)rl((uint32_t)v, L) | ((uint64_t)rl((uint32_t)(v >> 32), L) << 32);
+    };
+    unsigned long long live = __ballot(c != 0);
+    while (live) {
+        const int L = __ffsll((long long)live) - 1;
+        live &= live - 1ull;
+        const uint32_t w = (uint32_t)(w0 + (uint64_t)L);
+        uint64_t pos = rl64(l_pos, L);
+        const uint32_t small = rl(l_small, L);
+        if (small) {
+            for (uint32_t q = lane; q < small; q += kWave) {
+                item_vp[pos + q] = w;
+                item_q[pos + q] = q;
+            }
+            continue;
+        }
+        const uint32_t n2 = rl(l_n2, L), cw = rl(l_cw, L), R = rl(l_R, L), ngr = rl(l_ngr, L),
+                       nw = rl(l_nw, L);
+        const bool swap = rl(l_swap, L) != 0;
+        const unsigned long long *__restrict__ in = mask + rl64(l_wo, L);
+        {
+            // lane / cw through the reciprocal: lane < 64 and cw <= 64 keep
+            // (lane + 1/2) / cw at least 1 / 128 from an integer
+            const uint32_t r =
+                (uint32_t)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)cw));
+            tab[lane] = (uint16_t)((r << 8) | ((uint32_t)lane - r * cw));
+        }
+        for (uint32_t wb = 0; wb < nw; wb += kWave) {
+            const uint32_t k = wb + lane;
+            const unsigned long long m = k < nw ? in[k] : 0ull;
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            uint32_t incl = cnt;
 #pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t v = (uint32_t)__shfl_up((int)incl, o, kWave);
-            if (lane >= o) incl += v;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t v = (uint32_t)__shfl_up((int)incl, o, kWave);
+                if (lane >= o) incl += v;
+            }
+            const uint32_t total = rl(incl, kWave - 1);
+            const uint32_t ch = k / ngr, gr = k - ch * ngr;
+            sw[lane] = m;
+            si[lane] = incl;
+            sr[lane] = gr * R;
+            sc[lane] = ch * cw;
+            wave_sync();
+            for (uint32_t base = 0; base < total; base += kWave) {
+                const uint32_t idx = base + (uint32_t)lane;
+                if (idx < total) {
+                    // word: the number of prefixes <= idx (< 64 as si[63] > idx)
+                    uint32_t W = 0;
+#pragma unroll
+                    for (uint32_t st = kWave / 2; st >= 1; st >>= 1)
+                        if (si[W + st - 1] <= idx) W += st;
+                    unsigned long long x = sw[W];
+                    uint32_t rank = idx - (si[W] - (uint32_t)__popcll(x)), b = 0;
+#pragma unroll
+                    for (uint32_t h = 32; h >= 1; h >>= 1) {
+                        const uint32_t cb = (uint32_t)__popcll(x & ((1ull << h) - 1ull));
+                        if (rank >= cb) {
+                            rank -= cb;
+                            x >>= h;
+                            b += h;
+                        }
+                    }
+                    const uint32_t e = tab[b];
+                    const uint32_t row = sr[W] + (e >> 8), col = sc[W] + (e & 255u);
+                    item_vp[pos + idx] = w;
+                    item_q[pos + idx] = swap ? col * n2 + row : row * n2 + col;
+                }
+            }
+            wave_sync();
+            pos += total;
         }
-        const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
-        uint64_t at = pos + (incl - cnt);
-        const uint32_t ch = k / t.ngr, gr = k - ch * t.ngr;
-        const uint32_t row0 = gr * t.R, col0 = ch * t.cw;
-        while (m) {
-            const uint32_t b = (uint32_t)__ffsll((long long)m) - 1u;
-            m &= m - 1ull;
-            const uint32_t e = mytab[b];
-            const uint32_t row = row0 + (e >> 8), col = col0 + (e & 255u);
-            item_vp[at] = w;
-            item_q[at] = t.swap ? col * n2 + row : row * n2 + col;
-            at++;
-        }
-        pos += total;
     }
 }
 

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -43,12 +44,14 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     prm.precision = o->precision;
     prm.hausdorff_level = o->hausdorff_level;
     tdp_join *j = nullptr;
+    auto t0 = Clock::now();
     int rc = tdp_join_create_multi(ctx, d1, d2, &prm, N, prefix.data(), ids.data(), k_t, tau_t, &j);
     if (rc != TDP_OK) {
         err = tdp_last_error(ctx);
         return rc;
     }
-    auto t0 = Clock::now();
+    res->timing[7] = ms_since(t0);
+    t0 = Clock::now();
     uint64_t n_vp = 0;
     rc = tdp_join_filter(j, &n_vp);
     res->timing[2] = ms_since(t0);
@@ -61,9 +64,16 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
         res->stats[5]++;
     }
     res->timing[3] = ms_since(t0);
-    std::unique_ptr<uint8_t[]> rnd(new uint8_t[P ? P : 1]);
-    std::unique_ptr<double[]> mn(new double[P ? P : 1]), mx(new double[P ? P : 1]);
-    if (rc == TDP_OK) rc = tdp_join_results(j, rnd.get(), mn.get(), mx.get());
+    t0 = Clock::now();
+    // the per-pair results come down into pinned memory (kept by the context
+    // between joins): a DMA at full rate, no page faults
+    void *pin = nullptr;
+    if (rc == TDP_OK) rc = tdp_host_alloc(ctx, 17 * (P ? P : 1), &pin);
+    std::unique_ptr<void, std::function<void(void *)>> pin_hold(
+        pin, [ctx](void *p) { tdp_host_free(ctx, p); });
+    double *const mn = (double *)pin, *const mx = mn + P;
+    uint8_t *const rnd = (uint8_t *)(mx + P);
+    if (rc == TDP_OK) rc = tdp_join_results(j, rnd, mn, mx);
     uint64_t st[5] = {0, 0, 0, 0, 0};
     if (rc == TDP_OK) rc = tdp_join_stats(j, st);
     if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
@@ -74,6 +84,8 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
         return rc;
     }
     tdp_join_destroy(j);
+    res->timing[8] = ms_since(t0);
+    t0 = Clock::now();
     for (int k = 0; k < 4; k++) res->stats[k] = st[k];
     res->stats[6] = st[4];
     // per target: reported pairs ordered by (round, list position); targets
@@ -116,6 +128,7 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
             r = next;
         }
     }
+    res->timing[9] = ms_since(t0);
     return TDP_OK;
 }
 
@@ -625,16 +638,28 @@ uint64_t tdb_result_targets(const tdb_result *r) { return r ? r->n_targets : 0; 
 
 void tdb_result_get(const tdb_result *r, int32_t *target, int32_t *obj2, double *rmin,
                     double *rmax, uint8_t *round) {
-    const size_t n = r->target.size();
-    if (target) std::copy(r->target.begin(), r->target.end(), target);
-    if (obj2) std::copy(r->obj2.begin(), r->obj2.end(), obj2);
-    if (rmin) std::copy(r->rmin.begin(), r->rmin.end(), rmin);
-    if (rmax) std::copy(r->rmax.begin(), r->rmax.end(), rmax);
-    if (round) std::copy(r->round.begin(), r->round.begin() + n, round);
+    const long n = (long)r->target.size();
+    // in slices over the threads (the caller's arrays are often fresh)
+    const long sl = 1 << 16;
+#pragma omp parallel for schedule(static) if (n > 4 * sl)
+    for (long a = 0; a < n; a += sl) {
+        const long b = std::min(n, a + sl);
+        if (target) std::copy(r->target.begin() + a, r->target.begin() + b, target + a);
+        if (obj2) std::copy(r->obj2.begin() + a, r->obj2.begin() + b, obj2 + a);
+        if (rmin) std::copy(r->rmin.begin() + a, r->rmin.begin() + b, rmin + a);
+        if (rmax) std::copy(r->rmax.begin() + a, r->rmax.begin() + b, rmax + a);
+        if (round) std::copy(r->round.begin() + a, r->round.begin() + b, round + a);
+    }
 }
 
 void tdb_result_timing(const tdb_result *r, double *out7) {
     for (int k = 0; k < 7; k++) out7[k] = r->timing[k];
+}
+int tdb_result_timing_n(const tdb_result *r, double *out, int n) {
+    if (!r || !out || n < 0) return 0;
+    n = std::min(n, 10);
+    for (int k = 0; k < n; k++) out[k] = r->timing[k];
+    return n;
 }
 void tdb_result_stats(const tdb_result *r, uint64_t *out7) {
     for (int k = 0; k < 7; k++) out7[k] = r->stats[k];

@@ -36,12 +36,33 @@ struct tdb_engine {
     std::unordered_map<const tdb_tile *, tdp_tile *> resident;
 };
 
+// Leaves new elements uninitialised on resize (they are all written
+// afterwards, in parallel, which also spreads the page faults over threads).
+template <typename T>
+struct NoInit : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <typename U>
+    NoInit(const NoInit<U> &) {}
+    template <typename U>
+    void construct(U *) noexcept {}
+    template <typename U, typename... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <typename T>
+using ResultVec = std::vector<T, NoInit<T>>;
+
 struct tdb_result {
     uint64_t n_targets = 0;
-    std::vector<int32_t> target, obj2;
-    std::vector<double> rmin, rmax;
-    std::vector<uint8_t> round;
-    double timing[7] = {0, 0, 0, 0, 0, 0, 0};
+    ResultVec<int32_t> target, obj2;
+    ResultVec<double> rmin, rmax;
+    ResultVec<uint8_t> round;
+    double timing[10] = {};
     double stage_ms[TDP_STAGE_COUNT] = {};
     uint64_t stage_n[TDP_STAGE_COUNT] = {};
     uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};

@@ -382,7 +382,7 @@ void tdb_server::executor() {
                     r->rmax.push_back(all.rmax[k]);
                     r->round.push_back(all.round[k]);
                 }
-                std::copy(all.timing, all.timing + 7, r->timing);
+                std::copy(all.timing, all.timing + 10, r->timing);
                 r->timing[0] = d.latency_ms;
                 std::copy(all.stage_ms, all.stage_ms + TDP_STAGE_COUNT, r->stage_ms);
                 std::copy(all.stage_n, all.stage_n + TDP_STAGE_COUNT, r->stage_n);

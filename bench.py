@@ -303,8 +303,11 @@ def main():
         r = run(args.precision)
         times.append(time.perf_counter() - t0)
         results.append(r)
-        log(f"step {s}: {times[-1] * 1e3:.1f} ms (filter {r.timing_ms['filter']:.1f}, refine "
-            f"{r.timing_ms['refine']:.1f}, refine kernels {r.timing_ms['refine_kernels']:.1f})")
+        tm = r.timing_ms
+        log(f"step {s}: {times[-1] * 1e3:.1f} ms (candidates {tm['candidates']:.1f}, set-up "
+            f"{tm['setup']:.1f}, filter {tm['filter']:.1f}, refine {tm['refine']:.1f}, refine "
+            f"kernels {tm['refine_kernels']:.1f}, download {tm['download']:.1f}, assembly "
+            f"{tm['assembly']:.1f})")
     r = results[-1]
     ms = 1e3 * sum(times) / len(times)
     tri_pairs = r.stats["tri_pairs"]

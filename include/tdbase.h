@@ -156,6 +156,10 @@ void tdb_result_get(const tdb_result *r, int32_t *target, int32_t *obj2, double 
  * [3] refinement, [4] filter kernels (device), [5] refine kernels (device),
  * [6] tile upload */
 void tdb_result_timing(const tdb_result *r, double *out7);
+/* the first n (<= 10) timing figures: the seven above, then [7] join set-up
+ * (state and candidate lists to the device), [8] result download, [9] result
+ * assembly on the host. Returns the number written. */
+int tdb_result_timing_n(const tdb_result *r, double *out, int n);
 /* [0] triangle pairs covered, [1] voxel pairs evaluated, [2] FP64 TriDist
  * evaluations (mixed form), [3] voxel pairs after the filter, [4] candidate
  * object pairs, [5] rounds, [6] triangle pairs excluded by the FP32 bounds

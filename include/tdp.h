@@ -100,7 +100,10 @@ size_t tdp_tile_bytes(const tdp_tile *tile);
 int tdp_set_memory_limit(tdp_ctx *ctx, uint64_t bytes);
 /* bytes held now and the most held (since open, or the last reset) */
 int tdp_memory_usage(tdp_ctx *ctx, uint64_t *used, uint64_t *peak, int reset_peak);
-/* pinned host memory for the asynchronous fills */
+/* pinned host memory for the asynchronous fills and fast downloads; freed
+ * blocks are kept by the context (up to 1 GiB) and handed out again by later
+ * calls, so a per-call buffer costs no pinning after the first call. Free
+ * every block before tdp_close. */
 int tdp_host_alloc(tdp_ctx *ctx, size_t bytes, void **out);
 int tdp_host_free(tdp_ctx *ctx, void *p);
 /* wait until the work queued on ctx's stream (fills) has finished */
