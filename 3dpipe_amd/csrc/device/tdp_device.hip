@@ -93,6 +93,7 @@ struct tdp_join {
     float *mx_U = nullptr;
     uint32_t *mx_probe = nullptr, *mx_flags = nullptr;
     unsigned long long *mx_key = nullptr;
+    VpPlan *mx_plan = nullptr;  // per voxel pair: triangles and tiling (k_mx_plan)
     uint32_t *mx_nw = nullptr;   // candidate-mask words per voxel pair
     uint64_t *mx_woff = nullptr; // their exclusive scan
     size_t cap_mx = 0;
@@ -984,7 +985,8 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     void *ptrs[] = {j->work, j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
-                    j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q};
+                    j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q,
+                    j->mx_plan};
     for (void *p : ptrs) pfree(ctx, p);
     for (int b = 0; b < 2; b++) {
         pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
@@ -1155,9 +1157,9 @@ static int ensure_mx(tdp_join *j, size_t n) {
     tdp_ctx *ctx = j->ctx;
     const size_t cap = std::max(n, j->cap_vp);
     pfree(ctx, j->mx_U); pfree(ctx, j->mx_probe); pfree(ctx, j->mx_flags); pfree(ctx, j->mx_key);
-    pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff);
+    pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff); pfree(ctx, j->mx_plan);
     j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
-    j->mx_nw = nullptr; j->mx_woff = nullptr;
+    j->mx_nw = nullptr; j->mx_woff = nullptr; j->mx_plan = nullptr;
     j->cap_mx = 0;
     int rc = palloc(ctx, &j->mx_U, 3 * cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_probe, 3 * cap);
@@ -1165,6 +1167,7 @@ static int ensure_mx(tdp_join *j, size_t n) {
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_key, 3 * cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_nw, cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_woff, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_plan, cap);
     if (rc == TDP_OK) j->cap_mx = cap;
     return rc;
 }
@@ -1356,12 +1359,15 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         const dim3 gw = wave_grid(n);
         // quantities this round folds: d (bit 0), lo (bit 1), hi (bit 2)
         const uint32_t need = fold_need(j, is_last);
+        k_mx_plan<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+            n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
+            slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan,
+            j->mx_flags, j->mx_nw);
+        TDP_LAUNCHED(ctx);
         with_need(need, [&](auto nd) {
             k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-                n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd,
-                B->vox_tri_off, B->vox_tri_cnt, B->tri, B->bnd, slot, need,
-                (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
-                j->mx_flags, j->mx_key, j->mx_nw);
+                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, need, j->mx_U, j->mx_probe,
+                j->mx_key);
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_BOUNDS) != TDP_OK) return TDP_ERR_HIP;
@@ -1372,25 +1378,20 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         rc = ensure_mask(j, words);
         if (rc != TDP_OK) return rc;
         k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
-            3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-            A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
+            3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
         with_need(need, [&](auto nd) {
             k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-                n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd,
-                B->vox_tri_off, B->vox_tri_cnt, B->tri, B->bnd, slot, (uint32_t)j->tune.max_rows,
-                j->mx_U, j->mx_probe, j->mx_flags, j->mx_key, j->mx_woff, j->mx_mask, j->keep);
+                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
+                j->mx_woff, j->mx_mask, j->keep);
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
         if (j->tune.drain) {
             // persistent FP64 stage: no work list, no host read-back
             const int nb = j->tune.drain_blocks ? j->tune.drain_blocks : 4 * ctx->cu_count;
-            k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off,
-                                             A->vox_tri_cnt, A->tri, B->vox_tri_off,
-                                             B->vox_tri_cnt, B->tri, slot,
-                                             (uint32_t)j->tune.max_rows, j->mx_flags, j->mx_woff,
+            k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->mx_plan, A->tri, B->tri, j->mx_woff,
                                              j->mx_mask, j->mx_key, j->stats);
             TDP_LAUNCHED(ctx);
             if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
@@ -1404,15 +1405,13 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                 if (rc != TDP_OK) return rc;
                 // one wave per kWave voxel pairs
                 k_mx_expand<<<wave_grid((n + kWave - 1) / kWave), kBlock, 0, s>>>(
-                    n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off,
-                    B->vox_tri_cnt, slot, (uint32_t)j->tune.max_rows, j->mx_flags, j->mx_woff,
-                    j->mx_mask, j->keep, j->pos, j->item_vp, j->item_q);
+                    n, j->mx_plan, j->mx_woff, j->mx_mask, j->keep, j->pos, j->item_vp,
+                    j->item_q);
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
                 k_mx_eval<<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(
-                    items, j->item_vp, j->item_q, 0, j->v1[c], j->v2[c], A->vox_tri_off,
-                    A->vox_tri_cnt, A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot,
-                    j->mx_key, j->stats);
+                    items, j->item_vp, j->item_q, 0, j->mx_plan, A->tri, B->tri, j->mx_key,
+                    j->stats);
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
             }
@@ -1534,29 +1533,28 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_cnt,
                                                            B->vox_tri_cnt, slot, j->stats);
     launched();
+    k_mx_plan<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+        n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
+        slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan, j->mx_flags,
+        j->mx_nw);
+    launched();
     with_need(need, [&](auto nd) {
         k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-            n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd, B->vox_tri_off,
-            B->vox_tri_cnt, B->tri, B->bnd, slot, need, (uint32_t)j->tune.small_pairs,
-            (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe, j->mx_flags, j->mx_key, j->mx_nw);
+            n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, need, j->mx_U, j->mx_probe, j->mx_key);
     });
     launched();
     if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, j->d_words);
     k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
-        3 * n, nullptr, j->mx_probe, 1, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-        A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot, j->mx_key, j->stats);
+        3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
     launched();
     with_need(need, [&](auto nd) {
         k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-            n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, A->tri, A->bnd, B->vox_tri_off,
-            B->vox_tri_cnt, B->tri, B->bnd, slot, (uint32_t)j->tune.max_rows, j->mx_U, j->mx_probe,
-            j->mx_flags, j->mx_key, j->mx_woff, j->mx_mask, j->keep);
+            n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
+            j->mx_woff, j->mx_mask, j->keep);
     });
     launched();
-    k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt,
-                                     A->tri, B->vox_tri_off, B->vox_tri_cnt, B->tri, slot,
-                                     (uint32_t)j->tune.max_rows, j->mx_flags, j->mx_woff,
-                                     j->mx_mask, j->mx_key, j->stats);
+    k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->mx_plan, A->tri, B->tri, j->mx_woff, j->mx_mask,
+                                     j->mx_key, j->stats);
     launched();
     k_mx_fold<R><<<blocks_for(n, kBlock), kBlock, 0, s>>>(
         n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,

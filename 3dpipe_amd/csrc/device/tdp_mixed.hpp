@@ -201,6 +201,63 @@ __device__ __forceinline__ Tiling mx_tiling(const VpLod &g, uint32_t max_rows) {
 
 __device__ __forceinline__ uint32_t mx_words(const Tiling &t) { return t.nch * t.ngr; }
 
+// A voxel pair's figures for the round, worked out once (k_mx_plan, one
+// thread per voxel pair) and read by every pass in one load: where its
+// triangles are, how its grid is tiled, whether it is EMPTY or SMALL. (Each
+// wave working them out again cost a chain of dependent loads and the
+// tiling's integer divisions per pass.)
+struct VpPlan {
+    uint64_t o1, o2;    // first triangle of each voxel at the round's LOD
+    uint32_t n1, n2;    // triangles of each voxel
+    uint32_t nch, ngr;  // tiling (zero for EMPTY / SMALL)
+    uint32_t shape;     // cw | R << 8 | swap << 16 | flags << 24
+    uint32_t nw;        // mask words (zero for EMPTY / SMALL)
+};
+static_assert(sizeof(VpPlan) == 40, "VpPlan layout");
+
+__device__ __forceinline__ VpLod plan_lod(const VpPlan &p) { return VpLod{p.o1, p.o2, p.n1, p.n2}; }
+__device__ __forceinline__ uint32_t plan_flags(const VpPlan &p) { return p.shape >> 24; }
+__device__ __forceinline__ Tiling plan_tiling(const VpPlan &p) {
+    Tiling t;
+    t.cw = p.shape & 255u;
+    t.R = (p.shape >> 8) & 255u;
+    t.swap = ((p.shape >> 16) & 1u) != 0;
+    t.nch = p.nch;
+    t.ngr = p.ngr;
+    return t;
+}
+
+// Plans of the round's voxel pairs, with their flags and mask word counts:
+// EMPTY (a voxel without triangles at this LOD), SMALL (at most small_pairs
+// triangle pairs: all go to FP64, no mask), else tiled for the FP32 passes.
+__global__ void __launch_bounds__(kBlock) k_mx_plan(
+    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
+    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
+    const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, int slot,
+    uint32_t small_pairs, uint32_t max_rows, VpPlan *__restrict__ plan,
+    uint32_t *__restrict__ flags, uint32_t *__restrict__ nwords) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_vp) return;
+    const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+    VpPlan p{g.o1, g.o2, g.n1, g.n2, 0u, 0u, 0u, 0u};
+    uint32_t fl = 0;
+    if (g.n1 == 0 || g.n2 == 0) {
+        fl = MX_EMPTY;
+    } else if ((uint64_t)g.n1 * g.n2 <= small_pairs) {
+        fl = MX_SMALL;
+    } else {
+        const Tiling t = mx_tiling(g, max_rows);
+        p.nch = t.nch;
+        p.ngr = t.ngr;
+        p.shape = t.cw | (t.R << 8) | (t.swap ? 1u << 16 : 0u);
+        p.nw = mx_words(t);
+    }
+    p.shape |= fl << 24;
+    plan[w] = p;
+    flags[w] = fl;
+    nwords[w] = p.nw;
+}
+
 struct PairAt {
     uint32_t i, j;     // triangle indices within the two voxels
     uint32_t word;     // iteration number
@@ -290,37 +347,27 @@ __device__ __forceinline__ bool reaches_n(const float lw[3], const float U[3]) {
 // Per voxel pair: the smallest FP32 upper bound of each quantity the round
 // folds (vpU; -inf for the others) and up to three probe pairs, the pairs of
 // smallest score (centre distance less the radii, with the quantity's
-// Hausdorff terms), whose FP64 values usually are the minima. Also the flags
-// and the mask word count of the tiled walk. NEED as for reaches_n (and
-// `need` the same bits at run time).
+// Hausdorff terms), whose FP64 values usually are the minima. NEED as for
+// reaches_n (and `need` the same bits at run time).
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_bounds(
-    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
-    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
-    const float4 *__restrict__ rec1, const float4 *__restrict__ bnd1,
-    const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2,
-    const float4 *__restrict__ rec2, const float4 *__restrict__ bnd2, int slot, uint32_t need,
-    uint32_t small_pairs, uint32_t max_rows, float *__restrict__ vpU,
-    uint32_t *__restrict__ probe, uint32_t *__restrict__ flags,
-    unsigned long long *__restrict__ mkey, uint32_t *__restrict__ nwords) {
+    uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
+    const float4 *__restrict__ bnd1, const float4 *__restrict__ rec2,
+    const float4 *__restrict__ bnd2, uint32_t need, float *__restrict__ vpU,
+    uint32_t *__restrict__ probe, unsigned long long *__restrict__ mkey) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
     const int lane = lane_id();
-    const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+    const VpPlan pl = plan[w];
     if (lane < 3) {
         mkey[3 * (size_t)w + lane] = dkey(INFINITY);
         probe[3 * (size_t)w + lane] = kNoProbe;
         vpU[3 * (size_t)w + lane] = -INFINITY;
     }
-    if (g.n1 == 0 || g.n2 == 0 || g.n1 * g.n2 <= small_pairs) {
-        if (lane == 0) {
-            flags[w] = (g.n1 == 0 || g.n2 == 0) ? MX_EMPTY : MX_SMALL;
-            nwords[w] = 0;
-        }
-        return;
-    }
+    if (plan_flags(pl)) return;  // EMPTY or SMALL
+    const VpLod g = plan_lod(pl);
     const uint32_t n2 = g.n2;
-    const Tiling t = mx_tiling(g, max_rows);
+    const Tiling t = plan_tiling(pl);
     float up[3] = {INFINITY, INFINITY, INFINITY};
     float best[3] = {INFINITY, INFINITY, INFINITY};
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
@@ -363,11 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
         vpU[3 * (size_t)w + lane] = lane == 0 ? U[0] : (lane == 1 ? U[1] : U[2]);
         probe[3 * (size_t)w + lane] = lane == 0 ? pq[0] : (lane == 1 ? pq[1] : pq[2]);
     }
-    if (lane == 0) {
-        flags[w] = 0;
-        nwords[w] = mx_words(t);
-        if (lo_zero) mkey[3 * (size_t)w + 1] = dkey(0.0);  // the minimum is +0
-    }
+    if (lane == 0 && lo_zero) mkey[3 * (size_t)w + 1] = dkey(0.0);  // the minimum is +0
 }
 
 // ------------------------------------------------------------- FP64 lists
@@ -406,11 +449,9 @@ __device__ __forceinline__ void seg_min_fold(uint32_t w, unsigned long long a,
 // in voxel-pair order).
 __global__ void __launch_bounds__(kBlock) k_mx_eval(
     uint64_t n_items, const uint32_t *__restrict__ item_vp, const uint32_t *__restrict__ item_q,
-    int vp_from_index3, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
-    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
-    const float4 *__restrict__ rec1, const uint64_t *__restrict__ off2,
-    const uint32_t *__restrict__ cnt2, const float4 *__restrict__ rec2, int slot,
-    unsigned long long *__restrict__ mkey, unsigned long long *__restrict__ stats) {
+    int vp_from_index3, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
+    const float4 *__restrict__ rec2, unsigned long long *__restrict__ mkey,
+    unsigned long long *__restrict__ stats) {
     // grid-stride: the work list can exceed one launch's thread count
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const unsigned long long inf_key = dkey(INFINITY);
@@ -425,7 +466,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
             w = vp_from_index3 ? (uint32_t)(k / 3) : item_vp[k];
             if (q != kNoProbe) {
                 act = true;
-                const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+                const VpPlan &pl = plan[w];
+                const uint64_t o1 = pl.o1, o2 = pl.o2;
+                const uint32_t n2 = pl.n2;
+                const VpLod g{o1, o2, 0u, n2};
                 const uint32_t i = q / g.n2, j = q - i * g.n2;
                 double d, lo, hi;
                 pair_values<double>(rec1, g.o1 + i, rec2, g.o2 + j, d, lo, hi);
@@ -474,24 +518,22 @@ __device__ __forceinline__ bool mx_candidate(const TriS &A, const TriS &B, const
 // (U is -inf for the others).
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
-    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
-    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
-    const float4 *__restrict__ rec1, const float4 *__restrict__ bnd1,
-    const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2,
-    const float4 *__restrict__ rec2, const float4 *__restrict__ bnd2, int slot,
-    uint32_t max_rows, const float *__restrict__ vpU, const uint32_t *__restrict__ probe,
-    const uint32_t *__restrict__ flags, const unsigned long long *__restrict__ mkey,
+    uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
+    const float4 *__restrict__ bnd1, const float4 *__restrict__ rec2,
+    const float4 *__restrict__ bnd2, const float *__restrict__ vpU,
+    const uint32_t *__restrict__ probe, const unsigned long long *__restrict__ mkey,
     const uint64_t *__restrict__ woff,
     unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
     const int lane = lane_id();
-    const uint32_t fl = flags[w];
+    const VpPlan pl = plan[w];
+    const uint32_t fl = plan_flags(pl);
     if (fl & MX_EMPTY) {
         if (lane == 0) count[w] = 0;
         return;
     }
-    const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+    const VpLod g = plan_lod(pl);
     if (fl & MX_SMALL) {
         if (lane == 0) count[w] = g.n1 * g.n2;
         return;
@@ -500,8 +542,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     bounds_U(vpU, mkey, w, U);
     const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
                    p2 = probe[3 * (size_t)w + 2];
-    const Tiling t = mx_tiling(g, max_rows);
-    const uint32_t n2 = g.n2, nw = mx_words(t);
+    const Tiling t = plan_tiling(pl);
+    const uint32_t n2 = g.n2, nw = pl.nw;
     unsigned long long *__restrict__ out = mask + woff[w];
     uint32_t c = 0;
     unsigned long long mine = 0;  // lane k keeps word (k mod 64) of the current group
@@ -521,16 +563,13 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
 }
 
 // Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One wave
-// per kWave consecutive voxel pairs: lane i loads the figures of voxel pair
-// w0 + i and works out its tiling (all 64 at once), then the wave expands the
-// voxel pairs that have items one after another, reading each one's figures
-// from its lane into scalar registers. (One wave per voxel pair spent most of
-// its instructions on the per-pair set-up: the tiling's divisions.)
+// per kWave consecutive voxel pairs: lane i loads the plan of voxel pair
+// w0 + i (all 64 at once), then the wave expands the voxel pairs that have
+// items one after another, reading each one's figures from its lane into
+// scalar registers. (One wave per voxel pair spent most of its time on the
+// per-pair set-up.)
 __global__ void __launch_bounds__(kBlock) k_mx_expand(
-    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
-    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
-    const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, int slot,
-    uint32_t max_rows, const uint32_t *__restrict__ flags, const uint64_t *__restrict__ woff,
+    uint64_t n_vp, const VpPlan *__restrict__ plan, const uint64_t *__restrict__ woff,
     const unsigned long long *__restrict__ mask, const uint32_t *__restrict__ count,
     const uint64_t *__restrict__ offs, uint32_t *__restrict__ item_vp,
     uint32_t *__restrict__ item_q) {
@@ -544,19 +583,18 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
     if (wl < n_vp) {
         c = count[wl];
         if (c) {  // no items: no candidate, or empty voxels among them
-            const uint32_t fl = flags[wl];
-            const VpLod g = vp_lod(vp_v1[wl], vp_v2[wl], off1, cnt1, off2, cnt2, slot);
-            l_n2 = g.n2;
+            const VpPlan pl = plan[wl];
+            l_n2 = pl.n2;
             l_pos = offs[wl];
-            if (fl & MX_SMALL) {
-                l_small = g.n1 * g.n2;
+            if (plan_flags(pl) & MX_SMALL) {
+                l_small = pl.n1 * pl.n2;
             } else {
                 l_wo = woff[wl];
-                const Tiling t = mx_tiling(g, max_rows);
+                const Tiling t = plan_tiling(pl);
                 l_cw = t.cw;
                 l_R = t.R;
                 l_ngr = t.ngr;
-                l_nw = mx_words(t);
+                l_nw = pl.nw;
                 l_swap = t.swap;
             }
         }
@@ -667,11 +705,8 @@ constexpr uint32_t kDrainQueue = 2 * kWave;
 // lanes however few candidates a voxel pair has, and no global work list.
 // Minima fold as in k_mx_eval (the queue keeps voxel-pair order).
 __global__ void __launch_bounds__(kBlock) k_mx_drain(
-    uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
-    const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
-    const float4 *__restrict__ rec1, const uint64_t *__restrict__ off2,
-    const uint32_t *__restrict__ cnt2, const float4 *__restrict__ rec2, int slot,
-    uint32_t max_rows, const uint32_t *__restrict__ flags, const uint64_t *__restrict__ woff,
+    uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
+    const float4 *__restrict__ rec2, const uint64_t *__restrict__ woff,
     const unsigned long long *__restrict__ mask, unsigned long long *__restrict__ mkey,
     unsigned long long *__restrict__ stats) {
     __shared__ uint32_t qw[kWavesPerBlock][kDrainQueue], qq[kWavesPerBlock][kDrainQueue];
@@ -724,18 +759,18 @@ __global__ void __launch_bounds__(kBlock) k_mx_drain(
                     done = true;
                     break;
                 }
-                const uint32_t fl = flags[w];
+                const VpPlan pl = plan[w];
+                const uint32_t fl = plan_flags(pl);
                 if (fl & MX_EMPTY) continue;
-                const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
                 cur = w;
-                n2 = g.n2;
+                n2 = pl.n2;
                 small = (fl & MX_SMALL) != 0;
                 if (small) {
-                    total = g.n1 * g.n2;
+                    total = pl.n1 * pl.n2;
                     k = 0;
                 } else {
-                    t = mx_tiling(g, max_rows);
-                    nw = mx_words(t);
+                    t = plan_tiling(pl);
+                    nw = pl.nw;
                     in = mask + woff[w];
                     wb = 0;
                     wave_sync();
@@ -775,7 +810,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_drain(
         if ((uint32_t)lane < cnt) {
             w = mw[lane];
             const uint32_t q = mq[lane];
-            const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+            const VpPlan &pl = plan[w];
+            const VpLod g{pl.o1, pl.o2, 0u, pl.n2};
             const uint32_t i = q / g.n2, j = q - i * g.n2;
             double d, lo, hi;
             pair_values<double>(rec1, g.o1 + i, rec2, g.o2 + j, d, lo, hi);
