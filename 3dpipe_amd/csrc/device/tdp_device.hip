@@ -242,6 +242,14 @@ static void pool_release_all(tdp_ctx *ctx) {
 
 // join buffers come from the context's block cache (callers release them only
 // when no queued kernel uses them: after a stream synchronisation)
+// zero count float4 records at p (tile padding), synchronously
+static int zero_pad(tdp_ctx *ctx, float4 *p, size_t count) {
+    if (!count) return TDP_OK;
+    TDP_CHECK(ctx, hipMemsetAsync(p, 0, count * sizeof(float4), ctx->stream));
+    TDP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return TDP_OK;
+}
+
 template <typename T>
 static int palloc(tdp_ctx *ctx, T **p, size_t count) {
     *p = nullptr;
@@ -521,11 +529,21 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
         up(&t->obj_box, d->obj_box, 6 * d->n_obj);
         up(&t->obj_tr, d->obj_transform, 3 * d->n_obj);
     }
+    // records and bounds carry kTriPad zeroed triangles past the last one
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->tri, 3 * (d->n_tri + kTriPad));
+    if (rc == TDP_OK) {
+        t->bytes += 48 * (d->n_tri + kTriPad);
+        rc = zero_pad(ctx, t->tri + 3 * d->n_tri, 3 * kTriPad);
+    }
     if (!tri_parts) {
-        up(&t->tri, (const float4 *)d->tri, 3 * d->n_tri);
+        if (rc == TDP_OK && d->n_tri) {
+            hipError_t e = hipMemcpy(t->tri, d->tri, 48 * d->n_tri, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                ctx->err = std::string("tile upload: ") + hipGetErrorString(e);
+                rc = TDP_ERR_HIP;
+            }
+        }
     } else if (rc == TDP_OK) {
-        rc = dalloc(ctx, &t->tri, 3 * d->n_tri);
-        if (rc == TDP_OK) t->bytes += 48 * d->n_tri;
         size_t at = 0;
         for (size_t k = 0; rc == TDP_OK && k < n_parts; k++) {
             if (part_n_tri[k]) {
@@ -539,9 +557,12 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
             at += part_n_tri[k];
         }
     }
-    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * d->n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * (d->n_tri + kTriPad));
+    if (rc == TDP_OK) {
+        t->bytes += sizeof(float4) * kBnd * (d->n_tri + kTriPad);
+        rc = zero_pad(ctx, t->bnd + kBnd * d->n_tri, kBnd * kTriPad);
+    }
     if (rc == TDP_OK && d->n_tri) {
-        t->bytes += sizeof(float4) * kBnd * d->n_tri;
         k_tri_bounds<<<blocks_for(d->n_tri, kBlock), kBlock, 0, ctx->stream>>>(d->n_tri, t->tri, t->bnd);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -669,15 +690,20 @@ extern "C" int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t
     if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_obj, n_vox);
     if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_tri_off, 5 * n_vox);
     if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_tri_cnt, 5 * n_vox);
-    if (rc == TDP_OK) rc = dalloc(ctx, &t->tri, 3 * n_tri);
-    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * n_tri);
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->tri, 3 * (n_tri + kTriPad));
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * (n_tri + kTriPad));
+    // zeroed whole: the padding past the capacity, and records a smaller fill
+    // leaves untouched, are always readable values
+    if (rc == TDP_OK) rc = zero_pad(ctx, t->tri, 3 * (n_tri + kTriPad));
+    if (rc == TDP_OK) rc = zero_pad(ctx, t->bnd, kBnd * (n_tri + kTriPad));
     if (rc == TDP_OK) rc = dalloc(ctx, &t->obj_box, 6 * n_obj);
     if (rc == TDP_OK) rc = dalloc(ctx, &t->obj_tr, 3 * n_obj);
     if (rc != TDP_OK) {
         tdp_tile_free(t);
         return rc;
     }
-    t->bytes = (n_obj + 1) * 4 + n_obj * 76 + n_vox * (36 + 4 + 40 + 20) + n_tri * 48 * (1 + kBnd);
+    t->bytes = (n_obj + 1) * 4 + n_obj * 76 + n_vox * (36 + 4 + 40 + 20) +
+               (n_tri + kTriPad) * 16 * (3 + kBnd);
     *out = t;
     return TDP_OK;
 }

@@ -423,6 +423,10 @@ constexpr uint32_t kMixedMinPairs = 2 * kWave;  // below this, evaluate every pa
 // plane's even for thin triangles (the FP64 cross product's error relative
 // to |n| stays below 1e-9 under the 1e-6 aspect cut-off).
 constexpr int kBnd = 3;
+// zeroed triangles (records and bounds) after the last one of every device
+// tile: a tiled walk reads up to kWave - 1 rows past a voxel's last triangle
+// without clamping its index
+constexpr size_t kTriPad = kWave;
 
 // owning object of every voxel, from the per-object voxel offsets
 __global__ void __launch_bounds__(kBlock) k_vox_owner(size_t n_obj,

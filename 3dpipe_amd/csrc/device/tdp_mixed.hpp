@@ -69,9 +69,12 @@ struct TriS {
     float cx, cy, cz, r, nx, ny, nz, cm, ph, h, dg;  // dg = 1: no plane
 };
 
-__device__ __forceinline__ TriS load_s(const float4 *__restrict__ bnd, uint64_t t) {
-    const float4 a = bnd[kBnd * t], b = bnd[kBnd * t + 1], c = bnd[kBnd * t + 2];
+__device__ __forceinline__ TriS load_s_at(const float4 *__restrict__ p) {
+    const float4 a = p[0], b = p[1], c = p[2];
     return TriS{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z};
+}
+__device__ __forceinline__ TriS load_s(const float4 *__restrict__ bnd, uint64_t t) {
+    return load_s_at(bnd + kBnd * t);
 }
 
 // vertices of one triangle
@@ -79,9 +82,12 @@ struct TriV {
     float x0, y0, z0, x1, y1, z1, x2, y2, z2;
 };
 
-__device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t t) {
-    const float4 a = rec[3 * t], b = rec[3 * t + 1], c = rec[3 * t + 2];
+__device__ __forceinline__ TriV load_v_at(const float4 *__restrict__ p) {
+    const float4 a = p[0], b = p[1], c = p[2];
     return TriV{a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+}
+__device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t t) {
+    return load_v_at(rec + 3 * t);
 }
 
 // Disc interval of the FP64 TriDist value of (A, B), A from tile 1, from
@@ -291,13 +297,20 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const uint64_t lt = ol + (cv ? col : nl - 1);  // in bounds for idle lanes
         const TriS L = load_s(bl, lt);
         const TriV LV = load_v(rl, lt);
-        for (uint32_t gr = 0; gr < t.ngr; gr++, word++) {
+        // row records through pointers stepped R rows per group (no index
+        // arithmetic per iteration). Rows past the smaller voxel (last group,
+        // lanes outside the tile) are at most 63 past its last triangle: they
+        // read the next voxel's records or the tile's padding (kTriPad), and
+        // their cells are not valid.
+        const float4 *__restrict__ pb = bs + kBnd * (os + r);
+        const float4 *__restrict__ pr = rs + 3 * (os + r);
+        const size_t step_b = (size_t)kBnd * t.R, step_r = (size_t)3 * t.R;
+        for (uint32_t gr = 0; gr < t.ngr; gr++, word++, pb += step_b, pr += step_r) {
             const uint32_t row = gr * t.R + r;
             const bool valid = cv && row < ns;
-            const uint64_t st = os + (row < ns ? row : ns - 1);
-            const TriS S = load_s(bs, st);
+            const TriS S = load_s_at(pb);
             auto lane_v = [&]() { return LV; };
-            auto row_v = [&]() { return load_v(rs, st); };
+            auto row_v = [&]() { return load_v_at(pr); };
             if (SWAP)
                 f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v);
             else
@@ -326,6 +339,14 @@ __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int l
 }
 
 // ------------------------------------------------------------------ bounds
+
+// centre distance less both radii: a score only (it picks the probes),
+// contracted like the centre distance of disc_interval, which it then shares
+__device__ __forceinline__ float centre_score(const TriS &A, const TriS &B) {
+#pragma clang fp contract(fast)
+    const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
+    return __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - A.r - B.r;
+}
 
 // lowest q among the lanes holding the wave minimum of v (kNoProbe if none)
 __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
@@ -376,8 +397,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                   if (!p.valid) return;
                   float lw[3], uw[3];
                   disc_interval(A, B, lw, uw);
-                  const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
-                  const float sc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - A.r - B.r;
+                  const float sc = centre_score(A, B);
                   const float key[3] = {sc, sc - A.ph - B.ph, sc + A.h + B.h};
                   const uint32_t q = p.i * n2 + p.j;
 #pragma unroll
