@@ -1367,9 +1367,11 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                                                                 (KT *)j->acc_max);
     TDP_LAUNCHED(ctx);
     const tdp_tile *A = j->t1, *B = j->t2;
-    k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_cnt,
-                                                           B->vox_tri_cnt, slot, j->stats);
-    TDP_LAUNCHED(ctx);
+    if constexpr (MODE != MODE_MIXED) {  // the mixed form counts in k_mx_plan
+        k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+            n, j->v1[c], j->v2[c], A->vox_tri_cnt, B->vox_tri_cnt, slot, j->stats);
+        TDP_LAUNCHED(ctx);
+    }
     if (mark(TDP_STAGE_RESET) != TDP_OK) return TDP_ERR_HIP;
     if constexpr (MODE != MODE_MIXED) {
         k_refine<R, MODE><<<wave_grid(n), kBlock, 0, s>>>(
@@ -1388,7 +1390,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         k_mx_plan<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
             slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan,
-            j->mx_flags, j->mx_nw);
+            j->mx_flags, j->mx_nw, j->stats);
         TDP_LAUNCHED(ctx);
         with_need(need, [&](auto nd) {
             k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
@@ -1403,8 +1405,12 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         if (mark(TDP_STAGE_SCAN) != TDP_OK) return TDP_ERR_HIP;
         rc = ensure_mask(j, words);
         if (rc != TDP_OK) return rc;
-        k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
-            3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
+        with_need(need, [&](auto nd) {
+            k_mx_eval<decltype(nd)::value>
+                <<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
+                    3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key,
+                    j->stats);
+        });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
         with_need(need, [&](auto nd) {
@@ -1435,9 +1441,12 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                     j->item_q);
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EXPAND) != TDP_OK) return TDP_ERR_HIP;
-                k_mx_eval<<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(
-                    items, j->item_vp, j->item_q, 0, j->mx_plan, A->tri, B->tri, j->mx_key,
-                    j->stats);
+                with_need(need, [&](auto nd) {
+                    k_mx_eval<decltype(nd)::value>
+                        <<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(
+                            items, j->item_vp, j->item_q, 0, j->mx_plan, A->tri, B->tri,
+                            j->mx_key, j->stats);
+                });
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
             }
@@ -1556,13 +1565,10 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     k_reset_acc<R, KT><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (KT *)j->acc_min,
                                                                 (KT *)j->acc_max);
     launched();
-    k_round_stats<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->v1[c], j->v2[c], A->vox_tri_cnt,
-                                                           B->vox_tri_cnt, slot, j->stats);
-    launched();
     k_mx_plan<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
         n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
         slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan, j->mx_flags,
-        j->mx_nw);
+        j->mx_nw, j->stats);
     launched();
     with_need(need, [&](auto nd) {
         k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
@@ -1570,8 +1576,10 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     });
     launched();
     if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, j->d_words);
-    k_mx_eval<<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
-        3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
+    with_need(need, [&](auto nd) {
+        k_mx_eval<decltype(nd)::value><<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
+            3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
+    });
     launched();
     with_need(need, [&](auto nd) {
         k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(

@@ -236,32 +236,54 @@ __device__ __forceinline__ Tiling plan_tiling(const VpPlan &p) {
 // Plans of the round's voxel pairs, with their flags and mask word counts:
 // EMPTY (a voxel without triangles at this LOD), SMALL (at most small_pairs
 // triangle pairs: all go to FP64, no mask), else tiled for the FP32 passes.
+// Also the round's statistics as k_round_stats counts them (triangle pairs
+// covered, non-empty voxel pairs; one atomic per block).
 __global__ void __launch_bounds__(kBlock) k_mx_plan(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
     const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, int slot,
     uint32_t small_pairs, uint32_t max_rows, VpPlan *__restrict__ plan,
-    uint32_t *__restrict__ flags, uint32_t *__restrict__ nwords) {
+    uint32_t *__restrict__ flags, uint32_t *__restrict__ nwords,
+    unsigned long long *__restrict__ stats) {
+    __shared__ unsigned long long part[2][kWavesPerBlock];
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= n_vp) return;
-    const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
-    VpPlan p{g.o1, g.o2, g.n1, g.n2, 0u, 0u, 0u, 0u};
-    uint32_t fl = 0;
-    if (g.n1 == 0 || g.n2 == 0) {
-        fl = MX_EMPTY;
-    } else if ((uint64_t)g.n1 * g.n2 <= small_pairs) {
-        fl = MX_SMALL;
-    } else {
-        const Tiling t = mx_tiling(g, max_rows);
-        p.nch = t.nch;
-        p.ngr = t.ngr;
-        p.shape = t.cw | (t.R << 8) | (t.swap ? 1u << 16 : 0u);
-        p.nw = mx_words(t);
+    uint64_t pairs = 0;
+    if (w < n_vp) {
+        const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
+        VpPlan p{g.o1, g.o2, g.n1, g.n2, 0u, 0u, 0u, 0u};
+        uint32_t fl = 0;
+        pairs = (uint64_t)g.n1 * g.n2;
+        if (pairs == 0) {
+            fl = MX_EMPTY;
+        } else if (pairs <= small_pairs) {
+            fl = MX_SMALL;
+        } else {
+            const Tiling t = mx_tiling(g, max_rows);
+            p.nch = t.nch;
+            p.ngr = t.ngr;
+            p.shape = t.cw | (t.R << 8) | (t.swap ? 1u << 16 : 0u);
+            p.nw = mx_words(t);
+        }
+        p.shape |= fl << 24;
+        plan[w] = p;
+        flags[w] = fl;
+        nwords[w] = p.nw;
     }
-    p.shape |= fl << 24;
-    plan[w] = p;
-    flags[w] = fl;
-    nwords[w] = p.nw;
+    const uint64_t sp = wave_sum_u64(pairs), sv = wave_sum_u64(pairs ? 1u : 0u);
+    if (lane_id() == 0) {
+        part[0][threadIdx.x / kWave] = sp;
+        part[1][threadIdx.x / kWave] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int k = 0; k < kWavesPerBlock; k++) {
+            a += part[0][k];
+            b += part[1][k];
+        }
+        if (a) atomicAdd(&stats[0], a);
+        if (b) atomicAdd(&stats[1], b);
+    }
 }
 
 struct PairAt {
@@ -437,6 +459,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
 
 // Minima over the lanes of one voxel pair w (lanes are sorted by w; kNoProbe
 // marks an empty lane), folded with one atomic per voxel pair and quantity.
+// NEED: the quantities folded (bits 0-2: a, b, c); the others are left alone.
+template <uint32_t NEED = 7u>
 __device__ __forceinline__ void seg_min_fold(uint32_t w, unsigned long long a,
                                              unsigned long long b, unsigned long long c,
                                              unsigned long long *__restrict__ mkey) {
@@ -444,29 +468,36 @@ __device__ __forceinline__ void seg_min_fold(uint32_t w, unsigned long long a,
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
         const uint32_t ow = (uint32_t)__shfl_down((int)w, o, kWave);
-        const unsigned long long oa = __shfl_down(a, o, kWave);
-        const unsigned long long ob = __shfl_down(b, o, kWave);
-        const unsigned long long oc = __shfl_down(c, o, kWave);
-        if (lane + o < kWave && ow == w) {
-            a = min(a, oa);
-            b = min(b, ob);
-            c = min(c, oc);
+        const bool same = lane + o < kWave && ow == w;
+        if constexpr ((NEED & 1u) != 0) {
+            const unsigned long long oa = __shfl_down(a, o, kWave);
+            if (same) a = min(a, oa);
+        }
+        if constexpr ((NEED & 2u) != 0) {
+            const unsigned long long ob = __shfl_down(b, o, kWave);
+            if (same) b = min(b, ob);
+        }
+        if constexpr ((NEED & 4u) != 0) {
+            const unsigned long long oc = __shfl_down(c, o, kWave);
+            if (same) c = min(c, oc);
         }
     }
     const uint32_t pw = (uint32_t)__shfl_up((int)w, 1, kWave);
     const bool head = lane == 0 || pw != w;
     if (head && w != kNoProbe) {
         const unsigned long long inf_key = dkey(INFINITY);
-        if (a < inf_key) atomicMin(&mkey[3 * (size_t)w], a);
-        if (b < inf_key) atomicMin(&mkey[3 * (size_t)w + 1], b);
-        if (c < inf_key) atomicMin(&mkey[3 * (size_t)w + 2], c);
+        if ((NEED & 1u) && a < inf_key) atomicMin(&mkey[3 * (size_t)w], a);
+        if ((NEED & 2u) && b < inf_key) atomicMin(&mkey[3 * (size_t)w + 1], b);
+        if ((NEED & 4u) && c < inf_key) atomicMin(&mkey[3 * (size_t)w + 2], c);
     }
 }
 
 // FP64 TriDist of work items (voxel pair w, triangle pair q), min-folded into
 // the voxel pair's minima keys; items with q == kNoProbe are skipped. Items
 // arrive sorted by voxel pair (probe lists: w = k / 3; work lists: written
-// in voxel-pair order).
+// in voxel-pair order). NEED: the quantities the round folds (the others'
+// keys are not touched: the round does not use them).
+template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_eval(
     uint64_t n_items, const uint32_t *__restrict__ item_vp, const uint32_t *__restrict__ item_q,
     int vp_from_index3, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
@@ -498,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
                 c = dkey(hi);
             }
         }
-        seg_min_fold(w, a, b, c, mkey);
+        seg_min_fold<NEED>(w, a, b, c, mkey);
         n += (uint64_t)__popcll(__ballot(act));
     }
     if (lane_id() == 0 && n) atomicAdd(&stats[2], (unsigned long long)n);
