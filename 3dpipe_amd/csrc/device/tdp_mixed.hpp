@@ -29,6 +29,8 @@
 // that pass go on to the vertex-level interval.
 #pragma once
 
+#include <type_traits>
+
 #include "tdp_kernels.hpp"
 
 namespace tdp {
@@ -107,11 +109,7 @@ __device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t 
 // Sines are rounded up (16 ulp inside the root for the cosine's error); a
 // triangle without a plane has a zero normal, so only the other triangle's
 // plane can pass the guard.
-// Operation count on the separated path, as used by k_mx_mark (lower bounds
-// and the three comparisons; add/sub/mul/sqrt/rcp/min/max/compare = 1, abs
-// free): centre difference 3, dc 6, rs 1, m 5, e1 and e2 10, guard 5, disc
-// gap 21, du 2, s 5, lower bounds of lo and hi 7, comparisons with U 3 -- 68
-// per triangle pair.
+// k_mx_bounds uses the upper end; k_mx_mark the lower end (disc_lower).
 __device__ __forceinline__ void disc_interval(const TriS &A, const TriS &B, float lw[3],
                                               float uw[3]) {
     // fused multiply-adds round once where two operations round twice, so the
@@ -134,42 +132,6 @@ __device__ __forceinline__ void disc_interval(const TriS &A, const TriS &B, floa
         dl = fmaxf(dc - A.r * fminf(sa, 1.f) - B.r * fminf(sb, 1.f) - m, 0.f);
     }
     intervals(dl, dc + rs + m, A.ph, A.h, B.ph, B.h, lw, uw);
-}
-
-// Vertex-level interval (the arithmetic of pair_interval on register data).
-__device__ __forceinline__ void vertex_interval(const TriS &A, const TriV &a, const TriS &B,
-                                                const TriV &c, float lw[3], float uw[3]) {
-#pragma clang fp contract(fast)
-    const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
-    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-    const float rs = A.r + B.r;
-    const float m = 32.f * kU * (dc + rs + A.cm + B.cm) + 1e-6f;
-    const float du = dc + rs + m;
-    float dl = 0.f;
-    const float ax0 = a.x0 - A.cx, ay0 = a.y0 - A.cy, az0 = a.z0 - A.cz;
-    const float ax1 = a.x1 - A.cx, ay1 = a.y1 - A.cy, az1 = a.z1 - A.cz;
-    const float ax2 = a.x2 - A.cx, ay2 = a.y2 - A.cy, az2 = a.z2 - A.cz;
-    const float cx0 = c.x0 - A.cx, cy0 = c.y0 - A.cy, cz0 = c.z0 - A.cz;
-    const float cx1 = c.x1 - A.cx, cy1 = c.y1 - A.cy, cz1 = c.z1 - A.cz;
-    const float cx2 = c.x2 - A.cx, cy2 = c.y2 - A.cy, cz2 = c.z2 - A.cz;
-    const float t0 = A.nx * cx0 + A.ny * cy0 + A.nz * cz0;
-    const float t1 = A.nx * cx1 + A.ny * cy1 + A.nz * cz1;
-    const float t2 = A.nx * cx2 + A.ny * cy2 + A.nz * cz2;
-    const float w0 = B.nx * (ax0 - dx) + B.ny * (ay0 - dy) + B.nz * (az0 - dz);
-    const float w1 = B.nx * (ax1 - dx) + B.ny * (ay1 - dy) + B.nz * (az1 - dz);
-    const float w2 = B.nx * (ax2 - dx) + B.ny * (ay2 - dy) + B.nz * (az2 - dz);
-    const bool sep1 = fminf(t0, fminf(t1, t2)) > m || fmaxf(t0, fmaxf(t1, t2)) < -m;
-    const bool sep2 = fminf(w0, fminf(w1, w2)) > m || fmaxf(w0, fmaxf(w1, w2)) < -m;
-    if ((sep1 || sep2) && dc > 0.f) {
-        const float inv = __builtin_amdgcn_rcpf(dc);
-        const float ux = dx * inv, uy = dy * inv, uz = dz * inv;
-        const float smax = fmaxf(ux * ax0 + uy * ay0 + uz * az0,
-                                 fmaxf(ux * ax1 + uy * ay1 + uz * az1, ux * ax2 + uy * ay2 + uz * az2));
-        const float tmin = fminf(ux * cx0 + uy * cy0 + uz * cz0,
-                                 fminf(ux * cx1 + uy * cy1 + uz * cz1, ux * cx2 + uy * cy2 + uz * cz2));
-        dl = fmaxf(tmin - smax - m, 0.f);
-    }
-    intervals(dl, du, A.ph, A.h, B.ph, B.h, lw, uw);
 }
 
 // ------------------------------------------------------------ pair walking
@@ -292,17 +254,20 @@ struct PairAt {
     bool valid;
 };
 
-// Calls f(A, B, p, va, vc) for every cell of the tiling: A/B the tile-1 /
-// tile-2 sphere data, p the place, va()/vc() the vertices of A/B. All 64
-// lanes take part in every call (p.valid = false for cells outside the grid),
-// so f may use wave-wide operations. ROWS1: one row per iteration (R == 1),
-// the row triangle is wave-uniform.
-template <bool SWAP, bool ROWS1, typename F>
+// Calls f(A, B, p, va, vc, swap) for every cell of the tiling: A/B the
+// tile-1 / tile-2 sphere data, p the place, va()/vc() the vertices of A/B,
+// swap an std::integral_constant<bool> that is true when the column triangle
+// (the lane's, fixed over a column chunk) is A. All 64 lanes take part in
+// every call (p.valid = false for cells outside the grid), so f may use
+// wave-wide operations. fc(L) is called by every lane at the start of each
+// column chunk with the lane's column triangle. ROWS1: one row per iteration
+// (R == 1), the row triangle is wave-uniform.
+template <bool SWAP, bool ROWS1, typename FC, typename F>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
                                            const float4 *__restrict__ rec1,
                                            const float4 *__restrict__ bnd1,
                                            const float4 *__restrict__ rec2,
-                                           const float4 *__restrict__ bnd2, F &&f) {
+                                           const float4 *__restrict__ bnd2, FC &&fc, F &&f) {
     const uint32_t nl = SWAP ? g.n1 : g.n2, ns = SWAP ? g.n2 : g.n1;
     const uint64_t ol = SWAP ? g.o1 : g.o2, os = SWAP ? g.o2 : g.o1;
     const float4 *__restrict__ bl = SWAP ? bnd1 : bnd2;
@@ -319,6 +284,7 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const uint64_t lt = ol + (cv ? col : nl - 1);  // in bounds for idle lanes
         const TriS L = load_s(bl, lt);
         const TriV LV = load_v(rl, lt);
+        fc(L);
         // row records through pointers stepped R rows per group (no index
         // arithmetic per iteration). Rows past the smaller voxel (last group,
         // lanes outside the tile) are at most 63 past its last triangle: they
@@ -334,29 +300,31 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
             auto lane_v = [&]() { return LV; };
             auto row_v = [&]() { return load_v_at(pr); };
             if (SWAP)
-                f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v);
+                f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v,
+                  std::integral_constant<bool, true>{});
             else
-                f(S, L, PairAt{row, cv ? col : 0u, word, valid}, row_v, lane_v);
+                f(S, L, PairAt{row, cv ? col : 0u, word, valid}, row_v, lane_v,
+                  std::integral_constant<bool, false>{});
         }
     }
 }
 
-template <typename F>
+template <typename FC, typename F>
 __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int lane,
                                           const float4 *__restrict__ rec1,
                                           const float4 *__restrict__ bnd1,
                                           const float4 *__restrict__ rec2,
-                                          const float4 *__restrict__ bnd2, F &&f) {
+                                          const float4 *__restrict__ bnd2, FC &&fc, F &&f) {
     if (t.swap) {
         if (t.R == 1)
-            walk_tiles<true, true>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+            walk_tiles<true, true>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
         else
-            walk_tiles<true, false>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+            walk_tiles<true, false>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
     } else {
         if (t.R == 1)
-            walk_tiles<false, true>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+            walk_tiles<false, true>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
         else
-            walk_tiles<false, false>(g, t, lane, rec1, bnd1, rec2, bnd2, f);
+            walk_tiles<false, false>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
     }
 }
 
@@ -379,19 +347,13 @@ __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
     return c;
 }
 
-// the quantity tests a round needs: NEED (bit 0 d, 1 lo, 2 hi) is a launch
-// constant, so the arithmetic of the others drops out
-template <uint32_t NEED>
-__device__ __forceinline__ bool reaches_n(const float lw[3], const float U[3]) {
-    return ((NEED & 1u) && lw[0] <= U[0]) || ((NEED & 2u) && lw[1] <= U[1]) ||
-           ((NEED & 4u) && lw[2] <= U[2]);
-}
-
 // Per voxel pair: the smallest FP32 upper bound of each quantity the round
 // folds (vpU; -inf for the others) and up to three probe pairs, the pairs of
 // smallest score (centre distance less the radii, with the quantity's
-// Hausdorff terms), whose FP64 values usually are the minima. NEED as for
-// reaches_n (and `need` the same bits at run time).
+// Hausdorff terms), whose FP64 values usually are the minima. NEED: the
+// quantities the round folds (bit 0 d, 1 lo, 2 hi), a launch constant so that
+// the arithmetic of the others drops out (and `need` the same bits at run
+// time).
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
@@ -414,8 +376,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     float up[3] = {INFINITY, INFINITY, INFINITY};
     float best[3] = {INFINITY, INFINITY, INFINITY};
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
-    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
-              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&) {
+    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2, [](const TriS &) {},
+              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto) {
                   if (!p.valid) return;
                   float lw[3], uw[3];
                   disc_interval(A, B, lw, uw);
@@ -548,25 +510,120 @@ __device__ __forceinline__ void bounds_U(const float *vpU, const unsigned long l
     if (mkey[3 * w + 1] == dkey(0.0)) U[1] = -INFINITY;
 }
 
-// Candidate test: the disc interval first, the vertex interval for pairs
-// that pass it; probes (already evaluated) are never candidates.
-template <uint32_t NEED, typename VA, typename VC>
-__device__ __forceinline__ bool mx_candidate(const TriS &A, const TriS &B, const PairAt &p,
-                                             VA &&va, VC &&vc, const float U[3], uint32_t q,
-                                             uint32_t p0, uint32_t p1, uint32_t p2) {
-    if (!p.valid) return false;
-    float lw[3], uw[3];
-    disc_interval(A, B, lw, uw);
-    if (!reaches_n<NEED>(lw, U)) return false;
-    if (q == p0 || q == p1 || q == p2) return false;
-    vertex_interval(A, va(), B, vc(), lw, uw);
-    return reaches_n<NEED>(lw, U);
+// ------------------------------------------------------- candidate test
+//
+// k_mx_mark keeps a pair when a lower bound dl of its FP64 TriDist value d
+// does not exclude it from holding the minimum of a quantity the round
+// folds, i.e. dl <= T for T = max over the needed q of T_q, with
+//   T_d  = U_d,
+//   T_lo = U_lo + ph1 + ph2   (lo = max(d - ph1 - ph2, 0) <= U_lo implies
+//                              d <= U_lo + ph1 + ph2, U_lo >= 0),
+//   T_hi = U_hi - h1 - h2     (hi = d + h1 + h2 <= U_hi),
+// each rounded upwards: the column triangle's part is a per-lane constant of
+// the column chunk (MarkCol), so a pair pays one fused multiply-add per
+// quantity. Rounding: with H = 1 + 8u and G = 1 - 16u (u = 2^-24),
+//   fl(c H + fl(b H + fl(a H)))       >= (a + b + c)(1 + 4u),
+//   fl(-c G + fl(-b G + fl(a H)))     >= a - b - c + 3u a
+// for a, b, c >= 0 -- above the exact thresholds by more than the FP64
+// rounding of the quantities themselves (a few 2^-53 relative). U = -inf
+// (a quantity not folded, or lo known to be +0) gives T_q = -inf. dl is the
+// lower end of disc_interval (disc_lower) or the vertex-level bound of
+// pair_interval (vertex_lower), without the clamp at 0 (a smaller dl only
+// keeps more pairs).
+constexpr float kTH = 1.f + 8.f * kU;
+constexpr float kTG = 1.f - 16.f * kU;
+
+// per-lane constants of a column triangle L
+struct MarkCol {
+    float clo, chi;  // column parts of T_lo, T_hi
+    float mc;        // column part of the margin: 32u (r_L + cm_L) + 1e-6
+};
+
+__device__ __forceinline__ MarkCol mark_col(const TriS &L, float ulo_h, float uhi_h) {
+    MarkCol c;
+    c.clo = fmaf(L.ph, kTH, ulo_h);
+    c.chi = fmaf(L.h, -kTG, uhi_h);
+    c.mc = fmaf(32.f * kU, L.r + L.cm, 1e-6f);
+    return c;
+}
+
+// T for a pair: the row triangle's terms added to the column constants
+template <uint32_t NEED>
+__device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float ud) {
+    float T = -INFINITY;
+    if (NEED & 1u) T = ud;
+    if (NEED & 2u) T = fmaxf(T, fmaf(S.ph, kTH, c.clo));
+    if (NEED & 4u) T = fmaxf(T, fmaf(S.h, -kTG, c.chi));
+    return T;
+}
+
+// Lower end of disc_interval for the row triangle S and the column triangle L
+// (the same bound: the margin is 32u (dc + r_S + r_L + cm_S + cm_L) + 1e-6
+// as there, the sines are not clamped at 1 and 1 - c^2 + 16u is formed by
+// one fused multiply-add, which only lowers the bound or rounds once where
+// two roundings were).
+__device__ __forceinline__ float disc_lower(const TriS &S, const TriS &L, const MarkCol &c) {
+#pragma clang fp contract(fast)
+    const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
+    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    const float m = fmaf(32.f * kU, dc + (S.r + S.cm), c.mc);
+    const float eL = L.nx * dx + L.ny * dy + L.nz * dz;
+    const float eS = S.nx * dx + S.ny * dy + S.nz * dz;
+    const float m2 = 2.f * m;
+    float dl = 0.f;
+    if (fabsf(eL) > S.r + m2 || fabsf(eS) > L.r + m2) {  // dc > 0 here
+        const float inv = __builtin_amdgcn_rcpf(dc);
+        const float cl = eL * inv, cs = eS * inv;
+        const float sl = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cl, cl, 1.f + 16.f * kU), 16.f * kU));
+        const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
+        dl = fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m));
+    }
+    return dl;
+}
+
+// Vertex-level lower bound: the arithmetic of pair_interval() on register
+// data (its plane tests against the actual vertices and the gap of the
+// vertex projections on the centre axis), without the clamp at 0.
+__device__ __forceinline__ float vertex_lower(const TriS &A, const TriV &a, const TriS &B,
+                                              const TriV &c) {
+#pragma clang fp contract(fast)
+    const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
+    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    const float rs = A.r + B.r;
+    const float m = 32.f * kU * (dc + rs + A.cm + B.cm) + 1e-6f;
+    float dl = 0.f;
+    const float ax0 = a.x0 - A.cx, ay0 = a.y0 - A.cy, az0 = a.z0 - A.cz;
+    const float ax1 = a.x1 - A.cx, ay1 = a.y1 - A.cy, az1 = a.z1 - A.cz;
+    const float ax2 = a.x2 - A.cx, ay2 = a.y2 - A.cy, az2 = a.z2 - A.cz;
+    const float cx0 = c.x0 - A.cx, cy0 = c.y0 - A.cy, cz0 = c.z0 - A.cz;
+    const float cx1 = c.x1 - A.cx, cy1 = c.y1 - A.cy, cz1 = c.z1 - A.cz;
+    const float cx2 = c.x2 - A.cx, cy2 = c.y2 - A.cy, cz2 = c.z2 - A.cz;
+    const float t0 = A.nx * cx0 + A.ny * cy0 + A.nz * cz0;
+    const float t1 = A.nx * cx1 + A.ny * cy1 + A.nz * cz1;
+    const float t2 = A.nx * cx2 + A.ny * cy2 + A.nz * cz2;
+    const float w0 = B.nx * (ax0 - dx) + B.ny * (ay0 - dy) + B.nz * (az0 - dz);
+    const float w1 = B.nx * (ax1 - dx) + B.ny * (ay1 - dy) + B.nz * (az1 - dz);
+    const float w2 = B.nx * (ax2 - dx) + B.ny * (ay2 - dy) + B.nz * (az2 - dz);
+    const bool sep1 = fminf(t0, fminf(t1, t2)) > m || fmaxf(t0, fmaxf(t1, t2)) < -m;
+    const bool sep2 = fminf(w0, fminf(w1, w2)) > m || fmaxf(w0, fmaxf(w1, w2)) < -m;
+    if ((sep1 || sep2) && dc > 0.f) {
+        const float inv = __builtin_amdgcn_rcpf(dc);
+        const float ux = dx * inv, uy = dy * inv, uz = dz * inv;
+        const float smax = fmaxf(ux * ax0 + uy * ay0 + uz * az0,
+                                 fmaxf(ux * ax1 + uy * ay1 + uz * az1, ux * ax2 + uy * ay2 + uz * az2));
+        const float tmin = fminf(ux * cx0 + uy * cy0 + uz * cz0,
+                                 fminf(ux * cx1 + uy * cy1 + uz * cz1, ux * cx2 + uy * cy2 + uz * cz2));
+        dl = tmin - smax - m;
+    }
+    return dl;
 }
 
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
 // the ballot of iteration k of the tiled walk; count[w] = number of bits.
-// SMALL voxel pairs: every pair, no mask. NEED: the quantities of the round
-// (U is -inf for the others).
+// A pair is a candidate when its disc lower bound, and then its vertex lower
+// bound, reach T; probes (already evaluated) never are. SMALL voxel pairs:
+// every pair, no mask. NEED: the quantities of the round (U is -inf for the
+// others).
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
@@ -591,25 +648,33 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     }
     float U[3];
     bounds_U(vpU, mkey, w, U);
+    const float ud = U[0], ulo_h = U[1] * kTH, uhi_h = U[2] * kTH;
     const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
                    p2 = probe[3 * (size_t)w + 2];
     const Tiling t = plan_tiling(pl);
-    const uint32_t n2 = g.n2, nw = pl.nw;
+    const uint32_t n2 = g.n2;
     unsigned long long *__restrict__ out = mask + woff[w];
     uint32_t c = 0;
-    unsigned long long mine = 0;  // lane k keeps word (k mod 64) of the current group
-    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
-              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc) {
-                  const bool take =
-                      mx_candidate<NEED>(A, B, p, va, vc, U, p.i * n2 + p.j, p0, p1, p2);
-                  const unsigned long long m = __ballot(take);
-                  c += (uint32_t)__popcll(m);
-                  const uint32_t k = p.word & (kWave - 1);
-                  if ((uint32_t)lane == k) mine = m;
-                  if (k == kWave - 1 || p.word + 1 == nw) {
-                      if ((uint32_t)lane <= k) out[(p.word & ~(uint32_t)(kWave - 1)) + lane] = mine;
-                  }
-              });
+    MarkCol mc;
+    for_tiles(
+        g, t, lane, rec1, bnd1, rec2, bnd2, [&](const TriS &L) { mc = mark_col(L, ulo_h, uhi_h); },
+        [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap) {
+            constexpr bool col_is_a = decltype(swap)::value;
+            const TriS &S = col_is_a ? B : A;
+            const TriS &L = col_is_a ? A : B;
+            bool take = false;
+            if (p.valid) {
+                const float T = mark_T<NEED>(S, mc, ud);
+                if (disc_lower(S, L, mc) <= T) {
+                    const uint32_t q = p.i * n2 + p.j;
+                    if (q != p0 && q != p1 && q != p2)
+                        take = vertex_lower(A, va(), B, vc()) <= T;
+                }
+            }
+            const unsigned long long m = __ballot(take);
+            c += (uint32_t)__popcll(m);
+            if (lane == 0) out[p.word] = m;
+        });
     if (lane == 0) count[w] = c;
 }
 
