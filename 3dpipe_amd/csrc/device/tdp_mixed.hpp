@@ -259,9 +259,9 @@ struct PairAt {
 // swap an std::integral_constant<bool> that is true when the column triangle
 // (the lane's, fixed over a column chunk) is A. All 64 lanes take part in
 // every call (p.valid = false for cells outside the grid), so f may use
-// wave-wide operations. fc(L) is called by every lane at the start of each
-// column chunk with the lane's column triangle. ROWS1: one row per iteration
-// (R == 1), the row triangle is wave-uniform.
+// wave-wide operations. fc(L, LV) is called by every lane at the start of
+// each column chunk with the lane's column triangle (sphere data, vertices).
+// ROWS1: one row per iteration (R == 1), the row triangle is wave-uniform.
 template <bool SWAP, bool ROWS1, typename FC, typename F>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
                                            const float4 *__restrict__ rec1,
@@ -284,7 +284,7 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const uint64_t lt = ol + (cv ? col : nl - 1);  // in bounds for idle lanes
         const TriS L = load_s(bl, lt);
         const TriV LV = load_v(rl, lt);
-        fc(L);
+        fc(L, LV);
         // row records through pointers stepped R rows per group (no index
         // arithmetic per iteration). Rows past the smaller voxel (last group,
         // lanes outside the tile) are at most 63 past its last triangle: they
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     float up[3] = {INFINITY, INFINITY, INFINITY};
     float best[3] = {INFINITY, INFINITY, INFINITY};
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
-    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2, [](const TriS &) {},
+    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2, [](const TriS &, const TriV &) {},
               [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto) {
                   if (!p.valid) return;
                   float lw[3], uw[3];
@@ -526,10 +526,9 @@ __device__ __forceinline__ void bounds_U(const float *vpU, const unsigned long l
 //   fl(-c G + fl(-b G + fl(a H)))     >= a - b - c + 3u a
 // for a, b, c >= 0 -- above the exact thresholds by more than the FP64
 // rounding of the quantities themselves (a few 2^-53 relative). U = -inf
-// (a quantity not folded, or lo known to be +0) gives T_q = -inf. dl is the
-// lower end of disc_interval (disc_lower) or the vertex-level bound of
-// pair_interval (vertex_lower), without the clamp at 0 (a smaller dl only
-// keeps more pairs).
+// (a quantity not folded, or lo known to be +0) gives T_q = -inf. The lower
+// bounds of mark_pair are not clamped at 0 (a smaller bound only keeps more
+// pairs).
 constexpr float kTH = 1.f + 8.f * kU;
 constexpr float kTG = 1.f - 16.f * kU;
 
@@ -537,13 +536,18 @@ constexpr float kTG = 1.f - 16.f * kU;
 struct MarkCol {
     float clo, chi;  // column parts of T_lo, T_hi
     float mc;        // column part of the margin: 32u (r_L + cm_L) + 1e-6
+    float x0, y0, z0, x1, y1, z1, x2, y2, z2;  // L's vertices less its centre
 };
 
-__device__ __forceinline__ MarkCol mark_col(const TriS &L, float ulo_h, float uhi_h) {
+__device__ __forceinline__ MarkCol mark_col(const TriS &L, const TriV &V, float ulo_h,
+                                            float uhi_h) {
     MarkCol c;
     c.clo = fmaf(L.ph, kTH, ulo_h);
     c.chi = fmaf(L.h, -kTG, uhi_h);
     c.mc = fmaf(32.f * kU, L.r + L.cm, 1e-6f);
+    c.x0 = V.x0 - L.cx; c.y0 = V.y0 - L.cy; c.z0 = V.z0 - L.cz;
+    c.x1 = V.x1 - L.cx; c.y1 = V.y1 - L.cy; c.z1 = V.z1 - L.cz;
+    c.x2 = V.x2 - L.cx; c.y2 = V.y2 - L.cy; c.z2 = V.z2 - L.cz;
     return c;
 }
 
@@ -557,12 +561,27 @@ __device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float u
     return T;
 }
 
-// Lower end of disc_interval for the row triangle S and the column triangle L
-// (the same bound: the margin is 32u (dc + r_S + r_L + cm_S + cm_L) + 1e-6
-// as there, the sines are not clamped at 1 and 1 - c^2 + 16u is formed by
-// one fused multiply-add, which only lowers the bound or rounds once where
-// two roundings were).
-__device__ __forceinline__ float disc_lower(const TriS &S, const TriS &L, const MarkCol &c) {
+// Candidate test of one pair (row triangle S, column triangle L) against T:
+//  1. the lower end of disc_interval (the same bound: the margin is
+//     32u (dc + r_S + r_L + cm_S + cm_L) + 1e-6 as there, the sines are not
+//     clamped at 1 and 1 - c^2 + 16u is formed by one fused multiply-add,
+//     which only lowers the bound or rounds once where two roundings were);
+//  2. for pairs it keeps, the gap between the projections of the actual
+//     vertices on the centre axis (the vertex-level bound of pair_interval()).
+//     The disc guard, which holds for all but a few in 10^5 of the pairs
+//     reaching this step, already shows the triangles disjoint for TriDist,
+//     so the vertex-level plane tests are not repeated; a pair without the
+//     guard stays a candidate. The projections are taken on the unnormalised
+//     centre difference and scaled by 1/dc once; the rounding of the vertex
+//     differences, the dot products and the reciprocal is a few u (dc + r_S
+//     + r_L + cm_S + cm_L), inside the margin.
+// Operation count per pair through step 1 (add/sub/mul/sqrt/rcp/min/max/
+// compare = 1, fused multiply-add = 2, abs free): centre difference 3, dc 6,
+// margin 4, the two plane offsets 10, guard 5, sines and disc gap 16,
+// comparison 1 -- 45; T adds 5 in rounds that fold lo and hi (50).
+template <typename RV>
+__device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const MarkCol &c, float T,
+                                          bool probe, RV &&row_v) {
 #pragma clang fp contract(fast)
     const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
@@ -570,58 +589,33 @@ __device__ __forceinline__ float disc_lower(const TriS &S, const TriS &L, const 
     const float eL = L.nx * dx + L.ny * dy + L.nz * dz;
     const float eS = S.nx * dx + S.ny * dy + S.nz * dz;
     const float m2 = 2.f * m;
-    float dl = 0.f;
-    if (fabsf(eL) > S.r + m2 || fabsf(eS) > L.r + m2) {  // dc > 0 here
-        const float inv = __builtin_amdgcn_rcpf(dc);
+    const bool g = fabsf(eL) > S.r + m2 || fabsf(eS) > L.r + m2;
+    float dl = 0.f, inv = 0.f;
+    if (g) {  // dc > 0 here
+        inv = __builtin_amdgcn_rcpf(dc);
         const float cl = eL * inv, cs = eS * inv;
         const float sl = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cl, cl, 1.f + 16.f * kU), 16.f * kU));
         const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
         dl = fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m));
     }
-    return dl;
-}
-
-// Vertex-level lower bound: the arithmetic of pair_interval() on register
-// data (its plane tests against the actual vertices and the gap of the
-// vertex projections on the centre axis), without the clamp at 0.
-__device__ __forceinline__ float vertex_lower(const TriS &A, const TriV &a, const TriS &B,
-                                              const TriV &c) {
-#pragma clang fp contract(fast)
-    const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
-    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-    const float rs = A.r + B.r;
-    const float m = 32.f * kU * (dc + rs + A.cm + B.cm) + 1e-6f;
-    float dl = 0.f;
-    const float ax0 = a.x0 - A.cx, ay0 = a.y0 - A.cy, az0 = a.z0 - A.cz;
-    const float ax1 = a.x1 - A.cx, ay1 = a.y1 - A.cy, az1 = a.z1 - A.cz;
-    const float ax2 = a.x2 - A.cx, ay2 = a.y2 - A.cy, az2 = a.z2 - A.cz;
-    const float cx0 = c.x0 - A.cx, cy0 = c.y0 - A.cy, cz0 = c.z0 - A.cz;
-    const float cx1 = c.x1 - A.cx, cy1 = c.y1 - A.cy, cz1 = c.z1 - A.cz;
-    const float cx2 = c.x2 - A.cx, cy2 = c.y2 - A.cy, cz2 = c.z2 - A.cz;
-    const float t0 = A.nx * cx0 + A.ny * cy0 + A.nz * cz0;
-    const float t1 = A.nx * cx1 + A.ny * cy1 + A.nz * cz1;
-    const float t2 = A.nx * cx2 + A.ny * cy2 + A.nz * cz2;
-    const float w0 = B.nx * (ax0 - dx) + B.ny * (ay0 - dy) + B.nz * (az0 - dz);
-    const float w1 = B.nx * (ax1 - dx) + B.ny * (ay1 - dy) + B.nz * (az1 - dz);
-    const float w2 = B.nx * (ax2 - dx) + B.ny * (ay2 - dy) + B.nz * (az2 - dz);
-    const bool sep1 = fminf(t0, fminf(t1, t2)) > m || fmaxf(t0, fmaxf(t1, t2)) < -m;
-    const bool sep2 = fminf(w0, fminf(w1, w2)) > m || fmaxf(w0, fmaxf(w1, w2)) < -m;
-    if ((sep1 || sep2) && dc > 0.f) {
-        const float inv = __builtin_amdgcn_rcpf(dc);
-        const float ux = dx * inv, uy = dy * inv, uz = dz * inv;
-        const float smax = fmaxf(ux * ax0 + uy * ay0 + uz * az0,
-                                 fmaxf(ux * ax1 + uy * ay1 + uz * az1, ux * ax2 + uy * ay2 + uz * az2));
-        const float tmin = fminf(ux * cx0 + uy * cy0 + uz * cz0,
-                                 fminf(ux * cx1 + uy * cy1 + uz * cz1, ux * cx2 + uy * cy2 + uz * cz2));
-        dl = tmin - smax - m;
-    }
-    return dl;
+    if (!(dl <= T) || probe) return false;
+    if (!g) return true;
+    const TriV v = row_v();
+    // S's vertices lie beyond L's along d = cS - cL
+    const float s0 = dx * (v.x0 - L.cx) + dy * (v.y0 - L.cy) + dz * (v.z0 - L.cz);
+    const float s1 = dx * (v.x1 - L.cx) + dy * (v.y1 - L.cy) + dz * (v.z1 - L.cz);
+    const float s2 = dx * (v.x2 - L.cx) + dy * (v.y2 - L.cy) + dz * (v.z2 - L.cz);
+    const float l0 = dx * c.x0 + dy * c.y0 + dz * c.z0;
+    const float l1 = dx * c.x1 + dy * c.y1 + dz * c.z1;
+    const float l2 = dx * c.x2 + dy * c.y2 + dz * c.z2;
+    const float gap = (fminf(s0, fminf(s1, s2)) - fmaxf(l0, fmaxf(l1, l2))) * inv - m;
+    return gap <= T;
 }
 
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
 // the ballot of iteration k of the tiled walk; count[w] = number of bits.
-// A pair is a candidate when its disc lower bound, and then its vertex lower
-// bound, reach T; probes (already evaluated) never are. SMALL voxel pairs:
+// A pair is a candidate when mark_pair keeps it; probes (already evaluated)
+// never are. SMALL voxel pairs:
 // every pair, no mask. NEED: the quantities of the round (U is -inf for the
 // others).
 template <uint32_t NEED>
@@ -652,28 +646,38 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
                    p2 = probe[3 * (size_t)w + 2];
     const Tiling t = plan_tiling(pl);
-    const uint32_t n2 = g.n2;
+    const uint32_t n2 = g.n2, nw = pl.nw;
     unsigned long long *__restrict__ out = mask + woff[w];
+    // the words of the current group of 64 iterations, stored together
+    __shared__ unsigned long long s_words[kWavesPerBlock][kWave];
+    unsigned long long *const sw = s_words[threadIdx.x / kWave];
     uint32_t c = 0;
     MarkCol mc;
     for_tiles(
-        g, t, lane, rec1, bnd1, rec2, bnd2, [&](const TriS &L) { mc = mark_col(L, ulo_h, uhi_h); },
+        g, t, lane, rec1, bnd1, rec2, bnd2,
+        [&](const TriS &L, const TriV &LV) { mc = mark_col(L, LV, ulo_h, uhi_h); },
         [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap) {
             constexpr bool col_is_a = decltype(swap)::value;
             const TriS &S = col_is_a ? B : A;
             const TriS &L = col_is_a ? A : B;
             bool take = false;
             if (p.valid) {
-                const float T = mark_T<NEED>(S, mc, ud);
-                if (disc_lower(S, L, mc) <= T) {
-                    const uint32_t q = p.i * n2 + p.j;
-                    if (q != p0 && q != p1 && q != p2)
-                        take = vertex_lower(A, va(), B, vc()) <= T;
-                }
+                const uint32_t q = p.i * n2 + p.j;
+                const bool probe = q == p0 || q == p1 || q == p2;
+                if constexpr (col_is_a)
+                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), probe, vc);
+                else
+                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), probe, va);
             }
             const unsigned long long m = __ballot(take);
             c += (uint32_t)__popcll(m);
-            if (lane == 0) out[p.word] = m;
+            const uint32_t k = p.word & (kWave - 1);
+            if (lane == 0) sw[k] = m;
+            if (k == kWave - 1 || p.word + 1 == nw) {
+                wave_sync();
+                if ((uint32_t)lane <= k) out[(p.word & ~(uint32_t)(kWave - 1)) + lane] = sw[lane];
+                wave_sync();
+            }
         });
     if (lane == 0) count[w] = c;
 }
