@@ -417,7 +417,10 @@ constexpr uint32_t kMixedMinPairs = 2 * kWave;  // below this, evaluate every pa
 //                             FP32, radius measured from the rounded centre
 //                             with a relative margin);
 //   {unit normal.xyz, max |centre coord|}   normal 0 when near-degenerate;
-//   {proxy hausdorff, hausdorff, 1 if the normal is 0 else 0, 0}.
+//   {proxy hausdorff ph, hausdorff h, alo, ahi}: alo = r - ph + 8u t and
+//                             ahi = r + h + 8u t (t = r + ph + h, u = 2^-24),
+//                             the triangle's parts of the upper bounds of lo
+//                             and hi in k_mx_bounds.
 // The normal is formed in FP64 from the (exact) FP64 edge differences and
 // rounded once, so its direction is within a few FP32 ulps of the true
 // plane's even for thin triangles (the FP64 cross product's error relative
@@ -463,9 +466,12 @@ __global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *_
         fz = (float)(nz / nl);
     }
     const float cm = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+    // upper-bound constants of lo and hi, rounded upwards by 8u (r + ph + h)
+    const float ph = a.w, h = b.w, tt = (r + ph) + h;
+    const float alo = (r - ph) + 8.f * kU * tt, ahi = (r + h) + 8.f * kU * tt;
     bnd[kBnd * i] = make_float4(cx, cy, cz, r);
     bnd[kBnd * i + 1] = make_float4(fx, fy, fz, cm);
-    bnd[kBnd * i + 2] = make_float4(a.w, b.w, fx == 0.f && fy == 0.f && fz == 0.f ? 1.f : 0.f, 0.f);
+    bnd[kBnd * i + 2] = make_float4(ph, h, alo, ahi);
 }
 
 // (d, lo, hi) intervals from an interval [dl, du] of the FP64 TriDist value

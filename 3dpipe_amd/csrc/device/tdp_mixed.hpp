@@ -68,12 +68,13 @@ __device__ __forceinline__ uint32_t wave_item() {
 
 // sphere-level data of one triangle (first and third bound records)
 struct TriS {
-    float cx, cy, cz, r, nx, ny, nz, cm, ph, h, dg;  // dg = 1: no plane
+    float cx, cy, cz, r, nx, ny, nz, cm, ph, h;
+    float alo, ahi;  // upper-bound constants of lo and hi (k_tri_bounds)
 };
 
 __device__ __forceinline__ TriS load_s_at(const float4 *__restrict__ p) {
     const float4 a = p[0], b = p[1], c = p[2];
-    return TriS{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z};
+    return TriS{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
 }
 __device__ __forceinline__ TriS load_s(const float4 *__restrict__ bnd, uint64_t t) {
     return load_s_at(bnd + kBnd * t);
@@ -92,47 +93,25 @@ __device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t 
     return load_v_at(rec + 3 * t);
 }
 
-// Disc interval of the FP64 TriDist value of (A, B), A from tile 1, from
-// per-triangle data only. Each triangle lies in the disc of its plane around
-// its centre with its sphere's radius, so:
-//  upper: centre distance + radii;
-//  lower: the gap between the discs' projections on the centre axis u,
-//         dc - rA sin(u, nA) - rB sin(u, nB), used only when one triangle
-//         lies on one side of the other's plane by more than twice the
-//         margin: B lies within rB of its centre, so |nA . (cB - cA)| > rB
-//         + 2m puts every vertex of B on one side of A's plane (and the same
-//         with A and B exchanged). Then every vertex passes the vertex-level
-//         plane test of pair_interval(), which is what guarantees that
-//         TriDist's face test shows the triangles disjoint. (A tighter guard,
-//         B within rB sin(nA, nB) of the plane parallel to A's through B's
-//         centre, keeps 0.5% fewer pairs out and costs 19 more operations.)
-// Sines are rounded up (16 ulp inside the root for the cosine's error); a
-// triangle without a plane has a zero normal, so only the other triangle's
-// plane can pass the guard.
-// k_mx_bounds uses the upper end; k_mx_mark the lower end (disc_lower).
-__device__ __forceinline__ void disc_interval(const TriS &A, const TriS &B, float lw[3],
-                                              float uw[3]) {
-    // fused multiply-adds round once where two operations round twice, so the
-    // margins (per-operation error bounds) still hold
-#pragma clang fp contract(fast)
-    const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
-    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-    const float rs = A.r + B.r;
-    const float m = 32.f * kU * (dc + rs + A.cm + B.cm) + 1e-6f;
-    const float e1 = A.nx * dx + A.ny * dy + A.nz * dz;
-    const float e2 = B.nx * dx + B.ny * dy + B.nz * dz;
-    const float m2 = 2.f * m;
-    const bool g = fabsf(e1) > B.r + m2 || fabsf(e2) > A.r + m2;
-    float dl = 0.f;
-    if (g) {  // dc > 0 here
-        const float inv = __builtin_amdgcn_rcpf(dc);
-        const float c1 = e1 * inv, c2 = e2 * inv;
-        const float sa = __builtin_amdgcn_sqrtf(fmaxf(1.f - c1 * c1, 0.f) + 16.f * kU);
-        const float sb = __builtin_amdgcn_sqrtf(fmaxf(1.f - c2 * c2, 0.f) + 16.f * kU);
-        dl = fmaxf(dc - A.r * fminf(sa, 1.f) - B.r * fminf(sb, 1.f) - m, 0.f);
-    }
-    intervals(dl, dc + rs + m, A.ph, A.h, B.ph, B.h, lw, uw);
-}
+// The disc bound of a pair's FP64 TriDist value, from per-triangle data
+// only. Each triangle lies in the disc of its plane around its centre with
+// its sphere's radius, so the gap between the discs' projections on the
+// centre axis u, dc - rA sin(u, nA) - rB sin(u, nB), bounds the distance of
+// the triangles from below. It is used only when one triangle lies on one
+// side of the other's plane by more than twice the margin m: B lies within
+// rB of its centre, so |nA . (cB - cA)| > rB + 2m puts every vertex of B on
+// one side of A's plane (and the same with A and B exchanged). Then every
+// vertex passes the vertex-level plane test of pair_interval(), which is what
+// guarantees that TriDist's face test shows the triangles disjoint (no false
+// 0 return), so its value is a point-pair distance, at least the distance of
+// the triangles. (A tighter guard, B within rB sin(nA, nB) of the plane
+// parallel to A's through B's centre, keeps 0.5% fewer pairs out and costs
+// 19 more operations.) Sines are rounded up (16 ulp inside the root for the
+// cosine's error); a triangle without a plane has a zero normal, so only the
+// other triangle's plane can pass the guard. m = 32u (dc + rA + rB + cmA +
+// cmB) + 1e-6 covers the FP32 rounding of these operations, including the
+// rounded centroid's distance from its triangle's plane (u = 2^-24). The
+// bound is evaluated by mark_pair().
 
 // ------------------------------------------------------------ pair walking
 
@@ -330,14 +309,6 @@ __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int l
 
 // ------------------------------------------------------------------ bounds
 
-// centre distance less both radii: a score only (it picks the probes),
-// contracted like the centre distance of disc_interval, which it then shares
-__device__ __forceinline__ float centre_score(const TriS &A, const TriS &B) {
-#pragma clang fp contract(fast)
-    const float dx = B.cx - A.cx, dy = B.cy - A.cy, dz = B.cz - A.cz;
-    return __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - A.r - B.r;
-}
-
 // lowest q among the lanes holding the wave minimum of v (kNoProbe if none)
 __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
     const float vmin = wave_min(v);
@@ -347,13 +318,28 @@ __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
     return c;
 }
 
+// Upper bounds for k_mx_bounds. Every value TriDist returns is the distance
+// between two points of the triangles, so the FP64 d of a pair is at most
+// dc* + r_A + r_B, dc* the exact distance of the (FP32) centres: with dc the
+// FP32 centre distance (dc* <= dc (1 + 5u), u = 2^-24) and F = 1 + 20u,
+//   d  <= fl(dc F + fl(r_A G + fl(r_B G)))              (G = 1 + 4u),
+//   lo <= max(fl(dc F + fl(alo_A + alo_B)), 0),  alo = r - ph + 8u t,
+//   hi <=     fl(dc F + fl(ahi_A + ahi_B)),      ahi = r + h + 8u t,
+// t = r + ph + h (alo and ahi are formed once per triangle by k_tri_bounds):
+// each rounding is at most u of its result, which the factors and the 8u t
+// terms cover with room for the FP64 rounding of lo and hi themselves. The
+// clamp of lo at 0 is applied to the minimum.
+constexpr float kUF = 1.f + 20.f * kU;
+constexpr float kUG = 1.f + 4.f * kU;
+
 // Per voxel pair: the smallest FP32 upper bound of each quantity the round
 // folds (vpU; -inf for the others) and up to three probe pairs, the pairs of
 // smallest score (centre distance less the radii, with the quantity's
 // Hausdorff terms), whose FP64 values usually are the minima. NEED: the
 // quantities the round folds (bit 0 d, 1 lo, 2 hi), a launch constant so that
 // the arithmetic of the others drops out (and `need` the same bits at run
-// time).
+// time). The column triangle's parts of the bounds and scores are per-lane
+// constants of a column chunk; a pair adds the row triangle's.
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
@@ -376,24 +362,53 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     float up[3] = {INFINITY, INFINITY, INFINITY};
     float best[3] = {INFINITY, INFINITY, INFINITY};
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
-    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2, [](const TriS &, const TriV &) {},
-              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto) {
+    // column constants: r G, alo, ahi; the scores' -r, -(r + ph), h - r
+    float crg = 0.f, calo = 0.f, cahi = 0.f, ck0 = 0.f, ck1 = 0.f, ck2 = 0.f;
+    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
+              [&](const TriS &L, const TriV &) {
+                  crg = L.r * kUG;
+                  calo = L.alo;
+                  cahi = L.ahi;
+                  ck0 = -L.r;
+                  ck1 = -(L.r + L.ph);
+                  ck2 = L.h - L.r;
+              },
+              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto swap) {
                   if (!p.valid) return;
-                  float lw[3], uw[3];
-                  disc_interval(A, B, lw, uw);
-                  const float sc = centre_score(A, B);
-                  const float key[3] = {sc, sc - A.ph - B.ph, sc + A.h + B.h};
+                  constexpr bool col_is_a = decltype(swap)::value;
+                  const TriS &S = col_is_a ? B : A;
+                  const TriS &L = col_is_a ? A : B;
+                  const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
+                  float dc;
+                  {
+#pragma clang fp contract(fast)
+                      dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+                  }
+                  float x[3], key[3];
+                  if (NEED & 1u) {
+                      x[0] = fmaf(dc, kUF, fmaf(S.r, kUG, crg));
+                      key[0] = dc + (ck0 - S.r);
+                  }
+                  if (NEED & 2u) {
+                      x[1] = fmaf(dc, kUF, S.alo + calo);
+                      key[1] = dc + (ck1 - (S.r + S.ph));
+                  }
+                  if (NEED & 4u) {
+                      x[2] = fmaf(dc, kUF, S.ahi + cahi);
+                      key[2] = dc + (ck2 + (S.h - S.r));
+                  }
                   const uint32_t q = p.i * n2 + p.j;
 #pragma unroll
                   for (int f = 0; f < 3; f++) {
                       if (!((NEED >> f) & 1u)) continue;
-                      up[f] = fminf(up[f], uw[f]);
+                      up[f] = fminf(up[f], x[f]);
                       if (key[f] < best[f]) {
                           best[f] = key[f];
                           bq[f] = q;
                       }
                   }
               });
+    up[1] = fmaxf(up[1], 0.f);
     float U[3];
     uint32_t pq[3];
 #pragma unroll
@@ -562,10 +577,9 @@ __device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float u
 }
 
 // Candidate test of one pair (row triangle S, column triangle L) against T:
-//  1. the lower end of disc_interval (the same bound: the margin is
-//     32u (dc + r_S + r_L + cm_S + cm_L) + 1e-6 as there, the sines are not
-//     clamped at 1 and 1 - c^2 + 16u is formed by one fused multiply-add,
-//     which only lowers the bound or rounds once where two roundings were);
+//  1. the disc bound (described above; the sines are not clamped at 1 and
+//     1 - c^2 + 16u is formed by one fused multiply-add, which only lowers
+//     the bound or rounds once where two roundings were);
 //  2. for pairs it keeps, the gap between the projections of the actual
 //     vertices on the centre axis (the vertex-level bound of pair_interval()).
 //     The disc guard, which holds for all but a few in 10^5 of the pairs
