@@ -227,7 +227,7 @@ def main():
     ap.add_argument("--serve-rates", default="8,32",
                     help="query arrival rates (per second) of the serving runs")
     ap.add_argument("--tune", default="", help="kernel tuning, e.g. drain=0,max_rows=4")
-    ap.add_argument("--autotune", type=int, default=4000,
+    ap.add_argument("--autotune", type=int, default=40000,
                     help="autotune on this many targets before timing (0: off)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (the "
