@@ -442,8 +442,8 @@ class Engine:
                    knn: int = 1, within_dist: float = 1000.0,
                    max_objects1: Optional[int] = None):
         """Candidate lists as the join builds them (on the device for kNN
-        against at most 1024 objects): (prefix, ids, targets left to the
-        host octree)."""
+        and within against at most 1024 objects): (prefix, ids, targets left
+        to the host octree)."""
         tdb, _ = _load()
         o, _keep = _opts(query, knn, within_dist, None, 2, max_objects1, "mixed", 0)
         n = min(tile1.n_obj, max_objects1) if max_objects1 else tile1.n_obj

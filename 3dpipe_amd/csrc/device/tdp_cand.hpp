@@ -1,4 +1,5 @@
-// kNN candidate lists on the device (included after tdp_kernels.hpp).
+// kNN and within candidate lists on the device (included after
+// tdp_kernels.hpp).
 //
 // The octree's best-first query (octree.cpp:215-291) with add_candidate
 // (:120-154) returns, for a target with k wanted: every object whose box
@@ -143,6 +144,70 @@ __global__ void __launch_bounds__(kBlock) k_knn_candidates(
         const bool fb = tie || over || dup;
         fallback[t] = fb ? 1 : 0;
         count[t] = fb ? 0 : c;
+    }
+}
+
+// Within candidate lists on the device (octree.cpp:293-330, query_within):
+// the octree's depth-first walk visits a node when its box distance to the
+// target's box is at most tau and, in a leaf, takes every object (not the
+// target itself, not taken before) whose box distance is at most tau, in the
+// leaf's order. A child's box lies inside its parent's, and the FP32 box
+// distance is monotone in the box, so a leaf is reached exactly when its own
+// box passes: the walk is the leaves in depth-first order, each tested on its
+// own. One wave per target tests 64 leaves at a time, then walks the passing
+// ones in order, 64 objects at a time, with an LDS bit set of the objects
+// taken. Lists longer than cap are flagged for the host octree.
+__global__ void __launch_bounds__(kBlock) k_within_candidates(
+    uint64_t N, const float *__restrict__ box1, uint32_t n2, const float *__restrict__ box2,
+    int self, const float *__restrict__ tau_t, uint32_t n_leaves,
+    const float *__restrict__ leaf_box, const uint32_t *__restrict__ leaf_off,
+    const uint32_t *__restrict__ leaf_obj, uint32_t cap, uint32_t *__restrict__ count,
+    uint32_t *__restrict__ ids, uint8_t *__restrict__ fallback) {
+    __shared__ uint32_t seen_all[kWavesPerBlock][kCandMaxObj / 32];
+    const uint32_t t = wave_item();
+    if (t >= N) return;
+    const int lane = lane_id();
+    uint32_t *const seen = seen_all[threadIdx.x / kWave];
+    for (uint32_t i = (uint32_t)lane; i < kCandMaxObj / 32; i += kWave) seen[i] = 0u;
+    wave_sync();
+    float qb[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) qb[i] = box1[6 * (size_t)t + i];
+    const float tau = tau_t[t];
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t c = 0;
+    for (uint32_t l0 = 0; l0 < n_leaves; l0 += kWave) {
+        const uint32_t l = l0 + (uint32_t)lane;
+        const bool visit = l < n_leaves && cand_box_min(leaf_box + 6 * (size_t)l, qb) <= tau;
+        unsigned long long vm = __ballot(visit);
+        while (vm) {
+            const uint32_t leaf = l0 + (uint32_t)(__ffsll((long long)vm) - 1);
+            vm &= vm - 1ull;
+            const uint32_t k1 = leaf_off[leaf + 1];
+            for (uint32_t k0 = leaf_off[leaf]; k0 < k1; k0 += kWave) {
+                const uint32_t k = k0 + (uint32_t)lane;
+                uint32_t o = 0;
+                bool take = false;
+                if (k < k1) {
+                    o = leaf_obj[k];
+                    // an object appears once per leaf: no two lanes share o
+                    take = o < n2 && !(self && o == t) && !((seen[o >> 5] >> (o & 31u)) & 1u) &&
+                           cand_box_min(box2 + 6 * (size_t)o, qb) <= tau;
+                }
+                const unsigned long long tm = __ballot(take);
+                if (take) {
+                    const uint32_t at = c + (uint32_t)__popcll(tm & below);
+                    if (at < cap) ids[(size_t)t * cap + at] = o;
+                    atomicOr(&seen[o >> 5], 1u << (o & 31u));
+                }
+                c += (uint32_t)__popcll(tm);
+                wave_sync();
+            }
+        }
+    }
+    if (lane == 0) {
+        fallback[t] = c > cap ? 1 : 0;
+        count[t] = c > cap ? 0 : c;
     }
 }
 

@@ -1005,6 +1005,109 @@ extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, co
     return rc;
 }
 
+extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
+                                     const tdp_tile *t2, int self, const float *tau_target,
+                                     size_t n_leaves, const float *leaf_box,
+                                     const uint32_t *leaf_off, const uint32_t *leaf_obj,
+                                     uint32_t *prefix, uint32_t *ids, size_t ids_cap,
+                                     uint64_t *n_ids, uint8_t *fallback) {
+    if (!ctx || !t1 || !t2 || !n_ids || (N && (!tau_target || !prefix || !fallback)) ||
+        (n_leaves && (!leaf_box || !leaf_off || !leaf_obj)))
+        return TDP_ERR_ARG;
+    *n_ids = 0;
+    if (!t1->obj_box || !t2->obj_box) {
+        ctx->err = "tdp_within_candidates: tiles uploaded without object boxes";
+        return TDP_ERR_ARG;
+    }
+    if (t2->n_obj > (size_t)kCandMaxObj || N > t1->n_obj || n_leaves > 0xffffffffu) {
+        ctx->err = "tdp_within_candidates: tile 2 has more than 1024 objects";
+        return TDP_ERR_ARG;
+    }
+    const size_t n_lobj = n_leaves ? leaf_off[n_leaves] : 0;
+    for (size_t l = 0; l < n_leaves; l++)
+        if (leaf_off[l] > leaf_off[l + 1] || (l == 0 && leaf_off[0] != 0)) {
+            ctx->err = "tdp_within_candidates: leaf offsets must start at 0 and not decrease";
+            return TDP_ERR_ARG;
+        }
+    for (size_t i = 0; i < n_lobj; i++)
+        if (leaf_obj[i] >= t2->n_obj) {
+            ctx->err = "tdp_within_candidates: a leaf names an object outside tile 2";
+            return TDP_ERR_ARG;
+        }
+    prefix[0] = 0;
+    if (N == 0) return TDP_OK;
+    const uint32_t cap = kCandMaxCap;
+    float *dtau = nullptr, *dlbox = nullptr;
+    uint32_t *dloff = nullptr, *dlobj = nullptr, *dc = nullptr, *dl = nullptr, *dids = nullptr;
+    uint64_t *doff = nullptr;
+    uint8_t *df = nullptr;
+    int rc = palloc(ctx, &dtau, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &dlbox, 6 * n_leaves + 6);
+    if (rc == TDP_OK) rc = palloc(ctx, &dloff, n_leaves + 1);
+    if (rc == TDP_OK) rc = palloc(ctx, &dlobj, n_lobj + 1);
+    if (rc == TDP_OK) rc = palloc(ctx, &dc, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &dl, N * (size_t)cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &df, N);
+    if (rc == TDP_OK) rc = palloc(ctx, &doff, N);
+    hipError_t e = hipSuccess;
+    uint64_t total = 0;
+    if (rc == TDP_OK) {
+        e = hipMemcpyAsync(dtau, tau_target, 4 * N, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess && n_leaves) {
+            e = hipMemcpyAsync(dlbox, leaf_box, 24 * n_leaves, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dloff, leaf_off, 4 * (n_leaves + 1), hipMemcpyHostToDevice,
+                                   ctx->stream);
+            if (e == hipSuccess && n_lobj)
+                e = hipMemcpyAsync(dlobj, leaf_obj, 4 * n_lobj, hipMemcpyHostToDevice, ctx->stream);
+        }
+        if (e == hipSuccess) {
+            k_within_candidates<<<wave_grid(N), kBlock, 0, ctx->stream>>>(
+                N, t1->obj_box, (uint32_t)t2->n_obj, t2->obj_box, self, dtau, (uint32_t)n_leaves,
+                dlbox, dloff, dlobj, cap, dc, dl, df);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_within_candidates: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        }
+    }
+    if (rc == TDP_OK) rc = scan_exclusive<uint32_t>(ctx, dc, N, doff, &total);
+    *n_ids = total;
+    if (rc == TDP_OK && total > ids_cap) {
+        ctx->err = "tdp_within_candidates: ids array too small";
+        rc = TDP_ERR_ARG;
+    }
+    if (rc == TDP_OK) rc = palloc(ctx, &dids, total);
+    if (rc == TDP_OK) {
+        k_cand_compact<<<wave_grid(N), kBlock, 0, ctx->stream>>>(N, cap, dc, doff, dl, dids);
+        e = hipGetLastError();
+        if (e == hipSuccess && total)
+            e = hipMemcpyAsync(ids, dids, 4 * total, hipMemcpyDeviceToHost, ctx->stream);
+        std::vector<uint32_t> cnt(N);
+        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), dc, 4 * N, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(fallback, df, N, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("tdp_within_candidates: ") + hipGetErrorString(e);
+            rc = TDP_ERR_HIP;
+        } else {
+            for (size_t t = 0; t < N; t++) prefix[t + 1] = prefix[t] + cnt[t];
+        }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    pfree(ctx, dtau);
+    pfree(ctx, dlbox);
+    pfree(ctx, dloff);
+    pfree(ctx, dlobj);
+    pfree(ctx, dc);
+    pfree(ctx, dl);
+    pfree(ctx, df);
+    pfree(ctx, doff);
+    pfree(ctx, dids);
+    return rc;
+}
+
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
     tdp_ctx *ctx = j->ctx;

@@ -148,34 +148,53 @@ tdp_tile_desc tdb_desc_of(const tdb::Tile &t) {
     return d;
 }
 
-// Candidates with the device where it applies (kNN, tile 2 of at most 1024
-// objects, both tiles resident): the device lists, and the octree for the
-// targets the device leaves to it. Elsewhere the octree for all targets.
+// Candidates with the device where it applies (kNN or within, tile 2 of at
+// most 1024 objects, both tiles resident): the device lists, and the octree
+// for the targets the device leaves to it. Elsewhere the octree for all
+// targets.
 int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
                            const tdb_tile *t1, const tdb_tile *t2, bool self, size_t N,
                            const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
                            std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids,
                            uint64_t *n_fallback, std::string &err) {
     if (n_fallback) *n_fallback = 0;
-    if (!ctx || !d1 || !d2 || o->query != TDB_QUERY_NN || t2->t.n_obj > 1024 || N == 0) {
+    const bool knn = o->query == TDB_QUERY_NN, within = o->query == TDB_QUERY_WITHIN;
+    if (!ctx || !d1 || !d2 || !(knn || within) || t2->t.n_obj > 1024 || N == 0) {
         tdb_gen_candidates(t1, t2, self, N, o, k_t, tau_t, prefix, ids);
         return TDP_OK;
     }
-    std::vector<int32_t> kv(N);
-    for (size_t t = 0; t < N; t++) kv[t] = k_t ? k_t[t] : o->k;
+    const tdb::Octree &tree = t2->octree();
+    std::vector<int32_t> kv;
+    std::vector<float> tv;
+    std::vector<float> lbox;
+    std::vector<uint32_t> loff, lobj;
+    if (knn) {
+        kv.resize(N);
+        for (size_t t = 0; t < N; t++) kv[t] = k_t ? k_t[t] : o->k;
+    } else {
+        tv.resize(N);
+        for (size_t t = 0; t < N; t++) tv[t] = tau_t ? tau_t[t] : o->within_dist;
+        tree.leaves(lbox, loff, lobj);
+    }
     std::vector<uint32_t> dpre(N + 1);
     std::vector<uint8_t> fb(N);
     // room for about 16 per target; asked again with the exact count when short
     size_t room = 16 * N + 1024;
     std::unique_ptr<uint32_t[]> dids(new uint32_t[room]);
     uint64_t n_ids = 0;
-    int rc = tdp_knn_candidates(ctx, d1, N, d2, self ? 1 : 0, kv.data(), dpre.data(), dids.get(),
-                                room, &n_ids, fb.data());
+    auto run = [&]() {
+        if (knn)
+            return tdp_knn_candidates(ctx, d1, N, d2, self ? 1 : 0, kv.data(), dpre.data(),
+                                      dids.get(), room, &n_ids, fb.data());
+        return tdp_within_candidates(ctx, d1, N, d2, self ? 1 : 0, tv.data(), loff.size() - 1,
+                                     lbox.data(), loff.data(), lobj.data(), dpre.data(),
+                                     dids.get(), room, &n_ids, fb.data());
+    };
+    int rc = run();
     if (rc == TDP_ERR_ARG && n_ids > room) {
         room = n_ids;
         dids.reset(new uint32_t[room]);
-        rc = tdp_knn_candidates(ctx, d1, N, d2, self ? 1 : 0, kv.data(), dpre.data(), dids.get(),
-                                room, &n_ids, fb.data());
+        rc = run();
     }
     if (rc != TDP_OK) {
         err = tdp_last_error(ctx);
@@ -186,7 +205,6 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
         if (fb[t]) redo.push_back((long)t);
     if (n_fallback) *n_fallback = redo.size();
     std::vector<std::vector<uint32_t>> host(redo.size());
-    const tdb::Octree &tree = t2->octree();
 #ifdef _OPENMP
     const int nt = o->threads > 0 ? o->threads : omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
@@ -194,7 +212,12 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
     for (long r = 0; r < (long)redo.size(); r++) {
         const long i = redo[r];
         std::vector<std::pair<int, tdb::Range>> c;
-        tree.knn(&t1->t.obj_box[6 * i], &t1->t.obj_transform[3 * i], self ? (int)i : -1, kv[i], c);
+        if (knn)
+            tree.knn(&t1->t.obj_box[6 * i], &t1->t.obj_transform[3 * i], self ? (int)i : -1,
+                     kv[i], c);
+        else
+            tree.within(&t1->t.obj_box[6 * i], &t1->t.obj_transform[3 * i], self ? (int)i : -1,
+                        tv[i], c);
         for (auto &x : c) host[r].push_back((uint32_t)x.first);
     }
     if (redo.empty()) {

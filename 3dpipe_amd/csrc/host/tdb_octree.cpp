@@ -197,6 +197,25 @@ void Octree::within(const float *qbox, const float *qt, int self, float tau,
     within_rec(root_, qbox, qt, self, tau, out);
 }
 
+void Octree::leaves_rec(const Node *n, std::vector<float> &box, std::vector<uint32_t> &off,
+                        std::vector<uint32_t> &objs) const {
+    if (n->leaf) {
+        box.insert(box.end(), n->box, n->box + 6);
+        for (int o : n->objs) objs.push_back((uint32_t)o);
+        off.push_back((uint32_t)objs.size());
+    } else {
+        for (const Node *c : n->child) leaves_rec(c, box, off, objs);
+    }
+}
+
+void Octree::leaves(std::vector<float> &box, std::vector<uint32_t> &off,
+                    std::vector<uint32_t> &objs) const {
+    box.clear();
+    objs.clear();
+    off.assign(1, 0u);
+    if (root_) leaves_rec(root_, box, off, objs);
+}
+
 bool box_overlap(const float *a, const float *b) {
     return !(b[0] >= a[3] || b[3] <= a[0] || b[1] >= a[4] || b[4] <= a[1] || b[2] >= a[5] ||
              b[5] <= a[2]);

@@ -39,6 +39,10 @@ public:
     // (query_intersect, octree.cpp:332-349, then the sort and de-duplication
     // of mbb_intersect, IntersectionJoin.cpp:20-33)
     void intersect(const float *qbox, int self, std::vector<int> &out) const;
+    // the leaves in the depth-first order of within(): boxes (6 floats
+    // each), offsets into objs (n_leaves + 1) and each leaf's objects in order
+    void leaves(std::vector<float> &box, std::vector<uint32_t> &off,
+                std::vector<uint32_t> &objs) const;
 
 private:
     struct Node;
@@ -49,6 +53,8 @@ private:
     void within_rec(const Node *n, const float *qbox, const float *qt, int self, float tau,
                     std::vector<std::pair<int, Range>> &out) const;
     void intersect_rec(const Node *n, const float *qbox, int self, std::vector<int> &out) const;
+    void leaves_rec(const Node *n, std::vector<float> &box, std::vector<uint32_t> &off,
+                    std::vector<uint32_t> &objs) const;
 };
 
 // aab::distance (src/spatial/himesh_box.cpp:149-168): a is the box the

@@ -171,6 +171,23 @@ int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t n_targets, const
                        int self, const int32_t *k_target, uint32_t *prefix, uint32_t *ids,
                        size_t ids_cap, uint64_t *n_ids, uint8_t *fallback);
 
+/* Within candidate lists on the device, equal to the octree's query_within
+ * lists (octree.cpp:293-330) including their order: per target t of t1
+ * (t < n_targets), the objects o of t2 (o != t when self) whose box distance
+ * to the target's box is at most tau_target[t], in the order of the octree's
+ * depth-first walk, each once. The caller passes the octree's leaves in
+ * depth-first order: leaf_box[6 n_leaves] ({low.xyz, high.xyz}), leaf_off
+ * [n_leaves + 1] and leaf_obj[leaf_off[n_leaves]] (each leaf's objects in
+ * its order). Lists longer than 256 set fallback[t] = 1 and are left empty
+ * for the host octree. Both tiles need obj_box; t2 may hold at most 1024
+ * objects (TDP_ERR_ARG otherwise). Host arrays; ids, ids_cap and n_ids as
+ * for tdp_knn_candidates. */
+int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t n_targets, const tdp_tile *t2,
+                          int self, const float *tau_target, size_t n_leaves,
+                          const float *leaf_box, const uint32_t *leaf_off,
+                          const uint32_t *leaf_obj, uint32_t *prefix, uint32_t *ids,
+                          size_t ids_cap, uint64_t *n_ids, uint8_t *fallback);
+
 /* ---- tuning (performance only: results never depend on it) ---- */
 typedef struct {
     int small_pairs;  /* MIXED: voxel pairs of at most this many triangle pairs go to FP64

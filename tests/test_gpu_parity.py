@@ -327,6 +327,52 @@ def test_device_knn_candidates_equal_octree(tdp, eng, small_tiles, tmp_path, k):
     assert fb > 0
 
 
+@pytest.fixture(scope="module")
+def many_vessels(tdp, tmp_path_factory):
+    """300 vessels (an octree of many leaves, long vessels spanning several)
+    and 2 nuclei per vessel in a 3000^3 cube"""
+    d = tmp_path_factory.mktemp("many")
+    n, v = tdp.generate(seed=3, n_vessels=300, nuclei_per_vessel=2, cube=3000.0)
+    pn, pv = str(d / "n.dt"), str(d / "v.dt")
+    n.save(pn)
+    v.save(pv)
+    return pn, pv
+
+
+@pytest.mark.parametrize("tau", [0.0, 150.0, 600.0])
+def test_device_within_candidates_equal_octree(tdp, eng, many_vessels, tau):
+    """device within lists equal the octree's query_within lists, order included"""
+    pn, pv = many_vessels
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    hp, hi = tdp.candidates(n, v, "within", within_dist=tau)
+    dp, di, fb = eng.candidates(n, v, "within", within_dist=tau)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+    assert fb == 0
+    if tau > 0:
+        assert len(hi) > n.n_obj  # several candidates per target on average
+    # self-join of the nuclei (the target itself excluded)
+    hp, hi = tdp.candidates(n, None, "within", within_dist=tau)
+    dp, di, fb = eng.candidates(n, None, "within", within_dist=tau)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+
+
+def test_device_within_candidates_long_lists_fall_back(tdp, eng, many_vessels, small_tiles):
+    """lists longer than the device capacity go to the host octree; a within
+    join on device candidates equals the oracle's"""
+    pn, pv = many_vessels
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    hp, hi = tdp.candidates(n, v, "within", within_dist=1e6)
+    dp, di, fb = eng.candidates(n, v, "within", within_dist=1e6)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+    assert fb == n.n_obj and len(hi) == n.n_obj * v.n_obj
+    sn, sv = small_tiles
+    got = eng.join(tdp.Tile.load(sn), tdp.Tile.load(sv), query="within", within_dist=300.0,
+                   precision="fp64")
+    ot, oo, omn, omx, _ = orc.join(orc.OracleTile(sn), orc.OracleTile(sv), "within", 1, tau=300.0,
+                                   precision="f64")
+    assert orc.group(got.target, got.obj2, got.rmin, got.rmax) == orc.group(ot, oo, omn, omx)
+
+
 # ------------------------------------------------------------- tuning
 
 @pytest.mark.parametrize("tune", [dict(drain=0), dict(drain=1, drain_blocks=3),
