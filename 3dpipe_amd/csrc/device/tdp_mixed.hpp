@@ -233,6 +233,16 @@ struct PairAt {
     bool valid;
 };
 
+// Row records of the multi-row walk, staged in LDS per wave: kStageRows rows
+// of bound records and of triangle records. A tile holds R <= 13 rows (R
+// rows of cw >= 64 / R columns, cw at least a third of the larger voxel and
+// R at most the smaller), so a staged block covers several row groups.
+constexpr uint32_t kStageRows = 32;
+struct RowStage {
+    float4 s[kStageRows][kBnd];
+    float4 v[kStageRows][3];
+};
+
 // Calls f(A, B, p, va, vc, swap) for every cell of the tiling: A/B the
 // tile-1 / tile-2 sphere data, p the place, va()/vc() the vertices of A/B,
 // swap an std::integral_constant<bool> that is true when the column triangle
@@ -240,13 +250,19 @@ struct PairAt {
 // every call (p.valid = false for cells outside the grid), so f may use
 // wave-wide operations. fc(L, LV) is called by every lane at the start of
 // each column chunk with the lane's column triangle (sphere data, vertices).
-// ROWS1: one row per iteration (R == 1), the row triangle is wave-uniform.
-template <bool SWAP, bool ROWS1, typename FC, typename F>
+// ROWS1: one row per iteration (R == 1), the row triangle is wave-uniform and
+// its records come through scalar loads. Otherwise each lane reads its row
+// from `stage`, which the wave fills with the records of the next
+// R * floor(32 / R) rows whenever it starts a block of row groups (the row
+// vertices too when VERTS): one coalesced load per row instead of one per
+// row and iteration, and no memory latency inside the iterations.
+template <bool SWAP, bool ROWS1, bool VERTS, typename FC, typename F>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
                                            const float4 *__restrict__ rec1,
                                            const float4 *__restrict__ bnd1,
                                            const float4 *__restrict__ rec2,
-                                           const float4 *__restrict__ bnd2, FC &&fc, F &&f) {
+                                           const float4 *__restrict__ bnd2, RowStage *stage,
+                                           FC &&fc, F &&f) {
     const uint32_t nl = SWAP ? g.n1 : g.n2, ns = SWAP ? g.n2 : g.n1;
     const uint64_t ol = SWAP ? g.o1 : g.o2, os = SWAP ? g.o2 : g.o1;
     const float4 *__restrict__ bl = SWAP ? bnd1 : bnd2;
@@ -264,46 +280,73 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const TriS L = load_s(bl, lt);
         const TriV LV = load_v(rl, lt);
         fc(L, LV);
-        // row records through pointers stepped R rows per group (no index
-        // arithmetic per iteration). Rows past the smaller voxel (last group,
-        // lanes outside the tile) are at most 63 past its last triangle: they
-        // read the next voxel's records or the tile's padding (kTriPad), and
-        // their cells are not valid.
-        const float4 *__restrict__ pb = bs + kBnd * (os + r);
-        const float4 *__restrict__ pr = rs + 3 * (os + r);
-        const size_t step_b = (size_t)kBnd * t.R, step_r = (size_t)3 * t.R;
-        for (uint32_t gr = 0; gr < t.ngr; gr++, word++, pb += step_b, pr += step_r) {
-            const uint32_t row = gr * t.R + r;
+        auto call = [&](uint32_t row, const TriS &S, auto &&row_v) {
             const bool valid = cv && row < ns;
-            const TriS S = load_s_at(pb);
             auto lane_v = [&]() { return LV; };
-            auto row_v = [&]() { return load_v_at(pr); };
             if (SWAP)
                 f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v,
                   std::integral_constant<bool, true>{});
             else
                 f(S, L, PairAt{row, cv ? col : 0u, word, valid}, row_v, lane_v,
                   std::integral_constant<bool, false>{});
+        };
+        if (ROWS1) {
+            // row records through pointers stepped one row per iteration
+            const float4 *__restrict__ pb = bs + kBnd * os;
+            const float4 *__restrict__ pr = rs + 3 * os;
+            for (uint32_t gr = 0; gr < t.ngr; gr++, word++, pb += kBnd, pr += 3)
+                call(gr, load_s_at(pb), [&]() { return load_v_at(pr); });
+        } else {
+            // Blocks of gpb row groups. Rows staged past the smaller voxel
+            // (at most 31 past its last triangle) read the next voxel's
+            // records or the tile's padding (kTriPad); their cells are not
+            // valid. Lanes outside the tile read row R - 1 of a group.
+            const uint32_t gpb = kStageRows / t.R, rpb = gpb * t.R;
+            const uint32_t rr = min(r, t.R - 1u);
+            uint32_t k = 0;
+            for (uint32_t gr = 0; gr < t.ngr; gr++, word++) {
+                if (k == 0) {
+                    const uint64_t row0 = os + (uint64_t)gr * t.R;
+                    wave_sync();  // the previous block's reads are done
+                    if ((uint32_t)lane < rpb) {
+                        const float4 *p = bs + kBnd * (row0 + (uint32_t)lane);
+#pragma unroll
+                        for (int i = 0; i < kBnd; i++) stage->s[lane][i] = p[i];
+                    }
+                    if (VERTS && lane >= 32 && (uint32_t)lane - 32u < rpb) {
+                        const uint32_t q = (uint32_t)lane - 32u;
+                        const float4 *p = rs + 3 * (row0 + q);
+#pragma unroll
+                        for (int i = 0; i < 3; i++) stage->v[q][i] = p[i];
+                    }
+                    wave_sync();
+                }
+                const uint32_t slot = k * t.R + rr;
+                call(gr * t.R + r, load_s_at(stage->s[slot]),
+                     [&]() { return load_v_at(stage->v[slot]); });
+                if (++k == gpb) k = 0;
+            }
         }
     }
 }
 
-template <typename FC, typename F>
+template <bool VERTS, typename FC, typename F>
 __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int lane,
                                           const float4 *__restrict__ rec1,
                                           const float4 *__restrict__ bnd1,
                                           const float4 *__restrict__ rec2,
-                                          const float4 *__restrict__ bnd2, FC &&fc, F &&f) {
+                                          const float4 *__restrict__ bnd2, RowStage *stage,
+                                          FC &&fc, F &&f) {
     if (t.swap) {
         if (t.R == 1)
-            walk_tiles<true, true>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
+            walk_tiles<true, true, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
         else
-            walk_tiles<true, false>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
+            walk_tiles<true, false, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
     } else {
         if (t.R == 1)
-            walk_tiles<false, true>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
+            walk_tiles<false, true, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
         else
-            walk_tiles<false, false>(g, t, lane, rec1, bnd1, rec2, bnd2, fc, f);
+            walk_tiles<false, false, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
     }
 }
 
@@ -364,7 +407,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint32_t bq[3] = {kNoProbe, kNoProbe, kNoProbe};
     // column constants: r G, alo, ahi; the scores' -r, -(r + ph), h - r
     float crg = 0.f, calo = 0.f, cahi = 0.f, ck0 = 0.f, ck1 = 0.f, ck2 = 0.f;
-    for_tiles(g, t, lane, rec1, bnd1, rec2, bnd2,
+    __shared__ RowStage stage[kWavesPerBlock];
+    for_tiles<false>(g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
               [&](const TriS &L, const TriV &) {
                   crg = L.r * kUG;
                   calo = L.alo;
@@ -667,8 +711,9 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     unsigned long long *const sw = s_words[threadIdx.x / kWave];
     uint32_t c = 0;
     MarkCol mc;
-    for_tiles(
-        g, t, lane, rec1, bnd1, rec2, bnd2,
+    __shared__ RowStage stage[kWavesPerBlock];
+    for_tiles<true>(
+        g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
         [&](const TriS &L, const TriV &LV) { mc = mark_col(L, LV, ulo_h, uhi_h); },
         [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap) {
             constexpr bool col_is_a = decltype(swap)::value;
