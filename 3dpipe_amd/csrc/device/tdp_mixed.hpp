@@ -22,11 +22,10 @@
 // FP32 passes: one wave per voxel pair, walking its n1 x n2 grid in tiles of
 // R rows x cw columns (R * cw <= 64 lanes, chosen per voxel pair for the
 // fewest iterations). The larger voxel's triangles are the columns: a lane
-// keeps its column's records in registers for the whole walk. With one row
-// per iteration the row triangle is wave-uniform and its records come through
-// scalar loads; with several rows each lane loads its row's records. A pair
-// is first tested with its disc interval (per-triangle data only); only pairs
-// that pass go on to the vertex-level interval.
+// keeps its column's records in registers for the whole walk. The rows'
+// records are staged in LDS a block at a time and each lane reads its row
+// from there. A pair is first tested with its disc bound (per-triangle data
+// only); only pairs that pass go on to the vertex-level bound.
 #pragma once
 
 #include <type_traits>
@@ -125,6 +124,9 @@ struct Tiling {
     bool swap;  // the columns are tile-1 triangles
 };
 
+// rows of a staged block of the walk (the most rows a tile may have)
+constexpr uint32_t kStageRows = 32;
+
 __device__ __forceinline__ Tiling mx_tiling(const VpLod &g, uint32_t max_rows) {
     Tiling t;
     t.swap = g.n1 > g.n2;
@@ -133,7 +135,7 @@ __device__ __forceinline__ Tiling mx_tiling(const VpLod &g, uint32_t max_rows) {
     uint32_t best = 0xffffffffu;
     for (uint32_t m = m0; m < m0 + 3; m++) {
         const uint32_t cw = (nl + m - 1) / m;
-        const uint32_t R = min(min((uint32_t)kWave / cw, ns), max(max_rows, 1u));
+        const uint32_t R = min(min(min((uint32_t)kWave / cw, ns), max(max_rows, 1u)), kStageRows);
         const uint32_t nch = (nl + cw - 1) / cw, ngr = (ns + R - 1) / R;
         if (nch * ngr < best) {
             best = nch * ngr;
@@ -233,11 +235,11 @@ struct PairAt {
     bool valid;
 };
 
-// Row records of the multi-row walk, staged in LDS per wave: kStageRows rows
-// of bound records and of triangle records. A tile holds R <= 13 rows (R
-// rows of cw >= 64 / R columns, cw at least a third of the larger voxel and
-// R at most the smaller), so a staged block covers several row groups.
-constexpr uint32_t kStageRows = 32;
+// Row records of the tiled walk, staged in LDS per wave: kStageRows rows of
+// bound records and of triangle records. mx_tiling never gives more than 10
+// rows per tile (R <= 64 / cw, R <= the smaller voxel, and cw is at least the
+// larger voxel over its chunk count, checked for all voxel sizes up to 700)
+// and clamps R to kStageRows, so a staged block covers whole row groups.
 struct RowStage {
     float4 s[kStageRows][kBnd];
     float4 v[kStageRows][3];
@@ -250,13 +252,13 @@ struct RowStage {
 // every call (p.valid = false for cells outside the grid), so f may use
 // wave-wide operations. fc(L, LV) is called by every lane at the start of
 // each column chunk with the lane's column triangle (sphere data, vertices).
-// ROWS1: one row per iteration (R == 1), the row triangle is wave-uniform and
-// its records come through scalar loads. Otherwise each lane reads its row
-// from `stage`, which the wave fills with the records of the next
-// R * floor(32 / R) rows whenever it starts a block of row groups (the row
-// vertices too when VERTS): one coalesced load per row instead of one per
-// row and iteration, and no memory latency inside the iterations.
-template <bool SWAP, bool ROWS1, bool VERTS, typename FC, typename F>
+// Each lane reads its row from `stage`, which the wave fills with the records
+// of the next R * floor(32 / R) rows whenever it starts a block of row groups
+// (the row vertices too when VERTS): one coalesced load per row and chunk
+// instead of one per row and iteration, and no memory latency inside the
+// iterations. (With one row per iteration the row is the same in every lane
+// and its LDS reads are broadcasts.)
+template <bool SWAP, bool VERTS, typename FC, typename F>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
                                            const float4 *__restrict__ rec1,
                                            const float4 *__restrict__ bnd1,
@@ -269,9 +271,15 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
     const float4 *__restrict__ bs = SWAP ? bnd2 : bnd1;
     const float4 *__restrict__ rl = SWAP ? rec1 : rec2;
     const float4 *__restrict__ rs = SWAP ? rec2 : rec1;
-    const uint32_t r = ROWS1 ? 0u : (uint32_t)lane / t.cw;
-    const uint32_t c = ROWS1 ? (uint32_t)lane : (uint32_t)lane - r * t.cw;
-    const bool in_tile = ROWS1 ? c < t.cw : r < t.R;
+    const uint32_t r = (uint32_t)lane / t.cw;
+    const uint32_t c = (uint32_t)lane - r * t.cw;
+    const bool in_tile = r < t.R;
+    // Blocks of gpb row groups. Rows staged past the smaller voxel (at most
+    // 31 past its last triangle) read the next voxel's records or the tile's
+    // padding (kTriPad); their cells are not valid. Lanes outside the tile
+    // read row R - 1 of a group.
+    const uint32_t gpb = kStageRows / t.R, rpb = gpb * t.R;
+    const uint32_t rr = min(r, t.R - 1u);
     uint32_t word = 0;
     for (uint32_t ch = 0; ch < t.nch; ch++) {
         const uint32_t col = ch * t.cw + c;
@@ -280,52 +288,37 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const TriS L = load_s(bl, lt);
         const TriV LV = load_v(rl, lt);
         fc(L, LV);
-        auto call = [&](uint32_t row, const TriS &S, auto &&row_v) {
+        auto lane_v = [&]() { return LV; };
+        uint32_t k = 0;
+        for (uint32_t gr = 0; gr < t.ngr; gr++, word++) {
+            if (k == 0) {
+                const uint64_t row0 = os + (uint64_t)gr * t.R;
+                wave_sync();  // the previous block's reads are done
+                if ((uint32_t)lane < rpb) {
+                    const float4 *p = bs + kBnd * (row0 + (uint32_t)lane);
+#pragma unroll
+                    for (int i = 0; i < kBnd; i++) stage->s[lane][i] = p[i];
+                }
+                if (VERTS && lane >= 32 && (uint32_t)lane - 32u < rpb) {
+                    const uint32_t q = (uint32_t)lane - 32u;
+                    const float4 *p = rs + 3 * (row0 + q);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) stage->v[q][i] = p[i];
+                }
+                wave_sync();
+            }
+            const uint32_t slot = k * t.R + rr;
+            const uint32_t row = gr * t.R + r;
             const bool valid = cv && row < ns;
-            auto lane_v = [&]() { return LV; };
+            const TriS S = load_s_at(stage->s[slot]);
+            auto row_v = [&]() { return load_v_at(stage->v[slot]); };
             if (SWAP)
                 f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v,
                   std::integral_constant<bool, true>{});
             else
                 f(S, L, PairAt{row, cv ? col : 0u, word, valid}, row_v, lane_v,
                   std::integral_constant<bool, false>{});
-        };
-        if (ROWS1) {
-            // row records through pointers stepped one row per iteration
-            const float4 *__restrict__ pb = bs + kBnd * os;
-            const float4 *__restrict__ pr = rs + 3 * os;
-            for (uint32_t gr = 0; gr < t.ngr; gr++, word++, pb += kBnd, pr += 3)
-                call(gr, load_s_at(pb), [&]() { return load_v_at(pr); });
-        } else {
-            // Blocks of gpb row groups. Rows staged past the smaller voxel
-            // (at most 31 past its last triangle) read the next voxel's
-            // records or the tile's padding (kTriPad); their cells are not
-            // valid. Lanes outside the tile read row R - 1 of a group.
-            const uint32_t gpb = kStageRows / t.R, rpb = gpb * t.R;
-            const uint32_t rr = min(r, t.R - 1u);
-            uint32_t k = 0;
-            for (uint32_t gr = 0; gr < t.ngr; gr++, word++) {
-                if (k == 0) {
-                    const uint64_t row0 = os + (uint64_t)gr * t.R;
-                    wave_sync();  // the previous block's reads are done
-                    if ((uint32_t)lane < rpb) {
-                        const float4 *p = bs + kBnd * (row0 + (uint32_t)lane);
-#pragma unroll
-                        for (int i = 0; i < kBnd; i++) stage->s[lane][i] = p[i];
-                    }
-                    if (VERTS && lane >= 32 && (uint32_t)lane - 32u < rpb) {
-                        const uint32_t q = (uint32_t)lane - 32u;
-                        const float4 *p = rs + 3 * (row0 + q);
-#pragma unroll
-                        for (int i = 0; i < 3; i++) stage->v[q][i] = p[i];
-                    }
-                    wave_sync();
-                }
-                const uint32_t slot = k * t.R + rr;
-                call(gr * t.R + r, load_s_at(stage->s[slot]),
-                     [&]() { return load_v_at(stage->v[slot]); });
-                if (++k == gpb) k = 0;
-            }
+            if (++k == gpb) k = 0;
         }
     }
 }
@@ -337,17 +330,10 @@ __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int l
                                           const float4 *__restrict__ rec2,
                                           const float4 *__restrict__ bnd2, RowStage *stage,
                                           FC &&fc, F &&f) {
-    if (t.swap) {
-        if (t.R == 1)
-            walk_tiles<true, true, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
-        else
-            walk_tiles<true, false, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
-    } else {
-        if (t.R == 1)
-            walk_tiles<false, true, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
-        else
-            walk_tiles<false, false, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
-    }
+    if (t.swap)
+        walk_tiles<true, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
+    else
+        walk_tiles<false, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
 }
 
 // ------------------------------------------------------------------ bounds
