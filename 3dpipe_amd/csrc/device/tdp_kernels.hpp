@@ -22,6 +22,12 @@ constexpr int kFoldIntersect = 8;
 enum { MODE_REF32 = 0, MODE_FP64 = 1, MODE_MIXED = 2 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+// orders a wave's LDS writes before its later reads (compiler and memory)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 // one wave per item; the grid may be 2-D (x fastest) when the item count
 // exceeds what a 1-D launch can address (2^32 work-items per dimension)
 __device__ __forceinline__ size_t wave_global() {
@@ -116,31 +122,51 @@ __global__ void __launch_bounds__(kBlock) k_aggregate(
         const float *__restrict__ sb = sw ? vb2 + 9 * (size_t)b0 : vb1 + 9 * (size_t)a0;
         const uint32_t r = lane / cw, c = lane - r * cw;
         const bool in_tile = r < rows;
-        for (uint32_t ch = 0; ch < nch; ch++) {
-            const uint32_t col = ch * cw + c;
-            const bool cv = in_tile && col < nl;
-            float lv[9];
-            const float *src = lb + 9 * (size_t)(cv ? col : nl - 1);
+        // the rows' voxel records staged in LDS when they fit (one load per
+        // voxel instead of one per voxel and iteration)
+        __shared__ float stage_all[kWavesPerBlock][kWave * 9];
+        float *const stage = stage_all[threadIdx.x / kWave];
+        const bool staged = ns <= (uint32_t)kWave;
+        if (staged) {
+            if (lane < ns) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) lv[k] = src[k];
-            for (uint32_t gr = 0; gr < ngr; gr++) {
-                const uint32_t row = gr * rows + r;
-                const float *rv = sb + 9 * (size_t)(row < ns ? row : ns - 1);
-                float sv[9];
+                for (int k = 0; k < 9; k++) stage[9 * lane + k] = sb[9 * (size_t)lane + k];
+            }
+            wave_sync();
+        }
+        // rows from LDS or from memory: two copies of the walk, so that each
+        // reads through its own address space
+        auto walk = [&](auto row_ptr) {
+            for (uint32_t ch = 0; ch < nch; ch++) {
+                const uint32_t col = ch * cw + c;
+                const bool cv = in_tile && col < nl;
+                float lv[9];
+                const float *src = lb + 9 * (size_t)(cv ? col : nl - 1);
 #pragma unroll
-                for (int k = 0; k < 9; k++) sv[k] = rv[k];
-                R s, cd;
-                // the tile-1 voxel is always the first operand
-                if (sw)
-                    voxel_sq<R>(lv, sv, s, cd);
-                else
-                    voxel_sq<R>(sv, lv, s, cd);
-                if (cv && row < ns) {
-                    smin = rmin2(smin, s);
-                    cmin = rmin2(cmin, cd);
+                for (int k = 0; k < 9; k++) lv[k] = src[k];
+                for (uint32_t gr = 0; gr < ngr; gr++) {
+                    const uint32_t row = gr * rows + r;
+                    const float *rv = row_ptr(row < ns ? row : ns - 1);
+                    float sv[9];
+#pragma unroll
+                    for (int k = 0; k < 9; k++) sv[k] = rv[k];
+                    R s, cd;
+                    // the tile-1 voxel is always the first operand
+                    if (sw)
+                        voxel_sq<R>(lv, sv, s, cd);
+                    else
+                        voxel_sq<R>(sv, lv, s, cd);
+                    if (cv && row < ns) {
+                        smin = rmin2(smin, s);
+                        cmin = rmin2(cmin, cd);
+                    }
                 }
             }
-        }
+        };
+        if (staged)
+            walk([&](uint32_t rw) { return (const float *)(stage + 9 * rw); });
+        else
+            walk([&](uint32_t rw) { return sb + 9 * (size_t)rw; });
     }
     smin = wave_min(smin);
     cmin = wave_min(cmin);
@@ -535,11 +561,6 @@ __device__ __forceinline__ void pair_interval(const float4 *r1, const float4 *b1
     intervals(dl, du, a0.w, a1.w, c0.w, c1.w, lw, uw);
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // the next float above a non-negative x (x itself for inf / nan)
 __device__ __forceinline__ float ulp_up(float x) {
