@@ -468,6 +468,48 @@ def test_intersect_self_join(tdp, eng, touching_tiles, lods):
         assert len(ot) > 10
 
 
+@pytest.mark.parametrize("query,k,tau", [("nn", 3, 0.0), ("within", 1, 25.0)])
+@pytest.mark.parametrize("level", [0, 2])
+def test_join_touching_geometry_mixed_equals_oracle(tdp, eng, touching_tiles, query, k, tau,
+                                                    level):
+    """kNN and within joins where many nuclei touch or cross a vessel (lo = 0,
+    pairs at distance 0, coplanar contacts): the mixed form's lists and ranges
+    are the FP64 oracle's bit for bit"""
+    pn, pv, pd = touching_tiles
+    m = 120
+    ot, oo, omn, omx, _ = orc.join(orc.OracleTile(pn), orc.OracleTile(pv), query, k, tau=tau,
+                                   level=level, precision="f64", max_objects1=m)
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    for prec in ("fp64", "mixed"):
+        r = eng.join(n, v, query=query, knn=k, within_dist=tau, hausdorff_level=level,
+                     precision=prec, max_objects1=m)
+        _compare(r, ot, oo, omn, omx)
+    # densely packed nuclei against each other
+    ot, oo, omn, omx, _ = orc.join(orc.OracleTile(pd), None, query, k, tau=tau, level=level,
+                                   precision="f64", max_objects1=m)
+    d = tdp.Tile.load(pd)
+    r = eng.join(d, None, query=query, knn=k, within_dist=tau, hausdorff_level=level,
+                 precision="mixed", max_objects1=m)
+    _compare(r, ot, oo, omn, omx)
+
+
+def test_join_large_coordinates_mixed_equals_oracle(tdp, eng, tmp_path):
+    """objects near the corner of a 60000^3 cube: the FP32 bounds' margins
+    scale with the coordinates and the mixed form still equals the FP64
+    oracle"""
+    n, v = tdp.generate(seed=7, n_vessels=6, nuclei_per_vessel=15, cube=60000.0,
+                        place_min=15.0, place_max=90.0)
+    pn, pv = str(tmp_path / "n.dt"), str(tmp_path / "v.dt")
+    n.save(pn)
+    v.save(pv)
+    for query, k, tau in (("nn", 2, 0.0), ("within", 1, 60.0)):
+        ot, oo, omn, omx, _ = orc.join(orc.OracleTile(pn), orc.OracleTile(pv), query, k,
+                                       tau=tau, precision="f64")
+        assert len(ot) > 0
+        r = eng.join(n, v, query=query, knn=k, within_dist=tau, precision="mixed")
+        _compare(r, ot, oo, omn, omx)
+
+
 def test_intersect_captured_rounds_and_stream(tdp, touching_tiles):
     """the graph-captured round and the streaming join give the same pairs"""
     pn, pv, pd = touching_tiles
