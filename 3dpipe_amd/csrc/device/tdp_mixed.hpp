@@ -451,6 +451,23 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
             pq[f] = kNoProbe;
         }
     }
+    // Slots of quantities the round does not fold take further probes: the
+    // pairs next to the last folded quantity's probe in the voxel pair's
+    // order (neighbouring triangles of the smaller voxel's row, where the
+    // minimum often lies when the probe misses it).
+    {
+        constexpr int mq = (NEED & 4u) ? 2 : ((NEED & 2u) ? 1 : 0);
+        const uint32_t q0 = pq[mq], nq = g.n1 * n2;
+        int step = 1;
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            if ((((NEED & need) >> f) & 1u) || !(((NEED & need) >> mq) & 1u) || q0 == kNoProbe)
+                continue;
+            const int64_t qq = (int64_t)q0 + step;
+            pq[f] = (qq >= 0 && qq < (int64_t)nq) ? (uint32_t)qq : kNoProbe;
+            step = -step;
+        }
+    }
     const bool lo_zero = U[1] == 0.f;  // some pair certainly has lo = +0
     if (lo_zero) pq[1] = kNoProbe;
     if (pq[1] == pq[0]) pq[1] = kNoProbe;
