@@ -156,9 +156,10 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
                            const tdb_tile *t1, const tdb_tile *t2, bool self, size_t N,
                            const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
                            std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids,
-                           uint64_t *n_fallback, std::string &err) {
+                           uint64_t *n_fallback, std::string &err, bool device_within) {
     if (n_fallback) *n_fallback = 0;
-    const bool knn = o->query == TDB_QUERY_NN, within = o->query == TDB_QUERY_WITHIN;
+    const bool knn = o->query == TDB_QUERY_NN,
+               within = o->query == TDB_QUERY_WITHIN && device_within;
     if (!ctx || !d1 || !d2 || !(knn || within) || t2->t.n_obj > 1024 || N == 0) {
         tdb_gen_candidates(t1, t2, self, N, o, k_t, tau_t, prefix, ids);
         return TDP_OK;

@@ -87,13 +87,14 @@ void tdb_gen_candidates(const tdb_tile *t1, const tdb_tile *t2, bool self, size_
                         const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
                         std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids);
 
-// the same, on the device where it applies (kNN, tile 2 of at most 1024
-// objects; the octree for targets whose order the device leaves open)
+// the same, on the device where it applies (kNN, and within when
+// device_within, tile 2 of at most 1024 objects; the octree for targets whose
+// lists the device leaves to it)
 int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
                            const tdb_tile *t1, const tdb_tile *t2, bool self, size_t N,
                            const tdb_join_opts *o, const int32_t *k_t, const float *tau_t,
                            std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids,
-                           uint64_t *n_fallback, std::string &err);
+                           uint64_t *n_fallback, std::string &err, bool device_within = true);
 
 // Filter + LOD rounds on the device for the given candidates, results into
 // res (targets numbered 0..N-1, reported pairs ordered by (round, list

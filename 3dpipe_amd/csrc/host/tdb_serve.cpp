@@ -309,9 +309,12 @@ void tdb_server::preparer() {
         } else {
             ms_upload = ms_since(t0);
             t0 = Clock::now();
+            // within lists from the host octree here: the preparer runs while
+            // the executor's joins occupy the device, and a device pass would
+            // queue behind them
             if (tdb_gen_candidates_dev(prep_ctx, b->dev, res_dev, &b->merged, resident, false, N,
                                        &o, b->k_t.data(), b->tau_t.data(), b->prefix, b->ids,
-                                       nullptr, b->err) != TDP_OK &&
+                                       nullptr, b->err, false) != TDP_OK &&
                 b->err.empty())
                 b->err = "candidate generation failed";
         }

@@ -42,8 +42,8 @@ PEAK_FP64_TFLOPS = 78.6
 # factors at which the counters read a plain 1 GiB device copy,
 # --calibrate-copy 1024, as 1 GiB): k_mx_mark (mean over the 5 rounds), and
 # one refinement round (all its kernels).
-MARK_TRAFFIC_BYTES = 10.58e9
-ROUND_TRAFFIC_BYTES = 24.04e9
+MARK_TRAFFIC_BYTES = 12.03e9
+ROUND_TRAFFIC_BYTES = 25.79e9
 DEFAULT_TUNING = {"small_pairs": 8, "max_rows": 64, "drain": 0, "drain_blocks": 0,
                   "eval_blocks": 4096, "graph": 0}
 REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",
