@@ -31,6 +31,7 @@
  * invariants and by brute force at the final LOD on small inputs.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -741,6 +742,9 @@ void orc_result_get(const orc_result *r, int32_t *target, int32_t *obj2, double 
     }
 }
 /* tri_pairs, voxel_pairs, object_pairs, rounds */
+/* threads the join's refinement folds run on (OpenMP) */
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 void orc_result_stats(const orc_result *r, uint64_t *out4) {
     out4[0] = r->stats.tri_pairs;
     out4[1] = r->stats.voxel_pairs;

@@ -58,6 +58,7 @@ def lib():
         L.orc_result_get.argtypes = [P, P, P, P, P]
         L.orc_result_stats.argtypes = [P, P]
         L.orc_result_free.argtypes = [P]
+        L.orc_max_threads.restype = c.c_int
         _lib = L
     return _lib
 
@@ -165,6 +166,11 @@ def join(t1, t2, query="nn", k=1, tau=1000.0, lods=(20, 40, 60, 80, 100), level=
     L.orc_result_free(r)
     stats = dict(zip(["tri_pairs", "voxel_pairs", "object_pairs", "rounds"], st.tolist()))
     return t, o, mn, mx, stats
+
+
+def threads():
+    """OpenMP threads of the oracle's join refinement."""
+    return int(lib().orc_max_threads())
 
 
 def group(target, obj2, mn=None, mx=None):
