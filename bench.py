@@ -400,10 +400,10 @@ def main():
         log(f"plain FP64: {fp64_ms:.1f} ms, identical={same}")
     if args.cpu_baseline:
         with tempfile.TemporaryDirectory() as d:
-            # the first nuclei of the same workload (identical objects) and
-            # all vessels, written for the oracle
-            n_sub = min(nuc.n_obj, 2000)
-            sub, _ = tdp.generate(max_nuclei=n_sub, **gen_kw)
+            # the first nuclei of the same tile and all vessels, written for
+            # the oracle
+            n_sub = min(nuc.n_obj, 20000)
+            sub = tdp.Source.of_tile(nuc).decode(np.arange(n_sub))
             sub.save(os.path.join(d, "nuclei.dt"))
             ves.save(os.path.join(d, "vessels.dt"))
             del sub
