@@ -462,6 +462,23 @@ extern "C" int tdp_device_info(tdp_ctx *ctx, char *name, size_t name_len, int *c
 
 // ------------------------------------------------------------ tiles
 
+// What the kernels assume of a tile's voxels, checked before any upload:
+// voxel ranges that do not decrease, every voxel's records inside the record
+// array, and at most kMaxVoxTris triangles per voxel and LOD slot (the pair
+// walks number a voxel pair's n1 x n2 triangle pairs in 32 bits). Returns
+// nullptr or what is wrong.
+static const char *check_voxels(const tdp_tile_desc *d) {
+    for (size_t o = 0; o < d->n_obj; o++)
+        if (d->obj_vox_off[o + 1] < d->obj_vox_off[o]) return "obj_vox_off decreases";
+    for (size_t v = 0; v < 5 * d->n_vox; v++) {
+        if (d->vox_tri_cnt[v] > kMaxVoxTris)
+            return "a voxel holds more than 65535 triangles at one LOD";
+        if (d->vox_tri_off[v] > d->n_tri || d->vox_tri_cnt[v] > d->n_tri - d->vox_tri_off[v])
+            return "voxel triangles outside the record array";
+    }
+    return nullptr;
+}
+
 // tri_parts: the records in n_parts host buffers (concatenated in order), or
 // nullptr for d->tri
 static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const *tri_parts,
@@ -482,8 +499,10 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
             return TDP_ERR_ARG;
         }
     }
-    if (!d->obj_vox_off || !d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt ||
-        (d->n_tri && !d->tri && !tri_parts) || !d->obj_hd) {
+    // arrays may be null where their count is zero (a tile without objects
+    // is a valid tile: its joins have no candidates)
+    if (!d->obj_vox_off || (d->n_vox && (!d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt)) ||
+        (d->n_tri && !d->tri && !tri_parts) || (d->n_obj && !d->obj_hd)) {
         ctx->err = "tdp_tile_upload: null array";
         return TDP_ERR_ARG;
     }
@@ -491,11 +510,8 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
         ctx->err = "tdp_tile_upload: obj_vox_off does not span the voxels";
         return TDP_ERR_ARG;
     }
-    for (size_t v = 0; v < 5 * d->n_vox; v++)
-        if (d->vox_tri_off[v] + d->vox_tri_cnt[v] > d->n_tri) {
-            ctx->err = "tdp_tile_upload: voxel triangles outside the record array";
-            return TDP_ERR_ARG;
-        }
+    if (const char *bad = check_voxels(d))
+        return ctx->err = std::string("tdp_tile_upload: ") + bad, TDP_ERR_ARG;
     tdp_tile *t = new tdp_tile();
     t->ctx = ctx;
     t->n_obj = d->n_obj;
@@ -525,7 +541,7 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
     up(&t->vox_obj, vobj.data(), d->n_vox);
     up(&t->vox_tri_off, d->vox_tri_off, 5 * d->n_vox);
     up(&t->vox_tri_cnt, d->vox_tri_cnt, 5 * d->n_vox);
-    if (d->obj_box && d->obj_transform) {
+    if ((d->obj_box && d->obj_transform) || d->n_obj == 0) {
         up(&t->obj_box, d->obj_box, 6 * d->n_obj);
         up(&t->obj_tr, d->obj_transform, 3 * d->n_obj);
     }
@@ -719,7 +735,8 @@ extern "C" int tdp_tile_fill_async(tdp_tile *t, const tdp_tile_desc *d) {
         ctx->err = "tdp_tile_fill_async: sizes beyond the reserved capacity";
         return TDP_ERR_ARG;
     }
-    if (!d->obj_vox_off || !d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt || !d->obj_hd) {
+    if (!d->obj_vox_off || (d->n_vox && (!d->vox_box || !d->vox_tri_off || !d->vox_tri_cnt)) ||
+        (d->n_obj && !d->obj_hd)) {
         ctx->err = "tdp_tile_fill_async: null array";
         return TDP_ERR_ARG;
     }
@@ -727,16 +744,8 @@ extern "C" int tdp_tile_fill_async(tdp_tile *t, const tdp_tile_desc *d) {
         ctx->err = "tdp_tile_fill_async: obj_vox_off does not span the voxels";
         return TDP_ERR_ARG;
     }
-    for (size_t o = 0; o < d->n_obj; o++)
-        if (d->obj_vox_off[o + 1] < d->obj_vox_off[o]) {
-            ctx->err = "tdp_tile_fill_async: obj_vox_off decreases";
-            return TDP_ERR_ARG;
-        }
-    for (size_t v = 0; v < 5 * d->n_vox; v++)
-        if (d->vox_tri_off[v] + d->vox_tri_cnt[v] > d->n_tri) {
-            ctx->err = "tdp_tile_fill_async: voxel triangles outside the record array";
-            return TDP_ERR_ARG;
-        }
+    if (const char *bad = check_voxels(d))
+        return ctx->err = std::string("tdp_tile_fill_async: ") + bad, TDP_ERR_ARG;
     hipStream_t s = ctx->stream;
     auto cp = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;

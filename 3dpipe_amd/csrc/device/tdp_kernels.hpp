@@ -456,6 +456,9 @@ constexpr int kBnd = 3;
 // tile: a tiled walk reads up to kWave - 1 rows past a voxel's last triangle
 // without clamping its index
 constexpr size_t kTriPad = kWave;
+// most triangles of one voxel at one LOD: the pair walks number a voxel
+// pair's n1 x n2 triangle pairs in 32 bits (tiles are checked at upload)
+constexpr uint32_t kMaxVoxTris = 65535;
 
 // owning object of every voxel, from the per-object voxel offsets
 __global__ void __launch_bounds__(kBlock) k_vox_owner(size_t n_obj,

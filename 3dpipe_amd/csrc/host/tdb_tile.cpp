@@ -56,19 +56,24 @@ bool read_raw(const std::string &path, Tile &t, std::string &err) {
     t = Tile();
     t.obj_vox_off.push_back(0);
     size_t pos = 1;
+    // every size read from the file is checked against the bytes left
+    // (subtraction form: a huge count cannot wrap the position back)
     auto rd = [&](void *dst, size_t n) {
-        if (pos + n > buf.size()) return false;
+        if (n > buf.size() - pos) return false;
         std::memcpy(dst, buf.data() + pos, n);
         pos += n;
         return true;
     };
     while (pos < buf.size()) {
         uint64_t dsize = 0, nvox = 0;
-        if (!rd(&dsize, 8) || pos + dsize > buf.size()) { err = "truncated object data"; return false; }
+        if (!rd(&dsize, 8) || dsize > buf.size() - pos) { err = "truncated object data"; return false; }
         const char *data = buf.data() + pos;
         pos += dsize;
         float hd[10];
         if (!rd(&nvox, 8) || !rd(hd, 40)) { err = "truncated object meta"; return false; }
+        // each voxel's meta is 36 + 80 bytes; voxel ids are 32-bit
+        if (nvox > (buf.size() - pos) / 116) { err = "truncated voxel meta"; return false; }
+        if (nvox > (uint64_t)UINT32_MAX - t.n_vox()) { err = "too many voxels"; return false; }
         t.obj_hd.insert(t.obj_hd.end(), hd, hd + 10);
         size_t v0 = t.n_vox();
         std::vector<std::array<uint64_t, 10>> lodinfo(nvox);
@@ -87,7 +92,8 @@ bool read_raw(const std::string &path, Tile &t, std::string &err) {
         for (int s = 0; s < kLods; s++)
             for (uint64_t v = 0; v < nvox; v++) {
                 uint64_t off = lodinfo[v][2 * s], cnt = lodinfo[v][2 * s + 1];
-                if (off + 44 * cnt > dsize) { err = "voxel block outside object data"; return false; }
+                if (off > dsize || cnt > (dsize - off) / 44) { err = "voxel block outside object data"; return false; }
+                if (cnt > UINT32_MAX) { err = "voxel triangle count beyond 32 bits"; return false; }
                 t.vox_tri_off[5 * (v0 + v) + s] = t.n_tri();
                 t.vox_tri_cnt[5 * (v0 + v) + s] = (uint32_t)cnt;
                 const float *tp = (const float *)(data + off);

@@ -625,6 +625,26 @@ void orc_aab_distance(const float *a, const float *b, float *out2) {
     aab_distance(a, a + 3, b, b + 3, out2, out2 + 1);
 }
 
+/* kNN status of one target's m candidate ranges (2 doubles each, min and
+ * max) with cand_left still wanted: 1 = CONFIRMED, 2 = REMOVE, 0 = KEEP,
+ * in FP32 (the ranges rounded to float) or FP64 */
+void orc_knn_status(const double *ranges, int m, int cand_left, int fp64, int *status) {
+    if (fp64) {
+        range_d *r = (range_d *)malloc((m ? m : 1) * sizeof(range_d));
+        for (int i = 0; i < m; i++) { r[i].mn = ranges[2 * i]; r[i].mx = ranges[2 * i + 1]; }
+        knn_status_d(r, m, cand_left, status);
+        free(r);
+    } else {
+        range_f *r = (range_f *)malloc((m ? m : 1) * sizeof(range_f));
+        for (int i = 0; i < m; i++) {
+            r[i].mn = (float)ranges[2 * i];
+            r[i].mx = (float)ranges[2 * i + 1];
+        }
+        knn_status_f(r, m, cand_left, status);
+        free(r);
+    }
+}
+
 /* Octree over bare boxes (no tile data) for candidate-order tests: n boxes
  * given as low[3n], high[3n], transform[3n]. */
 typedef struct {

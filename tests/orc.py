@@ -40,6 +40,7 @@ def lib():
         L.orc_voxel_bound_f.argtypes = [P, P, P]
         L.orc_voxel_bound_d.argtypes = [P, P, P]
         L.orc_aab_distance.argtypes = [P, P, P]
+        L.orc_knn_status.argtypes = [P, c.c_int, c.c_int, c.c_int, P]
         L.orc_boxset_build.argtypes = [SZ, P, P, P, c.c_int]
         L.orc_boxset_build.restype = P
         for f in (L.orc_query_knn,):

@@ -421,6 +421,73 @@ def test_raw_reader_rejects_bad_files(tdp, tmp_path):
         tdp.Tile.load(str(tmp_path / "missing.dt"))
 
 
+def _raw_object(dsize, nvox, data=b"", voxels=b""):
+    """one RAW object record with the given header counts"""
+    return (dsize.to_bytes(8, "little") + data + nvox.to_bytes(8, "little") + b"\x00" * 40 +
+            voxels + b"\x00" * 12)
+
+
+def test_raw_reader_rejects_counts_that_overflow(tdp, tmp_path):
+    """sizes read from a corrupt file are checked against the bytes left, in
+    a form that a huge count cannot wrap (2^64 - 8 bytes of data, a voxel
+    count beyond the file, a voxel block past the object's data)"""
+    cases = {
+        "huge_dsize": b"\x01" + (2**64 - 8).to_bytes(8, "little") + b"\x00" * 200,
+        "huge_nvox": b"\x01" + _raw_object(0, 2**62),
+        "nvox_past_end": b"\x01" + _raw_object(0, 3, voxels=b"\x00" * 116),
+    }
+    # a voxel whose LOD-20 block claims 2^62 triangles from offset 0
+    vox = np.zeros(9, np.float32).tobytes() + np.array([0, 2**62] + [0, 0] * 4, np.uint64).tobytes()
+    cases["huge_count"] = b"\x01" + _raw_object(44, 1, data=b"\x00" * 44, voxels=vox)
+    vox = np.zeros(9, np.float32).tobytes() + np.array([2**64 - 4, 1] + [0, 0] * 4, np.uint64).tobytes()
+    cases["huge_offset"] = b"\x01" + _raw_object(44, 1, data=b"\x00" * 44, voxels=vox)
+    for name, blob in cases.items():
+        p = tmp_path / f"{name}.dt"
+        p.write_bytes(blob)
+        with pytest.raises(tdp.TdpError, match="truncated|outside|too many|beyond"):
+            tdp.Tile.load(str(p))
+    # a well-formed one-voxel object with one triangle at LOD 20 still loads
+    vox = np.zeros(9, np.float32).tobytes() + np.array([0, 1] + [0, 0] * 4, np.uint64).tobytes()
+    p = tmp_path / "ok.dt"
+    p.write_bytes(b"\x01" + _raw_object(44, 1, data=b"\x00" * 44, voxels=vox))
+    t = tdp.Tile.load(str(p))
+    assert (t.n_obj, t.n_vox, t.n_tri) == (1, 1, 1)
+
+
+def test_upload_rejects_voxels_over_65535_triangles(tdp):
+    """the pair walks number a voxel pair's n1 x n2 triangle pairs in 32 bits,
+    so a voxel of more than 65535 triangles at one LOD is refused with a named
+    error before anything reaches the device (checked here on a context whose
+    device open failed: the check runs before any HIP call)"""
+    import ctypes as c
+    _, lib = tdp._load()
+    ctx = c.c_void_p()
+    lib.tdp_open(0, c.byref(ctx))
+    assert ctx.value, "tdp_open returns its context even when it fails"
+
+    class Desc(c.Structure):
+        _fields_ = [("n_obj", c.c_size_t), ("n_vox", c.c_size_t), ("n_tri", c.c_size_t)] + \
+                   [(f, c.c_void_p) for f in ("obj_vox_off", "obj_hd", "vox_box", "vox_tri_off",
+                                              "vox_tri_cnt", "tri", "obj_box", "obj_transform")]
+    n_tri = 70000
+    off = np.array([0, 2], np.uint32)
+    hd = np.zeros(10, np.float32)
+    vb = np.zeros(18, np.float32)
+    cnt = np.array([70000, 0, 0, 0, 0, 1, 1, 1, 1, 1], np.uint32)
+    first = np.zeros(10, np.uint64)
+    tri = np.zeros(12 * n_tri, np.float32)
+    d = Desc(1, 2, n_tri, *[a.ctypes.data for a in (off, hd, vb, first, cnt, tri)], None, None)
+    lib.tdp_tile_upload.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+    out = c.c_void_p()
+    assert lib.tdp_tile_upload(ctx, c.byref(d), c.byref(out)) == -1  # TDP_ERR_ARG
+    assert b"65535" in lib.tdp_last_error(ctx)
+    cnt[0] = 65535
+    cnt[5] = 70000  # second voxel, LOD 20: records outside the array
+    assert lib.tdp_tile_upload(ctx, c.byref(d), c.byref(out)) == -1
+    assert b"65535" in lib.tdp_last_error(ctx)
+    lib.tdp_close(ctx)
+
+
 def test_generator_shapes(tdp):
     n, v = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=20, cube=2000.0)
     assert (n.n_obj, v.n_obj) == (200, 10)
