@@ -192,15 +192,23 @@ int cmd_join(int argc, char **argv) {
         const uint64_t n = tdb_result_count(r);
         std::vector<int32_t> a(n), b(n);
         std::vector<double> mn(n), mx(n);
-        tdb_result_get(r, a.data(), b.data(), mn.data(), mx.data(), nullptr);
+        std::vector<uint8_t> rnd(n);
+        tdb_result_get(r, a.data(), b.data(), mn.data(), mx.data(), rnd.data());
+        // --print_result prints what the reference prints: every report made
+        // through report_result (himesh_wrapper.cpp:195-202). The within join
+        // appends its refinement-round results without it
+        // (WithinJoin.cpp:779), so for -q within only the filter's reports
+        // (round 0) are printed; --print_distance prints every result with
+        // its distance range.
         const bool dist = p.has("print_distance");
+        const bool refine_silent = o.query == TDB_QUERY_WITHIN;
         if (p.has("print_result") || dist)
             for (uint64_t i = 0; i < n; i++) {
                 if (dist) std::printf("%d %d %.9g %.9g\n", a[i], b[i], mn[i], mx[i]);
-                else std::printf("%d %d\n", a[i], b[i]);
+                else if (!refine_silent || rnd[i] == 0) std::printf("%d %d\n", a[i], b[i]);
             }
         double tm[7];
-        uint64_t st[6];
+        uint64_t st[7];
         tdb_result_timing(r, tm);
         tdb_result_stats(r, st);
         std::fprintf(stderr,

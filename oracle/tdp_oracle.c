@@ -96,6 +96,7 @@ typedef struct {
 typedef struct {
     int target, obj2;
     double mn, mx;
+    int round; /* 0 = filter, r = r-th LOD round */
 } res_item;
 
 typedef struct {
@@ -115,7 +116,7 @@ static void res_push(orc_result *R, int target, int obj2, double mn, double mx) 
         R->cap = R->cap ? 2 * R->cap : 1024;
         R->items = (res_item *)realloc(R->items, R->cap * sizeof(res_item));
     }
-    res_item it = {target, obj2, mn, mx};
+    res_item it = {target, obj2, mn, mx, R->stats.rounds};
     R->items[R->n++] = it;
 }
 
@@ -760,6 +761,10 @@ void orc_result_get(const orc_result *r, int32_t *target, int32_t *obj2, double 
         mn[i] = r->items[i].mn;
         mx[i] = r->items[i].mx;
     }
+}
+/* the round of every report (0 = filter, r = r-th LOD round) */
+void orc_result_rounds(const orc_result *r, uint8_t *round) {
+    for (size_t i = 0; i < r->n; i++) round[i] = (uint8_t)r->items[i].round;
 }
 /* tri_pairs, voxel_pairs, object_pairs, rounds */
 /* threads the join's refinement folds run on (OpenMP) */

@@ -58,6 +58,7 @@ def lib():
         L.orc_result_count.restype = SZ
         L.orc_result_get.argtypes = [P, P, P, P, P]
         L.orc_result_stats.argtypes = [P, P]
+        L.orc_result_rounds.argtypes = [P, P]
         L.orc_result_free.argtypes = [P]
         L.orc_max_threads.restype = c.c_int
         _lib = L
@@ -148,8 +149,9 @@ class OracleTile:
 
 
 def join(t1, t2, query="nn", k=1, tau=1000.0, lods=(20, 40, 60, 80, 100), level=2,
-         precision="f32", max_objects1=0):
-    """Returns (target, obj2, min, max) arrays in report order and stats."""
+         precision="f32", max_objects1=0, rounds=False):
+    """Returns (target, obj2, min, max) arrays in report order and stats
+    (with rounds=True, also the round of every report: 0 = filter)."""
     L = lib()
     lods = np.array(lods, np.int32)
     qcode = {"nn": 1, "within": 0, "intersect": 2}[query]
@@ -162,10 +164,14 @@ def join(t1, t2, query="nn", k=1, tau=1000.0, lods=(20, 40, 60, 80, 100), level=
     t = np.zeros(n, np.int32); o = np.zeros(n, np.int32)
     mn = np.zeros(n, np.float64); mx = np.zeros(n, np.float64)
     L.orc_result_get(r, ptr(t), ptr(o), ptr(mn), ptr(mx))
+    rd = np.zeros(n, np.uint8)
+    L.orc_result_rounds(r, ptr(rd))
     st = np.zeros(4, np.uint64)
     L.orc_result_stats(r, ptr(st))
     L.orc_result_free(r)
     stats = dict(zip(["tri_pairs", "voxel_pairs", "object_pairs", "rounds"], st.tolist()))
+    if rounds:
+        return t, o, mn, mx, stats, rd
     return t, o, mn, mx, stats
 
 

@@ -1,0 +1,246 @@
+"""The HIP path against the CPU oracle on slices of the full-scale workloads,
+on the streaming join and the serving loop, on tiles without objects and
+through the `tdbase join` command (needs an MI355X).
+
+Bars as in test_gpu_parity.py: REF32 bit-exact against the oracle's FP32
+restatement; FP64 and mixed bit-exact against its FP64 restatement; lists,
+report order, report rounds and distance ranges all compared."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import orc
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_PREC = {"ref32": "f32", "fp64": "f64", "mixed": "f64"}
+
+
+def _check(res, want, what):
+    """res: a JoinResult; want: the oracle's (target, obj2, min, max, stats, rounds)"""
+    ot, oo, omn, omx, _, ord_ = want
+    got = orc.group(res.target, res.obj2, res.rmin, res.rmax)
+    exp = orc.group(ot, oo, omn, omx)
+    assert set(got) == set(exp), f"{what}: targets with results differ"
+    for t in exp:
+        assert got[t] == exp[t], f"{what}: target {t}: {got[t][:4]} vs {exp[t][:4]}"
+    gr = orc.group(res.target, res.round)
+    er = orc.group(ot, ord_)
+    assert gr == er, f"{what}: report rounds differ"
+
+
+def _save(tile, path):
+    tile.save(path)
+    return path
+
+
+@pytest.fixture(scope="module")
+def eng(tdp):
+    e = tdp.Engine(0)
+    yield e
+    e.close()
+
+
+# ------------------------------------------------------------ full-scale slices
+
+SLICE = 2000  # targets of the full-scale tiles checked against the oracle
+
+
+@pytest.fixture(scope="module")
+def full_scale(tdp, tmp_path_factory):
+    """The 10-NN workload's tiles (seed 1, 1000 vessels x 200 nuclei in a
+    20000^3 cube), its first SLICE nuclei as the targets."""
+    d = tmp_path_factory.mktemp("full")
+    n, v = tdp.generate(seed=1, n_vessels=1000, nuclei_per_vessel=200, cube=20000.0,
+                        max_nuclei=SLICE)
+    pn, pv = _save(n, str(d / "n.dt")), _save(v, str(d / "v.dt"))
+    return n, v, orc.OracleTile(pn), orc.OracleTile(pv)
+
+
+@pytest.fixture(scope="module")
+def uneven(tdp, tmp_path_factory):
+    """The within-400 workload's tiles: the same nuclei around vessels of
+    4..48 rings per segment (1.5k-18k triangles)."""
+    d = tmp_path_factory.mktemp("uneven")
+    n, v = tdp.generate(seed=1, n_vessels=1000, nuclei_per_vessel=200, cube=20000.0,
+                        rings_min=4, rings_max=48, max_nuclei=SLICE)
+    pn, pv = _save(n, str(d / "n.dt")), _save(v, str(d / "v.dt"))
+    return n, v, orc.OracleTile(pn), orc.OracleTile(pv)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "ref32"])
+def test_knn10_full_scale_slice(eng, full_scale, precision):
+    n, v, on, ov = full_scale
+    assert n.n_obj == SLICE and v.n_obj == 1000
+    want = orc.join(on, ov, "nn", 10, 0, precision=ORACLE_PREC[precision], rounds=True)
+    res = eng.join(n, v, query="nn", knn=10, precision=precision)
+    print(precision, len(res.target), res.stats)
+    assert len(res.target) >= 10 * SLICE // 2
+    assert res.stats["tri_pairs"] == want[4]["tri_pairs"]
+    _check(res, want, f"10-NN {precision}")
+
+
+@pytest.mark.parametrize("precision,tuning", [
+    ("mixed", dict(graph=1, drain=1, drain_blocks=2048, max_rows=64, small_pairs=8)),
+    ("mixed", dict(graph=0, drain=0, max_rows=4, small_pairs=64)),
+    ("fp64", {}), ("ref32", {})])
+def test_within400_uneven_slice(tdp, uneven, precision, tuning):
+    n, v, on, ov = uneven
+    e = tdp.Engine(0)
+    if tuning:
+        e.set_tuning(**tuning)
+    want = orc.join(on, ov, "within", 1, 400.0, precision=ORACLE_PREC[precision], rounds=True)
+    res = e.join(n, v, query="within", within_dist=400.0, precision=precision)
+    e.close()
+    print(precision, tuning, len(res.target), res.stats)
+    assert len(res.target) > SLICE
+    assert res.stats["tri_pairs"] == want[4]["tri_pairs"]
+    _check(res, want, f"within 400 {precision} {tuning}")
+
+
+def test_autotuned_within400_slice(tdp, uneven):
+    """the autotuner's pick (whatever it is) joins to the oracle's result"""
+    n, v, on, ov = uneven
+    e = tdp.Engine(0)
+    e.autotune(n, v, query="within", within_dist=400.0, sample_targets=500)
+    res = e.join(n, v, query="within", within_dist=400.0, precision="mixed")
+    e.close()
+    _check(res, orc.join(on, ov, "within", 1, 400.0, precision="f64", rounds=True), "autotuned")
+
+
+# ------------------------------------------------------------ streaming and serving
+
+NUC = dict(seed=4, n_vessels=0, nuclei_per_vessel=600, cube=1500.0, nucleus_r_min=6.0,
+           nucleus_r_max=14.0)
+MB = 1 << 20
+
+
+@pytest.mark.parametrize("query,k,tau,precision", [("nn", 10, 0.0, "mixed"),
+                                                   ("within", 1, 120.0, "mixed"),
+                                                   ("nn", 5, 0.0, "ref32")])
+def test_stream_self_join_against_oracle(tdp, eng, tmp_path, query, k, tau, precision):
+    """many chunks (a 64 MiB budget), stages overlapped, joins split to the
+    cap: each target's list equals the oracle's join of the whole tile"""
+    n, _ = tdp.generate(**NUC)
+    on = orc.OracleTile(_save(n, str(tmp_path / "n.dt")))
+    got = eng.stream_join(tdp.Source.generate(**NUC), None, query=query, knn=k, within_dist=tau,
+                          precision=precision, hbm_budget=64 * MB, overlap=True)
+    assert got.stream["chunks"] > 4
+    _check(got, orc.join(on, None, query, k, tau, precision=ORACLE_PREC[precision], rounds=True),
+           f"stream {query}")
+
+
+QUERIES = [(7, "nn", 1, 0.0), (30, "nn", 10, 0.0), (3, "nn", 5, 0.0), (45, "nn", 20, 0.0),
+           (12, "within", 1, 80.0), (25, "within", 1, 300.0), (18, "nn", 5, 0.0)]
+
+
+def _query_tile(tdp, i, size):
+    n, _ = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=max(1, -(-size // 10)),
+                        cube=2000.0, max_nuclei=size, nuclei_seed=1000 + i, emit_vessels=0)
+    return n
+
+
+def test_server_results_against_oracle(tdp, tmp_path):
+    """queries batched into shared joins: each result equals the oracle's
+    join of that query tile alone"""
+    _, vessels = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=1, cube=2000.0)
+    ov = orc.OracleTile(_save(vessels, str(tmp_path / "v.dt")))
+    want = []
+    for i, (size, q, k, tau) in enumerate(QUERIES):
+        oq = orc.OracleTile(_save(_query_tile(tdp, i, size), str(tmp_path / f"q{i}.dt")))
+        want.append(orc.join(oq, ov, q, k, tau, precision="f64", rounds=True))
+    srv = tdp.Server(vessels, max_batch=64, overlap=True, precision="mixed")
+    tickets = [srv.submit(_query_tile(tdp, i, size), query=q, knn=k, within_dist=tau)
+               for i, (size, q, k, tau) in enumerate(QUERIES)]
+    for i, tk in enumerate(tickets):
+        res, _ = srv.wait(tk)
+        _check(res, want[i], f"query {i}")
+    st = srv.stats()
+    srv.close()
+    assert st["max_batch"] >= 2
+
+
+# ------------------------------------------------------------ tiles without objects
+
+
+@pytest.mark.parametrize("precision", ["ref32", "fp64", "mixed"])
+def test_empty_tile_joins_to_empty_result(tdp, eng, small_tiles, tmp_path, precision):
+    """a tile holding only the type byte has no objects (tile_IO.cpp:37-45):
+    a join with it on either side has no results, as in the oracle"""
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    empty = tdp.Source.of_tile(n).decode(np.arange(0))
+    pe = _save(empty, str(tmp_path / "e.dt"))
+    assert open(pe, "rb").read() == b"\x01"
+    empty = tdp.Tile.load(pe)
+    oe, on, ov = orc.OracleTile(pe), orc.OracleTile(pn), orc.OracleTile(pv)
+    for query, k, tau in (("nn", 3, 0.0), ("within", 1, 150.0), ("intersect", 1, 0.0)):
+        for (a, b), (oa, ob) in (((empty, v), (oe, ov)), ((n, empty), (on, oe)),
+                                 ((empty, None), (oe, None))):
+            want = orc.join(oa, ob, query, k, tau, precision=ORACLE_PREC[precision], rounds=True)
+            res = eng.join(a, b, query=query, knn=k, within_dist=tau, precision=precision)
+            assert res.n_targets == a.n_obj
+            assert len(want[0]) == 0 and len(res.target) == 0
+    # the engine still joins afterwards
+    res = eng.join(n, v, query="nn", knn=3, precision=precision)
+    _check(res, orc.join(on, ov, "nn", 3, 0, precision=ORACLE_PREC[precision], rounds=True),
+           "after empty")
+
+
+def test_empty_tile_in_stream_and_server(tdp, eng, small_tiles):
+    pn, pv = small_tiles
+    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
+    empty = tdp.Source.of_tile(n).decode(np.arange(0))
+    got = eng.stream_join(tdp.Source.of_tile(empty), tdp.Source.of_tile(v), query="nn", knn=3)
+    assert len(got.target) == 0 and got.n_targets == 0
+    got = eng.stream_join(tdp.Source.of_tile(n), tdp.Source.of_tile(empty), query="nn", knn=3)
+    assert len(got.target) == 0 and got.n_targets == n.n_obj
+    srv = tdp.Server(v, max_batch=8)
+    tk = srv.submit(tdp.Source.of_tile(n).decode(np.arange(0)), query="nn", knn=3)
+    res, _ = srv.wait(tk)
+    srv.close()
+    assert len(res.target) == 0 and res.n_targets == 0
+
+
+# ------------------------------------------------------------ the command line
+
+
+def _cli(*args):
+    exe = os.path.join(ROOT, "3dpipe_amd", "bin", "tdbase")
+    p = subprocess.run([exe, "join", *args], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    return [tuple(int(x) for x in line.split()[:2]) for line in p.stdout.splitlines()
+            if line.strip() and line.split()[0].lstrip("-").isdigit()], p
+
+
+@pytest.mark.parametrize("precision", ["ref32", "mixed"])
+def test_cli_print_result_matches_oracle(small_tiles, precision):
+    """`tdbase join --print_result` prints what the reference prints: every
+    kNN report, and for within only the filter's reports (the refinement
+    rounds' results are appended without printing, WithinJoin.cpp:779);
+    --print_distance prints every result"""
+    pn, pv = small_tiles
+    on, ov = orc.OracleTile(pn), orc.OracleTile(pv)
+    op = ORACLE_PREC[precision]
+    got, _ = _cli("--tile1", pn, "--tile2", pv, "-q", "nn", "--knn", "3", "-g",
+                  "--print_result", "--precision", precision)
+    ot, oo, _, _, _ = orc.join(on, ov, "nn", 3, 0, precision=op)
+    assert orc.group(np.array([g[0] for g in got]), np.array([g[1] for g in got])) == \
+        orc.group(ot, oo)
+    got, _ = _cli("--tile1", pn, "--tile2", pv, "-q", "within", "--within_dist", "150",
+                  "--print_result", "--precision", precision)
+    ot, oo, _, _, _, rd = orc.join(on, ov, "within", 1, 150.0, precision=op, rounds=True)
+    assert (rd > 0).any() and (rd == 0).any()  # both kinds of report occur
+    assert sorted(got) == sorted(zip(ot[rd == 0].tolist(), oo[rd == 0].tolist()))
+    got, _ = _cli("--tile1", pn, "--tile2", pv, "-q", "within", "--within_dist", "150",
+                  "--print_distance", "--precision", precision)
+    assert sorted(got) == sorted(zip(ot.tolist(), oo.tolist()))
+    # self-join (no --tile2) with an LOD list carrying the parser's quirk
+    got, _ = _cli("--tile1", pn, "-q", "nn", "--knn", "2", "--lods", "60", "20", "20",
+                  "--print_result", "--precision", precision)
+    ot, oo, _, _, _ = orc.join(on, None, "nn", 2, 0, lods=(60, 20, 20), precision=op)
+    assert orc.group(np.array([g[0] for g in got]), np.array([g[1] for g in got])) == \
+        orc.group(ot, oo)
