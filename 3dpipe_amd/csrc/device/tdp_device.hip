@@ -943,8 +943,11 @@ extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, co
                                   int self, const int32_t *k_target, uint32_t *prefix,
                                   uint32_t *ids, size_t ids_cap, uint64_t *n_ids,
                                   uint8_t *fallback) {
-    if (!ctx || !t1 || !t2 || !n_ids || (N && (!k_target || !prefix || !fallback)))
+    if (!ctx) return TDP_ERR_ARG;
+    if (!t1 || !t2 || !n_ids || (N && (!k_target || !prefix || !fallback))) {
+        ctx->err = "tdp_knn_candidates: null argument";
         return TDP_ERR_ARG;
+    }
     *n_ids = 0;
     if (!t1->obj_box || !t2->obj_box || !t1->obj_tr || !t2->obj_tr) {
         ctx->err = "tdp_knn_candidates: tiles uploaded without object boxes";
@@ -1020,9 +1023,13 @@ extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
                                      const uint32_t *leaf_off, const uint32_t *leaf_obj,
                                      uint32_t *prefix, uint32_t *ids, size_t ids_cap,
                                      uint64_t *n_ids, uint8_t *fallback) {
-    if (!ctx || !t1 || !t2 || !n_ids || (N && (!tau_target || !prefix || !fallback)) ||
-        (n_leaves && (!leaf_box || !leaf_off || !leaf_obj)))
+    if (!ctx) return TDP_ERR_ARG;
+    // leaf_obj may be null when the leaves hold no objects (an empty tile 2)
+    if (!t1 || !t2 || !n_ids || (N && (!tau_target || !prefix || !fallback)) ||
+        (n_leaves && (!leaf_box || !leaf_off)) || (n_leaves && leaf_off[n_leaves] && !leaf_obj)) {
+        ctx->err = "tdp_within_candidates: null argument";
         return TDP_ERR_ARG;
+    }
     *n_ids = 0;
     if (!t1->obj_box || !t2->obj_box) {
         ctx->err = "tdp_within_candidates: tiles uploaded without object boxes";

@@ -152,8 +152,10 @@ def test_server_results_against_oracle(tdp, tmp_path):
     for i, (size, q, k, tau) in enumerate(QUERIES):
         oq = orc.OracleTile(_save(_query_tile(tdp, i, size), str(tmp_path / f"q{i}.dt")))
         want.append(orc.join(oq, ov, q, k, tau, precision="f64", rounds=True))
+    tiles = [_query_tile(tdp, i, size) for i, (size, _, _, _) in enumerate(QUERIES)]
     srv = tdp.Server(vessels, max_batch=64, overlap=True, precision="mixed")
-    tickets = [srv.submit(_query_tile(tdp, i, size), query=q, knn=k, within_dist=tau)
+    # submitted back to back: the queries that wait together share joins
+    tickets = [srv.submit(tiles[i], query=q, knn=k, within_dist=tau)
                for i, (size, q, k, tau) in enumerate(QUERIES)]
     for i, tk in enumerate(tickets):
         res, _ = srv.wait(tk)

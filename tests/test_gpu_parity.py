@@ -208,21 +208,6 @@ def test_knn_k_exceeds_objects(tdp, eng, small_tiles):
         _compare(r, ot, oo, omn, omx)
 
 
-def test_join_with_an_empty_tile_is_rejected(tdp, eng, small_tiles):
-    """a tile without objects on either side is refused at upload with an
-    error naming it (not joined to an empty result), in every form"""
-    pn, pv = small_tiles
-    n, v = tdp.Tile.load(pn), tdp.Tile.load(pv)
-    empty = tdp.Source.of_tile(n).decode(np.arange(0))
-    for precision in ("ref32", "fp64", "mixed"):
-        for a, b in ((empty, v), (n, empty), (empty, None)):
-            with pytest.raises(tdp.TdpError, match="null array"):
-                eng.join(a, b, query="nn", knn=3, precision=precision)
-    # the engine still joins afterwards
-    r = eng.join(n, v, query="nn", knn=3, precision="mixed")
-    assert len(r.target) > 0
-
-
 def test_within_zero_distance(tdp, eng, small_tiles):
     """tau = 0 (the reference's intersect-by-within runs): only touching objects"""
     pn, pv = small_tiles
