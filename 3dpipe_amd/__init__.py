@@ -141,6 +141,7 @@ def _load():
     tdp.tdp_last_error.restype = _c.c_char_p
     tdp.tdp_tridist_batch.argtypes = [_P, _c.c_int, _P, _P, sz, _P]
     tdp.tdp_meshdist_batch.argtypes = [_P, _c.c_int, _P, sz, _P, sz, _P]
+    tdp.tdp_meshdist_batch_noalloc.argtypes = [_P, _c.c_int, _P, sz, _P, sz, _P]
     tdp.tdp_knn_status.argtypes = [_P, sz, _P, _P, _P, _P, _c.c_int, _P]
     _libs["tdbase"], _libs["tdp"] = tdb, tdp
     return tdb, tdp
@@ -638,6 +639,16 @@ class Device:
                                   len(os_), _ptr(out)):
             raise self._err()
         return out
+
+    def meshdist_noalloc(self, d_rec: int, n_tri: int, d_offset_size: int, n_pairs: int,
+                         d_out: int, precision: str = "ref32") -> None:
+        """tdp_meshdist_batch_noalloc: the same per voxel pair values, on
+        device pointers the caller owns (records, pairs, output); the library
+        allocates no device memory."""
+        _, tdp = _load()
+        if tdp.tdp_meshdist_batch_noalloc(self._h, PREC[precision], _P(d_rec), int(n_tri),
+                                          _P(d_offset_size), int(n_pairs), _P(d_out)):
+            raise self._err()
 
     def knn_status(self, prefix, confirmed, rmin, rmax, k: int) -> np.ndarray:
         _, tdp = _load()

@@ -374,6 +374,35 @@ static int scan_exclusive(tdp_ctx *ctx, const In *in, size_t n, uint64_t *out, u
     return TDP_OK;
 }
 
+// ------------------------------------------------------------ allocation-free batch
+
+// Scratch of tdp_meshdist_batch_noalloc: the triangle bound records of up to
+// kNaMaxTri triangles (kBnd float4 each), then its counters (the mixed form's
+// statistics and the argument-check flag). One array per GPU, held from the
+// library's load to the process's end. Host code never uses the name itself
+// (on the host it is a stand-in, not the device address): tdp_open looks up
+// each device's address once, with that device current.
+constexpr size_t kNaMaxTri = (size_t)1 << 18;
+constexpr size_t kNaBndBytes = kNaMaxTri * kBnd * sizeof(float4);  // 12 MiB
+constexpr size_t kNaScratchBytes = kNaBndBytes + 256;
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mu[kNaMaxDevices];  // one call at a time per device in a process
+static void *g_na_ptr[kNaMaxDevices] = {};
+static std::once_flag g_na_once[kNaMaxDevices];
+static hipError_t g_na_err[kNaMaxDevices] = {};
+
+// the device address of `scratch` on the current device (once per device)
+static hipError_t na_scratch_setup(int device) {
+    std::call_once(g_na_once[device], [device] {
+        void *p = nullptr;
+        g_na_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (g_na_err[device] == hipSuccess) g_na_ptr[device] = p;
+    });
+    return g_na_err[device];
+}
+
 // ------------------------------------------------------------ context
 
 extern "C" int tdp_open(int device, tdp_ctx **out) {
@@ -399,6 +428,14 @@ extern "C" int tdp_open(int device, tdp_ctx **out) {
         return TDP_ERR_HIP;
     }
     ctx->cu_count = prop.multiProcessorCount;
+    if (device < kNaMaxDevices) {
+        e = na_scratch_setup(device);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            *out = ctx;
+            return TDP_ERR_HIP;
+        }
+    }
     *out = ctx;
     return TDP_OK;
 }
@@ -900,6 +937,64 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
     dfree(ctx, dO);
     dfree(ctx, dst);
     return rc;
+}
+
+extern "C" int tdp_meshdist_batch_noalloc(tdp_ctx *ctx, int precision, const float *tri,
+                                          size_t n_tri, const uint64_t *os, size_t P, void *out) {
+    if (!ctx) return TDP_ERR_ARG;
+    if ((P && (!os || !out)) || (n_tri && !tri)) {
+        ctx->err = "tdp_meshdist_batch_noalloc: null argument";
+        return TDP_ERR_ARG;
+    }
+    if (precision < 0 || precision > 2) {
+        ctx->err = "tdp_meshdist_batch_noalloc: bad precision";
+        return TDP_ERR_ARG;
+    }
+    if (n_tri > kNaMaxTri) {
+        ctx->err = "tdp_meshdist_batch_noalloc: " + std::to_string(n_tri) +
+                   " triangles, more than its scratch holds (" + std::to_string(kNaMaxTri) + ")";
+        return TDP_ERR_ARG;
+    }
+    if (P == 0) return TDP_OK;
+    if (ctx->device >= kNaMaxDevices || !g_na_ptr[ctx->device]) {
+        ctx->err = "tdp_meshdist_batch_noalloc: no scratch address for this device";
+        return TDP_ERR_STATE;
+    }
+    std::lock_guard<std::mutex> g(g_na_mu[ctx->device]);
+    TDP_CHECK(ctx, hipSetDevice(ctx->device));
+    unsigned char *base = (unsigned char *)g_na_ptr[ctx->device];
+    float4 *bnd = (float4 *)base;
+    unsigned long long *cnt = (unsigned long long *)(base + kNaBndBytes);  // [0..3] stats, [4] flag
+    hipStream_t s = ctx->stream;
+    // counters start at zero (the array keeps the last call's contents)
+    k_fill_u64<<<1, 64, 0, s>>>(cnt, 5, 0ull);
+    TDP_LAUNCHED(ctx);
+    // the pairs' ranges are checked where they live, on the device
+    k_check_pairs<<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, os, (uint64_t)n_tri, cnt + 4);
+    TDP_LAUNCHED(ctx);
+    unsigned long long bad = 0;
+    TDP_CHECK(ctx, hipMemcpyAsync(&bad, cnt + 4, sizeof bad, hipMemcpyDeviceToHost, s));
+    TDP_CHECK(ctx, hipStreamSynchronize(s));
+    if (bad) {
+        ctx->err = "tdp_meshdist_batch_noalloc: a pair lies outside the records or has more "
+                   "than 2^31 triangle pairs";
+        return TDP_ERR_ARG;
+    }
+    const float4 *rec = (const float4 *)tri;
+    if (n_tri) {
+        k_tri_bounds<<<blocks_for(n_tri, kBlock), kBlock, 0, s>>>(n_tri, rec, bnd);
+        TDP_LAUNCHED(ctx);
+    }
+    const dim3 gr = wave_grid(P);
+    if (precision == TDP_PREC_REF32)
+        k_meshdist_batch<float, MODE_REF32><<<gr, kBlock, 0, s>>>(P, os, rec, bnd, (float *)out, cnt);
+    else if (precision == TDP_PREC_FP64)
+        k_meshdist_batch<double, MODE_FP64><<<gr, kBlock, 0, s>>>(P, os, rec, bnd, (double *)out, cnt);
+    else
+        k_meshdist_batch<double, MODE_MIXED><<<gr, kBlock, 0, s>>>(P, os, rec, bnd, (double *)out, cnt);
+    TDP_LAUNCHED(ctx);
+    TDP_CHECK(ctx, hipStreamSynchronize(s));
+    return TDP_OK;
 }
 
 extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,

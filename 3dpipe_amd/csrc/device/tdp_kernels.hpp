@@ -1073,6 +1073,25 @@ __global__ void __launch_bounds__(kBlock) k_meshdist_batch(size_t P, const uint6
     }
 }
 
+// n 64-bit words set to v (a kernel rather than a memset: usable wherever a
+// caller's stream is captured)
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, uint32_t n, unsigned long long v) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// *bad = 1 when a pair {offset1, n1, offset2, n2} reaches past n_tri records
+// or has more than 2^31 triangle pairs (the kernels' 32-bit pair index)
+__global__ void __launch_bounds__(kBlock) k_check_pairs(size_t P, const uint64_t *__restrict__ os,
+                                                        uint64_t n_tri,
+                                                        unsigned long long *__restrict__ bad) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const uint64_t o1 = os[4 * p], n1 = os[4 * p + 1], o2 = os[4 * p + 2], n2 = os[4 * p + 3];
+    const bool out = o1 > n_tri || n1 > n_tri - o1 || o2 > n_tri || n2 > n_tri - o2 ||
+                     (n1 && n2 > 0x7fffffffull / n1);
+    if (out) atomicOr(bad, 1ull);
+}
+
 // kNN status for host-given lists (one thread per target)
 __global__ void __launch_bounds__(kBlock) k_knn_status(size_t N, const uint32_t *__restrict__ prefix,
                                                        const int32_t *__restrict__ confirmed,

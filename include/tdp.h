@@ -145,6 +145,22 @@ int tdp_tridist_batch(tdp_ctx *ctx, int precision, const float *S, const float *
 int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri, size_t n_tri,
                        const uint64_t *offset_size, size_t n_pairs, void *out);
 
+/* tdp_meshdist_batch on device-resident arrays, allocating no device memory.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc). tri
+ * (12-float records), offset_size (4 x u64 per pair) and out (3 values per
+ * pair, float for REF32, double otherwise) are device pointers the caller
+ * owns; the triangles' bound records and the counters live in a fixed
+ * per-GPU array of the library (12 MiB, held until the process ends), so
+ * n_tri may be at most 262144 (TDP_ERR_ARG beyond, before any work). Pairs
+ * reaching past n_tri records are refused with TDP_ERR_ARG (checked on the
+ * device). Results equal tdp_meshdist_batch's bit for bit. The call returns
+ * when out is written. Calls on one GPU share its array: within a process
+ * they are serialised by a lock; calls from threads that bypass it, or
+ * work queued on other streams that reads the array, would race. */
+int tdp_meshdist_batch_noalloc(tdp_ctx *ctx, int precision, const float *tri, size_t n_tri,
+                               const uint64_t *offset_size, size_t n_pairs, void *out);
+
 /* kNN join status (evaluate_knn_kernel, computing_gpu.cu:105-151, via
  * evaluation_kernel :6905-6950): per target t with candidates
  * [prefix[t], prefix[t+1]) and confirmed[t] already confirmed:
