@@ -111,6 +111,7 @@ struct tdp_join {
     uint64_t filter_vp = 0;
     tdp_tuning tune{};
     uint32_t *work = nullptr;  // work counter of the persistent stage
+    uint32_t *obj_u = nullptr;  // MIXED: per object pair, the last round's cap on U_d (keys)
     // graph-captured rounds: device totals, pinned read-back, the executable
     unsigned long long *words5 = nullptr;  // per-slot mask-word bound
     uint64_t *d_words = nullptr, *d_kept = nullptr;
@@ -1222,7 +1223,7 @@ extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
     tdp_ctx *ctx = j->ctx;
-    void *ptrs[] = {j->work, j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
+    void *ptrs[] = {j->work, j->obj_u, j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q,
@@ -1314,6 +1315,7 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
     if (rc == TDP_OK) rc = palloc(ctx, &j->offs, P);
     if (rc == TDP_OK) rc = palloc(ctx, &j->stats, 4);
     if (rc == TDP_OK) rc = palloc(ctx, &j->work, 4);
+    if (rc == TDP_OK && prm->precision == TDP_PREC_MIXED) rc = palloc(ctx, &j->obj_u, P);
     if (rc != TDP_OK) {
         tdp_join_destroy(j);
         return rc;
@@ -1553,6 +1555,32 @@ static void launch_decide(tdp_join *j, uint8_t rnd) {
             P, rmin, rmax, j->prm.hausdorff_level, j->state, j->round, rnd);
 }
 
+// The mark pass's caps on U (see k_mx_mark): the voxel pairs' ranges in
+// rounds that fold lo and hi into them, the object pairs' smallest U_d in the
+// last round; neither for the intersection join, whose rounds fold the
+// round's own values. Queues k_mx_obj_u when the last round's cap applies.
+struct MarkCaps {
+    const double *vp_min = nullptr, *vp_max = nullptr;
+    const uint32_t *vp_pair = nullptr, *obj_u = nullptr;
+};
+static MarkCaps mark_caps(tdp_join *j, int is_last, uint64_t n) {
+    MarkCaps m;
+    const int c = j->cur;
+    m.vp_pair = j->vpp[c];
+    if (j->prm.query == TDP_QUERY_INTERSECT) return m;
+    hipStream_t s = j->ctx->stream;
+    if (is_last) {
+        k_fill_u32<<<blocks_for(j->P, kBlock), kBlock, 0, s>>>(j->P, j->obj_u, 0xff800000u);
+        k_mx_obj_u<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->mx_plan, j->mx_U, j->mx_key,
+                                                             j->vpp[c], j->obj_u);
+        m.obj_u = j->obj_u;
+    } else if (fold_need(j, is_last) == 6u) {
+        m.vp_min = (const double *)j->vmin[c];
+        m.vp_max = (const double *)j->vmax[c];
+    }
+    return m;
+}
+
 template <typename R, int MODE>
 static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
     tdp_ctx *ctx = j->ctx;
@@ -1627,10 +1655,12 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
+        const MarkCaps cap = mark_caps(j, is_last, n);
+        TDP_LAUNCHED(ctx);
         with_need(need, [&](auto nd) {
             k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
                 n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-                j->mx_woff, j->mx_mask, j->keep);
+                j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, j->mx_mask, j->keep);
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
@@ -1795,10 +1825,12 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
             3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
     });
     launched();
+    const MarkCaps cap = mark_caps(j, is_last, n);
+    launched();
     with_need(need, [&](auto nd) {
         k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
             n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-            j->mx_woff, j->mx_mask, j->keep);
+            j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, j->mx_mask, j->keep);
     });
     launched();
     k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->mx_plan, A->tri, B->tri, j->mx_woff, j->mx_mask,

@@ -1092,6 +1092,12 @@ __global__ void __launch_bounds__(kBlock) k_check_pairs(size_t P, const uint64_t
     if (out) atomicOr(bad, 1ull);
 }
 
+// n 32-bit words set to v
+__global__ void __launch_bounds__(kBlock) k_fill_u32(size_t n, uint32_t *__restrict__ p, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // kNN status for host-given lists (one thread per target)
 __global__ void __launch_bounds__(kBlock) k_knn_status(size_t N, const uint32_t *__restrict__ prefix,
                                                        const int32_t *__restrict__ confirmed,

@@ -673,19 +673,53 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
     return gap <= T;
 }
 
+// Per object pair, the smallest U_d of its tiled voxel pairs (FP32 upper
+// bound, tightened by the probes' exact values) as an order-preserving key
+// (obj_u starts at the key of +inf). EMPTY and SMALL voxel pairs give no
+// bound here (their pairs are all evaluated in FP64 anyway).
+__global__ void __launch_bounds__(kBlock) k_mx_obj_u(
+    uint64_t n_vp, const VpPlan *__restrict__ plan, const float *__restrict__ vpU,
+    const unsigned long long *__restrict__ mkey, const uint32_t *__restrict__ vp_pair,
+    uint32_t *__restrict__ obj_u) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_vp || plan_flags(plan[w])) return;
+    const float u = fminf(vpU[3 * w], ulp_up((float)dunkey(mkey[3 * w])));
+    atomicMin(&obj_u[vp_pair[w]], fkey(u));
+}
+
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
 // the ballot of iteration k of the tiled walk; count[w] = number of bits.
 // A pair is a candidate when mark_pair keeps it; probes (already evaluated)
 // never are. SMALL voxel pairs:
 // every pair, no mask. NEED: the quantities of the round (U is -inf for the
 // others).
+//
+// Two caps tighten U where a round's fold does not need every voxel pair's
+// exact minima (both keep every pair that can change a folded value):
+//  * a round that folds lo and hi takes min(range min, min lo) and
+//    min(range max, min hi) (SpatialJoin fold at Hausdorff level 2): a pair
+//    whose lo exceeds the voxel pair's current range min cannot lower it,
+//    nor can one whose hi exceeds the range max, so U_lo and U_hi are capped
+//    by the range (vp_min / vp_max, rounded up to FP32; the minima found over
+//    the pairs that pass are then exact wherever they are below the range,
+//    and min(range, found) equals min(range, exact) in every case);
+//  * the last round folds only each object pair's minimum d over its voxel
+//    pairs ([d, d] per voxel pair, the object range their minimum), so U_d is
+//    capped by the smallest U_d of the object pair's voxel pairs (obj_u, keys
+//    of k_mx_obj_u): a pair above it cannot hold the object's minimum. A
+//    voxel pair's own minimum may then stay above its exact value, which
+//    changes neither the object range nor which voxel pairs the eviction
+//    keeps (a voxel pair whose exact minimum equals the object's has a pair
+//    at that value, below the cap, so it is found exactly).
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
     const float4 *__restrict__ bnd1, const float4 *__restrict__ rec2,
     const float4 *__restrict__ bnd2, const float *__restrict__ vpU,
     const uint32_t *__restrict__ probe, const unsigned long long *__restrict__ mkey,
-    const uint64_t *__restrict__ woff,
+    const uint64_t *__restrict__ woff, const double *__restrict__ vp_min,
+    const double *__restrict__ vp_max, const uint32_t *__restrict__ vp_pair,
+    const uint32_t *__restrict__ obj_u,
     unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
@@ -703,6 +737,11 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     }
     float U[3];
     bounds_U(vpU, mkey, w, U);
+    if (obj_u) U[0] = fminf(U[0], funkey(obj_u[vp_pair[w]]));
+    if (vp_min) {
+        U[1] = fminf(U[1], ulp_up((float)vp_min[w]));
+        U[2] = fminf(U[2], ulp_up((float)vp_max[w]));
+    }
     const float ud = U[0], ulo_h = U[1] * kTH, uhi_h = U[2] * kTH;
     const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
                    p2 = probe[3 * (size_t)w + 2];
