@@ -94,6 +94,7 @@ struct tdp_join {
     uint32_t *mx_probe = nullptr, *mx_flags = nullptr;
     unsigned long long *mx_key = nullptr;
     VpPlan *mx_plan = nullptr;  // per voxel pair: triangles and tiling (k_mx_plan)
+    unsigned long long *mx_pwb = nullptr;  // per voxel pair: its probes' mask word and bit
     uint32_t *mx_nw = nullptr;   // candidate-mask words per voxel pair
     uint64_t *mx_woff = nullptr; // their exclusive scan
     size_t cap_mx = 0;
@@ -1227,7 +1228,7 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q,
-                    j->mx_plan};
+                    j->mx_plan, j->mx_pwb};
     for (void *p : ptrs) pfree(ctx, p);
     for (int b = 0; b < 2; b++) {
         pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
@@ -1399,9 +1400,9 @@ static int ensure_mx(tdp_join *j, size_t n) {
     tdp_ctx *ctx = j->ctx;
     const size_t cap = std::max(n, j->cap_vp);
     pfree(ctx, j->mx_U); pfree(ctx, j->mx_probe); pfree(ctx, j->mx_flags); pfree(ctx, j->mx_key);
-    pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff); pfree(ctx, j->mx_plan);
+    pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff); pfree(ctx, j->mx_plan); pfree(ctx, j->mx_pwb);
     j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
-    j->mx_nw = nullptr; j->mx_woff = nullptr; j->mx_plan = nullptr;
+    j->mx_nw = nullptr; j->mx_woff = nullptr; j->mx_plan = nullptr; j->mx_pwb = nullptr;
     j->cap_mx = 0;
     int rc = palloc(ctx, &j->mx_U, 3 * cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_probe, 3 * cap);
@@ -1410,6 +1411,7 @@ static int ensure_mx(tdp_join *j, size_t n) {
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_nw, cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_woff, cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_plan, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_pwb, 3 * cap);
     if (rc == TDP_OK) j->cap_mx = cap;
     return rc;
 }
@@ -1562,6 +1564,7 @@ static void launch_decide(tdp_join *j, uint8_t rnd) {
 struct MarkCaps {
     const double *vp_min = nullptr, *vp_max = nullptr;
     const uint32_t *vp_pair = nullptr, *obj_u = nullptr;
+    int lo_free = 0;
 };
 static MarkCaps mark_caps(tdp_join *j, int is_last, uint64_t n) {
     MarkCaps m;
@@ -1634,9 +1637,16 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan,
             j->mx_flags, j->mx_nw, j->stats);
         TDP_LAUNCHED(ctx);
-        with_need(need, [&](auto nd) {
+        // After the first round a lo/hi round bounds only hi: U_lo then comes
+        // from the probes and the voxel pair's range min, which caps it in the
+        // mark pass (the minima of lo rise with the LOD, so the range min is
+        // the smaller one nearly always); the FP32 lo bound cost more than it
+        // saved.
+        const bool lo_free = need == 6u && j->rounds >= 1 && j->prm.query != TDP_QUERY_INTERSECT;
+        const uint32_t bneed = lo_free ? 4u : need;
+        with_need(bneed, [&](auto nd) {
             k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, need, j->mx_U, j->mx_probe,
+                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, bneed, j->mx_U, j->mx_probe,
                 j->mx_key);
         });
         TDP_LAUNCHED(ctx);
@@ -1651,7 +1661,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             k_mx_eval<decltype(nd)::value>
                 <<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
                     3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key,
-                    j->stats);
+                    j->stats, j->mx_pwb);
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
@@ -1660,7 +1670,8 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         with_need(need, [&](auto nd) {
             k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
                 n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-                j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, j->mx_mask, j->keep);
+                j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, lo_free ? 1 : 0,
+                j->mx_pwb, j->mx_mask, j->keep);
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
@@ -1689,7 +1700,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                     k_mx_eval<decltype(nd)::value>
                         <<<stride_grid(items, j->tune.eval_blocks), kBlock, 0, s>>>(
                             items, j->item_vp, j->item_q, 0, j->mx_plan, A->tri, B->tri,
-                            j->mx_key, j->stats);
+                            j->mx_key, j->stats, nullptr);
                 });
                 TDP_LAUNCHED(ctx);
                 if (mark(TDP_STAGE_EVAL) != TDP_OK) return TDP_ERR_HIP;
@@ -1814,15 +1825,18 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
         slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan, j->mx_flags,
         j->mx_nw, j->stats);
     launched();
-    with_need(need, [&](auto nd) {
+    const bool lo_free = need == 6u && j->rounds >= 1 && j->prm.query != TDP_QUERY_INTERSECT;
+    const uint32_t bneed = lo_free ? 4u : need;
+    with_need(bneed, [&](auto nd) {
         k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-            n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, need, j->mx_U, j->mx_probe, j->mx_key);
+            n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, bneed, j->mx_U, j->mx_probe, j->mx_key);
     });
     launched();
     if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, j->d_words);
     with_need(need, [&](auto nd) {
         k_mx_eval<decltype(nd)::value><<<stride_grid(3 * n, j->tune.eval_blocks), kBlock, 0, s>>>(
-            3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats);
+            3 * n, nullptr, j->mx_probe, 1, j->mx_plan, A->tri, B->tri, j->mx_key, j->stats,
+            j->mx_pwb);
     });
     launched();
     const MarkCaps cap = mark_caps(j, is_last, n);
@@ -1830,7 +1844,8 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     with_need(need, [&](auto nd) {
         k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
             n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-            j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, j->mx_mask, j->keep);
+            j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, lo_free ? 1 : 0,
+            j->mx_pwb, j->mx_mask, j->keep);
     });
     launched();
     k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->mx_plan, A->tri, B->tri, j->mx_woff, j->mx_mask,

@@ -521,12 +521,17 @@ __device__ __forceinline__ void seg_min_fold(uint32_t w, unsigned long long a,
 // arrive sorted by voxel pair (probe lists: w = k / 3; work lists: written
 // in voxel-pair order). NEED: the quantities the round folds (the others'
 // keys are not touched: the round does not use them).
+//
+// With vp_from_index3 (the probes: item k is probe k % 3 of voxel pair k / 3)
+// it also records where each probe lies in its voxel pair's candidate masks,
+// pwb[k] = word << 6 | bit (~0 for none), so that k_mx_mark can clear the
+// probes' bits without dividing per voxel pair.
 template <uint32_t NEED>
 __global__ void __launch_bounds__(kBlock) k_mx_eval(
     uint64_t n_items, const uint32_t *__restrict__ item_vp, const uint32_t *__restrict__ item_q,
     int vp_from_index3, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
     const float4 *__restrict__ rec2, unsigned long long *__restrict__ mkey,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, unsigned long long *__restrict__ pwb) {
     // grid-stride: the work list can exceed one launch's thread count
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const unsigned long long inf_key = dkey(INFINITY);
@@ -539,6 +544,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
         if (k < n_items) {
             const uint32_t q = item_q[k];
             w = vp_from_index3 ? (uint32_t)(k / 3) : item_vp[k];
+            unsigned long long at = ~0ull;
             if (q != kNoProbe) {
                 act = true;
                 const VpPlan &pl = plan[w];
@@ -546,12 +552,20 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
                 const uint32_t n2 = pl.n2;
                 const VpLod g{o1, o2, 0u, n2};
                 const uint32_t i = q / g.n2, j = q - i * g.n2;
+                if (vp_from_index3 && !plan_flags(pl)) {
+                    const Tiling t = plan_tiling(pl);
+                    const uint32_t row = t.swap ? j : i, col = t.swap ? i : j;
+                    const uint32_t ch = col / t.cw, gr = row / t.R;
+                    at = ((unsigned long long)(ch * t.ngr + gr) << 6) |
+                         ((row - gr * t.R) * t.cw + (col - ch * t.cw));
+                }
                 double d, lo, hi;
                 pair_values<double>(rec1, g.o1 + i, rec2, g.o2 + j, d, lo, hi);
                 a = dkey(d);
                 b = dkey(lo);
                 c = dkey(hi);
             }
+            if (vp_from_index3) pwb[k] = at;
         }
         seg_min_fold<NEED>(w, a, b, c, mkey);
         n += (uint64_t)__popcll(__ballot(act));
@@ -642,7 +656,7 @@ __device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float u
 // comparison 1 -- 45; T adds 5 in rounds that fold lo and hi (50).
 template <typename RV>
 __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const MarkCol &c, float T,
-                                          bool probe, RV &&row_v) {
+                                          RV &&row_v) {
 #pragma clang fp contract(fast)
     const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
@@ -659,7 +673,7 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
         const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
         dl = fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m));
     }
-    if (!(dl <= T) || probe) return false;
+    if (!(dl <= T)) return false;
     if (!g) return true;
     const TriV v = row_v();
     // S's vertices lie beyond L's along d = cS - cL
@@ -689,8 +703,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_obj_u(
 
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
 // the ballot of iteration k of the tiled walk; count[w] = number of bits.
-// A pair is a candidate when mark_pair keeps it; probes (already evaluated)
-// never are. SMALL voxel pairs:
+// A pair is a candidate when mark_pair keeps it; the probes (already
+// evaluated) never are: their bits are cleared from the ballot words (word
+// and bit worked out once per voxel pair, a scalar test per iteration
+// instead of three vector compares per pair). SMALL voxel pairs:
 // every pair, no mask. NEED: the quantities of the round (U is -inf for the
 // others).
 //
@@ -719,7 +735,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const uint32_t *__restrict__ probe, const unsigned long long *__restrict__ mkey,
     const uint64_t *__restrict__ woff, const double *__restrict__ vp_min,
     const double *__restrict__ vp_max, const uint32_t *__restrict__ vp_pair,
-    const uint32_t *__restrict__ obj_u,
+    const uint32_t *__restrict__ obj_u, int lo_free, const unsigned long long *__restrict__ pwb,
     unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
@@ -737,17 +753,26 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     }
     float U[3];
     bounds_U(vpU, mkey, w, U);
+    // lo without an FP32 bound this round: only the probes' exact values
+    if (lo_free && mkey[3 * w + 1] != dkey(0.0)) U[1] = ulp_up((float)dunkey(mkey[3 * w + 1]));
     if (obj_u) U[0] = fminf(U[0], funkey(obj_u[vp_pair[w]]));
     if (vp_min) {
         U[1] = fminf(U[1], ulp_up((float)vp_min[w]));
         U[2] = fminf(U[2], ulp_up((float)vp_max[w]));
     }
     const float ud = U[0], ulo_h = U[1] * kTH, uhi_h = U[2] * kTH;
-    const uint32_t p0 = probe[3 * (size_t)w], p1 = probe[3 * (size_t)w + 1],
-                   p2 = probe[3 * (size_t)w + 2];
     const Tiling t = plan_tiling(pl);
     const uint32_t n2 = g.n2, nw = pl.nw;
     unsigned long long *__restrict__ out = mask + woff[w];
+    // the probes' words and bits in the tiling (k_mx_eval)
+    uint32_t pword[3];
+    unsigned long long pbit[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const unsigned long long at = pwb[3 * (size_t)w + f];
+        pword[f] = at == ~0ull ? 0xffffffffu : (uint32_t)(at >> 6);
+        pbit[f] = at == ~0ull ? 0ull : 1ull << (at & 63u);
+    }
     // the words of the current group of 64 iterations, stored together
     __shared__ unsigned long long s_words[kWavesPerBlock][kWave];
     unsigned long long *const sw = s_words[threadIdx.x / kWave];
@@ -763,12 +788,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
             const TriS &L = col_is_a ? A : B;
             bool take = false;
             if (p.valid) {
-                const uint32_t q = p.i * n2 + p.j;
-                const bool probe = q == p0 || q == p1 || q == p2;
                 if constexpr (col_is_a)
-                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), probe, vc);
+                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), vc);
                 else
-                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), probe, va);
+                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), va);
             }
             const unsigned long long m = __ballot(take);
             c += (uint32_t)__popcll(m);
@@ -776,7 +799,21 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
             if (lane == 0) sw[k] = m;
             if (k == kWave - 1 || p.word + 1 == nw) {
                 wave_sync();
-                if ((uint32_t)lane <= k) out[(p.word & ~(uint32_t)(kWave - 1)) + lane] = sw[lane];
+                // the probes' bits among this group of words are cleared
+                // (once per 64 iterations instead of a test per iteration)
+                const uint32_t g0 = p.word & ~(uint32_t)(kWave - 1);
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    if (pword[f] - g0 <= k) {
+                        const unsigned long long x = sw[pword[f] - g0];
+                        if (x & pbit[f]) {
+                            c -= 1u;
+                            if (lane == 0) sw[pword[f] - g0] = x & ~pbit[f];
+                        }
+                        wave_sync();
+                    }
+                }
+                if ((uint32_t)lane <= k) out[g0 + lane] = sw[lane];
                 wave_sync();
             }
         });
