@@ -442,7 +442,9 @@ constexpr uint32_t kMixedMinPairs = 2 * kWave;  // below this, evaluate every pa
 //   {centre.xyz, radius}      sphere around the triangle (centre rounded to
 //                             FP32, radius measured from the rounded centre
 //                             with a relative margin);
-//   {unit normal.xyz, max |centre coord|}   normal 0 when near-degenerate;
+//   {unit normal.xyz, r + max |centre coord|}   normal 0 when near-degenerate
+//                             (the margins of the FP32 tests take r and the
+//                             centre's magnitude together, rounded upwards);
 //   {proxy hausdorff ph, hausdorff h, alo, ahi}: alo = r - ph + 8u t and
 //                             ahi = r + h + 8u t (t = r + ph + h, u = 2^-24),
 //                             the triangle's parts of the upper bounds of lo
@@ -494,7 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *_
         fy = (float)(ny / nl);
         fz = (float)(nz / nl);
     }
-    const float cm = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+    const float cm = (r + fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)))) * (1.f + 2.f * kU);
     // upper-bound constants of lo and hi, rounded upwards by 8u (r + ph + h)
     const float ph = a.w, h = b.w, tt = (r + ph) + h;
     const float alo = (r - ph) + 8.f * kU * tt, ahi = (r + h) + 8.f * kU * tt;
@@ -533,7 +535,7 @@ __device__ __forceinline__ void pair_interval(const float4 *r1, const float4 *b1
     const float dx = s2.x - s1.x, dy = s2.y - s1.y, dz = s2.z - s1.z;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
     const float rs = s1.w + s2.w;
-    const float m = 32.f * kU * (dc + rs + p1.w + p2.w) + 1e-6f;
+    const float m = 32.f * kU * (dc + p1.w + p2.w) + 1e-6f;  // p.w = r + max |centre coord|
     const float du = dc + rs + m;
     float dl = 0.f;
     // vertices relative to the first centre

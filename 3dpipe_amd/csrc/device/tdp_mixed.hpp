@@ -620,7 +620,7 @@ __device__ __forceinline__ MarkCol mark_col(const TriS &L, const TriV &V, float 
     MarkCol c;
     c.clo = fmaf(L.ph, kTH, ulo_h);
     c.chi = fmaf(L.h, -kTG, uhi_h);
-    c.mc = fmaf(32.f * kU, L.r + L.cm, 1e-6f);
+    c.mc = fmaf(32.f * kU, L.cm, 1e-6f);  // cm: r + max |centre coord|
     c.x0 = V.x0 - L.cx; c.y0 = V.y0 - L.cy; c.z0 = V.z0 - L.cz;
     c.x1 = V.x1 - L.cx; c.y1 = V.y1 - L.cy; c.z1 = V.z1 - L.cz;
     c.x2 = V.x2 - L.cx; c.y2 = V.y2 - L.cy; c.z2 = V.z2 - L.cz;
@@ -660,19 +660,19 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
 #pragma clang fp contract(fast)
     const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-    const float m = fmaf(32.f * kU, dc + (S.r + S.cm), c.mc);
+    const float m = fmaf(32.f * kU, dc + S.cm, c.mc);
     const float eL = L.nx * dx + L.ny * dy + L.nz * dz;
     const float eS = S.nx * dx + S.ny * dy + S.nz * dz;
     const float m2 = 2.f * m;
     const bool g = fabsf(eL) > S.r + m2 || fabsf(eS) > L.r + m2;
-    float dl = 0.f, inv = 0.f;
-    if (g) {  // dc > 0 here
-        inv = __builtin_amdgcn_rcpf(dc);
-        const float cl = eL * inv, cs = eS * inv;
-        const float sl = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cl, cl, 1.f + 16.f * kU), 16.f * kU));
-        const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
-        dl = fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m));
-    }
+    // the disc bound for every lane (no branch: nearly every iteration has a
+    // lane with the guard); without the guard the bound is 0. (dc = 0 gives
+    // no guard; its NaN cosines are dropped by fmaxf.)
+    const float inv = __builtin_amdgcn_rcpf(dc);
+    const float cl = eL * inv, cs = eS * inv;
+    const float sl = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cl, cl, 1.f + 16.f * kU), 16.f * kU));
+    const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
+    const float dl = g ? fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m)) : 0.f;
     if (!(dl <= T)) return false;
     if (!g) return true;
     const TriV v = row_v();
