@@ -124,6 +124,19 @@ struct Tiling {
     bool swap;  // the columns are tile-1 triangles
 };
 
+// four floats in LDS (32-bit addresses)
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) f4v lds_f4;
+
+__device__ __forceinline__ TriS load_s_lds(const lds_f4 *p) {
+    const f4v a = p[0], b = p[1], c = p[2];
+    return TriS{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+}
+__device__ __forceinline__ TriV load_v_lds(const lds_f4 *p) {
+    const f4v a = p[0], b = p[1], c = p[2];
+    return TriV{a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+}
+
 // rows of a staged block of the walk (the most rows a tile may have)
 constexpr uint32_t kStageRows = 32;
 
@@ -280,6 +293,8 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
     // read row R - 1 of a group.
     const uint32_t gpb = kStageRows / t.R, rpb = gpb * t.R;
     const uint32_t rr = min(r, t.R - 1u);
+    const lds_f4 *const s_base = (const lds_f4 *)&stage->s[0][0] + kBnd * rr;
+    const lds_f4 *const v_base = (const lds_f4 *)&stage->v[0][0] + 3 * rr;
     uint32_t word = 0;
     for (uint32_t ch = 0; ch < t.nch; ch++) {
         const uint32_t col = ch * t.cw + c;
@@ -307,11 +322,13 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
                 }
                 wave_sync();
             }
-            const uint32_t slot = k * t.R + rr;
             const uint32_t row = gr * t.R + r;
             const bool valid = cv && row < ns;
-            const TriS S = load_s_at(stage->s[slot]);
-            auto row_v = [&]() { return load_v_at(stage->v[slot]); };
+            // the lane's row slot: a 32-bit LDS address, the lane's part
+            // formed once
+            const uint32_t k0 = k * t.R;
+            const TriS S = load_s_lds(s_base + kBnd * k0);
+            auto row_v = [&]() { return load_v_lds(v_base + 3 * k0); };
             if (SWAP)
                 f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v,
                   std::integral_constant<bool, true>{});

@@ -55,30 +55,60 @@ def log(msg):
 
 
 def cpu_baseline(tiles_dir, n_targets_total, n_sub, query, knn, tau, budget_s=15.0):
-    """The oracle (the C restatement, FP64; its refinement folds in parallel
-    over targets on all host threads) on the first targets of the same
-    workload, grown until it has run for about budget_s / 3 or reaches
-    n_sub targets."""
+    """The oracle (the C restatement, FP64; candidates, filter and refinement
+    folds in parallel over targets on all host threads) on the first targets
+    of the same workload, grown until it has run for about budget_s / 3 or
+    reaches n_sub targets; the full join's time extrapolated linearly in the
+    targets."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
     t1 = orc.OracleTile(os.path.join(tiles_dir, "nuclei.dt"))
     t2 = orc.OracleTile(os.path.join(tiles_dir, "vessels.dt"))
-    n_targets_total = min(n_targets_total, n_sub)
+    n_cap = min(n_targets_total, n_sub)
     m = 20
     while True:
         t0 = time.perf_counter()
         _, _, _, _, st = orc.join(t1, t2, query, k=knn, tau=tau, precision="f64",
                                   max_objects1=m)
         dt = time.perf_counter() - t0
-        if dt >= budget_s / 3 or m >= n_targets_total:
+        if dt >= budget_s / 3 or m >= n_cap:
             break
-        m = min(n_targets_total, int(m * max(2.0, budget_s / 3 / max(dt, 1e-3))))
+        m = min(n_cap, int(m * max(2.0, budget_s / 3 / max(dt, 1e-3))))
     return {"value": st["tri_pairs"] / dt, "unit": "tri-pair dist/s", "cores": orc.threads(),
             "kind": "port",
             "sample": f"first {m} of {n_targets_total} nuclei vs all vessels, "
                       f"{'%d-NN' % knn if query == 'nn' else 'within %g' % tau}, FP64 oracle "
-                      f"(candidates and filter on one thread, refinement on {orc.threads()}), "
-                      f"{dt:.1f} s, {st['tri_pairs']} triangle pairs"}
+                      f"(candidates, filter and refinement on {orc.threads()} threads), "
+                      f"{dt:.1f} s, {st['tri_pairs']} triangle pairs",
+            "sample_targets": m, "sample_s": dt,
+            "extrapolated_full_join_s": dt * n_targets_total / m,
+            "extrapolation": "linear in the targets (the sample's nuclei lie around the first "
+                             "vessels; every target's join is independent)"}
+
+
+def list_agreement(a, b):
+    """How far result lists a agree with b (b the reference): entries in one
+    and not the other, targets whose lists differ, and the relative error of
+    a's distance ranges on the entries both hold."""
+    import numpy as np
+    ka = a.target.astype(np.int64) * (1 << 32) + a.obj2.astype(np.int64)
+    kb = b.target.astype(np.int64) * (1 << 32) + b.obj2.astype(np.int64)
+    common, ia, ib = np.intersect1d(ka, kb, assume_unique=True, return_indices=True)
+    only_a = np.setdiff1d(ka, kb, assume_unique=True)
+    only_b = np.setdiff1d(kb, ka, assume_unique=True)
+    bad_t = np.union1d(only_a >> 32, only_b >> 32)
+    out = {"entries": int(len(ka)), "reference_entries": int(len(kb)),
+           "missing": int(len(only_b)), "extra": int(len(only_a)),
+           "targets_differing": int(len(bad_t)),
+           "identical": bool(len(only_a) == 0 and len(only_b) == 0 and np.array_equal(a.target, b.target)
+                             and np.array_equal(a.obj2, b.obj2))}
+    for name, x, y in (("rmin", a.rmin[ia], b.rmin[ib]), ("rmax", a.rmax[ia], b.rmax[ib])):
+        fin = np.isfinite(y) & (y > 0)
+        rel = np.abs(x[fin] - y[fin]) / y[fin]
+        out[f"{name}_rel_err_max"] = float(rel.max()) if rel.size else 0.0
+        out[f"{name}_rel_err_median"] = float(np.median(rel)) if rel.size else 0.0
+        out[f"{name}_bit_identical"] = bool(np.array_equal(x.view(np.uint64), y.view(np.uint64)))
+    return out
 
 
 def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn):
@@ -225,6 +255,11 @@ def main():
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time one plain FP64 join and check identical lists")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--compare-ref32", type=int, default=1,
+                    help="also run the REF32 form and state its list agreement and distance "
+                         "error against FP64")
+    ap.add_argument("--cold-join", type=int, default=1,
+                    help="also time one join on a fresh engine (tile upload included)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--serve-queries", type=int, default=24,
                     help="serving run (resident vessels, Poisson arrivals) after the join bench; 0: off")
@@ -396,8 +431,33 @@ def main():
         same = (np.array_equal(rp.target, r.target) and np.array_equal(rp.obj2, r.obj2)
                 and np.array_equal(rp.rmin, r.rmin) and np.array_equal(rp.rmax, r.rmax))
         out["plain_fp64"] = {"ms": fp64_ms, "speedup": fp64_ms / ms, "identical_results": bool(same),
-                             "refine_kernel_ms": rp.timing_ms["refine_kernels"]}
+                             "refine_kernel_ms": rp.timing_ms["refine_kernels"],
+                             "agreement": list_agreement(r, rp)}
         log(f"plain FP64: {fp64_ms:.1f} ms, identical={same}")
+        if args.compare_ref32:
+            # the reference's own arithmetic (FP32 on every pair) against FP64
+            t0 = time.perf_counter()
+            r32 = run("ref32")
+            ref32_ms = 1e3 * (time.perf_counter() - t0)
+            out["ref32_vs_fp64"] = dict(list_agreement(r32, rp), ms=ref32_ms)
+            log(f"REF32: {ref32_ms:.1f} ms, {out['ref32_vs_fp64']}")
+            del r32
+        del rp
+    if args.cold_join:
+        # a join on a fresh engine: the tiles' upload (and bound records) included
+        e2 = tdp.Engine(0)
+        e2.set_tuning(**eng.tuning())
+        t0 = time.perf_counter()
+        rc = e2.join(nuc, ves, query=args.query, knn=args.knn, within_dist=args.tau,
+                     precision=args.precision)
+        cold_ms = 1e3 * (time.perf_counter() - t0)
+        e2.close()
+        out["cold_join"] = {"ms": cold_ms, "upload_ms": rc.timing_ms["upload"],
+                            "tile_bytes_h2d": int(48 * (nuc.n_tri + ves.n_tri)),
+                            "h2d_gbps": 48 * (nuc.n_tri + ves.n_tri) / (rc.timing_ms["upload"] * 1e6)
+                            if rc.timing_ms["upload"] else None}
+        log(f"cold join: {cold_ms:.1f} ms (upload {rc.timing_ms['upload']:.1f} ms)")
+        del rc
     if args.cpu_baseline:
         with tempfile.TemporaryDirectory() as d:
             # the first nuclei of the same tile and all vessels, written for
