@@ -41,9 +41,11 @@ PEAK_FP64_TFLOPS = 78.6
 # and WRITE_SIZE runs of this script, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the
 # factors at which the counters read a plain 1 GiB device copy,
 # --calibrate-copy 1024, as 1 GiB): k_mx_mark (mean over the 5 rounds), and
-# one refinement round (all its kernels).
-MARK_TRAFFIC_BYTES = 12.03e9
-ROUND_TRAFFIC_BYTES = 25.79e9
+# one refinement round (all its kernels). Measured in the run summarised in
+# TRAFFIC_SOURCE; reported only for that workload and tuning.
+MARK_TRAFFIC_BYTES = 12.57e9
+ROUND_TRAFFIC_BYTES = 24.86e9
+TRAFFIC_SOURCE = "profiles/knn10_200k_nuclei_1000_vessels_mixed_traffic_and_valu.md"
 DEFAULT_TUNING = {"small_pairs": 8, "max_rows": 64, "drain": 0, "drain_blocks": 0,
                   "eval_blocks": 4096, "graph": 0}
 REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",
@@ -389,6 +391,7 @@ def main():
         roofline["frac"] = achieved / roofline["peak"]
     if dom == "mark" and default_workload:
         roofline["traffic"] = MARK_TRAFFIC_BYTES
+        roofline["traffic_source"] = TRAFFIC_SOURCE
     # The reference's distance kernel computes TriDist on every triangle pair
     # (F_TRI each); the rate at which this pipeline gets through that work is
     # an effective figure (most pairs are ruled out without TriDist), not a
