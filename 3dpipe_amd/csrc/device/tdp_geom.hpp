@@ -4,6 +4,10 @@
 // float instantiation reproduces the reference's FP32 arithmetic bit for bit
 // (same operation order as src/geometry/computing_cpu.cpp:60-439 and
 // src/include/geometry.h:24-98; IEEE division and square root).
+// The TriDist algorithm (SegPoints, the edge-edge slab tests, the vertex-face
+// tests) is from PQP, Copyright 1999 The University of North Carolina at
+// Chapel Hill (E. Larsen et al.), as the reference's computing_cpu.cpp:1-37
+// states; the steps are restated here, the code is this project's.
 #pragma once
 
 #include <hip/hip_runtime.h>
