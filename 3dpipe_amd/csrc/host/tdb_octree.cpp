@@ -9,6 +9,11 @@
 
 namespace tdb {
 
+// Deepest level a node splits at: more halvings than a float box can take
+// (its exponent range and mantissa), so the cap changes no tree; it bounds the
+// recursion where more than leaf_size objects share one point.
+constexpr int kMaxLevel = 320;
+
 struct Octree::Node {
     float box[6];
     uint32_t size = 0;
@@ -66,35 +71,31 @@ void Octree::add(Node *n, int id) {
     }
     if (n->leaf) {
         n->objs.push_back(id);
-        if ((long)n->size > n->leaf_size && n->can_split) {
-            float mid[3];
+        if ((long)n->size > n->leaf_size && n->can_split && n->level < kMaxLevel) {
+            float mid[3], cb[8][6];
             for (int i = 0; i < 3; i++) mid[i] = (n->box[i] + n->box[3 + i]) / 2;
-            for (int c = 0; c < 8; c++) {
-                float b[6];
+            for (int c = 0; c < 8; c++)
                 for (int i = 0; i < 3; i++) {
                     const bool upper = (c >> i) & 1;
-                    b[i] = upper ? mid[i] : n->box[i];
-                    b[3 + i] = upper ? n->box[3 + i] : mid[i];
+                    cb[c][i] = upper ? mid[i] : n->box[i];
+                    cb[c][3 + i] = upper ? n->box[3 + i] : mid[i];
                 }
-                n->child[c] = new Node(b, n->level + 1, n->leaf_size);
-            }
-            int placed = 0;
+            // a split that duplicates most objects is undone until the node
+            // has grown by half again; the count of placements does not
+            // depend on the children's own splits, so it is taken first and
+            // an undone split builds no children (the reference builds them
+            // and deletes them, recursively: exponential work where many
+            // objects share a region)
+            uint32_t placed = 0;
             for (int o : n->objs)
-                for (int c = 0; c < 8; c++)
-                    if (overlaps(n->child[c]->box, boxes_ + 6 * o)) {
-                        add(n->child[c], o);
-                        placed++;
-                    }
-            // splitting that duplicates most objects is undone until the
-            // node has grown by half again
-            if ((uint32_t)placed >= 2 * (n->size - 1)) {
-                n->leaf = true;
+                for (int c = 0; c < 8; c++) placed += overlaps(cb[c], boxes_ + 6 * o) ? 1u : 0u;
+            if (placed >= 2 * (n->size - 1)) {
                 n->can_split = false;
-                for (Node *&c : n->child) {
-                    delete c;
-                    c = nullptr;
-                }
             } else {
+                for (int c = 0; c < 8; c++) n->child[c] = new Node(cb[c], n->level + 1, n->leaf_size);
+                for (int o : n->objs)
+                    for (int c = 0; c < 8; c++)
+                        if (overlaps(n->child[c]->box, boxes_ + 6 * o)) add(n->child[c], o);
                 n->objs.clear();
                 n->leaf = false;
             }

@@ -96,12 +96,15 @@ bool read_raw(const std::string &path, Tile &t, std::string &err) {
                 if (cnt > UINT32_MAX) { err = "voxel triangle count beyond 32 bits"; return false; }
                 t.vox_tri_off[5 * (v0 + v) + s] = t.n_tri();
                 t.vox_tri_cnt[5 * (v0 + v) + s] = (uint32_t)cnt;
-                const float *tp = (const float *)(data + off);
-                const float *hp = tp + 9 * cnt;
+                // the block sits at any byte offset of the file: copied, not
+                // read through float pointers
+                const char *tp = data + off, *hp = tp + 36 * cnt;
                 for (uint64_t i = 0; i < cnt; i++) {
-                    float r[12] = {tp[9 * i], tp[9 * i + 1], tp[9 * i + 2], hp[2 * i],
-                                   tp[9 * i + 3], tp[9 * i + 4], tp[9 * i + 5], hp[2 * i + 1],
-                                   tp[9 * i + 6], tp[9 * i + 7], tp[9 * i + 8], 0.f};
+                    float tri[9], hd2[2];
+                    std::memcpy(tri, tp + 36 * i, 36);
+                    std::memcpy(hd2, hp + 8 * i, 8);
+                    const float r[12] = {tri[0], tri[1], tri[2], hd2[0], tri[3], tri[4],
+                                         tri[5], hd2[1], tri[6], tri[7], tri[8], 0.f};
                     t.tri.insert(t.tri.end(), r, r + 12);
                 }
             }

@@ -200,37 +200,40 @@ static void oct_add(octnode *n, const orc_tile *t, int id) {
     }
     if (n->is_leaf) {
         oct_push(n, id);
-        if ((long)n->size > n->tile_size && n->can_split) {
-            float mid[3];
+        /* The undo decision (a split placing objects 2 (size - 1) times or
+           more) depends only on the child boxes, so it is counted before any
+           child is built; the reference builds the children recursively and
+           then deletes them, which does not end (or takes exponential time)
+           where more objects than a leaf holds share a region. The same tree
+           results. The depth cap (more halvings than a float box can take)
+           changes no tree. */
+        if ((long)n->size > n->tile_size && n->can_split && n->level < 320) {
+            float mid[3], clo[8][3], chi[8][3];
             for (int i = 0; i < 3; i++) mid[i] = (n->low[i] + n->high[i]) / 2;
-            for (int c = 0; c < 8; c++) {
-                float lo[3], hi[3];
+            for (int c = 0; c < 8; c++)
                 for (int i = 0; i < 3; i++) {
                     int upper = (c >> i) & 1;
-                    lo[i] = upper ? mid[i] : n->low[i];
-                    hi[i] = upper ? n->high[i] : mid[i];
+                    clo[c][i] = upper ? mid[i] : n->low[i];
+                    chi[c][i] = upper ? n->high[i] : mid[i];
                 }
-                n->child[c] = oct_new(lo, hi, n->level + 1, n->tile_size);
-            }
-            int total = 0;
+            uint32_t total = 0;
             for (size_t k = 0; k < n->nobj; k++) {
                 const orc_object *v = &t->obj[n->objs[k]];
-                for (int c = 0; c < 8; c++) {
-                    if (box_intersect(n->child[c]->low, n->child[c]->high, v->box_low,
-                                      v->box_high)) {
-                        oct_add(n->child[c], t, n->objs[k]);
-                        total += 1;
-                    }
-                }
+                for (int c = 0; c < 8; c++)
+                    total += box_intersect(clo[c], chi[c], v->box_low, v->box_high) ? 1u : 0u;
             }
-            if ((uint32_t)total >= 2 * (n->size - 1)) {
-                n->is_leaf = 1;
+            if (total >= 2 * (n->size - 1)) {
                 n->can_split = 0;
-                for (int c = 0; c < 8; c++) {
-                    oct_free(n->child[c]);
-                    n->child[c] = NULL;
-                }
             } else {
+                for (int c = 0; c < 8; c++)
+                    n->child[c] = oct_new(clo[c], chi[c], n->level + 1, n->tile_size);
+                for (size_t k = 0; k < n->nobj; k++) {
+                    const orc_object *v = &t->obj[n->objs[k]];
+                    for (int c = 0; c < 8; c++)
+                        if (box_intersect(n->child[c]->low, n->child[c]->high, v->box_low,
+                                          v->box_high))
+                            oct_add(n->child[c], t, n->objs[k]);
+                }
                 n->nobj = 0;
                 n->is_leaf = 0;
             }

@@ -488,6 +488,26 @@ def test_upload_rejects_voxels_over_65535_triangles(tdp):
     lib.tdp_close(ctx)
 
 
+def test_host_code_under_address_and_undefined_sanitizers(tmp_path):
+    """the host engine's CPU code (generator, RAW reader and writer, object
+    source, octree) built with -fsanitize=address,undefined from its sources
+    and run over intact, truncated and corrupted tiles and a degenerate box
+    set (tests/host_sanitize_driver.cpp); any memory error or undefined
+    behaviour aborts it"""
+    src = os.path.join(ROOT, "3dpipe_amd", "csrc", "host")
+    exe = str(tmp_path / "drv")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-fopenmp",
+                    "-o", exe, os.path.join(ROOT, "tests", "host_sanitize_driver.cpp"),
+                    os.path.join(src, "tdb_tile.cpp"), os.path.join(src, "tdb_octree.cpp")],
+                   check=True, timeout=600)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="print_stacktrace=1", OMP_NUM_THREADS="4")
+    p = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-4000:]
+    assert "0 failures" in p.stdout
+
+
 def test_generator_shapes(tdp):
     n, v = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=20, cube=2000.0)
     assert (n.n_obj, v.n_obj) == (200, 10)
