@@ -134,6 +134,7 @@ def _load():
     tdb.tdb_stream_defaults.argtypes = [_P]
     tdb.tdb_stream_join.argtypes = [_P, _P, _P, _c.POINTER(_JoinOpts), _P, _c.POINTER(_P)]
     tdb.tdb_result_stream.argtypes = [_P, _P]
+    tdb.tdb_result_stream_n.argtypes = [_P, _P, _c.c_int]
     # libtdp (kernel-level parity ops)
     tdp.tdp_open.argtypes = [_c.c_int, _c.POINTER(_P)]
     tdp.tdp_close.argtypes = [_P]
@@ -284,12 +285,13 @@ class Source:
 class _StreamOpts(_c.Structure):
     _fields_ = [("hbm_budget", _c.c_uint64), ("tile_share", _c.c_double),
                 ("max_chunk_targets", _c.c_uint64), ("decode_threads", _c.c_int),
-                ("overlap", _c.c_int)]
+                ("overlap", _c.c_int), ("lazy", _c.c_int), ("workers", _c.c_int)]
 
 
-# streaming figures of a result (tdb_result_stream)
+# streaming figures of a result (tdb_result_stream_n)
 STREAM = ["chunks", "h2d_bytes", "decode_ms", "copy_ms", "join_ms", "wall_ms", "overlap",
-          "peak_device_bytes", "index_ms", "candidates_ms", "objects_decoded", "splits"]
+          "peak_device_bytes", "index_ms", "candidates_ms", "objects_decoded", "splits",
+          "record_bytes_all", "record_bytes_shipped", "meta_bytes", "workers"]
 
 # ------------------------------------------------------------------ join
 
@@ -341,7 +343,7 @@ def _read_result(r) -> "JoinResult":
         sn = np.zeros(len(STAGES), np.uint64)
         tdb.tdb_result_stages(r, _ptr(sm), _ptr(sn), len(STAGES))
         sf = np.zeros(len(STREAM), np.float64)
-        tdb.tdb_result_stream(r, _ptr(sf))
+        tdb.tdb_result_stream_n(r, _ptr(sf), len(STREAM))
         nt = int(tdb.tdb_result_targets(r))
     finally:
         tdb.tdb_result_free(r)
@@ -480,10 +482,11 @@ class Engine:
                     max_objects1: Optional[int] = None, precision: str = "mixed",
                     threads: int = 0, hbm_budget: int = 0, tile_share: float = 0.5,
                     max_chunk_targets: int = 0, decode_threads: int = 0,
-                    overlap: bool = True) -> JoinResult:
+                    overlap: bool = True, lazy: bool = True, workers: int = 0) -> JoinResult:
         """Chunked join within a device budget (tdb_stream_join); the result
         equals join() on the same objects, with the streaming figures in
-        .stream."""
+        .stream. lazy: records shipped per LOD round for the voxels it reads;
+        workers: chunk workers side by side (0 = 3; 1 without overlap)."""
         tdb, _ = _load()
         o, keep = _opts(query, knn, within_dist, lods, hausdorff_level, max_objects1, precision,
                         threads)
@@ -492,6 +495,7 @@ class Engine:
         so.hbm_budget, so.tile_share = int(hbm_budget), float(tile_share)
         so.max_chunk_targets, so.decode_threads = int(max_chunk_targets), int(decode_threads)
         so.overlap = 1 if overlap else 0
+        so.lazy, so.workers = 1 if lazy else 0, int(workers)
         r = _P()
         rc = tdb.tdb_stream_join(self._h, src1._h, src2._h if src2 is not None else None,
                                  _c.byref(o), _c.byref(so), _c.byref(r))

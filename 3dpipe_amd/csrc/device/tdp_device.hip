@@ -68,6 +68,19 @@ struct tdp_tile {
     float *obj_box = nullptr, *obj_tr = nullptr;  // for candidate queries (optional)
     size_t bytes = 0;
     size_t cap_obj = 0, cap_vox = 0, cap_tri = 0;  // reserved tiles: capacity
+    // filled without records (tdp_tile_fill_meta_async): have[v] bit s = the
+    // records of voxel v at LOD slot s are held; the rest is the scratch of
+    // one round's list (tdp_join_lazy_needs / tdp_join_lazy_put)
+    bool lazy = false;
+    uint32_t *have = nullptr, *lz_flag = nullptr, *lz_list = nullptr, *lz_cnt = nullptr;
+    uint64_t *lz_pos = nullptr, *lz_off = nullptr;
+    size_t lz_cap = 0;
+    // listed by tdp_join_lazy_needs and not yet put: the tile is unusable
+    // until they are (or until it is filled again)
+    size_t lz_pending = 0;
+    uint64_t lz_records = 0;  // their records
+    int lz_slot = -1;
+    uint64_t put_records = 0;  // records put since the last fill
 };
 
 struct tdp_join {
@@ -127,6 +140,11 @@ struct tdp_join {
     int n_sev = 0;
     double stage_ms[TDP_STAGE_COUNT] = {};
     uint64_t stage_n[TDP_STAGE_COUNT] = {};
+    // tiles filled without records: per side, the LOD whose records were
+    // put (refine checks it), and the pinned copy of the last list
+    int lz_ready[2] = {-1, -1};
+    uint32_t *lz_host[2] = {nullptr, nullptr};
+    size_t lz_host_cap[2] = {0, 0};
 };
 
 // ------------------------------------------------------------ helpers
@@ -149,6 +167,23 @@ struct tdp_join {
         }                                                                            \
     } while (0)
 
+// Synchronous copies and fills on the context's own stream (never the legacy
+// stream: another thread's context may be capturing a graph meanwhile, and
+// legacy-stream work would have to wait for a capturing stream).
+static hipError_t copy_sync(tdp_ctx *ctx, void *dst, const void *src, size_t bytes,
+                            hipMemcpyKind kind) {
+    if (!bytes) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e;
+}
+static hipError_t set_sync(tdp_ctx *ctx, void *dst, int value, size_t bytes) {
+    if (!bytes) return hipSuccess;
+    hipError_t e = hipMemsetAsync(dst, value, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e;
+}
+
 static inline unsigned blocks_for(size_t n, size_t per_block) {
     return (unsigned)((n + per_block - 1) / per_block);
 }
@@ -170,6 +205,11 @@ static inline unsigned stride_grid(size_t n, int cap = 4096) {
 
 static void pool_release_all(tdp_ctx *ctx);
 
+// Held while a round is captured into a graph and around every device or
+// pinned allocation and free: an allocation call may synchronise the
+// device, which another thread's capture does not allow.
+static std::recursive_mutex g_capture_mu;
+
 // Every device allocation of the library: the context's cap on the bytes it
 // holds, its high-water mark, and one release of the block cache before an
 // allocation is refused.
@@ -183,7 +223,10 @@ static int dev_alloc(tdp_ctx *ctx, void **p, size_t bytes) {
         }
         hipError_t e = hipSuccess;
         if (!over) {
-            e = hipMalloc(p, bytes);
+            {
+                std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
+                e = hipMalloc(p, bytes);
+            }
             if (e == hipSuccess) {
                 std::lock_guard<std::mutex> g(ctx->mem_mu);
                 ctx->dsize[*p] = bytes;
@@ -210,6 +253,7 @@ static int dev_alloc(tdp_ctx *ctx, void **p, size_t bytes) {
 
 static void dev_free(tdp_ctx *ctx, void *p) {
     if (!p) return;
+    std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
     (void)hipFree(p);
     std::lock_guard<std::mutex> g(ctx->mem_mu);
     auto it = ctx->dsize.find(p);
@@ -421,7 +465,9 @@ extern "C" int tdp_open(int device, tdp_ctx **out) {
         return TDP_ERR_HIP;
     }
     e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
+    // non-blocking: no implicit order with the legacy stream, which the
+    // library never uses
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
@@ -461,6 +507,7 @@ extern "C" int tdp_set_tuning(tdp_ctx *ctx, const tdp_tuning *t) {
 
 extern "C" int tdp_close(tdp_ctx *ctx) {
     if (!ctx) return TDP_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
     for (auto &e : ctx->marks)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -567,7 +614,7 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
         if (rc != TDP_OK) return;
         t->bytes += count * sizeof(T);
         if (count) {
-            hipError_t e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+            hipError_t e = copy_sync(ctx, *dst, src, count * sizeof(T), hipMemcpyHostToDevice);
             if (e != hipSuccess) {
                 ctx->err = std::string("tile upload: ") + hipGetErrorString(e);
                 rc = TDP_ERR_HIP;
@@ -592,7 +639,7 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
     }
     if (!tri_parts) {
         if (rc == TDP_OK && d->n_tri) {
-            hipError_t e = hipMemcpy(t->tri, d->tri, 48 * d->n_tri, hipMemcpyHostToDevice);
+            hipError_t e = copy_sync(ctx, t->tri, d->tri, 48 * d->n_tri, hipMemcpyHostToDevice);
             if (e != hipSuccess) {
                 ctx->err = std::string("tile upload: ") + hipGetErrorString(e);
                 rc = TDP_ERR_HIP;
@@ -602,7 +649,7 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
         size_t at = 0;
         for (size_t k = 0; rc == TDP_OK && k < n_parts; k++) {
             if (part_n_tri[k]) {
-                hipError_t e = hipMemcpy(t->tri + 3 * at, tri_parts[k], 48 * part_n_tri[k],
+                hipError_t e = copy_sync(ctx, t->tri + 3 * at, tri_parts[k], 48 * part_n_tri[k],
                                          hipMemcpyHostToDevice);
                 if (e != hipSuccess) {
                     ctx->err = std::string("tile upload: ") + hipGetErrorString(e);
@@ -679,7 +726,10 @@ extern "C" int tdp_host_alloc(tdp_ctx *ctx, size_t bytes, void **out) {
         }
     }
     void *p = nullptr;
-    TDP_CHECK(ctx, hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    {
+        std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
+        TDP_CHECK(ctx, hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    }
     std::lock_guard<std::mutex> lk(ctx->mem_mu);
     ctx->host_size[p] = bytes;
     *out = p;
@@ -703,6 +753,7 @@ extern "C" int tdp_host_free(tdp_ctx *ctx, void *p) {
         }
         ctx->host_size.erase(it);
     }
+    std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
     TDP_CHECK(ctx, hipHostFree(p));
     return TDP_OK;
 }
@@ -806,7 +857,55 @@ extern "C" int tdp_tile_fill_async(tdp_tile *t, const tdp_tile_desc *d) {
     t->n_obj = d->n_obj;
     t->n_vox = d->n_vox;
     t->n_tri = d->n_tri;
+    t->lazy = false;
+    t->lz_pending = 0;
     if (d->tri) return tdp_tile_fill_tri_async(t, 0, d->n_tri, d->tri);
+    return TDP_OK;
+}
+
+extern "C" int tdp_tile_fill_meta_async(tdp_tile *t, const tdp_tile_desc *d) {
+    if (!t || !d) return TDP_ERR_ARG;
+    tdp_ctx *ctx = t->ctx;
+    if (!t->cap_obj) {
+        ctx->err = "tdp_tile_fill_meta_async: the tile was not made by tdp_tile_reserve";
+        return TDP_ERR_STATE;
+    }
+    if (t->lz_cap < t->cap_vox) {
+        for (void **p : {(void **)&t->have, (void **)&t->lz_flag, (void **)&t->lz_list,
+                         (void **)&t->lz_cnt, (void **)&t->lz_pos, (void **)&t->lz_off}) {
+            dfree(ctx, *p);
+            *p = nullptr;
+        }
+        t->lz_cap = 0;
+        const size_t n = t->cap_vox;
+        int rc = dalloc(ctx, &t->have, n);
+        if (rc == TDP_OK) rc = dalloc(ctx, &t->lz_flag, n);
+        if (rc == TDP_OK) rc = dalloc(ctx, &t->lz_list, n);
+        if (rc == TDP_OK) rc = dalloc(ctx, &t->lz_cnt, n);
+        if (rc == TDP_OK) rc = dalloc(ctx, &t->lz_pos, n);
+        if (rc == TDP_OK) rc = dalloc(ctx, &t->lz_off, n);
+        if (rc != TDP_OK) return rc;
+        t->lz_cap = n;
+        t->bytes += n * 32;
+    }
+    tdp_tile_desc m = *d;
+    m.tri = nullptr;
+    const int rc = tdp_tile_fill_async(t, &m);
+    if (rc != TDP_OK) return rc;
+    if (t->n_vox) {
+        k_fill_u32<<<blocks_for(t->n_vox, kBlock), kBlock, 0, ctx->stream>>>(t->n_vox, t->have, 0u);
+        TDP_LAUNCHED(ctx);
+    }
+    t->lazy = true;
+    t->lz_pending = 0;
+    t->lz_slot = -1;
+    t->put_records = 0;
+    return TDP_OK;
+}
+
+extern "C" int tdp_tile_put_records(const tdp_tile *t, uint64_t *records) {
+    if (!t || !records) return TDP_ERR_ARG;
+    *records = t->lazy ? t->put_records : t->n_tri;
     return TDP_OK;
 }
 
@@ -831,7 +930,9 @@ extern "C" int tdp_tile_free(tdp_tile *t) {
     tdp_ctx *ctx = t->ctx;
     for (void *p : {(void *)t->obj_vox_off, (void *)t->obj_hd, (void *)t->vox_box,
                     (void *)t->vox_obj, (void *)t->vox_tri_off, (void *)t->vox_tri_cnt,
-                    (void *)t->tri, (void *)t->bnd, (void *)t->obj_box, (void *)t->obj_tr})
+                    (void *)t->tri, (void *)t->bnd, (void *)t->obj_box, (void *)t->obj_tr,
+                    (void *)t->have, (void *)t->lz_flag, (void *)t->lz_list, (void *)t->lz_cnt,
+                    (void *)t->lz_pos, (void *)t->lz_off})
         dfree(ctx, p);
     delete t;
     return TDP_OK;
@@ -856,8 +957,8 @@ extern "C" int tdp_tridist_batch(tdp_ctx *ctx, int precision, const float *S, co
     if (rc == TDP_OK) rc = dalloc(ctx, &dT, 9 * n);
     if (rc == TDP_OK) rc = dalloc(ctx, (char **)&dO, n * osz);
     if (rc == TDP_OK) {
-        hipError_t e = hipMemcpy(dS, S, 9 * n * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dT, T, 9 * n * 4, hipMemcpyHostToDevice);
+        hipError_t e = copy_sync(ctx, dS, S, 9 * n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_sync(ctx, dT, T, 9 * n * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess) {
             if (precision == TDP_PREC_REF32)
                 k_tridist_batch<float><<<blocks_for(n, kBlock), kBlock, 0, ctx->stream>>>(
@@ -868,7 +969,7 @@ extern "C" int tdp_tridist_batch(tdp_ctx *ctx, int precision, const float *S, co
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipMemcpy(out, dO, n * osz, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = copy_sync(ctx, out, dO, n * osz, hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             ctx->err = std::string("tdp_tridist_batch: ") + hipGetErrorString(e);
             rc = TDP_ERR_HIP;
@@ -909,13 +1010,13 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
     if (rc == TDP_OK) rc = dalloc(ctx, (char **)&dO, 3 * P * osz);
     if (rc == TDP_OK) rc = dalloc(ctx, &dst, 4);
     if (rc == TDP_OK) {
-        hipError_t e = hipMemcpy(dR, tri, 48 * n_tri, hipMemcpyHostToDevice);
+        hipError_t e = copy_sync(ctx, dR, tri, 48 * n_tri, hipMemcpyHostToDevice);
         if (e == hipSuccess && n_tri) {
             k_tri_bounds<<<blocks_for(n_tri, kBlock), kBlock, 0, ctx->stream>>>(n_tri, dR, dB);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipMemcpy(dOS, os, 32 * P, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(dst, 0, 32);
+        if (e == hipSuccess) e = copy_sync(ctx, dOS, os, 32 * P, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = set_sync(ctx, dst, 0, 32);
         if (e == hipSuccess) {
             const dim3 g = wave_grid(P);
             if (precision == TDP_PREC_REF32)
@@ -927,7 +1028,7 @@ extern "C" int tdp_meshdist_batch(tdp_ctx *ctx, int precision, const float *tri,
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipMemcpy(out, dO, 3 * P * osz, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = copy_sync(ctx, out, dO, 3 * P * osz, hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             ctx->err = std::string("tdp_meshdist_batch: ") + hipGetErrorString(e);
             rc = TDP_ERR_HIP;
@@ -1015,16 +1116,16 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
     if (rc == TDP_OK) rc = dalloc(ctx, &dmn, M);
     if (rc == TDP_OK) rc = dalloc(ctx, &dmx, M);
     if (rc == TDP_OK) {
-        hipError_t e = hipMemcpy(dp, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dc, confirmed, 4 * N, hipMemcpyHostToDevice);
-        if (e == hipSuccess && M) e = hipMemcpy(dmn, rmin, 8 * M, hipMemcpyHostToDevice);
-        if (e == hipSuccess && M) e = hipMemcpy(dmx, rmax, 8 * M, hipMemcpyHostToDevice);
+        hipError_t e = copy_sync(ctx, dp, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_sync(ctx, dc, confirmed, 4 * N, hipMemcpyHostToDevice);
+        if (e == hipSuccess && M) e = copy_sync(ctx, dmn, rmin, 8 * M, hipMemcpyHostToDevice);
+        if (e == hipSuccess && M) e = copy_sync(ctx, dmx, rmax, 8 * M, hipMemcpyHostToDevice);
         if (e == hipSuccess) {
             k_knn_status<<<blocks_for(N, kBlock), kBlock, 0, ctx->stream>>>(N, dp, dc, dmn, dmx, k, ds);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess && M) e = hipMemcpy(status, ds, 4 * M, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && M) e = copy_sync(ctx, status, ds, 4 * M, hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             ctx->err = std::string("tdp_knn_status: ") + hipGetErrorString(e);
             rc = TDP_ERR_HIP;
@@ -1240,8 +1341,13 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     pfree(ctx, j->words5);
     pfree(ctx, j->d_words);
     pfree(ctx, j->d_kept);
-    if (j->h_kept) (void)hipHostFree(j->h_kept);
+    if (j->h_kept) {
+        std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
+        (void)hipHostFree(j->h_kept);
+    }
     if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
+    for (int k = 0; k < 2; k++)
+        if (j->lz_host[k]) (void)tdp_host_free(ctx, j->lz_host[k]);
     delete j;
     return TDP_OK;
 }
@@ -1321,8 +1427,8 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
         tdp_join_destroy(j);
         return rc;
     }
-    hipError_t e = hipMemcpy(j->prefix, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
-    if (e == hipSuccess && P) e = hipMemcpy(j->pair_obj2, cand, 4 * P, hipMemcpyHostToDevice);
+    hipError_t e = copy_sync(ctx, j->prefix, prefix, 4 * (N + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && P) e = copy_sync(ctx, j->pair_obj2, cand, 4 * P, hipMemcpyHostToDevice);
     if (e == hipSuccess && N) {
         k_pair_targets<<<blocks_for(N, kBlock), kBlock, 0, ctx->stream>>>(N, j->prefix,
                                                                         j->pair_target);
@@ -1335,13 +1441,13 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
             kv[t] = k_target ? k_target[t] : prm->k;
             tv[t] = tau_target ? tau_target[t] : prm->within_dist;
         }
-        e = hipMemcpy(j->k_t, kv.data(), 4 * N, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(j->tau_t, tv.data(), 4 * N, hipMemcpyHostToDevice);
+        e = copy_sync(ctx, j->k_t, kv.data(), 4 * N, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_sync(ctx, j->tau_t, tv.data(), 4 * N, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMemset(j->stats, 0, 32);
-    if (e == hipSuccess) e = hipMemset(j->confirmed, 0, 4 * (N ? N : 1));
-    if (e == hipSuccess) e = hipMemset(j->round, 0xff, P ? P : 1);
-    if (e == hipSuccess) e = hipMemset(j->state, 0, P ? P : 1);
+    if (e == hipSuccess) e = set_sync(ctx, j->stats, 0, 32);
+    if (e == hipSuccess) e = set_sync(ctx, j->confirmed, 0, 4 * (N ? N : 1));
+    if (e == hipSuccess) e = set_sync(ctx, j->round, 0xff, P ? P : 1);
+    if (e == hipSuccess) e = set_sync(ctx, j->state, 0, P ? P : 1);
     auto take_event = [&](hipEvent_t &ev) {
         if (e != hipSuccess) return;
         if (!ctx->ev_pool.empty()) {
@@ -1762,7 +1868,10 @@ static int plan_graph_rounds(tdp_join *j) {
     if (rc == TDP_OK) rc = palloc(ctx, &j->d_words, 1);
     if (rc == TDP_OK) rc = palloc(ctx, &j->d_kept, 1);
     if (rc != TDP_OK) return rc;
-    if (!j->h_kept) TDP_CHECK(ctx, hipHostMalloc((void **)&j->h_kept, sizeof(uint64_t)));
+    if (!j->h_kept) {
+        std::lock_guard<std::recursive_mutex> cg(g_capture_mu);
+        TDP_CHECK(ctx, hipHostMalloc((void **)&j->h_kept, sizeof(uint64_t)));
+    }
     k_zero_u32<<<1, 64, 0, s>>>((uint32_t *)j->words5, 10);
     TDP_LAUNCHED(ctx);
     const tdp_tile *A = j->t1, *B = j->t2;
@@ -1808,6 +1917,7 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     const uint32_t need = fold_need(j, is_last);
     const int nb = j->tune.drain_blocks ? j->tune.drain_blocks : 4 * ctx->cu_count;
 
+    std::unique_lock<std::recursive_mutex> capture_lock(g_capture_mu);
     TDP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = TDP_OK;
     auto launched = [&]() {
@@ -1877,6 +1987,7 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     }
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(s, &graph);
+    capture_lock.unlock();
     if (e != hipSuccess && rc == TDP_OK) {
         ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
         rc = TDP_ERR_HIP;
@@ -1918,6 +2029,117 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     return TDP_OK;
 }
 
+static int lod_slot(int lod) {
+    for (int k = 0; k < 5; k++)
+        if (lod == 20 * (k + 1)) return k;
+    return -1;
+}
+
+// the tile of join side 1 or 2 when it is filled without records and the
+// side has its own list (a self-join's tile is listed once, on side 1)
+static tdp_tile *lazy_side(tdp_join *j, int side) {
+    const tdp_tile *t = side == 1 ? j->t1 : j->t2;
+    if (!t->lazy || (side == 2 && j->t2 == j->t1)) return nullptr;
+    return const_cast<tdp_tile *>(t);
+}
+
+extern "C" int tdp_join_lazy_needs(tdp_join *j, int lod, int side, const uint32_t **voxels,
+                                   size_t *n_voxels, uint64_t *n_records) {
+    if (!j || (side != 1 && side != 2) || !voxels || !n_voxels) return TDP_ERR_ARG;
+    tdp_ctx *ctx = j->ctx;
+    *voxels = nullptr;
+    *n_voxels = 0;
+    if (n_records) *n_records = 0;
+    if (!j->filtered) {
+        ctx->err = "tdp_join_lazy_needs: call tdp_join_filter first";
+        return TDP_ERR_STATE;
+    }
+    j->lz_ready[side - 1] = -1;
+    tdp_tile *t = lazy_side(j, side);
+    const int slot = lod_slot(lod);
+    if (!t || slot < 0 || j->n_vp == 0 || t->n_vox == 0) {
+        j->lz_ready[side - 1] = lod;
+        if (t) t->lz_slot = slot;
+        return TDP_OK;
+    }
+    if (t->lz_pending) {
+        ctx->err = "tdp_join_lazy_needs: voxels listed before were never put (fill the tile again)";
+        return TDP_ERR_STATE;
+    }
+    hipStream_t s = ctx->stream;
+    const int mask = side == 1 ? (j->t2 == j->t1 ? 3 : 1) : 2;
+    k_fill_u32<<<blocks_for(t->n_vox, kBlock), kBlock, 0, s>>>(t->n_vox, t->lz_flag, 0u);
+    TDP_LAUNCHED(ctx);
+    k_lazy_flag<<<blocks_for(j->n_vp, kBlock), kBlock, 0, s>>>(
+        j->n_vp, j->v1[j->cur], j->v2[j->cur], mask, slot, t->have, t->lz_flag);
+    TDP_LAUNCHED(ctx);
+    uint64_t n = 0, recs = 0;
+    int rc = scan_exclusive<uint32_t>(ctx, t->lz_flag, t->n_vox, t->lz_pos, &n);
+    if (rc != TDP_OK) return rc;
+    if (n) {
+        k_lazy_list<<<blocks_for(t->n_vox, kBlock), kBlock, 0, s>>>(
+            t->n_vox, slot, t->lz_flag, t->lz_pos, t->vox_tri_cnt, t->have, t->lz_list, t->lz_cnt);
+        TDP_LAUNCHED(ctx);
+        rc = scan_exclusive<uint32_t>(ctx, t->lz_cnt, n, t->lz_off, &recs);
+        if (rc != TDP_OK) return rc;
+        uint32_t *&h = j->lz_host[side - 1];
+        if (j->lz_host_cap[side - 1] < n) {
+            const size_t cap = std::max<size_t>(n, 2 * j->lz_host_cap[side - 1]);
+            if (h) tdp_host_free(ctx, h);
+            h = nullptr;
+            j->lz_host_cap[side - 1] = 0;
+            rc = tdp_host_alloc(ctx, 4 * cap, (void **)&h);
+            if (rc != TDP_OK) return rc;
+            j->lz_host_cap[side - 1] = cap;
+        }
+        TDP_CHECK(ctx, hipMemcpyAsync(h, t->lz_list, 4 * n, hipMemcpyDeviceToHost, s));
+        TDP_CHECK(ctx, hipStreamSynchronize(s));
+        *voxels = h;
+    }
+    *n_voxels = n;
+    if (n_records) *n_records = recs;
+    t->lz_pending = n;
+    t->lz_records = recs;
+    t->lz_slot = slot;
+    if (n == 0) j->lz_ready[side - 1] = lod;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_lazy_put(tdp_join *j, int lod, int side, const float *records) {
+    if (!j || (side != 1 && side != 2)) return TDP_ERR_ARG;
+    tdp_ctx *ctx = j->ctx;
+    tdp_tile *t = lazy_side(j, side);
+    const int slot = lod_slot(lod);
+    if (!t || !t->lz_pending) {
+        if (!t || t->lz_slot == slot) j->lz_ready[side - 1] = lod;
+        return TDP_OK;
+    }
+    if (t->lz_slot != slot) {
+        ctx->err = "tdp_join_lazy_put: the listed voxels are of another LOD";
+        return TDP_ERR_STATE;
+    }
+    if (!records) {
+        ctx->err = "tdp_join_lazy_put: null records";
+        return TDP_ERR_ARG;
+    }
+    void *dev = nullptr;
+    hipError_t e = hipHostGetDevicePointer(&dev, const_cast<float *>(records), 0);
+    if (e != hipSuccess || !dev) {
+        (void)hipGetLastError();
+        ctx->err = "tdp_join_lazy_put: the records are not in pinned memory from tdp_host_alloc";
+        return TDP_ERR_ARG;
+    }
+    const size_t n = t->lz_pending;
+    k_lazy_put<<<stride_grid(n * kWave, 2048), kBlock, 0, ctx->stream>>>(
+        n, slot, t->lz_list, t->lz_off, t->vox_tri_off, t->vox_tri_cnt, (const float4 *)dev,
+        t->tri, t->bnd);
+    TDP_LAUNCHED(ctx);
+    t->put_records += t->lz_records;
+    t->lz_pending = 0;
+    j->lz_ready[side - 1] = lod;
+    return TDP_OK;
+}
+
 extern "C" int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pairs_out) {
     if (!j) return TDP_ERR_ARG;
     if (!j->filtered) {
@@ -1928,6 +2150,12 @@ extern "C" int tdp_join_refine(tdp_join *j, int lod, int is_last, uint64_t *pair
         if (pairs_out) *pairs_out = 0;
         return TDP_OK;
     }
+    for (int side = 1; side <= 2; side++)
+        if (lazy_side(j, side) && j->lz_ready[side - 1] != lod) {
+            j->ctx->err = "tdp_join_refine: a tile filled without records needs this round's "
+                          "records first (tdp_join_lazy_needs, tdp_join_lazy_put)";
+            return TDP_ERR_STATE;
+        }
     switch (j->prm.precision) {
         case TDP_PREC_REF32: return join_refine_t<float, MODE_REF32>(j, lod, is_last, pairs_out);
         case TDP_PREC_FP64: return join_refine_t<double, MODE_FP64>(j, lod, is_last, pairs_out);
@@ -1943,17 +2171,17 @@ extern "C" int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin
     const size_t P = j->P;
     if (P == 0) return TDP_OK;
     if (report_round)
-        TDP_CHECK(ctx, hipMemcpy(report_round, j->round, P, hipMemcpyDeviceToHost));
+        TDP_CHECK(ctx, copy_sync(ctx, report_round, j->round, P, hipMemcpyDeviceToHost));
     if (j->rsz == 8) {  // straight into the caller's arrays
-        if (rmin) TDP_CHECK(ctx, hipMemcpy(rmin, j->rmin, P * 8, hipMemcpyDeviceToHost));
-        if (rmax) TDP_CHECK(ctx, hipMemcpy(rmax, j->rmax, P * 8, hipMemcpyDeviceToHost));
+        if (rmin) TDP_CHECK(ctx, copy_sync(ctx, rmin, j->rmin, P * 8, hipMemcpyDeviceToHost));
+        if (rmax) TDP_CHECK(ctx, copy_sync(ctx, rmax, j->rmax, P * 8, hipMemcpyDeviceToHost));
         return TDP_OK;
     }
     std::unique_ptr<float[]> a(new float[P]);
     for (int k = 0; k < 2; k++) {
         double *dst = k ? rmax : rmin;
         if (!dst) continue;
-        TDP_CHECK(ctx, hipMemcpy(a.get(), k ? j->rmax : j->rmin, P * 4, hipMemcpyDeviceToHost));
+        TDP_CHECK(ctx, copy_sync(ctx, a.get(), k ? j->rmax : j->rmin, P * 4, hipMemcpyDeviceToHost));
         for (size_t p = 0; p < P; p++) dst[p] = a[p];
     }
     return TDP_OK;
@@ -1962,7 +2190,7 @@ extern "C" int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin
 extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
     if (!j || !out5) return TDP_ERR_ARG;
     unsigned long long s[4];
-    TDP_CHECK(j->ctx, hipMemcpy(s, j->stats, sizeof s, hipMemcpyDeviceToHost));
+    TDP_CHECK(j->ctx, copy_sync(j->ctx, s, j->stats, sizeof s, hipMemcpyDeviceToHost));
     out5[0] = s[0];
     out5[1] = s[1];
     out5[2] = s[2];

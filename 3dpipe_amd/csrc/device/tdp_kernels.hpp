@@ -471,11 +471,9 @@ __global__ void __launch_bounds__(kBlock) k_vox_owner(size_t n_obj,
     for (uint32_t v = obj_vox_off[o]; v < obj_vox_off[o + 1]; v++) vox_obj[v] = (uint32_t)o;
 }
 
-__global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *__restrict__ rec,
-                                                       float4 *__restrict__ bnd) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = rec[3 * i], b = rec[3 * i + 1], c = rec[3 * i + 2];
+// the bound record of the triangle (a, b, c) (records carry ph in a.w and h in b.w)
+__device__ __forceinline__ void tri_bound(const float4 a, const float4 b, const float4 c,
+                                          float4 *__restrict__ out) {
     const float cx = ((a.x + b.x) + c.x) * (1.f / 3.f);
     const float cy = ((a.y + b.y) + c.y) * (1.f / 3.f);
     const float cz = ((a.z + b.z) + c.z) * (1.f / 3.f);
@@ -500,9 +498,87 @@ __global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *_
     // upper-bound constants of lo and hi, rounded upwards by 8u (r + ph + h)
     const float ph = a.w, h = b.w, tt = (r + ph) + h;
     const float alo = (r - ph) + 8.f * kU * tt, ahi = (r + h) + 8.f * kU * tt;
-    bnd[kBnd * i] = make_float4(cx, cy, cz, r);
-    bnd[kBnd * i + 1] = make_float4(fx, fy, fz, cm);
-    bnd[kBnd * i + 2] = make_float4(ph, h, alo, ahi);
+    out[0] = make_float4(cx, cy, cz, r);
+    out[1] = make_float4(fx, fy, fz, cm);
+    out[2] = make_float4(ph, h, alo, ahi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_tri_bounds(size_t n, const float4 *__restrict__ rec,
+                                                       float4 *__restrict__ bnd) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tri_bound(rec[3 * i], rec[3 * i + 1], rec[3 * i + 2], bnd + kBnd * i);
+}
+
+// ---- records supplied per LOD round (tiles filled without their records)
+//
+// have[v] bit s: the records of voxel v at LOD slot s are in the device
+// arrays (or listed to be put there before the round runs).
+
+// flag[v] = 1 for every voxel a live voxel pair names (v1 when side bit 0,
+// v2 when side bit 1) whose records at `slot` the tile does not hold
+__global__ void __launch_bounds__(kBlock) k_lazy_flag(size_t n, const uint32_t *__restrict__ v1,
+                                                      const uint32_t *__restrict__ v2, int side,
+                                                      int slot, const uint32_t *__restrict__ have,
+                                                      uint32_t *__restrict__ flag) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t bit = 1u << slot;
+    if (side & 1) {
+        const uint32_t v = v1[i];
+        if (!(have[v] & bit)) flag[v] = 1u;
+    }
+    if (side & 2) {
+        const uint32_t v = v2[i];
+        if (!(have[v] & bit)) flag[v] = 1u;
+    }
+}
+
+// the flagged voxels in voxel order (pos: exclusive scan of flag), now held,
+// with their record counts at `slot`
+__global__ void __launch_bounds__(kBlock) k_lazy_list(size_t n_vox, int slot,
+                                                      const uint32_t *__restrict__ flag,
+                                                      const uint64_t *__restrict__ pos,
+                                                      const uint32_t *__restrict__ vox_tri_cnt,
+                                                      uint32_t *__restrict__ have,
+                                                      uint32_t *__restrict__ list,
+                                                      uint32_t *__restrict__ cnt) {
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_vox || !flag[v]) return;
+    const uint64_t k = pos[v];
+    list[k] = (uint32_t)v;
+    cnt[k] = vox_tri_cnt[5 * v + slot];
+    have[v] |= 1u << slot;
+}
+
+// Listed voxels' records from `packed` (host memory the device reads: each
+// voxel's records at `slot`, voxel after voxel, voxel k's first at off[k])
+// into the tile's record array, with their bound records. One wave per
+// voxel, grid-stride.
+__global__ void __launch_bounds__(kBlock) k_lazy_put(size_t n_list, int slot,
+                                                     const uint32_t *__restrict__ list,
+                                                     const uint64_t *__restrict__ off,
+                                                     const uint64_t *__restrict__ vox_tri_off,
+                                                     const uint32_t *__restrict__ vox_tri_cnt,
+                                                     const float4 *__restrict__ packed,
+                                                     float4 *__restrict__ tri,
+                                                     float4 *__restrict__ bnd) {
+    const unsigned lane = threadIdx.x % kWave;
+    const size_t waves = (size_t)gridDim.x * (kBlock / kWave);
+    for (size_t k = (size_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; k < n_list;
+         k += waves) {
+        const uint32_t v = list[k];
+        const uint64_t dst = vox_tri_off[5 * (size_t)v + slot], src = off[k];
+        const uint32_t c = vox_tri_cnt[5 * (size_t)v + slot];
+        for (uint32_t t = lane; t < c; t += kWave) {
+            const float4 a = packed[3 * (src + t)], b = packed[3 * (src + t) + 1],
+                         cc = packed[3 * (src + t) + 2];
+            tri[3 * (dst + t)] = a;
+            tri[3 * (dst + t) + 1] = b;
+            tri[3 * (dst + t) + 2] = cc;
+            tri_bound(a, b, cc, bnd + kBnd * (dst + t));
+        }
+    }
 }
 
 // (d, lo, hi) intervals from an interval [dl, du] of the FP64 TriDist value

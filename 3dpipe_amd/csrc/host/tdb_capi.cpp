@@ -28,9 +28,68 @@ static void set_err(char *err, size_t len, const std::string &msg) {
 }
 
 
+// the records of the next round's voxels at `lod` that the tiles lack
+static int feed_round(tdp_ctx *ctx, tdp_join *j, int lod, LazyFeed &f, std::string &err) {
+    const int slot = tdb::lod_slot(lod);
+    for (int side = 1; side <= 2; side++) {
+        const uint32_t *vox = nullptr;
+        size_t n = 0;
+        uint64_t nrec = 0;
+        int rc = tdp_join_lazy_needs(j, lod, side, &vox, &n, &nrec);
+        if (rc != TDP_OK) {
+            err = tdp_last_error(ctx);
+            return rc;
+        }
+        if (n) {
+            const int w = side - 1;
+            const uint32_t *cnt = f.vox_tri_cnt[w];
+            if (!cnt || slot < 0) {
+                err = "lazy feed: no record counts for side " + std::to_string(side);
+                return TDP_ERR_STATE;
+            }
+            // packed offsets from the host's own counts
+            f.off.resize(n);
+            uint64_t at = 0;
+            for (size_t k = 0; k < n; k++) {
+                f.off[k] = at;
+                at += cnt[5 * (size_t)vox[k] + slot];
+            }
+            if (at != nrec) {
+                err = "lazy feed: the host's record counts differ from the tile's";
+                return TDP_ERR_STATE;
+            }
+            if (f.cap[w] < nrec) {
+                const size_t cap = std::max<size_t>(nrec, f.cap[w] + f.cap[w] / 2);
+                if (f.buf[w]) tdp_host_free(f.ctx, f.buf[w]);
+                f.buf[w] = nullptr;
+                f.cap[w] = 0;
+                rc = tdp_host_alloc(f.ctx, 48 * cap, (void **)&f.buf[w]);
+                if (rc != TDP_OK) {
+                    err = tdp_last_error(f.ctx);
+                    return rc;
+                }
+                f.cap[w] = cap;
+            }
+            const auto t0 = Clock::now();
+            if (!f.pack(side, slot, vox, n, f.off.data(), f.buf[w], err)) return TDP_ERR_STATE;
+            f.pack_ms += ms_since(t0);
+            f.put_records += nrec;
+        }
+        const auto t1 = Clock::now();
+        rc = tdp_join_lazy_put(j, lod, side, n ? f.buf[side - 1] : nullptr);
+        f.put_ms += ms_since(t1);
+        if (rc != TDP_OK) {
+            err = tdp_last_error(ctx);
+            return rc;
+        }
+    }
+    return TDP_OK;
+}
+
 int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *o,
                  const std::vector<uint32_t> &prefix, const std::vector<uint32_t> &ids,
-                 const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err) {
+                 const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err,
+                 LazyFeed *feed) {
     const size_t N = prefix.size() - 1, P = ids.size();
     res->n_targets = N;
     res->stats[4] = P;
@@ -58,12 +117,22 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     t0 = Clock::now();
     const std::vector<int> lods = tdb_lod_list(o);
     const int highest = lods.empty() ? 0 : lods.back();
+    std::string ferr;
     for (size_t li = 0; rc == TDP_OK && li < lods.size(); li++) {
         if (n_vp == 0) break;
+        if (feed) {
+            rc = feed_round(ctx, j, lods[li], *feed, ferr);
+            if (rc != TDP_OK) break;
+        }
         rc = tdp_join_refine(j, lods[li], lods[li] == highest, &n_vp);
         res->stats[5]++;
     }
     res->timing[3] = ms_since(t0);
+    if (rc != TDP_OK && !ferr.empty()) {
+        err = ferr;
+        tdp_join_destroy(j);
+        return rc;
+    }
     t0 = Clock::now();
     // the per-pair results come down into pinned memory (kept by the context
     // between joins): a DMA at full rate, no page faults

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <chrono>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -66,7 +67,7 @@ struct tdb_result {
     double stage_ms[TDP_STAGE_COUNT] = {};
     uint64_t stage_n[TDP_STAGE_COUNT] = {};
     uint64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
-    double stream[12] = {};  // streaming figures (tdb_result_stream)
+    double stream[16] = {};  // streaming figures (tdb_result_stream_n)
 };
 
 using Clock = std::chrono::steady_clock;
@@ -96,9 +97,40 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
                            std::vector<uint32_t> &prefix, std::vector<uint32_t> &ids,
                            uint64_t *n_fallback, std::string &err, bool device_within = true);
 
+// Records for tiles filled without them (tdp_tile_fill_meta_async): before
+// each LOD round the join lists, per side, the voxels that round reads and
+// the tile does not hold; pack(side, slot, voxels, n, packed_off, out) writes
+// their records at that slot into out (pinned, 12 floats per record), voxel
+// k's first at packed_off[k], and returns false on failure (err set). The
+// pinned buffers are the feed's, grown as needed (free them with
+// release()).
+struct LazyFeed {
+    std::function<bool(int side, int slot, const uint32_t *vox, size_t n, const uint64_t *off,
+                       float *out, std::string &err)>
+        pack;
+    // per side: the tile's record count of each voxel at each slot (5 per voxel)
+    const uint32_t *vox_tri_cnt[2] = {nullptr, nullptr};
+    tdp_ctx *ctx = nullptr;  // owner of the pinned buffers
+    float *buf[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};  // records
+    std::vector<uint64_t> off;
+    double pack_ms = 0, put_ms = 0;
+    uint64_t put_records = 0;
+    void release() {
+        for (int k = 0; k < 2; k++) {
+            if (buf[k]) tdp_host_free(ctx, buf[k]);
+            buf[k] = nullptr;
+            cap[k] = 0;
+        }
+    }
+};
+
 // Filter + LOD rounds on the device for the given candidates, results into
 // res (targets numbered 0..N-1, reported pairs ordered by (round, list
-// position)); timing [2..5] and stats filled. Returns a TDP_* status.
+// position)); timing [2..5] and stats filled. feed: the records of tiles
+// filled without them, round by round (nullptr: the tiles hold theirs).
+// Returns a TDP_* status.
 int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *o,
                  const std::vector<uint32_t> &prefix, const std::vector<uint32_t> &ids,
-                 const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err);
+                 const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err,
+                 LazyFeed *feed = nullptr);

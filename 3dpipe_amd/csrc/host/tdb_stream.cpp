@@ -10,21 +10,26 @@
 // neighbours share chunks, together with every object their candidate lists
 // name, form one device tile; the chunk's join (filter + every LOD round)
 // runs on that tile alone, so nothing a chunk needs lives outside it and a
-// chunk's results are those of the same targets in one big join. Three
-// stages run at once on consecutive chunks:
-//   decode  host threads write chunk i+2's arrays into pinned staging
-//           (ObjectSource::decode, the LOD decode of the reference's
-//           HiMesh_Wrapper::decode_to, himesh_wrapper.cpp:137-161),
-//   copy    chunk i+1's staging goes to a reserved device tile
-//           (tdp_tile_fill_async on the copy context's stream),
-//   join    chunk i is joined on the compute context.
-// Two staging sets and two device tiles alternate. The device budget covers
-// both tiles (reserved once) and the join buffers (the compute context's
-// cap); a chunk's join that would pass the cap is split into target ranges on
-// the same tile.
+// chunk's results are those of the same targets in one big join.
+//
+// Workers -- each a device context (stream) with its own reserved tiles,
+// pinned staging and join buffers -- take chunks in turn; a worker decodes
+// its chunk on host threads (ObjectSource, the LOD decode of the reference's
+// HiMesh_Wrapper::decode_to, himesh_wrapper.cpp:137-161), fills its tiles
+// and joins them, so one worker's host work runs beside another's kernels.
+// By default a chunk's tiles go over without their triangle records
+// (tdp_tile_fill_meta_async): before each LOD round the device lists the
+// voxels the round's live voxel pairs read, and the worker decodes just those
+// objects' records at that LOD (ObjectSource::decode_slot) and hands the
+// listed voxels' records over (tdp_join_lazy_put) -- the reference's per-LOD
+// packing of live candidates' voxels (KNNJoin.cpp:1584-1617,
+// SpatialJoin.cpp:169-248). Voxel pairs pruned at LOD 20 never ship their
+// higher LODs. The device budget covers every worker's tiles and join
+// buffers; a chunk's join that would pass its worker's cap is split into
+// target ranges on the same tile.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
-#include <condition_variable>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -44,9 +49,10 @@ struct tdb_source {
 namespace {
 
 // device bytes of one object in a reserved tile: records + bounds, voxel
-// arrays, object arrays
+// arrays (with the per-round lists of a tile filled without records),
+// object arrays
 inline uint64_t obj_bytes(const tdb::ObjectSource &s, size_t o) {
-    return 96 * s.n_tri(o) + 100 * (uint64_t)s.n_vox(o) + 48;
+    return 96 * s.n_tri(o) + 132 * (uint64_t)s.n_vox(o) + 48;
 }
 
 struct Chunk {
@@ -213,9 +219,14 @@ void free_staging(tdp_ctx *ctx, Staging &st) {
     st = Staging();
 }
 
-// the objects' arrays into pinned staging, laid out as one tile
+// the objects' arrays into pinned staging, laid out as one tile (meta_only:
+// without the records, which then come round by round)
+// meta_only: without the records, which then come round by round, and the
+// objects' decode state for those later decodes in state (object i's at
+// state_off[i])
 void decode_tile(const tdb::ObjectSource &s, const std::vector<uint32_t> &objs, Staging &st,
-                 int threads) {
+                 int threads, bool meta_only, std::vector<float> *state = nullptr,
+                 std::vector<uint64_t> *state_off = nullptr) {
     const size_t n = objs.size();
     std::vector<uint64_t> toff(n + 1, 0);
     st.obj_vox_off[0] = 0;
@@ -223,14 +234,22 @@ void decode_tile(const tdb::ObjectSource &s, const std::vector<uint32_t> &objs, 
         st.obj_vox_off[i + 1] = st.obj_vox_off[i] + s.n_vox(objs[i]);
         toff[i + 1] = toff[i] + s.n_tri(objs[i]);
     }
+    if (meta_only && state) {
+        state_off->assign(n + 1, 0);
+        for (size_t i = 0; i < n; i++) (*state_off)[i + 1] = (*state_off)[i] + s.state_floats(objs[i]);
+        if (state->size() < state_off->back()) state->resize(state_off->back());
+    }
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
 #endif
     for (long i = 0; i < (long)n; i++) {
         const uint32_t v0 = st.obj_vox_off[i], nv = st.obj_vox_off[i + 1] - v0;
         tdb::ObjArrays out{st.vox_box + 9 * (size_t)v0, st.vox_tri_cnt + 5 * (size_t)v0,
-                           st.obj_hd + 10 * (size_t)i, st.tri + 12 * toff[i]};
-        s.decode(objs[i], out);
+                           st.obj_hd + 10 * (size_t)i, meta_only ? nullptr : st.tri + 12 * toff[i]};
+        if (meta_only)
+            s.decode_meta(objs[i], out, state ? state->data() + (*state_off)[i] : nullptr);
+        else
+            s.decode(objs[i], out);
         uint64_t cur = toff[i];
         for (int sl = 0; sl < tdb::kLods; sl++)
             for (uint32_t v = 0; v < nv; v++) {
@@ -248,33 +267,32 @@ void decode_tile(const tdb::ObjectSource &s, const std::vector<uint32_t> &objs, 
     d.vox_box = st.vox_box;
     d.vox_tri_off = st.vox_tri_off;
     d.vox_tri_cnt = st.vox_tri_cnt;
-    d.tri = st.tri;
+    d.tri = meta_only ? nullptr : st.tri;
 }
 
-// pipeline progress shared by the three stages
+// one pipeline worker: a context (stream) with its tile slots, pinned
+// staging and record feed, and what it did
+struct Worker {
+    tdp_ctx *ctx = nullptr;
+    tdp_tile *tile[2] = {nullptr, nullptr};
+    Staging stg[2];
+    LazyFeed feed;
+    std::vector<uint32_t> vobj[2];  // chunk-local object of each voxel (lazy)
+    std::vector<float> state[2];    // the chunk objects' decode state (lazy)
+    std::vector<uint64_t> state_off[2];
+    double decode_ms = 0, copy_ms = 0, join_ms = 0;
+    uint64_t h2d = 0, n_decoded = 0, splits = 0, meta_bytes = 0, all_records = 0;
+};
+
+// the first failure of any worker (the others stop at their next chunk)
 struct Progress {
     std::mutex mu;
-    std::condition_variable cv;
-    size_t decoded = 0, uploaded = 0, joined = 0;
-    bool failed = false;
+    std::atomic<bool> failed{false};
     std::string err;
     void fail(const std::string &e) {
         std::lock_guard<std::mutex> g(mu);
         if (!failed) err = e;
         failed = true;
-        cv.notify_all();
-    }
-    // wait until pred() holds (true) or a stage failed (false)
-    template <typename F>
-    bool wait(F pred) {
-        std::unique_lock<std::mutex> g(mu);
-        cv.wait(g, [&] { return failed || pred(); });
-        return !failed;
-    }
-    void bump(size_t &c) {
-        std::lock_guard<std::mutex> g(mu);
-        c++;
-        cv.notify_all();
     }
 };
 
@@ -386,10 +404,19 @@ void tdb_stream_defaults(tdb_stream_opts *o) {
     o->max_chunk_targets = 0;
     o->decode_threads = 0;
     o->overlap = 1;
+    o->lazy = 1;
+    o->workers = 0;
 }
 
 void tdb_result_stream(const tdb_result *r, double *out12) {
     for (int k = 0; k < 12; k++) out12[k] = r ? r->stream[k] : 0.0;
+}
+
+int tdb_result_stream_n(const tdb_result *r, double *out, int n) {
+    if (!out || n < 0) return 0;
+    n = std::min(n, 16);
+    for (int k = 0; k < n; k++) out[k] = r ? r->stream[k] : 0.0;
+    return n;
 }
 
 int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src2,
@@ -472,7 +499,9 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         }
         budget = fr / 10 * 9;
     }
-    const uint64_t slot_bytes = (uint64_t)(budget * so->tile_share / 2);
+    const bool lazy = so->lazy != 0;
+    const int n_workers = so->overlap ? std::max(1, so->workers > 0 ? so->workers : 3) : 1;
+    const uint64_t slot_bytes = (uint64_t)(budget * so->tile_share / n_workers);
     const uint64_t max_targets = so->max_chunk_targets ? so->max_chunk_targets : UINT64_MAX;
     std::vector<Chunk> chunks;
     if (!plan_chunks(s1, s2, self, order, prefix, ids, slot_bytes, max_targets, chunks, e->err))
@@ -494,78 +523,151 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
             cap_tri[w] = std::max<size_t>(cap_tri[w], c.n_tri[w]);
         }
 
-    // contexts: copies on one stream, joins on another (the engine's tuning)
-    tdp_ctx *cctx = nullptr, *jctx = nullptr;
-    int rc = tdp_open(e->device, &cctx);
-    if (rc == TDP_OK) rc = tdp_open(e->device, &jctx);
-    tdp_tuning tune;
-    if (rc == TDP_OK) rc = tdp_get_tuning(e->ctx, &tune);
-    if (rc == TDP_OK) rc = tdp_set_tuning(jctx, &tune);
-    tdp_tile *slot[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    Staging stg[2][2];
+    // workers: each its own context (stream), tiles and staging, the
+    // engine's tuning; they take chunks in turn
+    std::vector<Worker> W(n_workers);
     auto cleanup = [&]() {
-        for (int b = 0; b < 2; b++)
-            for (int w = 0; w < 2; w++) {
-                if (slot[b][w]) tdp_tile_free(slot[b][w]);
-                if (cctx) free_staging(cctx, stg[b][w]);
+        for (Worker &w : W) {
+            if (w.ctx) w.feed.release();
+            for (int k = 0; k < 2; k++) {
+                if (w.tile[k]) tdp_tile_free(w.tile[k]);
+                if (w.ctx) free_staging(w.ctx, w.stg[k]);
             }
-        if (jctx) tdp_close(jctx);
-        if (cctx) tdp_close(cctx);
-    };
-    auto fail = [&](tdp_ctx *c, const std::string &what) {
-        e->err = what + (c ? std::string(": ") + tdp_last_error(c) : std::string());
-    };
-    if (rc != TDP_OK) {
-        fail(cctx ? (jctx ? jctx : cctx) : nullptr, "stream join: device contexts");
-        cleanup();
-        return rc;
-    }
-    for (int b = 0; b < 2 && rc == TDP_OK; b++)
-        for (int w = 0; w < n_tiles && rc == TDP_OK; w++) {
-            rc = tdp_tile_reserve(cctx, cap_obj[w], cap_vox[w], cap_tri[w], &slot[b][w]);
-            if (rc == TDP_OK) rc = alloc_staging(cctx, cap_obj[w], cap_vox[w], cap_tri[w], stg[b][w]);
+            if (w.ctx) tdp_close(w.ctx);
         }
+    };
+    tdp_tuning tune;
+    int rc = tdp_get_tuning(e->ctx, &tune);
     uint64_t tiles_bytes = 0;
-    tdp_memory_usage(cctx, &tiles_bytes, nullptr, 0);
+    for (int k = 0; k < n_workers && rc == TDP_OK; k++) {
+        Worker &w = W[k];
+        rc = tdp_open(e->device, &w.ctx);
+        if (rc == TDP_OK) rc = tdp_set_tuning(w.ctx, &tune);
+        for (int t = 0; t < n_tiles && rc == TDP_OK; t++) {
+            rc = tdp_tile_reserve(w.ctx, cap_obj[t], cap_vox[t], cap_tri[t], &w.tile[t]);
+            if (rc == TDP_OK)
+                rc = alloc_staging(w.ctx, cap_obj[t], cap_vox[t], lazy ? 0 : cap_tri[t], w.stg[t]);
+        }
+        uint64_t used = 0;
+        if (w.ctx) tdp_memory_usage(w.ctx, &used, nullptr, 0);
+        tiles_bytes += used;
+        w.feed.ctx = w.ctx;
+    }
     if (rc == TDP_OK && tiles_bytes >= budget) {
         e->err = "stream join: the tile slots alone exceed the device budget";
         rc = TDP_ERR_NOMEM;
     }
     if (rc != TDP_OK) {
-        if (e->err.empty()) fail(cctx, "stream join: tile slots");
+        if (e->err.empty()) {
+            tdp_ctx *c = nullptr;
+            for (Worker &w : W)
+                if (w.ctx) c = w.ctx;
+            e->err = std::string("stream join: worker set-up") +
+                     (c ? std::string(": ") + tdp_last_error(c) : std::string());
+        }
         cleanup();
         return rc;
     }
-    tdp_set_memory_limit(jctx, budget - tiles_bytes);
+    for (Worker &w : W) tdp_set_memory_limit(w.ctx, tiles_bytes / n_workers + (budget - tiles_bytes) / n_workers);
 
-    // the three stages
     const size_t nc = chunks.size();
     Progress pg;
-    double decode_ms = 0, copy_ms = 0, join_ms = 0;
-    uint64_t h2d = 0, n_decoded = 0, splits = 0;
     std::vector<tdb_result> parts(nc);
-    auto decode = [&](size_t i) {
+    const int nt_w = std::max(1, nt / n_workers);
+
+    // chunk i's arrays into worker w's staging: without records (lazy) or whole
+    auto decode = [&](size_t i, Worker &w) {
         const auto a = Clock::now();
         const Chunk &c = chunks[i];
-        for (int w = 0; w < n_tiles; w++) decode_tile(w == 0 ? s1 : s2, c.objs[w], stg[i % 2][w], nt);
-        decode_ms += ms_since(a);
-        n_decoded += c.objs[0].size() + (self ? 0 : c.objs[1].size());
-    };
-    auto copy = [&](size_t i) -> bool {
-        const auto a = Clock::now();
-        for (int w = 0; w < n_tiles; w++) {
-            const Staging &st = stg[i % 2][w];
-            if (tdp_tile_fill_async(slot[i % 2][w], &st.desc) != TDP_OK) return false;
-            h2d += 48 * st.desc.n_tri + 100 * st.desc.n_vox + 44 * st.desc.n_obj + 4;
+        for (int t = 0; t < n_tiles; t++) {
+            decode_tile(t == 0 ? s1 : s2, c.objs[t], w.stg[t], nt_w, lazy, &w.state[t],
+                        &w.state_off[t]);
+            if (lazy) {
+                const Staging &st = w.stg[t];
+                w.vobj[t].resize(st.desc.n_vox);
+                for (size_t o = 0; o < st.desc.n_obj; o++)
+                    for (uint32_t v = st.obj_vox_off[o]; v < st.obj_vox_off[o + 1]; v++)
+                        w.vobj[t][v] = (uint32_t)o;
+            }
         }
-        if (tdp_sync(cctx) != TDP_OK) return false;
-        copy_ms += ms_since(a);
+        w.decode_ms += ms_since(a);
+        w.n_decoded += c.objs[0].size() + (self ? 0 : c.objs[1].size());
+    };
+    auto copy = [&](size_t i, Worker &w) -> bool {
+        const auto a = Clock::now();
+        for (int t = 0; t < n_tiles; t++) {
+            const tdp_tile_desc &d = w.stg[t].desc;
+            if ((lazy ? tdp_tile_fill_meta_async(w.tile[t], &d) : tdp_tile_fill_async(w.tile[t], &d)) !=
+                TDP_OK)
+                return false;
+            const uint64_t meta = 100 * d.n_vox + 44 * d.n_obj + 4;
+            w.meta_bytes += meta;
+            w.h2d += meta + (lazy ? 0 : 48 * d.n_tri);
+            w.all_records += d.n_tri;
+        }
+        if (tdp_sync(w.ctx) != TDP_OK) return false;
+        w.copy_ms += ms_since(a);
         return true;
     };
-    auto join = [&](size_t i) -> bool {
+    // lazy: the records of a round's voxels, decoded per object at that slot
+    auto set_feed = [&](size_t i, Worker &w) {
+        w.feed.vox_tri_cnt[0] = w.stg[0].vox_tri_cnt;
+        w.feed.vox_tri_cnt[1] = self ? nullptr : w.stg[1].vox_tri_cnt;
+        w.feed.pack = [&, i](int side, int slot, const uint32_t *vox, size_t n, const uint64_t *off,
+                             float *out, std::string &err) -> bool {
+            const int t = side - 1;
+            if (t >= n_tiles) {
+                err = "lazy feed: side 2 of a self-join";
+                return false;
+            }
+            const tdb::ObjectSource &s = t == 0 ? s1 : s2;
+            const std::vector<uint32_t> &objs = chunks[i].objs[t];
+            const Staging &st = w.stg[t];
+            const std::vector<uint32_t> &vo = w.vobj[t];
+            // runs of one object in the (ascending) list
+            std::vector<size_t> run{0};
+            for (size_t k = 1; k < n; k++)
+                if (vo[vox[k]] != vo[vox[k - 1]]) run.push_back(k);
+            run.push_back(n);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt_w)
+#endif
+            {
+                std::vector<float> tmp;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+                for (long r = 0; r < (long)run.size() - 1; r++) {
+                    const size_t a = run[r], b = run[r + 1];
+                    const uint32_t o = vo[vox[a]];
+                    const uint32_t v0 = st.obj_vox_off[o], nv = st.obj_vox_off[o + 1] - v0;
+                    const float *sto = w.state_off[t][o + 1] > w.state_off[t][o]
+                                           ? w.state[t].data() + w.state_off[t][o]
+                                           : nullptr;
+                    if (b - a == nv) {  // every voxel: the slot's block as decoded
+                        s.decode_slot(objs[o], slot, out + 12 * off[a], sto);
+                        continue;
+                    }
+                    const uint64_t base = st.vox_tri_off[5 * (size_t)v0 + slot];
+                    const size_t last = 5 * (size_t)(v0 + nv - 1) + slot;
+                    tmp.resize(12 * (st.vox_tri_off[last] + st.vox_tri_cnt[last] - base));
+                    s.decode_slot(objs[o], slot, tmp.data(), sto);
+                    for (size_t k = a; k < b; k++) {
+                        const size_t q = 5 * (size_t)vox[k] + slot;
+                        std::memcpy(out + 12 * off[k], tmp.data() + 12 * (st.vox_tri_off[q] - base),
+                                    48 * (size_t)st.vox_tri_cnt[q]);
+                    }
+                }
+            }
+            return true;
+        };
+    };
+    auto join = [&](size_t i, Worker &w) -> bool {
         const auto a = Clock::now();
         const Chunk &c = chunks[i];
-        tdp_tile *d1 = slot[i % 2][0], *d2 = self ? d1 : slot[i % 2][1];
+        tdp_tile *d1 = w.tile[0], *d2 = self ? d1 : w.tile[1];
+        if (lazy) set_feed(i, w);
+        const double feed0 = w.feed.pack_ms + w.feed.put_ms;
         // target ranges [lo, hi); halved while a join passes the memory cap
         std::vector<std::pair<size_t, size_t>> todo{{0, c.targets.size()}};
         tdb_result &R = parts[i];
@@ -577,9 +679,10 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
             std::vector<uint32_t> id(c.ids.begin() + c.prefix[lo], c.ids.begin() + c.prefix[hi]);
             tdb_result r;
             std::string err;
-            int jr = tdb_run_join(jctx, d1, d2, o, pf, id, nullptr, nullptr, &r, err);
+            int jr = tdb_run_join(w.ctx, d1, d2, o, pf, id, nullptr, nullptr, &r, err,
+                                  lazy ? &w.feed : nullptr);
             if (jr == TDP_ERR_NOMEM && hi - lo > 1) {
-                splits++;
+                w.splits++;
                 const size_t mid = lo + (hi - lo) / 2;
                 todo.push_back({mid, hi});
                 todo.push_back({lo, mid});
@@ -606,52 +709,50 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
                 R.stage_n[k] += r.stage_n[k];
             }
         }
-        join_ms += ms_since(a);
+        w.join_ms += ms_since(a) - (w.feed.pack_ms + w.feed.put_ms - feed0);
         return true;
     };
 
     const auto t_pipe = Clock::now();
-    if (!so->overlap) {
-        for (size_t i = 0; i < nc && !pg.failed; i++) {
-            decode(i);
-            if (!copy(i)) {
-                pg.fail(std::string("stream join: copy: ") + tdp_last_error(cctx));
-                break;
+    std::atomic<size_t> next{0};
+    auto run_worker = [&](Worker &w) {
+        for (;;) {
+            if (pg.failed) return;
+            const size_t i = next.fetch_add(1);
+            if (i >= nc) return;
+            decode(i, w);
+            if (!copy(i, w)) {
+                pg.fail(std::string("stream join: copy: ") + tdp_last_error(w.ctx));
+                return;
             }
-            join(i);
+            if (!join(i, w)) return;
         }
+    };
+    if (n_workers == 1) {
+        run_worker(W[0]);
     } else {
-        std::thread dec([&] {
-            for (size_t i = 0; i < nc; i++) {
-                // staging i % 2 is free once chunk i - 2 has been copied
-                if (!pg.wait([&] { return pg.uploaded + 1 >= i; })) return;
-                decode(i);
-                pg.bump(pg.decoded);
-            }
-        });
-        std::thread cpy([&] {
-            for (size_t i = 0; i < nc; i++) {
-                // tile i % 2 is free once chunk i - 2 has been joined
-                if (!pg.wait([&] { return pg.decoded > i && pg.joined + 1 >= i; })) return;
-                if (!copy(i)) {
-                    pg.fail(std::string("stream join: copy: ") + tdp_last_error(cctx));
-                    return;
-                }
-                pg.bump(pg.uploaded);
-            }
-        });
-        for (size_t i = 0; i < nc; i++) {
-            if (!pg.wait([&] { return pg.uploaded > i; })) break;
-            if (!join(i)) break;
-            pg.bump(pg.joined);
-        }
-        dec.join();
-        cpy.join();
+        std::vector<std::thread> th;
+        for (Worker &w : W) th.emplace_back([&run_worker, &w] { run_worker(w); });
+        for (auto &t : th) t.join();
     }
     const double wall = ms_since(t_pipe);
-    uint64_t peak_c = 0, peak_j = 0;
-    tdp_memory_usage(cctx, nullptr, &peak_c, 0);
-    tdp_memory_usage(jctx, nullptr, &peak_j, 0);
+    double decode_ms = 0, copy_ms = 0, join_ms = 0;
+    uint64_t h2d = 0, n_decoded = 0, splits = 0, meta_bytes = 0, all_records = 0, put_records = 0,
+             peak = 0;
+    for (Worker &w : W) {
+        decode_ms += w.decode_ms + w.feed.pack_ms;
+        copy_ms += w.copy_ms + w.feed.put_ms;
+        join_ms += w.join_ms;
+        h2d += w.h2d + 48 * w.feed.put_records;
+        n_decoded += w.n_decoded;
+        splits += w.splits;
+        meta_bytes += w.meta_bytes;
+        all_records += w.all_records;
+        put_records += lazy ? w.feed.put_records : w.all_records;
+        uint64_t p = 0;
+        tdp_memory_usage(w.ctx, nullptr, &p, 0);
+        peak += p;
+    }
     cleanup();
     if (pg.failed) {
         e->err = pg.err;
@@ -704,9 +805,13 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     res->stream[4] = join_ms;
     res->stream[5] = wall;
     res->stream[6] = serial > slowest ? std::min(1.0, std::max(0.0, (serial - wall) / (serial - slowest))) : 1.0;
-    res->stream[7] = (double)(peak_c + peak_j);
+    res->stream[7] = (double)peak;
     res->stream[10] = (double)n_decoded;
     res->stream[11] = (double)splits;
+    res->stream[12] = 48.0 * (double)all_records;
+    res->stream[13] = 48.0 * (double)put_records;
+    res->stream[14] = (double)meta_bytes;
+    res->stream[15] = (double)n_workers;
     res->timing[0] = ms_since(t_start);
     *out = res.release();
     return 0;

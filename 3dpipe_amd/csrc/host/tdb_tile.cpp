@@ -263,10 +263,17 @@ struct Voxelized {
     std::vector<double> hd[kLods];           // per slot, per face: proxy, hausdorff
     std::vector<int> order[kLods];           // per slot: faces by voxel, then face index
     std::vector<uint32_t> vcnt[kLods];       // per slot, per voxel: faces
+    // decode shortcuts (set_order): per slot the largest proxy / hausdorff
+    // bound (-1 when the slot has no faces); per voxel the distinct vertices
+    // of its LOD-100 faces (vox_vert_off[v] .. [v + 1]); the vertices LOD-100
+    // faces use, ascending
+    double hd_max[kLods][2] = {};
+    std::vector<uint32_t> vox_vert_off, vox_vert, used_vert;
 };
 
-// the record order of an object: slot -> voxel -> face
-void set_order(Voxelized &vz) {
+// the record order of an object: slot -> voxel -> face; and the decode
+// shortcuts
+void set_order(const LodMesh &m, Voxelized &vz) {
     for (int s = 0; s < kLods; s++) {
         const size_t nf = vz.face_vox[s].size();
         vz.vcnt[s].assign(vz.nvox, 0);
@@ -275,7 +282,31 @@ void set_order(Voxelized &vz) {
         for (int v = 0; v < vz.nvox; v++) at[v + 1] = at[v] + vz.vcnt[s][v];
         vz.order[s].assign(nf, 0);
         for (size_t i = 0; i < nf; i++) vz.order[s][at[vz.face_vox[s][i]]++] = (int)i;
+        vz.hd_max[s][0] = vz.hd_max[s][1] = -1;
+        if (s < 4)
+            for (size_t i = 0; i < nf; i++) {
+                vz.hd_max[s][0] = std::max(vz.hd_max[s][0], vz.hd[s][2 * i]);
+                vz.hd_max[s][1] = std::max(vz.hd_max[s][1], vz.hd[s][2 * i + 1]);
+            }
     }
+    std::vector<std::vector<uint32_t>> vv(vz.nvox);
+    std::vector<uint8_t> used(m.v.size(), 0);
+    for (size_t i = 0; i < m.f[4].size(); i++)
+        for (int k = 0; k < 3; k++) {
+            vv[vz.face_vox[4][i]].push_back((uint32_t)m.f[4][i][k]);
+            used[m.f[4][i][k]] = 1;
+        }
+    vz.vox_vert_off.assign(1, 0);
+    vz.vox_vert.clear();
+    for (auto &l : vv) {
+        std::sort(l.begin(), l.end());
+        l.erase(std::unique(l.begin(), l.end()), l.end());
+        vz.vox_vert.insert(vz.vox_vert.end(), l.begin(), l.end());
+        vz.vox_vert_off.push_back((uint32_t)vz.vox_vert.size());
+    }
+    vz.used_vert.clear();
+    for (size_t i = 0; i < used.size(); i++)
+        if (used[i]) vz.used_vert.push_back((uint32_t)i);
 }
 
 V3 bary(const LodMesh &m, const std::array<int, 3> &f) {
@@ -451,7 +482,7 @@ Voxelized voxelize(const LodMesh &m, int voxel_tris) {
             vz.hd[s][2 * i + 1] = h;
         }
     }
-    set_order(vz);
+    set_order(m, vz);
     return vz;
 }
 
@@ -476,25 +507,25 @@ size_t total_faces(const LodMesh &m) {
     return n;
 }
 
-// Voxel boxes from the LOD-100 faces, cores from the core face's vertices,
-// and the max-distance proxy point: the LOD-100 vertex nearest the box centre.
+// Voxel boxes from the LOD-100 faces' vertices, cores from the core face's
+// vertices, and the max-distance proxy point: the LOD-100 vertex nearest the
+// box centre (the first such in vertex order).
 void emit_boxes(const LodMesh &m, const std::vector<float> &vf, const Voxelized &vz,
                 float *vox_box, float *transform) {
     const int nv = vz.nvox;
     auto P = [&](int i, int d) { return vf[3 * i + d]; };
-    for (int v = 0; v < nv; v++)
-        for (int d = 0; d < 3; d++) {
-            vox_box[9 * v + d] = INFINITY;
-            vox_box[9 * v + 3 + d] = -INFINITY;
-        }
-    for (size_t i = 0; i < m.f[4].size(); i++) {
-        int v = vz.face_vox[4][i];
-        for (int k = 0; k < 3; k++)
+    for (int v = 0; v < nv; v++) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t q = vz.vox_vert_off[v]; q < vz.vox_vert_off[v + 1]; q++)
             for (int d = 0; d < 3; d++) {
-                float c = P(m.f[4][i][k], d);
-                vox_box[9 * v + d] = std::min(vox_box[9 * v + d], c);
-                vox_box[9 * v + 3 + d] = std::max(vox_box[9 * v + 3 + d], c);
+                const float c = P((int)vz.vox_vert[q], d);
+                lo[d] = std::min(lo[d], c);
+                hi[d] = std::max(hi[d], c);
             }
+        for (int d = 0; d < 3; d++) {
+            vox_box[9 * v + d] = lo[d];
+            vox_box[9 * v + 3 + d] = hi[d];
+        }
     }
     for (int v = 0; v < nv; v++) {
         const auto &cf = m.f[4][vz.core_face[v]];
@@ -504,20 +535,15 @@ void emit_boxes(const LodMesh &m, const std::vector<float> &vf, const Voxelized 
         }
     }
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    std::vector<uint8_t> used(vf.size() / 3, 0);
-    for (auto &f : m.f[4])
-        for (int k = 0; k < 3; k++) used[f[k]] = 1;
-    for (size_t i = 0; i < used.size(); i++)
-        if (used[i])
-            for (int d = 0; d < 3; d++) {
-                lo[d] = std::min(lo[d], P((int)i, d));
-                hi[d] = std::max(hi[d], P((int)i, d));
-            }
+    for (uint32_t i : vz.used_vert)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], P((int)i, d));
+            hi[d] = std::max(hi[d], P((int)i, d));
+        }
     float c[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
     float best = INFINITY;
     int bi = 0;
-    for (size_t i = 0; i < used.size(); i++) {
-        if (!used[i]) continue;
+    for (uint32_t i : vz.used_vert) {
         float dx = P((int)i, 0) - c[0], dy = P((int)i, 1) - c[1], dz = P((int)i, 2) - c[2];
         float d2 = dx * dx + dy * dy + dz * dz;
         if (d2 < best) { best = d2; bi = (int)i; }
@@ -525,34 +551,69 @@ void emit_boxes(const LodMesh &m, const std::vector<float> &vf, const Voxelized 
     for (int d = 0; d < 3; d++) transform[d] = P(bi, d);
 }
 
+// The margin of the records' Hausdorff bounds: covers rounding the vertices
+// to float (bounds in `vz` are multiplied by `scale`).
+double record_margin(const std::vector<float> &vf, double scale) {
+    float maxc = 0;
+    for (float c : vf) maxc = std::max(maxc, std::fabs(c));
+    return 8.0 * ulpf(maxc) + 1e-6 * scale;
+}
+
+// the (proxy hausdorff, hausdorff) pair of face i at slot s (0 at LOD 100)
+inline void face_hd(const Voxelized &vz, int s, int i, double scale, double margin, float &ph,
+                    float &h) {
+    ph = 0;
+    h = 0;
+    if (s < 4) {
+        ph = (float)(vz.hd[s][2 * i] * scale + margin);
+        h = (float)(vz.hd[s][2 * i + 1] * scale + margin);
+    }
+}
+
+// Per-voxel counts and the object Hausdorff pairs: the largest of the
+// records' (hausdorff, proxy) values, 0 at LOD 100 and for a slot without
+// faces. A record's value is the float of (bound * scale + margin), a
+// non-decreasing map of the bound, so the largest record value is the map
+// of the largest bound.
+void emit_counts_hd(const Voxelized &vz, double scale, double margin, uint32_t *cnt, float *hd) {
+    for (int s = 0; s < kLods; s++) {
+        hd[2 * s] = 0;
+        hd[2 * s + 1] = 0;
+        for (int v = 0; v < vz.nvox; v++) cnt[5 * v + s] = vz.vcnt[s][v];
+        if (s < 4 && !vz.order[s].empty()) {
+            hd[2 * s] = std::max(0.f, (float)(vz.hd_max[s][1] * scale + margin));      // hausdorff
+            hd[2 * s + 1] = std::max(0.f, (float)(vz.hd_max[s][0] * scale + margin));  // proxy
+        }
+    }
+}
+
+// the triangle records of slot s (voxel -> face order)
+void emit_slot(const LodMesh &m, const float *vf, const Voxelized &vz, double scale,
+               double margin, int s, float *rec) {
+    auto P = [&](int i, int d) { return vf[3 * i + d]; };
+    size_t w = 0;
+    for (int i : vz.order[s]) {
+        const auto &f = m.f[s][i];
+        float ph, h;
+        face_hd(vz, s, i, scale, margin, ph, h);
+        float r[12] = {P(f[0], 0), P(f[0], 1), P(f[0], 2), ph, P(f[1], 0), P(f[1], 1),
+                       P(f[1], 2), h,          P(f[2], 0), P(f[2], 1), P(f[2], 2), 0.f};
+        std::memcpy(rec + 12 * w, r, sizeof r);
+        w++;
+    }
+}
+
 // Triangle records (slot -> voxel -> face), per-voxel counts and the object
 // Hausdorff pairs; bounds in `vz` are multiplied by `scale` and widened by a
 // margin that covers rounding the vertices to float.
 void emit_records(const LodMesh &m, const std::vector<float> &vf, const Voxelized &vz,
                   double scale, uint32_t *cnt, float *rec, float *hd) {
-    float maxc = 0;
-    for (float c : vf) maxc = std::max(maxc, std::fabs(c));
-    const double margin = 8.0 * ulpf(maxc) + 1e-6 * scale;
-    auto P = [&](int i, int d) { return vf[3 * i + d]; };
+    const double margin = record_margin(vf, scale);
+    emit_counts_hd(vz, scale, margin, cnt, hd);
     size_t w = 0;
     for (int s = 0; s < kLods; s++) {
-        hd[2 * s] = 0;
-        hd[2 * s + 1] = 0;
-        for (int v = 0; v < vz.nvox; v++) cnt[5 * v + s] = vz.vcnt[s][v];
-        for (int i : vz.order[s]) {
-            const auto &f = m.f[s][i];
-            float ph = 0, h = 0;
-            if (s < 4) {
-                ph = (float)(vz.hd[s][2 * i] * scale + margin);
-                h = (float)(vz.hd[s][2 * i + 1] * scale + margin);
-            }
-            float r[12] = {P(f[0], 0), P(f[0], 1), P(f[0], 2), ph, P(f[1], 0), P(f[1], 1),
-                           P(f[1], 2), h,          P(f[2], 0), P(f[2], 1), P(f[2], 2), 0.f};
-            std::memcpy(rec + 12 * w, r, sizeof r);
-            w++;
-            hd[2 * s] = std::max(hd[2 * s], h);           // object hausdorff
-            hd[2 * s + 1] = std::max(hd[2 * s + 1], ph);  // object proxy
-        }
+        emit_slot(m, vf.data(), vz, scale, margin, s, rec + 12 * w);
+        w += vz.order[s].size();
     }
 }
 
@@ -965,6 +1026,36 @@ class NucleusSource final : public ObjectSource {
         emit_boxes(b_.protos[k], vf, b_.pvz[k], out.vox_box, tr);
         emit_records(b_.protos[k], vf, b_.pvz[k], b_.inst[o].radius, out.cnt, out.rec, out.hd);
     }
+    // state: the float vertices, then the records' margin (as a double's two halves)
+    size_t state_floats(size_t o) const override {
+        return 3 * b_.protos[b_.inst[o].proto].v.size() + 2;
+    }
+    void decode_meta(size_t o, const ObjArrays &out, float *state) const override {
+        const int k = b_.inst[o].proto;
+        std::vector<float> vf;
+        float tr[3];
+        nucleus_vertices(b_, o, vf);
+        emit_boxes(b_.protos[k], vf, b_.pvz[k], out.vox_box, tr);
+        const double scale = b_.inst[o].radius, margin = record_margin(vf, scale);
+        emit_counts_hd(b_.pvz[k], scale, margin, out.cnt, out.hd);
+        if (state) {
+            std::memcpy(state, vf.data(), 4 * vf.size());
+            std::memcpy(state + vf.size(), &margin, sizeof margin);
+        }
+    }
+    void decode_slot(size_t o, int slot, float *rec, const float *state) const override {
+        const int k = b_.inst[o].proto;
+        const double scale = b_.inst[o].radius;
+        if (state) {
+            double margin;
+            std::memcpy(&margin, state + 3 * b_.protos[k].v.size(), sizeof margin);
+            emit_slot(b_.protos[k], state, b_.pvz[k], scale, margin, slot, rec);
+            return;
+        }
+        std::vector<float> vf;
+        nucleus_vertices(b_, o, vf);
+        emit_slot(b_.protos[k], vf.data(), b_.pvz[k], scale, record_margin(vf, scale), slot, rec);
+    }
 
   private:
     NucleusBank b_;
@@ -1000,12 +1091,47 @@ class TileSource final : public ObjectSource {
                 w += n;
             }
     }
+    void decode_meta(size_t o, const ObjArrays &out, float *) const override {
+        const uint32_t v0 = t_.obj_vox_off[o], nv = t_.obj_vox_off[o + 1] - v0;
+        std::memcpy(out.vox_box, &t_.vox_box[9 * (size_t)v0], 36 * (size_t)nv);
+        std::memcpy(out.cnt, &t_.vox_tri_cnt[5 * (size_t)v0], 20 * (size_t)nv);
+        std::memcpy(out.hd, &t_.obj_hd[10 * o], 40);
+    }
+    void decode_slot(size_t o, int slot, float *rec, const float *) const override {
+        const uint32_t v0 = t_.obj_vox_off[o], nv = t_.obj_vox_off[o + 1] - v0;
+        size_t w = 0;
+        for (uint32_t v = 0; v < nv; v++) {
+            const size_t q = 5 * (size_t)(v0 + v) + slot, n = t_.vox_tri_cnt[q];
+            std::memcpy(rec + 12 * w, &t_.tri[12 * t_.vox_tri_off[q]], 48 * n);
+            w += n;
+        }
+    }
 
   private:
     const Tile &t_;
 };
 
 }  // namespace
+
+// generic forms: a whole decode into scratch
+void ObjectSource::decode_meta(size_t o, const ObjArrays &out, float *) const {
+    std::vector<float> rec(12 * n_tri(o));
+    decode(o, ObjArrays{out.vox_box, out.cnt, out.hd, rec.data()});
+}
+
+void ObjectSource::decode_slot(size_t o, int slot, float *rec, const float *) const {
+    const uint32_t nv = n_vox(o);
+    std::vector<float> box(9 * (size_t)nv), hd(10), all(12 * n_tri(o));
+    std::vector<uint32_t> cnt(5 * (size_t)nv);
+    decode(o, ObjArrays{box.data(), cnt.data(), hd.data(), all.data()});
+    size_t at = 0, n = 0;
+    for (int s = 0; s < kLods; s++)
+        for (uint32_t v = 0; v < nv; v++) {
+            if (s < slot) at += cnt[5 * v + s];
+            if (s == slot) n += cnt[5 * v + s];
+        }
+    std::memcpy(rec, all.data() + 12 * at, 48 * n);
+}
 
 std::unique_ptr<ObjectSource> nucleus_source(const GenParams &p) {
     return std::unique_ptr<ObjectSource>(new NucleusSource(p));

@@ -94,6 +94,14 @@ class ObjectSource {
     virtual uint64_t n_tri(size_t o) const = 0;
     virtual void meta(size_t o, float box[6], float transform[3]) const = 0;
     virtual void decode(size_t o, const ObjArrays &out) const = 0;
+    // decode without the records (out.rec unused): voxel boxes, counts and
+    // Hausdorff values, as decode writes them; with `state` (state_floats(o)
+    // floats) it also leaves what later decode_slot calls reuse
+    virtual void decode_meta(size_t o, const ObjArrays &out, float *state = nullptr) const;
+    // the records of LOD slot `slot` alone, voxel after voxel, as decode
+    // writes that slot's part of out.rec (state: from decode_meta, or null)
+    virtual void decode_slot(size_t o, int slot, float *rec, const float *state = nullptr) const;
+    virtual size_t state_floats(size_t o) const { return 0; }
 };
 
 // the generator's nuclei (equal, object for object, to generate()'s nuclei tile)

@@ -113,10 +113,14 @@ def list_agreement(a, b):
     return out
 
 
-def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn):
+def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn, lazy=True,
+                 workers=0):
     """Chunked streaming 10-NN self-join of n_nuclei nuclei (radii 6-14,
-    uniform in the cube) within a device budget: host decode into pinned
-    buffers, copies and joins overlapped (tdb_stream_join)."""
+    uniform in the cube) within a device budget (tdb_stream_join): workers
+    side by side (three by default), each decoding a chunk's voxel and object arrays
+    into pinned buffers and joining it; lazily (the default) only the records
+    of the voxels each LOD round reads are decoded and shipped, otherwise
+    every record of every chunk object."""
     t0 = time.perf_counter()
     src = tdp.Source.generate(seed=seed, n_vessels=0, nuclei_per_vessel=n_nuclei, cube=cube,
                               nucleus_r_min=6.0, nucleus_r_max=14.0)
@@ -124,20 +128,29 @@ def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn):
     if tuning:
         eng.set_tuning(**tuning)
     budget = int(budget_gib * (1 << 30))
-    r = eng.stream_join(src, None, query="nn", knn=knn, precision="mixed", hbm_budget=budget)
+    r = eng.stream_join(src, None, query="nn", knn=knn, precision="mixed", hbm_budget=budget,
+                        lazy=lazy, workers=workers)
     wall = time.perf_counter() - t0
     eng.close()
     st = r.stream
     h2d = st["h2d_bytes"]
-    log_fn(f"stream: {n_nuclei} nuclei, {st['chunks']:.0f} chunks, {wall:.1f} s, "
-           f"copy {h2d / 1e9:.1f} GB at {h2d / (st['copy_ms'] * 1e6):.1f} GB/s, "
+    log_fn(f"stream ({'lazy' if lazy else 'every record'}): {n_nuclei} nuclei, "
+           f"{st['chunks']:.0f} chunks, {wall:.1f} s, h2d {h2d / 1e9:.1f} GB "
+           f"(records {st['record_bytes_shipped'] / 1e9:.1f} of {st['record_bytes_all'] / 1e9:.1f} GB), "
            f"overlap {st['overlap']:.2f}, peak {st['peak_device_bytes'] / 2**30:.1f} GiB")
     return {
         "workload": f"{knn}-NN self-join of {n_nuclei} nuclei (radii 6-14) in a {cube:g}^3 cube, "
                     f"LODs 20-100, mixed precision, device budget {budget_gib:g} GiB",
+        "records": "per LOD round, the voxels it reads" if lazy else "every record",
+        "workers": int(st["workers"]),
         "seed": seed, "wall_s": wall, "setup_s": wall - st["wall_ms"] / 1e3,
         "chunks": int(st["chunks"]), "objects_decoded": int(st["objects_decoded"]),
         "h2d_bytes": int(h2d),
+        "h2d_record_bytes": int(st["record_bytes_shipped"]),
+        "h2d_meta_bytes": int(st["meta_bytes"]),
+        "record_bytes_of_chunk_objects": int(st["record_bytes_all"]),
+        "record_share_shipped": (st["record_bytes_shipped"] / st["record_bytes_all"]
+                                 if st["record_bytes_all"] else None),
         "pcie_gbps": h2d / (st["copy_ms"] * 1e6) if st["copy_ms"] else None,
         "end_to_end_gbps": h2d / (st["wall_ms"] * 1e6) if st["wall_ms"] else None,
         "decode_gbps": h2d / (st["decode_ms"] * 1e6) if st["decode_ms"] else None,
@@ -281,6 +294,12 @@ def main():
                     help="run only the serving benchmark and print its figures")
     ap.add_argument("--stream-cube", type=float, default=50000.0)
     ap.add_argument("--stream-budget-gib", type=float, default=24.0)
+    ap.add_argument("--stream-workers", type=str, default="0",
+                    help="streaming workers side by side (0 = the library's default; a comma "
+                         "list with --only-stream runs each)")
+    ap.add_argument("--stream-every-record", type=int, default=0,
+                    help="also run the streaming self-join shipping every record of every "
+                         "chunk object, for comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the join shards by "
@@ -313,9 +332,18 @@ def main():
                                      args.seed + 100, log)), flush=True)
         return
     if args.only_stream:
-        print(json.dumps(stream_bench(tdp, args.stream_nuclei, args.stream_cube,
-                                      args.stream_budget_gib, args.knn, args.seed, None, log)),
-              flush=True)
+        ws = [int(x) for x in args.stream_workers.split(",")]
+        out = stream_bench(tdp, args.stream_nuclei, args.stream_cube, args.stream_budget_gib,
+                           args.knn, args.seed, None, log, workers=ws[0])
+        for w in ws[1:]:
+            out[f"workers_{w}"] = stream_bench(tdp, args.stream_nuclei, args.stream_cube,
+                                               args.stream_budget_gib, args.knn, args.seed, None,
+                                               log, workers=w)
+        if args.stream_every_record:
+            out["every_record"] = stream_bench(tdp, args.stream_nuclei, args.stream_cube,
+                                               args.stream_budget_gib, args.knn, args.seed, None,
+                                               log, lazy=False)
+        print(json.dumps(out), flush=True)
         return
 
     t0 = time.perf_counter()

@@ -183,7 +183,15 @@ void tdb_result_free(tdb_result *r);
  * chunked pipeline (nearest_neighbor_chunking / within_chunking,
  * KNNJoin.cpp:1524-1811, WithinJoin.cpp:644-813, with the packing producer
  * KNNJoin.cpp:1586-1617 and SpatialJoin.cpp:169-248). The result equals
- * tdb_join's on the same objects. */
+ * tdb_join's on the same objects.
+ *
+ * Workers (each with its own device stream, tile slots and pinned staging)
+ * take chunks in turn, so one worker's host work runs beside another's
+ * kernels. With `lazy` a chunk's tiles go to the device without their
+ * triangle records: before each LOD round the device lists the voxels that
+ * the round's live voxel pairs read, and only those voxels' records at that
+ * LOD are decoded (ObjectSource::decode_slot) and shipped -- the reference's
+ * per-LOD packing of live candidates' voxels (KNNJoin.cpp:1584-1617). */
 typedef struct tdb_source tdb_source;
 /* the generator's nuclei (the vessel tile is not produced), object for object
  * equal to tdb_generate's nuclei tile */
@@ -198,10 +206,13 @@ void tdb_source_free(tdb_source *s);
 typedef struct {
     uint64_t hbm_budget;        /* device bytes the join may hold: both tile slots and the
                                    join buffers (0 = 90% of the free memory) */
-    double tile_share;          /* share of the budget for the two tile slots, in (0, 1) */
+    double tile_share;          /* share of the budget for the workers' tile slots, in (0, 1) */
     uint64_t max_chunk_targets; /* most targets per chunk (0 = as many as fit a slot) */
-    int decode_threads;         /* host threads decoding (0 = all) */
-    int overlap;                /* 1: the decode, copy and join stages overlap; 0: in turn */
+    int decode_threads;         /* host threads decoding, shared by the workers (0 = all) */
+    int overlap;                /* 1: `workers` workers side by side; 0: one, stages in turn */
+    int lazy;                   /* 1: records shipped per LOD round for the voxels it reads
+                                   (default); 0: every record of every chunk object */
+    int workers;                /* workers when overlapping (0 = 3) */
 } tdb_stream_opts;
 
 void tdb_stream_defaults(tdb_stream_opts *o);
@@ -217,6 +228,12 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *s1, const tdb_source *s2,
  * boxes), [9] candidate ms, [10] objects decoded, [11] chunk joins split to
  * fit the budget */
 void tdb_result_stream(const tdb_result *r, double *out12);
+/* the same figures and more, n of them (at most 16; returns how many): [0..11]
+ * as above with [2] decode ms including the per-round record decodes and [3]
+ * copy ms including the per-round record puts, then [12] record bytes of the
+ * chunks' objects (what shipping every record would copy), [13] record bytes
+ * shipped, [14] voxel and object array bytes shipped, [15] workers */
+int tdb_result_stream_n(const tdb_result *r, double *out, int n);
 
 /* ---- serving: queries against one resident tile, continuously batched ----
  *

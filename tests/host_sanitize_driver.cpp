@@ -108,20 +108,55 @@ int main(int argc, char **argv) {
         }
     }
 
-    // the object source decodes the generator's objects
+    // the object source decodes the generator's objects; a decode without
+    // records plus one decode per LOD slot gives the whole decode exactly
+    // (the generator's source, a tile's, and the generic forms)
     auto src = tdb::nucleus_source(g);
     CHECK(src->n_obj() == n.n_obj);
-    for (size_t o = 0; o < src->n_obj(); o++) {
-        const uint32_t nv = src->n_vox(o);
-        const uint64_t nt = src->n_tri(o);
-        std::vector<float> box(9 * nv), hd(10), rec(12 * nt);
-        std::vector<uint32_t> cnt(5 * nv);
-        tdb::ObjArrays a{box.data(), cnt.data(), hd.data(), rec.data()};
-        src->decode(o, a);
-        float b6[6], tr[3];
-        src->meta(o, b6, tr);
-        for (int i = 0; i < 6; i++) CHECK(b6[i] == n.obj_box[6 * o + i]);
-    }
+    auto ts = tdb::tile_source(n);
+    struct WholeOnly final : tdb::ObjectSource {  // only decode(): the generic forms
+        const tdb::ObjectSource &s;
+        explicit WholeOnly(const tdb::ObjectSource &x) : s(x) {}
+        size_t n_obj() const override { return s.n_obj(); }
+        uint32_t n_vox(size_t o) const override { return s.n_vox(o); }
+        uint64_t n_tri(size_t o) const override { return s.n_tri(o); }
+        void meta(size_t o, float b[6], float t[3]) const override { s.meta(o, b, t); }
+        void decode(size_t o, const tdb::ObjArrays &a) const override { s.decode(o, a); }
+    } whole(*src);
+    const tdb::ObjectSource *sources[3] = {src.get(), ts.get(), &whole};
+    for (const tdb::ObjectSource *S : sources)
+        for (size_t o = 0; o < S->n_obj(); o++) {
+            const uint32_t nv = S->n_vox(o);
+            const uint64_t nt = S->n_tri(o);
+            std::vector<float> box(9 * nv), hd(10), rec(12 * nt);
+            std::vector<uint32_t> cnt(5 * nv);
+            tdb::ObjArrays a{box.data(), cnt.data(), hd.data(), rec.data()};
+            S->decode(o, a);
+            float b6[6], tr[3];
+            S->meta(o, b6, tr);
+            for (int i = 0; i < 6; i++) CHECK(b6[i] == n.obj_box[6 * o + i]);
+            std::vector<float> box2(9 * nv), hd2(10), rec2(12 * nt, -1.f);
+            std::vector<uint32_t> cnt2(5 * nv);
+            S->decode_meta(o, tdb::ObjArrays{box2.data(), cnt2.data(), hd2.data(), nullptr});
+            CHECK(box2 == box && cnt2 == cnt && hd2 == hd);
+            size_t at = 0;
+            for (int s = 0; s < tdb::kLods; s++) {
+                S->decode_slot(o, s, rec2.data() + 12 * at);
+                for (uint32_t v = 0; v < nv; v++) at += cnt[5 * v + s];
+            }
+            CHECK(at == nt && rec2 == rec);
+            // the same with the decode state kept from the meta decode
+            std::vector<float> state(S->state_floats(o)), rec3(12 * nt, -1.f);
+            S->decode_meta(o, tdb::ObjArrays{box2.data(), cnt2.data(), hd2.data(), nullptr},
+                           state.empty() ? nullptr : state.data());
+            CHECK(box2 == box && cnt2 == cnt && hd2 == hd);
+            at = 0;
+            for (int s = 0; s < tdb::kLods; s++) {
+                S->decode_slot(o, s, rec3.data() + 12 * at, state.empty() ? nullptr : state.data());
+                for (uint32_t v = 0; v < nv; v++) at += cnt[5 * v + s];
+            }
+            CHECK(rec3 == rec);
+        }
 
     // octree queries over the nuclei (self-join) and the vessels
     for (const tdb::Tile *t : {&n, &v}) {
