@@ -125,6 +125,7 @@ def _load():
     tdb.tdb_server_stats.argtypes = [_P, _P]
     tdb.tdb_server_close.argtypes = [_P]
     tdb.tdb_server_stage_ms.argtypes = [_P, _P]
+    tdb.tdb_server_figures.argtypes = [_P, _P, _c.c_int]
     tdb.tdb_source_generate.argtypes = [_c.POINTER(_GenParams), _c.POINTER(_P)]
     tdb.tdb_source_tile.argtypes = [_P, _c.POINTER(_P)]
     tdb.tdb_source_objects.argtypes = [_P]
@@ -517,7 +518,8 @@ class Engine:
 
 class _ServerOpts(_c.Structure):
     _fields_ = [("device", _c.c_int), ("hbm_budget", _c.c_uint64), ("max_batch", _c.c_int),
-                ("overlap", _c.c_int), ("precision", _c.c_int), ("threads", _c.c_int)]
+                ("overlap", _c.c_int), ("precision", _c.c_int), ("threads", _c.c_int),
+                ("lazy", _c.c_int), ("executors", _c.c_int)]
 
 
 class Server:
@@ -527,12 +529,13 @@ class Server:
 
     def __init__(self, resident: Tile, device: int = 0, hbm_budget: int = 0,
                  max_batch: int = 64, overlap: bool = True, precision: str = "mixed",
-                 threads: int = 0):
+                 threads: int = 0, lazy: bool = True, executors: int = 0):
         tdb, _ = _load()
         o = _ServerOpts()
         tdb.tdb_server_defaults(_c.byref(o))
         o.device, o.hbm_budget, o.max_batch = int(device), int(hbm_budget), int(max_batch)
         o.overlap, o.precision, o.threads = int(bool(overlap)), PREC[precision], int(threads)
+        o.lazy, o.executors = int(bool(lazy)), int(executors)
         self._resident = resident  # must outlive the server
         h = _P()
         rc = tdb.tdb_server_open(_c.byref(o), resident._h, _c.byref(h))
@@ -571,7 +574,13 @@ class Server:
         tdb.tdb_server_stage_ms(self._h, _ptr(ms))
         out = dict(zip(["batches", "queries", "max_batch", "peak_device_bytes"],
                        (int(x) for x in st)))
-        out["stage_ms"] = dict(zip(["merge", "upload", "candidates", "join"], ms.tolist()))
+        fg = np.zeros(12, np.float64)
+        tdb.tdb_server_figures(self._h, _ptr(fg), 12)
+        out["stage_ms"] = dict(zip(["merge", "upload", "candidates", "join", "record_copy",
+                                    "record_put"], fg[:6].tolist()))
+        out["h2d_meta_bytes"], out["h2d_record_bytes"] = int(fg[6]), int(fg[7])
+        out["record_bytes_of_queries"] = int(fg[8])
+        out["executors"], out["most_batches_at_once"] = int(fg[9]), int(fg[10])
         return out
 
     def close(self):

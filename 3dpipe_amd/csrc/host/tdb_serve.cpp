@@ -2,31 +2,37 @@
 // queries (fresh target tiles) against one resident tile, with continuous
 // batching.
 //
-// Two threads and two device contexts (streams):
+// One preparer and E executors, each a device context (stream):
 //   preparer  takes the queries waiting at that moment -- as many as fit the
 //             device-memory budget and share the query type, LOD list and
 //             Hausdorff level -- merges their tiles, builds the candidates
 //             (per-target k / within distance) and uploads the merged tile on
 //             its own stream;
-//   executor  runs each prepared batch as ONE join on the other stream and
-//             splits the result lists back per query.
-// While the executor runs batch n, the preparer builds batch n+1 from the
-// queries that arrived meanwhile (host merge, octree, upload overlap the
-// kernels), and batch n+1 starts as soon as batch n ends. Each target's join
-// depends only on that target and the resident tile, so a query's results
-// are the same alone and in any mix.
+//   executor  runs a prepared batch as ONE join on its stream and splits the
+//             result lists back per query.
+// A query that arrives while batches run is admitted as soon as a tile slot
+// and the memory budget allow: the preparer builds its batch beside the
+// running joins and a free executor starts it at once, so the joins of
+// different batches overlap on the device (E + 1 tile slots: one per
+// executor and one being prepared). Each target's join depends only on that
+// target and the resident tile, so a query's results are the same alone and
+// in any mix.
 //
-// Uploads go into two device tiles reserved once (grown when a batch needs
-// more) that the preparer and the executor hand back and forth, through
-// pinned staging: the object and voxel arrays in one pinned block, the
-// triangle records in chunks copied by host threads into two pinned buffers
-// that alternate while the previous chunk's DMA runs.
+// By default a batch's tile goes up without its triangle records: before
+// each LOD round the executor's join lists the query voxels that round reads
+// and copies just their records at that LOD from the query tiles into pinned
+// memory for the device (tdp_join_lazy_needs / tdp_join_lazy_put), the
+// reference's per-LOD packing of live candidates' voxels
+// (KNNJoin.cpp:1584-1617). Otherwise every record goes up with the tile,
+// through pinned chunks copied by host threads while the previous chunk's
+// DMA runs.
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 #include <cstring>
 #include <algorithm>
 #include <condition_variable>
+#include <array>
 #include <deque>
 #include <thread>
 #include <unordered_map>
@@ -53,6 +59,7 @@ struct Batch {
     std::vector<int32_t> k_t;
     std::vector<float> tau_t;
     std::vector<size_t> first;  // first target of each query in the merged tile
+    std::vector<size_t> vfirst, tfirst;  // its first voxel and first record there
     uint64_t dev_bytes = 0;
     std::string err;
 };
@@ -80,7 +87,8 @@ bool same_kind(const Query &a, const Query &b) {
 
 // the batch's tile without its triangle records (those go to the device
 // straight from each query's buffer)
-void merge_tiles(const std::vector<Query> &qs, tdb::Tile &m, std::vector<size_t> &first) {
+void merge_tiles(const std::vector<Query> &qs, tdb::Tile &m, std::vector<size_t> &first,
+                 std::vector<size_t> &vfirst, std::vector<size_t> &tfirst) {
     size_t no = 0, nv = 0, nt = 0;
     for (auto &q : qs) {
         no += q.tile->t.n_obj;
@@ -102,6 +110,8 @@ void merge_tiles(const std::vector<Query> &qs, tdb::Tile &m, std::vector<size_t>
     for (auto &q : qs) {
         const tdb::Tile &t = q.tile->t;
         first.push_back(oo);
+        vfirst.push_back(vo);
+        tfirst.push_back(to);
         for (size_t i = 1; i <= t.n_obj; i++) m.obj_vox_off.push_back((uint32_t)(vo + t.obj_vox_off[i]));
         m.obj_box.insert(m.obj_box.end(), t.obj_box.begin(), t.obj_box.end());
         m.obj_transform.insert(m.obj_transform.end(), t.obj_transform.begin(), t.obj_transform.end());
@@ -120,7 +130,10 @@ void merge_tiles(const std::vector<Query> &qs, tdb::Tile &m, std::vector<size_t>
 
 struct tdb_server {
     tdb_server_opts opts{};
-    tdp_ctx *run_ctx = nullptr, *prep_ctx = nullptr;
+    int n_exec = 1;
+    tdp_ctx *prep_ctx = nullptr;
+    std::vector<tdp_ctx *> run_ctx;  // one per executor
+    std::vector<LazyFeed> feed;      // one per executor
     const tdb_tile *resident = nullptr;
     tdp_tile *res_dev = nullptr;
     uint64_t budget = 0, res_bytes = 0;
@@ -132,25 +145,30 @@ struct tdb_server {
     std::unordered_map<int64_t, Done> done;
     int64_t next_ticket = 0;
     uint64_t in_use = 0;        // bytes of prepared and running batches
+    int running = 0;            // batches being joined
     bool stop = false, prep_exit = false;
     std::string err;
     uint64_t n_batches = 0, n_queries = 0, max_batch_seen = 0, peak_bytes = 0;
+    int max_running = 0;
     // device tile slots (guarded by mu: busy) and the preparer's pinned staging
     static constexpr size_t kChunkTri = (size_t)1 << 20;  // records per staging chunk
-    tdp_tile *slot[2] = {nullptr, nullptr};
-    size_t slot_cap[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    bool slot_busy[2] = {false, false};
+    std::vector<tdp_tile *> slot;
+    std::vector<std::array<size_t, 3>> slot_cap;
+    std::vector<char> slot_busy;
     char *meta_pin = nullptr;
     size_t meta_cap = 0;
     float *stage[2] = {nullptr, nullptr};
     int upload(Batch &b);
     // host time of the stages (ms, summed over batches): merge, upload,
-    // candidates (preparer); join (executor)
-    double t_merge = 0, t_upload = 0, t_cand = 0, t_join = 0;
-    std::thread prep, exec;
+    // candidates (preparer); join, record packing and puts (executors);
+    // bytes to the device
+    double t_merge = 0, t_upload = 0, t_cand = 0, t_join = 0, t_pack = 0, t_put = 0;
+    uint64_t h2d_meta = 0, h2d_rec = 0, rec_all = 0;
+    std::thread prep;
+    std::vector<std::thread> exec;
 
     void preparer();
-    void executor();
+    void executor(int e);
 };
 
 // the merged tile into a free slot through pinned staging (preparer thread)
@@ -162,13 +180,18 @@ int tdb_server::upload(Batch &b) {
     int s = -1;
     {
         std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] { return !slot_busy[0] || !slot_busy[1]; });
-        s = slot_busy[0] ? 1 : 0;
-        slot_busy[s] = true;
+        cv.wait(lk, [&] {
+            for (char x : slot_busy)
+                if (!x) return true;
+            return false;
+        });
+        while (slot_busy[s + 1]) s++;
+        s++;
+        slot_busy[s] = 1;
     }
     b.slot = s;
     int rc = TDP_OK;
-    size_t *cap = slot_cap[s];
+    size_t *cap = slot_cap[s].data();
     if (!slot[s] || cap[0] < no || cap[1] < nv || cap[2] < nt) {
         if (slot[s]) tdp_tile_free(slot[s]);
         slot[s] = nullptr;
@@ -181,7 +204,7 @@ int tdb_server::upload(Batch &b) {
             return rc;
         }
     }
-    if (!stage[0]) {
+    if (!stage[0] && !opts.lazy) {
         for (int k = 0; k < 2 && rc == TDP_OK; k++)
             rc = tdp_host_alloc(prep_ctx, 48 * kChunkTri, (void **)&stage[k]);
         if (rc != TDP_OK) return rc;
@@ -217,6 +240,16 @@ int tdb_server::upload(Batch &b) {
     d.vox_tri_cnt = (const uint32_t *)(meta_pin + at[4]);
     d.obj_box = (const float *)(meta_pin + at[5]);
     d.obj_transform = (const float *)(meta_pin + at[6]);
+    h2d_meta += need;
+    rec_all += 48 * (uint64_t)nt;
+    if (opts.lazy) {
+        // the records come round by round, from the query tiles
+        rc = tdp_tile_fill_meta_async(slot[s], &d);
+        if (rc == TDP_OK) rc = tdp_sync(prep_ctx);
+        if (rc == TDP_OK) b.dev = slot[s];
+        return rc;
+    }
+    h2d_rec += 48 * (uint64_t)nt;
     rc = tdp_tile_fill_async(slot[s], &d);
     // triangle records: host threads copy chunk k while chunk k-1 is on the wire
     uint64_t mk[2] = {0, 0};
@@ -260,8 +293,10 @@ void tdb_server::preparer() {
                     cv.notify_all();
                     return;
                 }
-                // look ahead at most one batch; without overlap, only when idle
-                const bool room = opts.overlap ? ready.empty() : (ready.empty() && in_use == 0);
+                // batches in flight (running or ready) at most one more than
+                // the executors; without overlap, one at a time
+                const bool room = opts.overlap ? (int)ready.size() + running < n_exec + 1
+                                               : (ready.empty() && in_use == 0);
                 if (!pending.empty() && room) {
                     const Query &q = pending.front();
                     if (in_use == 0 || res_bytes + in_use + q.dev_bytes <= budget) break;
@@ -288,7 +323,7 @@ void tdb_server::preparer() {
         }
         // merge, candidates, upload: outside the lock, beside the running join
         auto t0 = Clock::now();
-        merge_tiles(b->qs, b->merged.t, b->first);
+        merge_tiles(b->qs, b->merged.t, b->first, b->vfirst, b->tfirst);
         const size_t N = b->merged.t.n_obj;
         b->k_t.resize(N);
         b->tau_t.resize(N);
@@ -330,7 +365,9 @@ void tdb_server::preparer() {
     }
 }
 
-void tdb_server::executor() {
+void tdb_server::executor(int e) {
+    tdp_ctx *ctx = run_ctx[e];
+    LazyFeed &f = feed[e];
     for (;;) {
         std::unique_ptr<Batch> b;
         {
@@ -339,6 +376,8 @@ void tdb_server::executor() {
             if (ready.empty()) return;
             b = std::move(ready.front());
             ready.pop_front();
+            running++;
+            max_running = std::max(max_running, running);
         }
         cv.notify_all();  // the preparer may form the next batch now
         tdb_result all;
@@ -350,17 +389,49 @@ void tdb_server::executor() {
             o.n_lods = (int)b->qs[0].lods.size();
             o.precision = opts.precision;
             o.max_objects1 = 0;
+            // query voxels' records at a round's slot, from the query tiles
+            const tdb::Tile &m = b->merged.t;
+            const Batch &B = *b;
+            f.vox_tri_cnt[0] = m.vox_tri_cnt.data();
+            f.vox_tri_cnt[1] = nullptr;
+            f.pack = [&](int side, int slot, const uint32_t *vox, size_t n, const uint64_t *off,
+                         float *out, std::string &err) -> bool {
+                if (side != 1) {
+                    err = "serving feed: the resident tile holds its records";
+                    return false;
+                }
+#ifdef _OPENMP
+                const int nth = opts.threads > 0 ? opts.threads : std::min(16, omp_get_max_threads());
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 256)
+#endif
+                for (long k = 0; k < (long)n; k++) {
+                    const size_t v = vox[k];
+                    const size_t q = std::upper_bound(B.vfirst.begin(), B.vfirst.end(), v) -
+                                     B.vfirst.begin() - 1;
+                    const size_t at = 5 * v + slot;
+                    const tdb::Tile &t = B.qs[q].tile->t;
+                    std::memcpy(out + 12 * off[k], t.tri.data() + 12 * (m.vox_tri_off[at] - B.tfirst[q]),
+                                48 * (size_t)m.vox_tri_cnt[at]);
+                }
+                return true;
+            };
+            const double pk0 = f.pack_ms, pt0 = f.put_ms;
+            const uint64_t pr0 = f.put_records;
             const auto t0 = Clock::now();
-            rc = tdb_run_join(run_ctx, b->dev, res_dev, &o, b->prefix, b->ids, b->k_t.data(),
-                              b->tau_t.data(), &all, msg);
+            rc = tdb_run_join(ctx, b->dev, res_dev, &o, b->prefix, b->ids, b->k_t.data(),
+                              b->tau_t.data(), &all, msg, opts.lazy ? &f : nullptr);
             const double ms = ms_since(t0);
             std::lock_guard<std::mutex> lk(mu);
-            t_join += ms;
+            t_join += ms - (f.pack_ms - pk0) - (f.put_ms - pt0);
+            t_pack += f.pack_ms - pk0;
+            t_put += f.put_ms - pt0;
+            h2d_rec += 48 * (f.put_records - pr0);
         }
         b->dev = nullptr;
-        if (b->slot >= 0) {
+        {
             std::lock_guard<std::mutex> lk(mu);
-            slot_busy[b->slot] = false;
+            if (b->slot >= 0) slot_busy[b->slot] = 0;
+            running--;
         }
         cv.notify_all();
         // per query results and latencies
@@ -376,9 +447,9 @@ void tdb_server::executor() {
                 tdb_result *r = new tdb_result();
                 r->n_targets = n;
                 // the joined lists are grouped by target in ascending order
-                const auto f = std::lower_bound(all.target.begin(), all.target.end(), (int32_t)lo);
-                const auto l = std::lower_bound(all.target.begin(), all.target.end(), (int32_t)hi);
-                for (size_t k = f - all.target.begin(); k < (size_t)(l - all.target.begin()); k++) {
+                const auto fi = std::lower_bound(all.target.begin(), all.target.end(), (int32_t)lo);
+                const auto la = std::lower_bound(all.target.begin(), all.target.end(), (int32_t)hi);
+                for (size_t k = fi - all.target.begin(); k < (size_t)(la - all.target.begin()); k++) {
                     r->target.push_back(all.target[k] - (int32_t)lo);
                     r->obj2.push_back(all.obj2[k]);
                     r->rmin.push_back(all.rmin[k]);
@@ -417,6 +488,8 @@ void tdb_server_defaults(tdb_server_opts *o) {
     o->overlap = 1;
     o->precision = TDP_PREC_MIXED;
     o->threads = 0;
+    o->lazy = 1;
+    o->executors = 0;
 }
 
 int tdb_server_open(const tdb_server_opts *o, const tdb_tile *resident, tdb_server **out) {
@@ -425,32 +498,45 @@ int tdb_server_open(const tdb_server_opts *o, const tdb_tile *resident, tdb_serv
     std::unique_ptr<tdb_server> s(new tdb_server());
     s->opts = *o;
     s->resident = resident;
-    int rc = tdp_open(o->device, &s->run_ctx);
+    s->n_exec = o->overlap ? (o->executors > 0 ? o->executors : 2) : 1;
+    s->run_ctx.assign(s->n_exec, nullptr);
+    s->feed.resize(s->n_exec);
+    int rc = TDP_OK;
+    for (int e = 0; e < s->n_exec && rc == TDP_OK; e++) rc = tdp_open(o->device, &s->run_ctx[e]);
     if (rc == TDP_OK) rc = tdp_open(o->device, &s->prep_ctx);
     uint64_t free_b = 0, total_b = 0;
-    if (rc == TDP_OK) rc = tdp_device_memory(s->run_ctx, &free_b, &total_b);
+    if (rc == TDP_OK) rc = tdp_device_memory(s->prep_ctx, &free_b, &total_b);
     const tdp_tile_desc d = tdb_desc_of(resident->t);
-    if (rc == TDP_OK) rc = tdp_tile_upload(s->run_ctx, &d, &s->res_dev);
+    if (rc == TDP_OK) rc = tdp_tile_upload(s->prep_ctx, &d, &s->res_dev);
     if (rc != TDP_OK) {
-        s->err = s->run_ctx ? tdp_last_error(s->run_ctx) : "tdp_open failed";
+        tdp_ctx *c = s->prep_ctx;
+        for (tdp_ctx *x : s->run_ctx)
+            if (!c && x) c = x;
+        s->err = c ? tdp_last_error(c) : "tdp_open failed";
         if (s->res_dev) tdp_tile_free(s->res_dev);
         if (s->prep_ctx) tdp_close(s->prep_ctx);
-        if (s->run_ctx) tdp_close(s->run_ctx);
-        s->run_ctx = s->prep_ctx = nullptr;
+        for (tdp_ctx *x : s->run_ctx)
+            if (x) tdp_close(x);
+        s->run_ctx.clear();
+        s->prep_ctx = nullptr;
         *out = s.release();  // so that the message can be read
         return rc;
     }
+    for (int e = 0; e < s->n_exec; e++) s->feed[e].ctx = s->run_ctx[e];
+    s->slot.assign(s->n_exec + 1, nullptr);
+    s->slot_cap.assign(s->n_exec + 1, {0, 0, 0});
+    s->slot_busy.assign(s->n_exec + 1, 0);
     s->res_bytes = tdp_tile_bytes(s->res_dev);
     s->budget = o->hbm_budget ? o->hbm_budget : free_b / 10 * 9 + s->res_bytes;
     tdb_server *p = s.get();
     s->prep = std::thread([p] { p->preparer(); });
-    s->exec = std::thread([p] { p->executor(); });
+    for (int e = 0; e < s->n_exec; e++) s->exec.emplace_back([p, e] { p->executor(e); });
     *out = s.release();
     return TDP_OK;
 }
 
 int64_t tdb_server_submit(tdb_server *s, tdb_tile *query, const tdb_join_opts *o) {
-    if (!s || !query || !o || !s->run_ctx) return TDP_ERR_ARG;
+    if (!s || !query || !o || s->run_ctx.empty()) return TDP_ERR_ARG;
     if ((o->query != TDB_QUERY_NN && o->query != TDB_QUERY_WITHIN &&
          o->query != TDB_QUERY_INTERSECT) ||
         (o->query == TDB_QUERY_NN && o->k < 1)) {
@@ -519,6 +605,17 @@ void tdb_server_stage_ms(tdb_server *s, double *out4) {
     out4[3] = s->t_join;
 }
 
+int tdb_server_figures(tdb_server *s, double *out, int n) {
+    if (!s || !out || n < 0) return 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const double v[12] = {s->t_merge, s->t_upload, s->t_cand, s->t_join, s->t_pack, s->t_put,
+                          (double)s->h2d_meta, (double)s->h2d_rec, (double)s->rec_all,
+                          (double)s->n_exec, (double)s->max_running, (double)s->n_batches};
+    n = std::min(n, 12);
+    for (int k = 0; k < n; k++) out[k] = v[k];
+    return n;
+}
+
 void tdb_server_close(tdb_server *s) {
     if (!s) return;
     {
@@ -527,17 +624,21 @@ void tdb_server_close(tdb_server *s) {
     }
     s->cv.notify_all();
     if (s->prep.joinable()) s->prep.join();
-    if (s->exec.joinable()) s->exec.join();
+    for (auto &t : s->exec)
+        if (t.joinable()) t.join();
     for (auto &kv : s->done) delete kv.second.res;
     for (auto &q : s->pending) tdb_tile_free(q.tile);
-    for (int k = 0; k < 2; k++) {
-        if (s->slot[k]) tdp_tile_free(s->slot[k]);
+    for (tdp_tile *t : s->slot)
+        if (t) tdp_tile_free(t);
+    for (int k = 0; k < 2; k++)
         if (s->stage[k]) tdp_host_free(s->prep_ctx, s->stage[k]);
-    }
     if (s->meta_pin) tdp_host_free(s->prep_ctx, s->meta_pin);
+    for (LazyFeed &f : s->feed)
+        if (f.ctx) f.release();
     if (s->res_dev) tdp_tile_free(s->res_dev);
     if (s->prep_ctx) tdp_close(s->prep_ctx);
-    if (s->run_ctx) tdp_close(s->run_ctx);
+    for (tdp_ctx *x : s->run_ctx)
+        if (x) tdp_close(x);
     delete s;
 }
 

@@ -500,12 +500,19 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         budget = fr / 10 * 9;
     }
     const bool lazy = so->lazy != 0;
-    const int n_workers = so->overlap ? std::max(1, so->workers > 0 ? so->workers : 3) : 1;
-    const uint64_t slot_bytes = (uint64_t)(budget * so->tile_share / n_workers);
+    // fewer workers (larger tile slots) when a target with its candidates
+    // does not fit the slot of as many as were asked for
+    int n_workers = so->overlap ? std::max(1, so->workers > 0 ? so->workers : 3) : 1;
     const uint64_t max_targets = so->max_chunk_targets ? so->max_chunk_targets : UINT64_MAX;
     std::vector<Chunk> chunks;
-    if (!plan_chunks(s1, s2, self, order, prefix, ids, slot_bytes, max_targets, chunks, e->err))
-        return TDP_ERR_NOMEM;
+    for (;; n_workers--) {
+        const uint64_t slot_bytes = (uint64_t)(budget * so->tile_share / n_workers);
+        chunks.clear();
+        e->err.clear();
+        if (plan_chunks(s1, s2, self, order, prefix, ids, slot_bytes, max_targets, chunks, e->err))
+            break;
+        if (n_workers == 1) return TDP_ERR_NOMEM;
+    }
     {
         std::vector<uint32_t> kpos(s1.n_obj(), 0);
         for (size_t k = 0; k < N; k++) kpos[order[k]] = (uint32_t)k;

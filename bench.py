@@ -170,9 +170,11 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
     tiles of log-uniform size 100..50000 (seeded), each kNN with k in
     {1, 5, 10, 20} or within with tau U(50, 500), arriving as a Poisson
     process (unit-rate gaps scaled to each of `rates` queries/s). At each rate
-    run twice on the same schedule: the batched mixed server and
-    one-query-at-a-time plain FP64. Returns, per rate, latency percentiles,
-    throughput and whether every query's results agree."""
+    run on the same schedule: the batched mixed server (records per LOD
+    round, batches joined side by side), the same uploading every record
+    with its tile, and one-query-at-a-time plain FP64 (every record
+    uploaded). Returns, per rate, latency percentiles, throughput, bytes sent
+    to the device and whether every query's results agree."""
     import threading
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -196,8 +198,9 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
     log_fn(f"serving: {n_queries} query tiles ({n_nuclei} nuclei) generated in "
            f"{time.perf_counter() - t0:.1f} s")
 
-    def run(precision, max_batch, overlap, gaps):
-        srv = tdp.Server(vessels, max_batch=max_batch, overlap=overlap, precision=precision)
+    def run(precision, max_batch, overlap, gaps, lazy=True):
+        srv = tdp.Server(vessels, max_batch=max_batch, overlap=overlap, precision=precision,
+                         lazy=lazy)
         copies = [t.clone() for t in tiles]  # each server owns what it is given
         tickets = [None] * n_queries
         start = time.perf_counter()
@@ -227,6 +230,11 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
                      "mean_ms": float(lat.mean()), "queries_per_s": n_queries / makespan,
                      "nuclei_per_s": n_nuclei / makespan, "batches": st["batches"],
                      "max_batch": st["max_batch"], "peak_device_gb": st["peak_device_bytes"] / 1e9,
+                     "executors": st["executors"],
+                     "most_batches_at_once": st["most_batches_at_once"],
+                     "h2d_bytes": st["h2d_meta_bytes"] + st["h2d_record_bytes"],
+                     "h2d_record_bytes": st["h2d_record_bytes"],
+                     "record_bytes_of_queries": st["record_bytes_of_queries"],
                      "host_stage_ms": st["stage_ms"], "makespan_s": makespan}
 
     out = {"queries": n_queries, "nuclei": n_nuclei,
@@ -237,15 +245,21 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
         res_b, batched = run("mixed", 64, True, gaps)
         log_fn(f"serving {rate:g} q/s, batched mixed: p50 {batched['p50_ms']:.1f} ms, p99 "
                f"{batched['p99_ms']:.1f} ms, {batched['queries_per_s']:.1f} q/s, "
-               f"{batched['batches']} batches")
-        res_s, single = run("fp64", 1, False, gaps)
+               f"{batched['batches']} batches (largest {batched['max_batch']}), "
+               f"{batched['most_batches_at_once']} at once, h2d {batched['h2d_bytes'] / 1e9:.1f} GB")
+        res_e, every = run("mixed", 64, True, gaps, lazy=False)
+        log_fn(f"serving {rate:g} q/s, batched mixed, every record uploaded: p50 "
+               f"{every['p50_ms']:.1f} ms, p99 {every['p99_ms']:.1f} ms, "
+               f"{every['queries_per_s']:.1f} q/s, h2d {every['h2d_bytes'] / 1e9:.1f} GB")
+        res_s, single = run("fp64", 1, False, gaps, lazy=False)
         log_fn(f"serving {rate:g} q/s, one-at-a-time FP64: p50 {single['p50_ms']:.1f} ms, p99 "
                f"{single['p99_ms']:.1f} ms, {single['queries_per_s']:.1f} q/s")
         same = all(np.array_equal(a.target, b.target) and np.array_equal(a.obj2, b.obj2)
                    and np.array_equal(a.rmin, b.rmin) and np.array_equal(a.rmax, b.rmax)
-                   for a, b in zip(res_b, res_s))
+                   for r in (res_b, res_e) for a, b in zip(r, res_s))
         out["by_rate"].append({
-            "arrival_rate_per_s": rate, "batched_mixed": batched, "one_at_a_time_fp64": single,
+            "arrival_rate_per_s": rate, "batched_mixed": batched,
+            "batched_mixed_every_record": every, "one_at_a_time_fp64": single,
             "p50_speedup": single["p50_ms"] / batched["p50_ms"],
             "p99_speedup": single["p99_ms"] / batched["p99_ms"],
             "throughput_speedup": batched["queries_per_s"] / single["queries_per_s"],

@@ -212,7 +212,8 @@ typedef struct {
     int overlap;                /* 1: `workers` workers side by side; 0: one, stages in turn */
     int lazy;                   /* 1: records shipped per LOD round for the voxels it reads
                                    (default); 0: every record of every chunk object */
-    int workers;                /* workers when overlapping (0 = 3) */
+    int workers;                /* workers when overlapping (0 = 3); fewer when a target with
+                                   its candidates would not fit their tile slots */
 } tdb_stream_opts;
 
 void tdb_stream_defaults(tdb_stream_opts *o);
@@ -239,12 +240,16 @@ int tdb_result_stream_n(const tdb_result *r, double *out, int n);
  *
  * No counterpart in the reference (which runs one join per process). A
  * server keeps `resident` (tile 2 of every query) on the device and joins
- * each submitted query tile (tile 1) against it. Queries waiting when the
- * device frees up are merged into one join -- those sharing the query type,
+ * each submitted query tile (tile 1) against it. Queries waiting when a tile
+ * slot frees up are merged into one join -- those sharing the query type,
  * LOD list and Hausdorff level, as many as fit the device-memory budget; k and
- * the within distance may differ per query -- and the next batch is prepared
- * (tile merge, octree candidates, upload) on a second stream while one runs.
- * A query's result is the same alone and in any batch. */
+ * the within distance may differ per query. Batches are prepared (tile merge,
+ * octree candidates, upload) on their own stream while others run, and up to
+ * `executors` batches are joined at once, each on its own stream, so a query
+ * that arrives while others are on the device starts beside them. With
+ * `lazy` a batch's triangle records go to the device round by round, only
+ * for the query voxels each LOD round reads. A query's result is the same
+ * alone and in any batch. */
 typedef struct tdb_server tdb_server;
 typedef struct {
     int device;
@@ -253,7 +258,10 @@ typedef struct {
     int max_batch;        /* most queries in one join (1 = one at a time) */
     int overlap;          /* 1: prepare the next batch while one runs */
     int precision;        /* TDP_PREC_* of tdp.h */
-    int threads;          /* host threads for candidates (0 = all) */
+    int threads;          /* host threads for candidates and record copies (0 = all) */
+    int lazy;             /* 1: query records per LOD round for the voxels it reads
+                             (default); 0: every record with the tile */
+    int executors;        /* batches joined at once when overlapping (0 = 2) */
 } tdb_server_opts;
 
 void tdb_server_defaults(tdb_server_opts *o);
@@ -273,6 +281,13 @@ void tdb_server_stats(tdb_server *s, uint64_t *out4);
 /* host time (ms, summed over batches) of [0] tile merge, [1] upload, [2]
  * candidates (the preparer) and [3] joins (the executor) */
 void tdb_server_stage_ms(tdb_server *s, double *out4);
+/* more figures, n of them (at most 12; returns how many): [0..3] as
+ * tdb_server_stage_ms with [3] excluding [4] the record copies into pinned
+ * memory and [5] the record puts (executors, ms), [6] bytes of voxel and
+ * object arrays and [7] of triangle records sent to the device, [8] record
+ * bytes of the queries' tiles (what shipping every record would send),
+ * [9] executors, [10] most batches joined at once, [11] batches run */
+int tdb_server_figures(tdb_server *s, double *out, int n);
 /* finishes the queued queries, then stops */
 void tdb_server_close(tdb_server *s);
 

@@ -33,8 +33,11 @@ def vessels(tdp):
     return v
 
 
-@pytest.mark.parametrize("precision,max_batch,overlap", [("mixed", 64, True), ("fp64", 1, False)])
-def test_server_matches_joins_alone(tdp, vessels, precision, max_batch, overlap):
+@pytest.mark.parametrize("precision,max_batch,overlap,lazy,executors", [
+    ("mixed", 64, True, True, 0), ("mixed", 64, True, False, 0), ("mixed", 64, True, True, 3),
+    ("mixed", 2, True, True, 1), ("fp64", 1, False, True, 0), ("fp64", 1, False, False, 0),
+    ("ref32", 64, True, True, 2)])
+def test_server_matches_joins_alone(tdp, vessels, precision, max_batch, overlap, lazy, executors):
     eng = tdp.Engine(0)
     alone = []
     for i, (size, q, k, tau) in enumerate(QUERIES):
@@ -43,7 +46,8 @@ def test_server_matches_joins_alone(tdp, vessels, precision, max_batch, overlap)
         alone.append(eng.join(t, vessels, query=q, knn=k, within_dist=tau, precision=precision))
     eng.close()
     tiles = [_query_tile(tdp, i, size) for i, (size, _, _, _) in enumerate(QUERIES)]
-    srv = tdp.Server(vessels, max_batch=max_batch, overlap=overlap, precision=precision)
+    srv = tdp.Server(vessels, max_batch=max_batch, overlap=overlap, precision=precision,
+                     lazy=lazy, executors=executors)
     # submitted back to back: the queries that wait together share joins
     tickets = [srv.submit(tiles[i], query=q, knn=k, within_dist=tau)
                for i, (size, q, k, tau) in enumerate(QUERIES)]
@@ -54,12 +58,19 @@ def test_server_matches_joins_alone(tdp, vessels, precision, max_batch, overlap)
         assert _same(res, alone[i]), f"query {i} differs from its join alone"
     st = srv.stats()
     srv.close()
+    print(st)
     assert st["queries"] == len(QUERIES)
+    assert st["executors"] == (1 if not overlap else (executors or 2))
     if max_batch == 1:
         assert st["batches"] == len(QUERIES) and st["max_batch"] == 1
     else:
         # all queries were waiting together: at least two shared a join
         assert st["max_batch"] >= 2 and st["batches"] < len(QUERIES)
+    # lazily only the records of voxels some round read went over
+    if lazy:
+        assert 0 < st["h2d_record_bytes"] < st["record_bytes_of_queries"]
+    else:
+        assert st["h2d_record_bytes"] == st["record_bytes_of_queries"]
 
 
 def test_server_budget_rejects_oversized_query(tdp, vessels):
