@@ -487,7 +487,7 @@ class Engine:
         """Chunked join within a device budget (tdb_stream_join); the result
         equals join() on the same objects, with the streaming figures in
         .stream. lazy: records shipped per LOD round for the voxels it reads;
-        workers: chunk workers side by side (0 = 3; 1 without overlap)."""
+        workers: chunk workers side by side (0 = 4; 1 without overlap)."""
         tdb, _ = _load()
         o, keep = _opts(query, knn, within_dist, lods, hausdorff_level, max_objects1, precision,
                         threads)

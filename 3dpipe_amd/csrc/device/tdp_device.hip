@@ -145,6 +145,8 @@ struct tdp_join {
     int lz_ready[2] = {-1, -1};
     uint32_t *lz_host[2] = {nullptr, nullptr};
     size_t lz_host_cap[2] = {0, 0};
+    float4 *lz_stage = nullptr;  // device copy of the records being put
+    size_t lz_stage_cap = 0;     // records
 };
 
 // ------------------------------------------------------------ helpers
@@ -1348,6 +1350,7 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     if (j->gexec) (void)hipGraphExecDestroy(j->gexec);
     for (int k = 0; k < 2; k++)
         if (j->lz_host[k]) (void)tdp_host_free(ctx, j->lz_host[k]);
+    pfree(ctx, j->lz_stage);
     delete j;
     return TDP_OK;
 }
@@ -2062,10 +2065,9 @@ extern "C" int tdp_join_lazy_needs(tdp_join *j, int lod, int side, const uint32_
         if (t) t->lz_slot = slot;
         return TDP_OK;
     }
-    if (t->lz_pending) {
-        ctx->err = "tdp_join_lazy_needs: voxels listed before were never put (fill the tile again)";
-        return TDP_ERR_STATE;
-    }
+    // voxels listed before and never put are simply listed again (they are
+    // marked held only once their records are in place)
+    t->lz_pending = 0;
     hipStream_t s = ctx->stream;
     const int mask = side == 1 ? (j->t2 == j->t1 ? 3 : 1) : 2;
     k_fill_u32<<<blocks_for(t->n_vox, kBlock), kBlock, 0, s>>>(t->n_vox, t->lz_flag, 0u);
@@ -2078,7 +2080,7 @@ extern "C" int tdp_join_lazy_needs(tdp_join *j, int lod, int side, const uint32_
     if (rc != TDP_OK) return rc;
     if (n) {
         k_lazy_list<<<blocks_for(t->n_vox, kBlock), kBlock, 0, s>>>(
-            t->n_vox, slot, t->lz_flag, t->lz_pos, t->vox_tri_cnt, t->have, t->lz_list, t->lz_cnt);
+            t->n_vox, slot, t->lz_flag, t->lz_pos, t->vox_tri_cnt, t->lz_list, t->lz_cnt);
         TDP_LAUNCHED(ctx);
         rc = scan_exclusive<uint32_t>(ctx, t->lz_cnt, n, t->lz_off, &recs);
         if (rc != TDP_OK) return rc;
@@ -2129,13 +2131,32 @@ extern "C" int tdp_join_lazy_put(tdp_join *j, int lod, int side, const float *re
         ctx->err = "tdp_join_lazy_put: the records are not in pinned memory from tdp_host_alloc";
         return TDP_ERR_ARG;
     }
+    // the packed records go to device staging by DMA (the copy engines, at
+    // the link's rate) in pieces of at most kPiece records, each scattered
+    // into place from device memory
+    constexpr uint64_t kPiece = uint64_t(1) << 20;
+    const uint64_t recs = t->lz_records, cap = std::min(std::max<uint64_t>(recs, 1), kPiece);
     const size_t n = t->lz_pending;
-    k_lazy_put<<<stride_grid(n * kWave, 2048), kBlock, 0, ctx->stream>>>(
-        n, slot, t->lz_list, t->lz_off, t->vox_tri_off, t->vox_tri_cnt, (const float4 *)dev,
-        t->tri, t->bnd);
-    TDP_LAUNCHED(ctx);
-    t->put_records += t->lz_records;
-    t->lz_pending = 0;
+    t->lz_pending = 0;  // on failure below nothing is marked held
+    if (j->lz_stage_cap < cap) {
+        pfree(ctx, j->lz_stage);
+        j->lz_stage = nullptr;
+        j->lz_stage_cap = 0;
+        const int rc = palloc(ctx, &j->lz_stage, 3 * cap);
+        if (rc != TDP_OK) return rc;
+        j->lz_stage_cap = cap;
+    }
+    for (uint64_t p0 = 0; p0 < std::max<uint64_t>(recs, 1); p0 += kPiece) {
+        const uint64_t p1 = std::min(recs, p0 + kPiece);
+        if (p1 > p0)
+            TDP_CHECK(ctx, hipMemcpyAsync(j->lz_stage, records + 12 * p0, 48 * (p1 - p0),
+                                          hipMemcpyHostToDevice, ctx->stream));
+        k_lazy_put<<<stride_grid(n * kWave, 2048), kBlock, 0, ctx->stream>>>(
+            n, slot, t->lz_list, t->lz_off, t->vox_tri_off, t->vox_tri_cnt, j->lz_stage, p0, p1,
+            p1 >= recs ? 1 : 0, t->tri, t->bnd, t->have);
+        TDP_LAUNCHED(ctx);
+    }
+    t->put_records += recs;
     j->lz_ready[side - 1] = lod;
     return TDP_OK;
 }

@@ -534,13 +534,12 @@ __global__ void __launch_bounds__(kBlock) k_lazy_flag(size_t n, const uint32_t *
     }
 }
 
-// the flagged voxels in voxel order (pos: exclusive scan of flag), now held,
-// with their record counts at `slot`
+// the flagged voxels in voxel order (pos: exclusive scan of flag) with their
+// record counts at `slot`
 __global__ void __launch_bounds__(kBlock) k_lazy_list(size_t n_vox, int slot,
                                                       const uint32_t *__restrict__ flag,
                                                       const uint64_t *__restrict__ pos,
                                                       const uint32_t *__restrict__ vox_tri_cnt,
-                                                      uint32_t *__restrict__ have,
                                                       uint32_t *__restrict__ list,
                                                       uint32_t *__restrict__ cnt) {
     const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -548,21 +547,23 @@ __global__ void __launch_bounds__(kBlock) k_lazy_list(size_t n_vox, int slot,
     const uint64_t k = pos[v];
     list[k] = (uint32_t)v;
     cnt[k] = vox_tri_cnt[5 * v + slot];
-    have[v] |= 1u << slot;
 }
 
-// Listed voxels' records from `packed` (host memory the device reads: each
-// voxel's records at `slot`, voxel after voxel, voxel k's first at off[k])
-// into the tile's record array, with their bound records. One wave per
-// voxel, grid-stride.
+// Listed voxels' records into the tile's record array, with their bound
+// records: the packed records (each voxel's records at `slot`, voxel after
+// voxel, voxel k's first at off[k]) numbered [p0, p1) are in `piece`. The
+// last piece (last != 0) marks the listed voxels held. One wave per voxel,
+// grid-stride.
 __global__ void __launch_bounds__(kBlock) k_lazy_put(size_t n_list, int slot,
                                                      const uint32_t *__restrict__ list,
                                                      const uint64_t *__restrict__ off,
                                                      const uint64_t *__restrict__ vox_tri_off,
                                                      const uint32_t *__restrict__ vox_tri_cnt,
-                                                     const float4 *__restrict__ packed,
+                                                     const float4 *__restrict__ piece, uint64_t p0,
+                                                     uint64_t p1, int last,
                                                      float4 *__restrict__ tri,
-                                                     float4 *__restrict__ bnd) {
+                                                     float4 *__restrict__ bnd,
+                                                     uint32_t *__restrict__ have) {
     const unsigned lane = threadIdx.x % kWave;
     const size_t waves = (size_t)gridDim.x * (kBlock / kWave);
     for (size_t k = (size_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; k < n_list;
@@ -570,9 +571,14 @@ __global__ void __launch_bounds__(kBlock) k_lazy_put(size_t n_list, int slot,
         const uint32_t v = list[k];
         const uint64_t dst = vox_tri_off[5 * (size_t)v + slot], src = off[k];
         const uint32_t c = vox_tri_cnt[5 * (size_t)v + slot];
-        for (uint32_t t = lane; t < c; t += kWave) {
-            const float4 a = packed[3 * (src + t)], b = packed[3 * (src + t) + 1],
-                         cc = packed[3 * (src + t) + 2];
+        if (last && lane == 0) have[v] |= 1u << slot;
+        if (src >= p1 || src + c <= p0) continue;
+        // this voxel's records inside the piece
+        const uint32_t t0 = src < p0 ? (uint32_t)(p0 - src) : 0u;
+        const uint32_t t1 = src + c > p1 ? (uint32_t)(p1 - src) : c;
+        for (uint32_t t = t0 + lane; t < t1; t += kWave) {
+            const uint64_t q = src + t - p0;
+            const float4 a = piece[3 * q], b = piece[3 * q + 1], cc = piece[3 * q + 2];
             tri[3 * (dst + t)] = a;
             tri[3 * (dst + t) + 1] = b;
             tri[3 * (dst + t) + 2] = cc;

@@ -270,16 +270,23 @@ void decode_tile(const tdb::ObjectSource &s, const std::vector<uint32_t> &objs, 
     d.tri = meta_only ? nullptr : st.tri;
 }
 
-// one pipeline worker: a context (stream) with its tile slots, pinned
-// staging and record feed, and what it did
+// one chunk decoded on the host, per tile: pinned arrays, and (lazy) the
+// chunk-local object of each voxel and the objects' decode state
+struct Decoded {
+    Staging stg[2];
+    std::vector<uint32_t> vobj[2];
+    std::vector<float> state[2];
+    std::vector<uint64_t> state_off[2];
+};
+
+// one pipeline worker: a context (stream) with its tile slots, two decode
+// sets (the next chunk is decoded while the current one joins) and record
+// feed, and what it did
 struct Worker {
     tdp_ctx *ctx = nullptr;
     tdp_tile *tile[2] = {nullptr, nullptr};
-    Staging stg[2];
+    Decoded set[2];
     LazyFeed feed;
-    std::vector<uint32_t> vobj[2];  // chunk-local object of each voxel (lazy)
-    std::vector<float> state[2];    // the chunk objects' decode state (lazy)
-    std::vector<uint64_t> state_off[2];
     double decode_ms = 0, copy_ms = 0, join_ms = 0;
     uint64_t h2d = 0, n_decoded = 0, splits = 0, meta_bytes = 0, all_records = 0;
 };
@@ -502,7 +509,7 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     const bool lazy = so->lazy != 0;
     // fewer workers (larger tile slots) when a target with its candidates
     // does not fit the slot of as many as were asked for
-    int n_workers = so->overlap ? std::max(1, so->workers > 0 ? so->workers : 3) : 1;
+    int n_workers = so->overlap ? std::max(1, so->workers > 0 ? so->workers : 4) : 1;
     const uint64_t max_targets = so->max_chunk_targets ? so->max_chunk_targets : UINT64_MAX;
     std::vector<Chunk> chunks;
     for (;; n_workers--) {
@@ -538,7 +545,8 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
             if (w.ctx) w.feed.release();
             for (int k = 0; k < 2; k++) {
                 if (w.tile[k]) tdp_tile_free(w.tile[k]);
-                if (w.ctx) free_staging(w.ctx, w.stg[k]);
+                for (Decoded &d : w.set)
+                    if (w.ctx) free_staging(w.ctx, d.stg[k]);
             }
             if (w.ctx) tdp_close(w.ctx);
         }
@@ -552,8 +560,9 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         if (rc == TDP_OK) rc = tdp_set_tuning(w.ctx, &tune);
         for (int t = 0; t < n_tiles && rc == TDP_OK; t++) {
             rc = tdp_tile_reserve(w.ctx, cap_obj[t], cap_vox[t], cap_tri[t], &w.tile[t]);
-            if (rc == TDP_OK)
-                rc = alloc_staging(w.ctx, cap_obj[t], cap_vox[t], lazy ? 0 : cap_tri[t], w.stg[t]);
+            for (int b = 0; b < 2 && rc == TDP_OK; b++)
+                rc = alloc_staging(w.ctx, cap_obj[t], cap_vox[t], lazy ? 0 : cap_tri[t],
+                                   w.set[b].stg[t]);
         }
         uint64_t used = 0;
         if (w.ctx) tdp_memory_usage(w.ctx, &used, nullptr, 0);
@@ -582,28 +591,28 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     std::vector<tdb_result> parts(nc);
     const int nt_w = std::max(1, nt / n_workers);
 
-    // chunk i's arrays into worker w's staging: without records (lazy) or whole
-    auto decode = [&](size_t i, Worker &w) {
+    // chunk i's arrays into decode set D: without records (lazy) or whole;
+    // returns the host ms it took
+    auto decode = [&](size_t i, Decoded &D) -> double {
         const auto a = Clock::now();
         const Chunk &c = chunks[i];
         for (int t = 0; t < n_tiles; t++) {
-            decode_tile(t == 0 ? s1 : s2, c.objs[t], w.stg[t], nt_w, lazy, &w.state[t],
-                        &w.state_off[t]);
+            decode_tile(t == 0 ? s1 : s2, c.objs[t], D.stg[t], nt_w, lazy, &D.state[t],
+                        &D.state_off[t]);
             if (lazy) {
-                const Staging &st = w.stg[t];
-                w.vobj[t].resize(st.desc.n_vox);
+                const Staging &st = D.stg[t];
+                D.vobj[t].resize(st.desc.n_vox);
                 for (size_t o = 0; o < st.desc.n_obj; o++)
                     for (uint32_t v = st.obj_vox_off[o]; v < st.obj_vox_off[o + 1]; v++)
-                        w.vobj[t][v] = (uint32_t)o;
+                        D.vobj[t][v] = (uint32_t)o;
             }
         }
-        w.decode_ms += ms_since(a);
-        w.n_decoded += c.objs[0].size() + (self ? 0 : c.objs[1].size());
+        return ms_since(a);
     };
-    auto copy = [&](size_t i, Worker &w) -> bool {
+    auto copy = [&](size_t i, Worker &w, const Decoded &D) -> bool {
         const auto a = Clock::now();
         for (int t = 0; t < n_tiles; t++) {
-            const tdp_tile_desc &d = w.stg[t].desc;
+            const tdp_tile_desc &d = D.stg[t].desc;
             if ((lazy ? tdp_tile_fill_meta_async(w.tile[t], &d) : tdp_tile_fill_async(w.tile[t], &d)) !=
                 TDP_OK)
                 return false;
@@ -617,9 +626,9 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         return true;
     };
     // lazy: the records of a round's voxels, decoded per object at that slot
-    auto set_feed = [&](size_t i, Worker &w) {
-        w.feed.vox_tri_cnt[0] = w.stg[0].vox_tri_cnt;
-        w.feed.vox_tri_cnt[1] = self ? nullptr : w.stg[1].vox_tri_cnt;
+    auto set_feed = [&](size_t i, Worker &w, const Decoded &D) {
+        w.feed.vox_tri_cnt[0] = D.stg[0].vox_tri_cnt;
+        w.feed.vox_tri_cnt[1] = self ? nullptr : D.stg[1].vox_tri_cnt;
         w.feed.pack = [&, i](int side, int slot, const uint32_t *vox, size_t n, const uint64_t *off,
                              float *out, std::string &err) -> bool {
             const int t = side - 1;
@@ -629,8 +638,8 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
             }
             const tdb::ObjectSource &s = t == 0 ? s1 : s2;
             const std::vector<uint32_t> &objs = chunks[i].objs[t];
-            const Staging &st = w.stg[t];
-            const std::vector<uint32_t> &vo = w.vobj[t];
+            const Staging &st = D.stg[t];
+            const std::vector<uint32_t> &vo = D.vobj[t];
             // runs of one object in the (ascending) list
             std::vector<size_t> run{0};
             for (size_t k = 1; k < n; k++)
@@ -648,8 +657,8 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
                     const size_t a = run[r], b = run[r + 1];
                     const uint32_t o = vo[vox[a]];
                     const uint32_t v0 = st.obj_vox_off[o], nv = st.obj_vox_off[o + 1] - v0;
-                    const float *sto = w.state_off[t][o + 1] > w.state_off[t][o]
-                                           ? w.state[t].data() + w.state_off[t][o]
+                    const float *sto = D.state_off[t][o + 1] > D.state_off[t][o]
+                                           ? D.state[t].data() + D.state_off[t][o]
                                            : nullptr;
                     if (b - a == nv) {  // every voxel: the slot's block as decoded
                         s.decode_slot(objs[o], slot, out + 12 * off[a], sto);
@@ -669,11 +678,11 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
             return true;
         };
     };
-    auto join = [&](size_t i, Worker &w) -> bool {
+    auto join = [&](size_t i, Worker &w, const Decoded &D) -> bool {
         const auto a = Clock::now();
         const Chunk &c = chunks[i];
         tdp_tile *d1 = w.tile[0], *d2 = self ? d1 : w.tile[1];
-        if (lazy) set_feed(i, w);
+        if (lazy) set_feed(i, w, D);
         const double feed0 = w.feed.pack_ms + w.feed.put_ms;
         // target ranges [lo, hi); halved while a join passes the memory cap
         std::vector<std::pair<size_t, size_t>> todo{{0, c.targets.size()}};
@@ -722,17 +731,37 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
 
     const auto t_pipe = Clock::now();
     std::atomic<size_t> next{0};
+    // a worker decodes the next chunk it takes (on a helper thread) while it
+    // copies and joins the current one
     auto run_worker = [&](Worker &w) {
+        size_t cur = next.fetch_add(1);
+        if (cur >= nc) return;
+        w.decode_ms += decode(cur, w.set[0]);
+        int b = 0;
         for (;;) {
             if (pg.failed) return;
-            const size_t i = next.fetch_add(1);
-            if (i >= nc) return;
-            decode(i, w);
-            if (!copy(i, w)) {
-                pg.fail(std::string("stream join: copy: ") + tdp_last_error(w.ctx));
-                return;
+            const size_t nxt = so->overlap ? next.fetch_add(1) : nc;
+            double ahead_ms = 0;
+            std::thread helper;
+            if (nxt < nc)
+                helper = std::thread([&, nxt, b] { ahead_ms = decode(nxt, w.set[1 - b]); });
+            bool ok = copy(cur, w, w.set[b]);
+            if (!ok) pg.fail(std::string("stream join: copy: ") + tdp_last_error(w.ctx));
+            else ok = join(cur, w, w.set[b]);
+            if (helper.joinable()) helper.join();
+            w.decode_ms += ahead_ms;
+            w.n_decoded += chunks[cur].objs[0].size() + (self ? 0 : chunks[cur].objs[1].size());
+            if (!ok) return;
+            if (nxt >= nc) {
+                // without overlap: the next chunk in turn
+                if (so->overlap) return;
+                cur = next.fetch_add(1);
+                if (cur >= nc) return;
+                w.decode_ms += decode(cur, w.set[b]);
+                continue;
             }
-            if (!join(i, w)) return;
+            cur = nxt;
+            b = 1 - b;
         }
     };
     if (n_workers == 1) {

@@ -117,7 +117,7 @@ def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn, laz
                  workers=0):
     """Chunked streaming 10-NN self-join of n_nuclei nuclei (radii 6-14,
     uniform in the cube) within a device budget (tdb_stream_join): workers
-    side by side (three by default), each decoding a chunk's voxel and object arrays
+    side by side (four by default), each decoding a chunk's voxel and object arrays
     into pinned buffers and joining it; lazily (the default) only the records
     of the voxels each LOD round reads are decoded and shipped, otherwise
     every record of every chunk object."""

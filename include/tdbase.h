@@ -212,7 +212,7 @@ typedef struct {
     int overlap;                /* 1: `workers` workers side by side; 0: one, stages in turn */
     int lazy;                   /* 1: records shipped per LOD round for the voxels it reads
                                    (default); 0: every record of every chunk object */
-    int workers;                /* workers when overlapping (0 = 3); fewer when a target with
+    int workers;                /* workers when overlapping (0 = 4); fewer when a target with
                                    its candidates would not fit their tile slots */
 } tdb_stream_opts;
 

@@ -135,8 +135,8 @@ int tdp_tile_fill_tri_async(tdp_tile *tile, size_t first, size_t n, const float 
  * tdp_tile_fill_meta_async holds its voxel and object arrays but no
  * records; before each tdp_join_refine the caller asks, per side, for the
  * voxels that round reads and the tile does not hold yet, and hands their
- * records over packed in pinned memory, which the device reads while it
- * writes them into place (with their bound records).
+ * records over packed in pinned memory: one DMA takes them to device
+ * staging, and a kernel writes them into place with their bound records.
  *
  * Refill a reserved tile without its records (desc->tri ignored; arrays
  * pinned, as for tdp_tile_fill_async). */
@@ -146,14 +146,14 @@ int tdp_tile_fill_meta_async(tdp_tile *tile, const tdp_tile_desc *desc);
  * the join owns, valid until the next call for that side), *n_voxels of
  * them holding *n_records records at that LOD. A self-join lists its one
  * tile on side 1 (side 2 lists nothing). A tile with records lists nothing.
- * The listed voxels count as held from here on: the next call on the side
- * is tdp_join_lazy_put. */
+ * The next call on the side is tdp_join_lazy_put (the listed voxels count
+ * as held once it has run; listed again by a later call if it failed). */
 int tdp_join_lazy_needs(tdp_join *j, int lod, int side, const uint32_t **voxels,
                         size_t *n_voxels, uint64_t *n_records);
 /* The listed voxels' records at `lod`, voxel after voxel in list order, each
  * voxel's records in its (voxel, LOD) order: 12 floats per record, in
  * pinned memory from tdp_host_alloc that stays untouched until the next
- * tdp_join_refine on j has returned (the device reads it). tdp_join_refine
+ * tdp_join_refine on j has returned (the copy is queued). tdp_join_refine
  * fails with TDP_ERR_STATE on a side whose records for its LOD were not put. */
 int tdp_join_lazy_put(tdp_join *j, int lod, int side, const float *records);
 /* records a tile holds: all of them, or for a tile filled without records

@@ -56,7 +56,9 @@ def test_stream_self_join_equals_join(tdp, nuclei, query, k, tau, overlap, share
         "record_bytes_shipped", "meta_bytes", "h2d_bytes", "workers")})
     assert st["chunks"] > 4
     assert st["peak_device_bytes"] <= budget
-    assert st["workers"] == (1 if not overlap else (workers or 3))
+    # as many workers as asked for (4 by default), fewer where a target with
+    # its candidates would not fit their tile slots
+    assert 1 <= st["workers"] <= (1 if not overlap else (workers or 4))
     assert got.n_targets == want.n_targets == nuclei.n_obj
     assert len(got.target) > 0
     assert _same(got, want)
