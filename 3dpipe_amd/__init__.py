@@ -519,7 +519,7 @@ class Engine:
 class _ServerOpts(_c.Structure):
     _fields_ = [("device", _c.c_int), ("hbm_budget", _c.c_uint64), ("max_batch", _c.c_int),
                 ("overlap", _c.c_int), ("precision", _c.c_int), ("threads", _c.c_int),
-                ("lazy", _c.c_int), ("executors", _c.c_int)]
+                ("lazy", _c.c_int), ("executors", _c.c_int), ("mix_kinds", _c.c_int)]
 
 
 class Server:
@@ -529,13 +529,14 @@ class Server:
 
     def __init__(self, resident: Tile, device: int = 0, hbm_budget: int = 0,
                  max_batch: int = 64, overlap: bool = True, precision: str = "mixed",
-                 threads: int = 0, lazy: bool = True, executors: int = 0):
+                 threads: int = 0, lazy: bool = True, executors: int = 0,
+                 mix_kinds: bool = True):
         tdb, _ = _load()
         o = _ServerOpts()
         tdb.tdb_server_defaults(_c.byref(o))
         o.device, o.hbm_budget, o.max_batch = int(device), int(hbm_budget), int(max_batch)
         o.overlap, o.precision, o.threads = int(bool(overlap)), PREC[precision], int(threads)
-        o.lazy, o.executors = int(bool(lazy)), int(executors)
+        o.lazy, o.executors, o.mix_kinds = int(bool(lazy)), int(executors), int(bool(mix_kinds))
         self._resident = resident  # must outlive the server
         h = _P()
         rc = tdb.tdb_server_open(_c.byref(o), resident._h, _c.byref(h))
@@ -574,13 +575,14 @@ class Server:
         tdb.tdb_server_stage_ms(self._h, _ptr(ms))
         out = dict(zip(["batches", "queries", "max_batch", "peak_device_bytes"],
                        (int(x) for x in st)))
-        fg = np.zeros(12, np.float64)
-        tdb.tdb_server_figures(self._h, _ptr(fg), 12)
+        fg = np.zeros(13, np.float64)
+        tdb.tdb_server_figures(self._h, _ptr(fg), 13)
         out["stage_ms"] = dict(zip(["merge", "upload", "candidates", "join", "record_copy",
                                     "record_put"], fg[:6].tolist()))
         out["h2d_meta_bytes"], out["h2d_record_bytes"] = int(fg[6]), int(fg[7])
         out["record_bytes_of_queries"] = int(fg[8])
         out["executors"], out["most_batches_at_once"] = int(fg[9]), int(fg[10])
+        out["mixed_kind_batches"] = int(fg[12])
         return out
 
     def close(self):

@@ -92,6 +92,7 @@ struct tdp_join {
     uint32_t *prefix = nullptr, *pair_obj2 = nullptr, *pair_target = nullptr;
     int32_t *k_t = nullptr;  // per target: k (kNN)
     float *tau_t = nullptr;  // per target: within distance
+    uint8_t *within_t = nullptr;  // kNN and within targets in one join: 1 for within
     void *rmin = nullptr, *rmax = nullptr;
     uint8_t *state = nullptr, *round = nullptr, *scratch = nullptr;
     int32_t *confirmed = nullptr;
@@ -1327,7 +1328,7 @@ extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
 extern "C" int tdp_join_destroy(tdp_join *j) {
     if (!j) return TDP_ERR_ARG;
     tdp_ctx *ctx = j->ctx;
-    void *ptrs[] = {j->work, j->obj_u, j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
+    void *ptrs[] = {j->within_t, j->work, j->obj_u, j->prefix, j->pair_obj2, j->pair_target, j->k_t, j->tau_t, j->rmin, j->rmax, j->state,
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q,
@@ -1355,10 +1356,12 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
     return TDP_OK;
 }
 
-extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
-                                     const tdp_join_params *prm, size_t N, const uint32_t *prefix,
-                                     const uint32_t *cand, const int32_t *k_target,
-                                     const float *tau_target, tdp_join **out) {
+// within_target: NULL, or per target 1 (within) / 0 (kNN) in a join whose
+// params->query is kNN or within
+static int join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                       const tdp_join_params *prm, size_t N, const uint32_t *prefix,
+                       const uint32_t *cand, const int32_t *k_target, const float *tau_target,
+                       const uint8_t *within_target, tdp_join **out) {
     if (!ctx || !t1 || !t2 || !prm || !out || (N && !prefix)) return TDP_ERR_ARG;
     *out = nullptr;
     if (prm->query != TDP_QUERY_KNN && prm->query != TDP_QUERY_WITHIN &&
@@ -1366,9 +1369,17 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
         ctx->err = "tdp_join_create: bad query";
         return TDP_ERR_ARG;
     }
-    if (prm->query == TDP_QUERY_KNN) {
-        bool ok = k_target ? true : prm->k >= 1;
-        for (size_t t = 0; k_target && t < N; t++) ok = ok && k_target[t] >= 1;
+    if (within_target && prm->query == TDP_QUERY_INTERSECT) {
+        ctx->err = "tdp_join_create_kinds: kNN and within targets only";
+        return TDP_ERR_ARG;
+    }
+    if (prm->query == TDP_QUERY_KNN || within_target) {
+        bool ok = true;
+        for (size_t t = 0; t < N; t++) {
+            const bool knn = within_target ? within_target[t] == 0 : true;
+            if (knn) ok = ok && (k_target ? k_target[t] : prm->k) >= 1;
+        }
+        if (!within_target && !k_target) ok = ok && prm->k >= 1;
         if (!ok) {
             ctx->err = "tdp_join_create: k must be >= 1";
             return TDP_ERR_ARG;
@@ -1447,6 +1458,15 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
         e = copy_sync(ctx, j->k_t, kv.data(), 4 * N, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = copy_sync(ctx, j->tau_t, tv.data(), 4 * N, hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess && within_target && N) {
+        if (palloc(ctx, &j->within_t, N) != TDP_OK) {
+            tdp_join_destroy(j);
+            return TDP_ERR_NOMEM;
+        }
+        std::vector<uint8_t> wv(within_target, within_target + N);
+        for (auto &x : wv) x = x ? 1 : 0;
+        e = copy_sync(ctx, j->within_t, wv.data(), N, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = set_sync(ctx, j->stats, 0, 32);
     if (e == hipSuccess) e = set_sync(ctx, j->confirmed, 0, 4 * (N ? N : 1));
     if (e == hipSuccess) e = set_sync(ctx, j->round, 0xff, P ? P : 1);
@@ -1469,6 +1489,26 @@ extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp
     }
     *out = j;
     return TDP_OK;
+}
+
+extern "C" int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                                     const tdp_join_params *prm, size_t N, const uint32_t *prefix,
+                                     const uint32_t *cand, const int32_t *k_target,
+                                     const float *tau_target, tdp_join **out) {
+    return join_create(ctx, t1, t2, prm, N, prefix, cand, k_target, tau_target, nullptr, out);
+}
+
+extern "C" int tdp_join_create_kinds(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                                     const tdp_join_params *prm, size_t N, const uint32_t *prefix,
+                                     const uint32_t *cand, const uint8_t *within_target,
+                                     const int32_t *k_target, const float *tau_target,
+                                     tdp_join **out) {
+    if (ctx && N && !within_target) {
+        ctx->err = "tdp_join_create_kinds: null within_target";
+        return TDP_ERR_ARG;
+    }
+    return join_create(ctx, t1, t2, prm, N, prefix, cand, k_target, tau_target, within_target,
+                       out);
 }
 
 extern "C" int tdp_join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
@@ -1571,23 +1611,24 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
             j->t2->obj_vox_off, j->t2->vox_box, rmin, rmax);
         TDP_LAUNCHED(ctx);
-        if (mode == FILTER_KNN) {
-            if (N) {
-                k_filter_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
-                    N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed,
-                    j->scratch);
-                TDP_LAUNCHED(ctx);
-            }
-        } else {
+        // a join of kNN and within targets runs both rules, each on its own
+        if ((mode == FILTER_KNN || j->within_t) && N) {
+            k_filter_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
+                N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch,
+                j->within_t);
+            TDP_LAUNCHED(ctx);
+        }
+        if (mode == FILTER_WITHIN || j->within_t) {
             k_filter_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
-                P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round);
+                P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, j->within_t);
             TDP_LAUNCHED(ctx);
         }
     }
     if (P) {
         k_survivor_count<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
-            j->t2->obj_vox_off, j->t2->vox_box, rmax, mode, j->tau_t, j->state, j->round, j->count);
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, mode, j->tau_t, j->within_t, j->state,
+            j->round, j->count);
         TDP_LAUNCHED(ctx);
     }
     uint64_t total = 0;
@@ -1599,7 +1640,8 @@ static int join_filter_t(tdp_join *j, uint64_t *pairs_out) {
     if (total) {
         k_survivor_write<R><<<wave_grid(P), kBlock, 0, s>>>(
             P, j->pair_target, j->pair_obj2, j->t1->obj_vox_off, j->t1->vox_box,
-            j->t2->obj_vox_off, j->t2->vox_box, rmax, mode, j->tau_t, j->state, j->offs, j->v1[0],
+            j->t2->obj_vox_off, j->t2->vox_box, rmax, mode, j->tau_t, j->within_t, j->state,
+            j->offs, j->v1[0],
             j->v2[0], j->vpp[0], (R *)j->vmin[0], (R *)j->vmax[0]);
         TDP_LAUNCHED(ctx);
     }
@@ -1655,12 +1697,19 @@ static void launch_decide(tdp_join *j, uint8_t rnd) {
     hipStream_t s = j->ctx->stream;
     const size_t P = j->P, N = j->N;
     R *rmin = (R *)j->rmin, *rmax = (R *)j->rmax;
-    if (j->prm.query == TDP_QUERY_KNN)
+    if (j->within_t) {  // kNN and within targets: both rules, each on its own
         k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
-            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd);
+            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd,
+            j->within_t);
+        k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
+            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd, j->within_t);
+    } else if (j->prm.query == TDP_QUERY_KNN)
+        k_decide_knn<R><<<blocks_for(N, kBlock), kBlock, 0, s>>>(
+            N, j->prefix, rmin, rmax, j->k_t, j->state, j->round, j->confirmed, j->scratch, rnd,
+            nullptr);
     else if (j->prm.query == TDP_QUERY_WITHIN)
         k_decide_within<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
-            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd);
+            P, rmin, rmax, j->pair_target, j->tau_t, j->state, j->round, rnd, nullptr);
     else
         k_decide_intersect<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(
             P, rmin, rmax, j->prm.hausdorff_level, j->state, j->round, rnd);

@@ -208,9 +208,11 @@ __global__ void __launch_bounds__(kBlock) k_filter_knn(size_t N, const uint32_t 
                                                        uint8_t *__restrict__ state,
                                                        uint8_t *__restrict__ round,
                                                        int32_t *__restrict__ confirmed,
-                                                       uint8_t *__restrict__ scratch) {
+                                                       uint8_t *__restrict__ scratch,
+                                                       const uint8_t *__restrict__ within_t) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
+    if (within_t && within_t[t]) return;  // a within target of a mixed join
     const int K = k_t[t];
     const uint32_t b = prefix[t], e = prefix[t + 1];
     const int m = (int)(e - b);
@@ -248,9 +250,11 @@ __global__ void __launch_bounds__(kBlock) k_filter_within(size_t P, const R *__r
                                                           const uint32_t *__restrict__ pair_target,
                                                           const float *__restrict__ tau_t,
                                                           uint8_t *__restrict__ state,
-                                                          uint8_t *__restrict__ round) {
+                                                          uint8_t *__restrict__ round,
+                                                          const uint8_t *__restrict__ within_t) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
+    if (within_t && !within_t[pair_target[p]]) return;  // a kNN target of a mixed join
     const R tau = (R)tau_t[pair_target[p]];
     uint8_t st = ST_DEAD, rd = 255;
     if (rmin[p] > tau) st = ST_DEAD;
@@ -283,8 +287,8 @@ __global__ void __launch_bounds__(kBlock) k_survivor_count(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, const R *__restrict__ rmax,
-    int mode, const float *__restrict__ tau_t, uint8_t *__restrict__ state,
-    uint8_t *__restrict__ round, uint32_t *__restrict__ count) {
+    int mode, const float *__restrict__ tau_t, const uint8_t *__restrict__ within_t,
+    uint8_t *__restrict__ state, uint8_t *__restrict__ round, uint32_t *__restrict__ count) {
     const size_t p = wave_global();
     if (p >= P) return;
     const int lane = lane_id();
@@ -292,6 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_survivor_count(
         if (lane == 0) count[p] = 0;
         return;
     }
+    if (within_t) mode = within_t[pair_target[p]] ? FILTER_WITHIN : FILTER_KNN;
     const uint32_t a0 = vo1[pair_target[p]], n1 = vo1[pair_target[p] + 1] - a0;
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
     const uint32_t total = n1 * n2;
@@ -329,13 +334,14 @@ __global__ void __launch_bounds__(kBlock) k_survivor_write(
     size_t P, const uint32_t *__restrict__ pair_target, const uint32_t *__restrict__ pair_obj2,
     const uint32_t *__restrict__ vo1, const float *__restrict__ vb1,
     const uint32_t *__restrict__ vo2, const float *__restrict__ vb2, const R *__restrict__ rmax,
-    int mode, const float *__restrict__ tau_t, const uint8_t *__restrict__ state,
-    const uint64_t *__restrict__ offs,
+    int mode, const float *__restrict__ tau_t, const uint8_t *__restrict__ within_t,
+    const uint8_t *__restrict__ state, const uint64_t *__restrict__ offs,
     uint32_t *__restrict__ vp_v1, uint32_t *__restrict__ vp_v2, uint32_t *__restrict__ vp_pair,
     R *__restrict__ vp_min, R *__restrict__ vp_max) {
     const size_t p = wave_global();
     if (p >= P) return;
     if (state[p] != ST_ALIVE) return;
+    if (within_t) mode = within_t[pair_target[p]] ? FILTER_WITHIN : FILTER_KNN;
     const int lane = lane_id();
     const uint32_t a0 = vo1[pair_target[p]], n1 = vo1[pair_target[p] + 1] - a0;
     const uint32_t b0 = vo2[pair_obj2[p]], n2 = vo2[pair_obj2[p] + 1] - b0;
@@ -925,9 +931,11 @@ __global__ void __launch_bounds__(kBlock) k_decide_knn(size_t N, const uint32_t 
                                                        uint8_t *__restrict__ state,
                                                        uint8_t *__restrict__ round,
                                                        int32_t *__restrict__ confirmed,
-                                                       uint8_t *__restrict__ scratch, uint8_t rnd) {
+                                                       uint8_t *__restrict__ scratch, uint8_t rnd,
+                                                       const uint8_t *__restrict__ within_t) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
+    if (within_t && within_t[t]) return;  // a within target of a mixed join
     const int K = k_t[t];
     const uint32_t b = prefix[t], e = prefix[t + 1];
     bool any = false;
@@ -1002,10 +1010,12 @@ __global__ void __launch_bounds__(kBlock) k_decide_within(size_t P, const R *__r
                                                           const uint32_t *__restrict__ pair_target,
                                                           const float *__restrict__ tau_t,
                                                           uint8_t *__restrict__ state,
-                                                          uint8_t *__restrict__ round, uint8_t rnd) {
+                                                          uint8_t *__restrict__ round, uint8_t rnd,
+                                                          const uint8_t *__restrict__ within_t) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     if (state[p] != ST_ALIVE) return;
+    if (within_t && !within_t[pair_target[p]]) return;  // a kNN target of a mixed join
     const R tau = (R)tau_t[pair_target[p]];
     if (rmax[p] <= tau) { state[p] = ST_REPORTED; round[p] = rnd; }
     else if (rmin[p] > tau) state[p] = ST_DEAD;

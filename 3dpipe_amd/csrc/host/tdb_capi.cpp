@@ -89,7 +89,7 @@ static int feed_round(tdp_ctx *ctx, tdp_join *j, int lod, LazyFeed &f, std::stri
 int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *o,
                  const std::vector<uint32_t> &prefix, const std::vector<uint32_t> &ids,
                  const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err,
-                 LazyFeed *feed) {
+                 LazyFeed *feed, const uint8_t *within_t) {
     const size_t N = prefix.size() - 1, P = ids.size();
     res->n_targets = N;
     res->stats[4] = P;
@@ -104,7 +104,10 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     prm.hausdorff_level = o->hausdorff_level;
     tdp_join *j = nullptr;
     auto t0 = Clock::now();
-    int rc = tdp_join_create_multi(ctx, d1, d2, &prm, N, prefix.data(), ids.data(), k_t, tau_t, &j);
+    int rc = within_t ? tdp_join_create_kinds(ctx, d1, d2, &prm, N, prefix.data(), ids.data(),
+                                              within_t, k_t, tau_t, &j)
+                      : tdp_join_create_multi(ctx, d1, d2, &prm, N, prefix.data(), ids.data(),
+                                              k_t, tau_t, &j);
     if (rc != TDP_OK) {
         err = tdp_last_error(ctx);
         return rc;

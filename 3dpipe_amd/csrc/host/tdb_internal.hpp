@@ -130,7 +130,9 @@ struct LazyFeed {
 // position)); timing [2..5] and stats filled. feed: the records of tiles
 // filled without them, round by round (nullptr: the tiles hold theirs).
 // Returns a TDP_* status.
+// within_t: kNN and within targets in one join (1 = within; o->query is
+// then either), or nullptr for o->query's kind throughout.
 int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *o,
                  const std::vector<uint32_t> &prefix, const std::vector<uint32_t> &ids,
                  const int32_t *k_t, const float *tau_t, tdb_result *res, std::string &err,
-                 LazyFeed *feed = nullptr);
+                 LazyFeed *feed = nullptr, const uint8_t *within_t = nullptr);

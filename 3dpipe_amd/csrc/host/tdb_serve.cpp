@@ -4,10 +4,11 @@
 //
 // One preparer and E executors, each a device context (stream):
 //   preparer  takes the queries waiting at that moment -- as many as fit the
-//             device-memory budget and share the query type, LOD list and
-//             Hausdorff level -- merges their tiles, builds the candidates
-//             (per-target k / within distance) and uploads the merged tile on
-//             its own stream;
+//             device-memory budget and share the LOD list and Hausdorff level
+//             (kNN and within queries together, each target keeping its
+//             kind's rules; intersection queries among themselves) -- merges
+//             their tiles, builds the candidates (per-target kind, k and
+//             within distance) and uploads the merged tile on its own stream;
 //   executor  runs a prepared batch as ONE join on its stream and splits the
 //             result lists back per query.
 // A query that arrives while batches run is admitted as soon as a tile slot
@@ -58,6 +59,7 @@ struct Batch {
     std::vector<uint32_t> prefix, ids;
     std::vector<int32_t> k_t;
     std::vector<float> tau_t;
+    std::vector<uint8_t> within_t;  // kNN and within queries together: 1 for within targets
     std::vector<size_t> first;  // first target of each query in the merged tile
     std::vector<size_t> vfirst, tfirst;  // its first voxel and first record there
     uint64_t dev_bytes = 0;
@@ -80,8 +82,13 @@ uint64_t estimate_bytes(const tdb::Tile &t) {
     return tile + tile / 2 + ((uint64_t)16 << 20);
 }
 
-bool same_kind(const Query &a, const Query &b) {
-    return a.o.query == b.o.query && a.lods == b.lods &&
+// queries that may share a join: the same LOD list and Hausdorff level, and
+// the same query type -- or kNN and within together when mix_kinds
+bool same_kind(const Query &a, const Query &b, bool mix_kinds) {
+    const bool kinds = a.o.query == b.o.query ||
+                       (mix_kinds && a.o.query != TDB_QUERY_INTERSECT &&
+                        b.o.query != TDB_QUERY_INTERSECT);
+    return kinds && a.lods == b.lods &&
            a.o.hausdorff_level == b.o.hausdorff_level;
 }
 
@@ -163,7 +170,7 @@ struct tdb_server {
     // candidates (preparer); join, record packing and puts (executors);
     // bytes to the device
     double t_merge = 0, t_upload = 0, t_cand = 0, t_join = 0, t_pack = 0, t_put = 0;
-    uint64_t h2d_meta = 0, h2d_rec = 0, rec_all = 0;
+    uint64_t h2d_meta = 0, h2d_rec = 0, rec_all = 0, mixed_batches = 0;
     std::thread prep;
     std::vector<std::thread> exec;
 
@@ -308,7 +315,7 @@ void tdb_server::preparer() {
             uint64_t bytes = 0;
             for (auto it = pending.begin(); it != pending.end();) {
                 if ((int)b->qs.size() >= std::max(1, opts.max_batch)) break;
-                if (!same_kind(*it, head)) {
+                if (!same_kind(*it, head, opts.mix_kinds != 0)) {
                     ++it;
                     continue;
                 }
@@ -327,11 +334,17 @@ void tdb_server::preparer() {
         const size_t N = b->merged.t.n_obj;
         b->k_t.resize(N);
         b->tau_t.resize(N);
+        bool mixed = false;
+        for (auto &q : b->qs) mixed = mixed || q.o.query != b->qs[0].o.query;
+        if (mixed) b->within_t.assign(N, 0);
         for (size_t q = 0; q < b->qs.size(); q++) {
             const size_t e = q + 1 < b->qs.size() ? b->first[q + 1] : N;
+            const bool within = b->qs[q].o.query == TDB_QUERY_WITHIN;
             for (size_t t = b->first[q]; t < e; t++) {
-                b->k_t[t] = b->qs[q].o.k;
+                // (a within target's k only feeds the kNN candidate pass)
+                b->k_t[t] = within && mixed ? 1 : b->qs[q].o.k;
                 b->tau_t[t] = b->qs[q].o.within_dist;
+                if (mixed) b->within_t[t] = within ? 1 : 0;
             }
         }
         tdb_join_opts o = b->qs[0].o;
@@ -347,11 +360,35 @@ void tdb_server::preparer() {
             // within lists from the host octree here: the preparer runs while
             // the executor's joins occupy the device, and a device pass would
             // queue behind them
-            if (tdb_gen_candidates_dev(prep_ctx, b->dev, res_dev, &b->merged, resident, false, N,
-                                       &o, b->k_t.data(), b->tau_t.data(), b->prefix, b->ids,
-                                       nullptr, b->err, false) != TDP_OK &&
-                b->err.empty())
-                b->err = "candidate generation failed";
+            auto lists = [&](const tdb_join_opts &oq, std::vector<uint32_t> &pf,
+                             std::vector<uint32_t> &id) {
+                if (tdb_gen_candidates_dev(prep_ctx, b->dev, res_dev, &b->merged, resident, false,
+                                           N, &oq, b->k_t.data(), b->tau_t.data(), pf, id,
+                                           nullptr, b->err, false) != TDP_OK &&
+                    b->err.empty())
+                    b->err = "candidate generation failed";
+            };
+            if (!mixed) {
+                lists(o, b->prefix, b->ids);
+            } else {
+                // each target's list of its own kind
+                std::vector<uint32_t> pk, ik, pw, iw;
+                tdb_join_opts ok = o, ow = o;
+                ok.query = TDB_QUERY_NN;
+                ow.query = TDB_QUERY_WITHIN;
+                lists(ok, pk, ik);
+                if (b->err.empty()) lists(ow, pw, iw);
+                if (b->err.empty()) {
+                    b->prefix.assign(N + 1, 0);
+                    b->ids.clear();
+                    for (size_t t = 0; t < N; t++) {
+                        const auto &p = b->within_t[t] ? pw : pk;
+                        const auto &id = b->within_t[t] ? iw : ik;
+                        b->ids.insert(b->ids.end(), id.begin() + p[t], id.begin() + p[t + 1]);
+                        b->prefix[t + 1] = (uint32_t)b->ids.size();
+                    }
+                }
+            }
         }
         const double ms_cand = ms_since(t0);
         {
@@ -359,6 +396,7 @@ void tdb_server::preparer() {
             t_merge += ms_merge;
             t_upload += ms_upload;
             t_cand += ms_cand;
+            mixed_batches += mixed ? 1 : 0;
             ready.push_back(std::move(b));
         }
         cv.notify_all();
@@ -419,7 +457,8 @@ void tdb_server::executor(int e) {
             const uint64_t pr0 = f.put_records;
             const auto t0 = Clock::now();
             rc = tdb_run_join(ctx, b->dev, res_dev, &o, b->prefix, b->ids, b->k_t.data(),
-                              b->tau_t.data(), &all, msg, opts.lazy ? &f : nullptr);
+                              b->tau_t.data(), &all, msg, opts.lazy ? &f : nullptr,
+                              b->within_t.empty() ? nullptr : b->within_t.data());
             const double ms = ms_since(t0);
             std::lock_guard<std::mutex> lk(mu);
             t_join += ms - (f.pack_ms - pk0) - (f.put_ms - pt0);
@@ -490,6 +529,7 @@ void tdb_server_defaults(tdb_server_opts *o) {
     o->threads = 0;
     o->lazy = 1;
     o->executors = 0;
+    o->mix_kinds = 1;
 }
 
 int tdb_server_open(const tdb_server_opts *o, const tdb_tile *resident, tdb_server **out) {
@@ -608,10 +648,11 @@ void tdb_server_stage_ms(tdb_server *s, double *out4) {
 int tdb_server_figures(tdb_server *s, double *out, int n) {
     if (!s || !out || n < 0) return 0;
     std::lock_guard<std::mutex> lk(s->mu);
-    const double v[12] = {s->t_merge, s->t_upload, s->t_cand, s->t_join, s->t_pack, s->t_put,
+    const double v[13] = {s->t_merge, s->t_upload, s->t_cand, s->t_join, s->t_pack, s->t_put,
                           (double)s->h2d_meta, (double)s->h2d_rec, (double)s->rec_all,
-                          (double)s->n_exec, (double)s->max_running, (double)s->n_batches};
-    n = std::min(n, 12);
+                          (double)s->n_exec, (double)s->max_running, (double)s->n_batches,
+                          (double)s->mixed_batches};
+    n = std::min(n, 13);
     for (int k = 0; k < n; k++) out[k] = v[k];
     return n;
 }

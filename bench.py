@@ -232,6 +232,7 @@ def serve_bench(tdp, vessels, gen_kw, n_queries, rates, seed, log_fn):
                      "max_batch": st["max_batch"], "peak_device_gb": st["peak_device_bytes"] / 1e9,
                      "executors": st["executors"],
                      "most_batches_at_once": st["most_batches_at_once"],
+                     "mixed_kind_batches": st["mixed_kind_batches"],
                      "h2d_bytes": st["h2d_meta_bytes"] + st["h2d_record_bytes"],
                      "h2d_record_bytes": st["h2d_record_bytes"],
                      "record_bytes_of_queries": st["record_bytes_of_queries"],

@@ -241,9 +241,11 @@ int tdb_result_stream_n(const tdb_result *r, double *out, int n);
  * No counterpart in the reference (which runs one join per process). A
  * server keeps `resident` (tile 2 of every query) on the device and joins
  * each submitted query tile (tile 1) against it. Queries waiting when a tile
- * slot frees up are merged into one join -- those sharing the query type,
- * LOD list and Hausdorff level, as many as fit the device-memory budget; k and
- * the within distance may differ per query. Batches are prepared (tile merge,
+ * slot frees up are merged into one join -- those sharing the LOD list and
+ * Hausdorff level (kNN and within queries together, each target with its
+ * kind's rules, tdp_join_create_kinds; intersection queries among
+ * themselves), as many as fit the device-memory budget; k and the within
+ * distance may differ per query. Batches are prepared (tile merge,
  * octree candidates, upload) on their own stream while others run, and up to
  * `executors` batches are joined at once, each on its own stream, so a query
  * that arrives while others are on the device starts beside them. With
@@ -262,6 +264,8 @@ typedef struct {
     int lazy;             /* 1: query records per LOD round for the voxels it reads
                              (default); 0: every record with the tile */
     int executors;        /* batches joined at once when overlapping (0 = 2) */
+    int mix_kinds;        /* 1: kNN and within queries share joins (default); 0: a batch
+                             holds one query type */
 } tdb_server_opts;
 
 void tdb_server_defaults(tdb_server_opts *o);
@@ -281,12 +285,13 @@ void tdb_server_stats(tdb_server *s, uint64_t *out4);
 /* host time (ms, summed over batches) of [0] tile merge, [1] upload, [2]
  * candidates (the preparer) and [3] joins (the executor) */
 void tdb_server_stage_ms(tdb_server *s, double *out4);
-/* more figures, n of them (at most 12; returns how many): [0..3] as
+/* more figures, n of them (at most 13; returns how many): [0..3] as
  * tdb_server_stage_ms with [3] excluding [4] the record copies into pinned
  * memory and [5] the record puts (executors, ms), [6] bytes of voxel and
  * object arrays and [7] of triangle records sent to the device, [8] record
  * bytes of the queries' tiles (what shipping every record would send),
- * [9] executors, [10] most batches joined at once, [11] batches run */
+ * [9] executors, [10] most batches joined at once, [11] batches run,
+ * [12] batches holding both kNN and within queries */
 int tdb_server_figures(tdb_server *s, double *out, int n);
 /* finishes the queued queries, then stops */
 void tdb_server_close(tdb_server *s);

@@ -292,6 +292,21 @@ int tdp_join_create_multi(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
                           const tdp_join_params *params, size_t n_targets,
                           const uint32_t *cand_prefix, const uint32_t *cand_obj,
                           const int32_t *k_target, const float *tau_target, tdp_join **out);
+/* The same with kNN and within targets in one join: within_target[t] = 1
+ * marks a within target (its within distance from tau_target or
+ * params->within_dist), 0 a kNN target (its k from k_target or params->k);
+ * params->query is TDP_QUERY_KNN or TDP_QUERY_WITHIN (its rounds fold the
+ * same quantities and evict the same voxel pairs for both kinds). Each
+ * target gets the kNN or within rules of the reference at the filter and
+ * after every round (KNNJoin.cpp:133-202, 497-851; WithinJoin.cpp:178-252,
+ * 769-792), so its results are those of a join of its kind alone. No
+ * counterpart in the reference (serving loops merging concurrent queries of
+ * both kinds). */
+int tdp_join_create_kinds(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
+                          const tdp_join_params *params, size_t n_targets,
+                          const uint32_t *cand_prefix, const uint32_t *cand_obj,
+                          const uint8_t *within_target, const int32_t *k_target,
+                          const float *tau_target, tdp_join **out);
 
 /* Filter stage on the device: object-level voxel bounds, the filter-time kNN
  * status / within decisions and compaction of surviving voxel pairs.

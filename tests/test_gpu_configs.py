@@ -143,9 +143,11 @@ def _query_tile(tdp, i, size):
     return n
 
 
-def test_server_results_against_oracle(tdp, tmp_path):
-    """queries batched into shared joins: each result equals the oracle's
-    join of that query tile alone"""
+@pytest.mark.parametrize("mix_kinds", [True, False])
+def test_server_results_against_oracle(tdp, tmp_path, mix_kinds):
+    """queries batched into shared joins (kNN and within queries in one join
+    when mix_kinds): each result equals the oracle's join of that query tile
+    alone"""
     _, vessels = tdp.generate(seed=0, n_vessels=10, nuclei_per_vessel=1, cube=2000.0)
     ov = orc.OracleTile(_save(vessels, str(tmp_path / "v.dt")))
     want = []
@@ -153,7 +155,7 @@ def test_server_results_against_oracle(tdp, tmp_path):
         oq = orc.OracleTile(_save(_query_tile(tdp, i, size), str(tmp_path / f"q{i}.dt")))
         want.append(orc.join(oq, ov, q, k, tau, precision="f64", rounds=True))
     tiles = [_query_tile(tdp, i, size) for i, (size, _, _, _) in enumerate(QUERIES)]
-    srv = tdp.Server(vessels, max_batch=64, overlap=True, precision="mixed")
+    srv = tdp.Server(vessels, max_batch=64, overlap=True, precision="mixed", mix_kinds=mix_kinds)
     # submitted back to back: the queries that wait together share joins
     tickets = [srv.submit(tiles[i], query=q, knn=k, within_dist=tau)
                for i, (size, q, k, tau) in enumerate(QUERIES)]
@@ -162,7 +164,10 @@ def test_server_results_against_oracle(tdp, tmp_path):
         _check(res, want[i], f"query {i}")
     st = srv.stats()
     srv.close()
+    print(st)
     assert st["max_batch"] >= 2
+    if not mix_kinds:
+        assert st["mixed_kind_batches"] == 0
 
 
 # ------------------------------------------------------------ tiles without objects

@@ -36,7 +36,7 @@ def vessels(tdp):
 @pytest.mark.parametrize("precision,max_batch,overlap,lazy,executors", [
     ("mixed", 64, True, True, 0), ("mixed", 64, True, False, 0), ("mixed", 64, True, True, 3),
     ("mixed", 2, True, True, 1), ("fp64", 1, False, True, 0), ("fp64", 1, False, False, 0),
-    ("ref32", 64, True, True, 2)])
+    ("ref32", 64, True, True, 2), ("mixed", 64, False, True, 0)])
 def test_server_matches_joins_alone(tdp, vessels, precision, max_batch, overlap, lazy, executors):
     eng = tdp.Engine(0)
     alone = []
@@ -66,6 +66,10 @@ def test_server_matches_joins_alone(tdp, vessels, precision, max_batch, overlap,
     else:
         # all queries were waiting together: at least two shared a join
         assert st["max_batch"] >= 2 and st["batches"] < len(QUERIES)
+    if max_batch > 1 and not overlap:
+        # one batch at a time: the queries waiting behind the first form one
+        # batch, kNN and within queries together
+        assert st["mixed_kind_batches"] >= 1
     # lazily only the records of voxels some round read went over
     if lazy:
         assert 0 < st["h2d_record_bytes"] < st["record_bytes_of_queries"]
