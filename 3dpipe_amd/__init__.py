@@ -54,7 +54,8 @@ class _GenParams(_c.Structure):
 
 class _Tuning(_c.Structure):
     _fields_ = [("small_pairs", _c.c_int), ("max_rows", _c.c_int), ("drain", _c.c_int),
-                ("drain_blocks", _c.c_int), ("eval_blocks", _c.c_int), ("graph", _c.c_int)]
+                ("drain_blocks", _c.c_int), ("eval_blocks", _c.c_int), ("graph", _c.c_int),
+                ("row_cull", _c.c_int)]
 
 
 class _JoinOpts(_c.Structure):

@@ -23,7 +23,7 @@ struct tdp_ctx {
     int device = 0;
     int cu_count = 256;
     hipStream_t stream = nullptr;
-    tdp_tuning tune{(int)tdp::kMxSmallPairs, 64, 0, 0, 4096, 0};
+    tdp_tuning tune{(int)tdp::kMxSmallPairs, 64, 0, 0, 4096, 0, 1};
     std::string err;
     // grow-only scratch for scans
     std::vector<void *> scan_buf;
@@ -65,6 +65,12 @@ struct tdp_tile {
     uint32_t *vox_tri_cnt = nullptr;
     float4 *tri = nullptr;
     float4 *bnd = nullptr;  // kBnd per triangle: sphere, plane, hausdorff pair
+    // kSph per voxel and LOD slot: the voxel's sphere (k_vox_sph), read by
+    // the mixed mark pass. Formed at upload, by the record puts of tiles
+    // filled without records, and before the first join after a fill with
+    // records (sph_stale)
+    float4 *sph = nullptr;
+    mutable bool sph_stale = false;
     float *obj_box = nullptr, *obj_tr = nullptr;  // for candidate queries (optional)
     size_t bytes = 0;
     size_t cap_obj = 0, cap_vox = 0, cap_tri = 0;  // reserved tiles: capacity
@@ -667,8 +673,12 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
         t->bytes += sizeof(float4) * kBnd * (d->n_tri + kTriPad);
         rc = zero_pad(ctx, t->bnd + kBnd * d->n_tri, kBnd * kTriPad);
     }
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->sph, kSph * 5 * std::max<size_t>(d->n_vox, 1));
+    if (rc == TDP_OK) t->bytes += sizeof(float4) * kSph * 5 * d->n_vox;
     if (rc == TDP_OK && d->n_tri) {
         k_tri_bounds<<<blocks_for(d->n_tri, kBlock), kBlock, 0, ctx->stream>>>(d->n_tri, t->tri, t->bnd);
+        k_vox_sph<<<stride_grid(5 * d->n_vox * kWave, 4096), kBlock, 0, ctx->stream>>>(
+            d->n_vox, t->vox_tri_off, t->vox_tri_cnt, t->tri, t->sph);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
@@ -801,6 +811,7 @@ extern "C" int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t
     if (rc == TDP_OK) rc = dalloc(ctx, &t->vox_tri_cnt, 5 * n_vox);
     if (rc == TDP_OK) rc = dalloc(ctx, &t->tri, 3 * (n_tri + kTriPad));
     if (rc == TDP_OK) rc = dalloc(ctx, &t->bnd, kBnd * (n_tri + kTriPad));
+    if (rc == TDP_OK) rc = dalloc(ctx, &t->sph, kSph * 5 * n_vox);
     // zeroed whole: the padding past the capacity, and records a smaller fill
     // leaves untouched, are always readable values
     if (rc == TDP_OK) rc = zero_pad(ctx, t->tri, 3 * (n_tri + kTriPad));
@@ -811,7 +822,7 @@ extern "C" int tdp_tile_reserve(tdp_ctx *ctx, size_t n_obj, size_t n_vox, size_t
         tdp_tile_free(t);
         return rc;
     }
-    t->bytes = (n_obj + 1) * 4 + n_obj * 76 + n_vox * (36 + 4 + 40 + 20) +
+    t->bytes = (n_obj + 1) * 4 + n_obj * 76 + n_vox * (36 + 4 + 40 + 20 + 16 * kSph * 5) +
                (n_tri + kTriPad) * 16 * (3 + kBnd);
     *out = t;
     return TDP_OK;
@@ -862,6 +873,7 @@ extern "C" int tdp_tile_fill_async(tdp_tile *t, const tdp_tile_desc *d) {
     t->n_tri = d->n_tri;
     t->lazy = false;
     t->lz_pending = 0;
+    t->sph_stale = true;
     if (d->tri) return tdp_tile_fill_tri_async(t, 0, d->n_tri, d->tri);
     return TDP_OK;
 }
@@ -921,6 +933,7 @@ extern "C" int tdp_tile_fill_tri_async(tdp_tile *t, size_t first, size_t n, cons
     }
     if (!n) return TDP_OK;
     hipStream_t s = ctx->stream;
+    t->sph_stale = true;
     TDP_CHECK(ctx, hipMemcpyAsync(t->tri + 3 * first, records, 48 * n, hipMemcpyHostToDevice, s));
     k_tri_bounds<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, t->tri + 3 * first,
                                                           t->bnd + kBnd * first);
@@ -933,8 +946,8 @@ extern "C" int tdp_tile_free(tdp_tile *t) {
     tdp_ctx *ctx = t->ctx;
     for (void *p : {(void *)t->obj_vox_off, (void *)t->obj_hd, (void *)t->vox_box,
                     (void *)t->vox_obj, (void *)t->vox_tri_off, (void *)t->vox_tri_cnt,
-                    (void *)t->tri, (void *)t->bnd, (void *)t->obj_box, (void *)t->obj_tr,
-                    (void *)t->have, (void *)t->lz_flag, (void *)t->lz_list, (void *)t->lz_cnt,
+                    (void *)t->tri, (void *)t->bnd, (void *)t->sph, (void *)t->obj_box,
+                    (void *)t->obj_tr, (void *)t->have, (void *)t->lz_flag, (void *)t->lz_list, (void *)t->lz_cnt,
                     (void *)t->lz_pos, (void *)t->lz_off})
         dfree(ctx, p);
     delete t;
@@ -1663,6 +1676,19 @@ extern "C" int tdp_join_filter(tdp_join *j, uint64_t *pairs_out) {
         j->ctx->err = "tdp_join_filter: already filtered";
         return TDP_ERR_STATE;
     }
+    // the mixed mark pass reads the voxel spheres: a tile filled with its
+    // records since its spheres were formed gets them now (tiles filled
+    // without records get each voxel's with its records)
+    if (j->prm.precision == TDP_PREC_MIXED)
+        for (const tdp_tile *t : {j->t1, j->t2})
+            if (t->sph_stale && !t->lazy) {
+                if (t->n_vox)
+                    k_vox_sph<<<stride_grid(5 * t->n_vox * kWave, 4096), kBlock, 0,
+                                j->ctx->stream>>>(t->n_vox, t->vox_tri_off, t->vox_tri_cnt,
+                                                  t->tri, t->sph);
+                TDP_LAUNCHED(j->ctx);
+                t->sph_stale = false;
+            }
     return j->rsz == 4 ? join_filter_t<float>(j, pairs_out) : join_filter_t<double>(j, pairs_out);
 }
 
@@ -1678,6 +1704,11 @@ static uint32_t fold_need(const tdp_join *j, int is_last) {
     if (j->prm.query == TDP_QUERY_INTERSECT) return 4u | (lv == 2 ? 2u : 0u) | (lv == 1 ? 1u : 0u);
     return is_last || lv != 2 ? 1u : 6u;
 }
+
+// The mark pass's row test (tuning row_cull) where it pays: rounds that fold
+// d alone. In rounds that fold lo and hi it rules out few rows (their
+// thresholds carry the Hausdorff terms) and its registers cost occupancy.
+static bool row_cull(const tdp_join *j, uint32_t need) { return j->tune.row_cull && need == 1u; }
 
 // calls f(std::integral_constant<uint32_t, NEED>) for the round's quantity set
 template <typename F>
@@ -1719,25 +1750,30 @@ static void launch_decide(tdp_join *j, uint8_t rnd) {
 // rounds that fold lo and hi into them, the object pairs' smallest U_d in the
 // last round; neither for the intersection join, whose rounds fold the
 // round's own values. Queues k_mx_obj_u when the last round's cap applies.
+// The kernel reads the arrays whether or not a cap applies (its loads are
+// issued together, before any branch); `caps` says which apply: 1 the range
+// caps, 2 the object cap.
 struct MarkCaps {
     const double *vp_min = nullptr, *vp_max = nullptr;
     const uint32_t *vp_pair = nullptr, *obj_u = nullptr;
-    int lo_free = 0;
+    int caps = 0;
 };
 static MarkCaps mark_caps(tdp_join *j, int is_last, uint64_t n) {
     MarkCaps m;
     const int c = j->cur;
     m.vp_pair = j->vpp[c];
+    m.vp_min = (const double *)j->vmin[c];
+    m.vp_max = (const double *)j->vmax[c];
+    m.obj_u = j->obj_u;
     if (j->prm.query == TDP_QUERY_INTERSECT) return m;
     hipStream_t s = j->ctx->stream;
     if (is_last) {
         k_fill_u32<<<blocks_for(j->P, kBlock), kBlock, 0, s>>>(j->P, j->obj_u, 0xff800000u);
         k_mx_obj_u<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->mx_plan, j->mx_U, j->mx_key,
                                                              j->vpp[c], j->obj_u);
-        m.obj_u = j->obj_u;
+        m.caps = 2;
     } else if (fold_need(j, is_last) == 6u) {
-        m.vp_min = (const double *)j->vmin[c];
-        m.vp_max = (const double *)j->vmax[c];
+        m.caps = 1;
     }
     return m;
 }
@@ -1826,10 +1862,15 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         const MarkCaps cap = mark_caps(j, is_last, n);
         TDP_LAUNCHED(ctx);
         with_need(need, [&](auto nd) {
-            k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-                j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, lo_free ? 1 : 0,
-                j->mx_pwb, j->mx_mask, j->keep);
+            auto go = [&](auto cull) {
+                k_mx_mark<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(
+                    n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe,
+                    j->mx_key, j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u,
+                    cap.caps, lo_free ? 1 : 0, j->mx_pwb, j->v1[c], j->v2[c], A->sph, B->sph,
+                    slot, j->mx_mask, j->keep);
+            };
+            if (row_cull(j, need)) go(std::true_type{});
+            else go(std::false_type{});
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_MARK) != TDP_OK) return TDP_ERR_HIP;
@@ -2004,10 +2045,15 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     const MarkCaps cap = mark_caps(j, is_last, n);
     launched();
     with_need(need, [&](auto nd) {
-        k_mx_mark<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-            n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-            j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, lo_free ? 1 : 0,
-            j->mx_pwb, j->mx_mask, j->keep);
+        auto go = [&](auto cull) {
+            k_mx_mark<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(
+                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
+                j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, cap.caps,
+                lo_free ? 1 : 0, j->mx_pwb, j->v1[c], j->v2[c], A->sph, B->sph, slot,
+                j->mx_mask, j->keep);
+        };
+        if (row_cull(j, need)) go(std::true_type{});
+        else go(std::false_type{});
     });
     launched();
     k_mx_drain<<<nb, kBlock, 0, s>>>(n, j->mx_plan, A->tri, B->tri, j->mx_woff, j->mx_mask,
@@ -2205,6 +2251,10 @@ extern "C" int tdp_join_lazy_put(tdp_join *j, int lod, int side, const float *re
             p1 >= recs ? 1 : 0, t->tri, t->bnd, t->have);
         TDP_LAUNCHED(ctx);
     }
+    // the listed voxels' spheres at this slot, from their records in place
+    k_vox_sph_list<<<stride_grid(n * kWave, 2048), kBlock, 0, ctx->stream>>>(
+        n, slot, t->lz_list, t->vox_tri_off, t->vox_tri_cnt, t->tri, t->sph);
+    TDP_LAUNCHED(ctx);
     t->put_records += recs;
     j->lz_ready[side - 1] = lod;
     return TDP_OK;

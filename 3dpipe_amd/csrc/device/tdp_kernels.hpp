@@ -593,6 +593,100 @@ __global__ void __launch_bounds__(kBlock) k_lazy_put(size_t n_list, int slot,
     }
 }
 
+// ---- voxel spheres: per voxel and LOD slot, a ball holding every vertex of
+// the voxel's triangles at that LOD, for the row test of the mixed mark pass
+// (k_mx_mark). Two records per (voxel, slot), at sph[kSph * (5 v + slot)]:
+//   {C.xyz, R}   every vertex lies within R of C (C is the middle of the
+//                vertices' box; any centre would do, R is what the test
+//                relies on: its square is summed in FP64 from exact vertex
+//                differences, and the root is rounded upwards to FP32);
+//   {R + max |C coord| (rounded upwards), the largest proxy Hausdorff ph,
+//    the smallest Hausdorff h, 0}   the margin term and the extreme per-
+//                triangle Hausdorff values that bound the threshold of any
+//                pair with one of these triangles.
+// A voxel without triangles at the slot gets zeros (its voxel pairs are
+// EMPTY and never read it).
+constexpr int kSph = 2;
+
+__device__ __forceinline__ float wave_max_f(float v) { return -wave_min<float>(-v); }
+
+// one wave: the sphere records of the cnt triangles at off
+__device__ __forceinline__ void vox_sphere(const float4 *__restrict__ tri, uint64_t off,
+                                           uint32_t cnt, float4 *__restrict__ out) {
+    const uint32_t lane = (uint32_t)lane_id();
+    if (cnt == 0) {
+        if (lane < kSph) out[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    float lx = INFINITY, ly = INFINITY, lz = INFINITY;
+    float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY, mph = -INFINITY, mh = INFINITY;
+    for (uint32_t t = lane; t < cnt; t += kWave) {
+        const float4 a = tri[3 * (off + t)], b = tri[3 * (off + t) + 1], c = tri[3 * (off + t) + 2];
+        lx = fminf(lx, fminf(a.x, fminf(b.x, c.x)));
+        ly = fminf(ly, fminf(a.y, fminf(b.y, c.y)));
+        lz = fminf(lz, fminf(a.z, fminf(b.z, c.z)));
+        hx = fmaxf(hx, fmaxf(a.x, fmaxf(b.x, c.x)));
+        hy = fmaxf(hy, fmaxf(a.y, fmaxf(b.y, c.y)));
+        hz = fmaxf(hz, fmaxf(a.z, fmaxf(b.z, c.z)));
+        mph = fmaxf(mph, a.w);  // records carry ph in a.w and h in b.w
+        mh = fminf(mh, b.w);
+    }
+    lx = wave_min<float>(lx);
+    ly = wave_min<float>(ly);
+    lz = wave_min<float>(lz);
+    hx = wave_max_f(hx);
+    hy = wave_max_f(hy);
+    hz = wave_max_f(hz);
+    mph = wave_max_f(mph);
+    mh = wave_min<float>(mh);
+    const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
+    // FP32 differences are exact in FP64 and so are their squares; the sum
+    // and the root add a few 2^-53, which the factor below covers
+    double r2 = 0.0;
+    for (uint32_t t = lane; t < cnt; t += kWave) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float4 v = tri[3 * (off + t) + k];
+            const double dx = (double)v.x - cx, dy = (double)v.y - cy, dz = (double)v.z - cz;
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        }
+    }
+    float r = (float)sqrt(r2);  // nearest: at most u below
+    r = wave_max_f(r) * (1.f + 4.f * kU) + 1e-30f;
+    const float cm = (r + fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)))) * (1.f + 2.f * kU);
+    if (lane == 0) {
+        out[0] = make_float4(cx, cy, cz, r);
+        out[1] = make_float4(cm, mph, mh, 0.f);
+    }
+}
+
+// every (voxel, slot) of a tile, one wave each
+__global__ void __launch_bounds__(kBlock) k_vox_sph(size_t n_vox,
+                                                    const uint64_t *__restrict__ vox_tri_off,
+                                                    const uint32_t *__restrict__ vox_tri_cnt,
+                                                    const float4 *__restrict__ tri,
+                                                    float4 *__restrict__ sph) {
+    const size_t waves = (size_t)gridDim.x * kWavesPerBlock;
+    for (size_t k = (size_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < 5 * n_vox;
+         k += waves)
+        vox_sphere(tri, vox_tri_off[k], vox_tri_cnt[k], sph + kSph * k);
+}
+
+// the listed voxels at one slot (records put for a round), one wave each
+__global__ void __launch_bounds__(kBlock) k_vox_sph_list(size_t n_list, int slot,
+                                                         const uint32_t *__restrict__ list,
+                                                         const uint64_t *__restrict__ vox_tri_off,
+                                                         const uint32_t *__restrict__ vox_tri_cnt,
+                                                         const float4 *__restrict__ tri,
+                                                         float4 *__restrict__ sph) {
+    const size_t waves = (size_t)gridDim.x * kWavesPerBlock;
+    for (size_t k = (size_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n_list;
+         k += waves) {
+        const size_t vs = 5 * (size_t)list[k] + slot;
+        vox_sphere(tri, vox_tri_off[vs], vox_tri_cnt[vs], sph + kSph * vs);
+    }
+}
+
 // (d, lo, hi) intervals from an interval [dl, du] of the FP64 TriDist value
 __device__ __forceinline__ void intervals(float dl, float du, float ph1, float h1, float ph2,
                                           float h2, float lw[3], float uw[3]) {
@@ -657,9 +751,10 @@ __device__ __forceinline__ void pair_interval(const float4 *r1, const float4 *b1
 
 // the next float above a non-negative x (x itself for inf / nan)
 __device__ __forceinline__ float ulp_up(float x) {
-    if (!(x < INFINITY)) return x;
+    // selects, not branches: +inf and NaN stay as they are
     const uint32_t b = __float_as_uint(x);
-    return x >= 0.f ? __uint_as_float(b + 1u) : __uint_as_float(b - 1u);
+    const float up = __uint_as_float(x >= 0.f ? b + 1u : b - 1u);
+    return x < INFINITY ? up : x;
 }
 
 __device__ __forceinline__ bool reaches(const float lw[3], const float U[3]) {

@@ -176,6 +176,7 @@ struct VpPlan {
     uint32_t nw;        // mask words (zero for EMPTY / SMALL)
 };
 static_assert(sizeof(VpPlan) == 40, "VpPlan layout");
+static_assert(kBnd == 3, "a bound record and a vertex record stage alike (3 float4)");
 
 __device__ __forceinline__ VpLod plan_lod(const VpPlan &p) { return VpLod{p.o1, p.o2, p.n1, p.n2}; }
 __device__ __forceinline__ uint32_t plan_flags(const VpPlan &p) { return p.shape >> 24; }
@@ -246,6 +247,7 @@ struct PairAt {
     uint32_t i, j;     // triangle indices within the two voxels
     uint32_t word;     // iteration number
     bool valid;
+    uint32_t k;        // iteration number within the staged block
 };
 
 // Row records of the tiled walk, staged in LDS per wave: kStageRows rows of
@@ -271,13 +273,21 @@ struct RowStage {
 // instead of one per row and iteration, and no memory latency inside the
 // iterations. (With one row per iteration the row is the same in every lane
 // and its LDS reads are broadcasts.)
-template <bool SWAP, bool VERTS, typename FC, typename F>
+//
+// With CULL, each staging lane also applies rt(row's bound record) to its row
+// (rows past the smaller voxel count as passed over), and an iteration all of
+// whose rows rt passes over is not evaluated: f gets skip = true (wave-
+// uniform) and an empty row record. Without CULL, skip is always false.
+//
+// fb(w0, nb) is called by every lane after each staged block, whose
+// iterations were w0 .. w0 + nb - 1 (nb <= 32).
+template <bool SWAP, bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
                                            const float4 *__restrict__ rec1,
                                            const float4 *__restrict__ bnd1,
                                            const float4 *__restrict__ rec2,
                                            const float4 *__restrict__ bnd2, RowStage *stage,
-                                           FC &&fc, F &&f) {
+                                           FC &&fc, F &&f, RT &&rt, FB &&fb) {
     const uint32_t nl = SWAP ? g.n1 : g.n2, ns = SWAP ? g.n2 : g.n1;
     const uint64_t ol = SWAP ? g.o1 : g.o2, os = SWAP ? g.o2 : g.o1;
     const float4 *__restrict__ bl = SWAP ? bnd1 : bnd2;
@@ -295,6 +305,7 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
     const uint32_t rr = min(r, t.R - 1u);
     const lds_f4 *const s_base = (const lds_f4 *)&stage->s[0][0] + kBnd * rr;
     const lds_f4 *const v_base = (const lds_f4 *)&stage->v[0][0] + 3 * rr;
+    const uint64_t gmask = (t.R >= 64u) ? ~0ull : ((1ull << t.R) - 1ull);
     uint32_t word = 0;
     for (uint32_t ch = 0; ch < t.nch; ch++) {
         const uint32_t col = ch * t.cw + c;
@@ -304,54 +315,72 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const TriV LV = load_v(rl, lt);
         fc(L, LV);
         auto lane_v = [&]() { return LV; };
-        uint32_t k = 0;
-        for (uint32_t gr = 0; gr < t.ngr; gr++, word++) {
-            if (k == 0) {
-                const uint64_t row0 = os + (uint64_t)gr * t.R;
-                wave_sync();  // the previous block's reads are done
-                if ((uint32_t)lane < rpb) {
-                    const float4 *p = bs + kBnd * (row0 + (uint32_t)lane);
+        // staged blocks of gpb row groups, then the block's iterations in a
+        // loop of their own (no per-iteration block test)
+        for (uint32_t gb = 0; gb < t.ngr; gb += gpb) {
+            const uint64_t row0 = os + (uint64_t)gb * t.R;
+            wave_sync();  // the previous block's reads are done
+            // the block's records: bound records to lanes 0..rpb-1, vertex
+            // records (VERTS) to lanes 32..32+rpb-1
+            const bool is_s = lane < 32;
+            const uint32_t q = is_s ? (uint32_t)lane : (uint32_t)lane - 32u;
+            const bool ld = q < rpb && (is_s || VERTS);
+            bool over = false;
+            if (ld) {
+                const float4 *p = is_s ? bs + kBnd * (row0 + q) : rs + 3 * (row0 + q);
+                float4 rq[3];
 #pragma unroll
-                    for (int i = 0; i < kBnd; i++) stage->s[lane][i] = p[i];
-                }
-                if (VERTS && lane >= 32 && (uint32_t)lane - 32u < rpb) {
-                    const uint32_t q = (uint32_t)lane - 32u;
-                    const float4 *p = rs + 3 * (row0 + q);
+                for (int i = 0; i < 3; i++) rq[i] = p[i];
+                float4 *dst = is_s ? &stage->s[q][0] : &stage->v[q][0];
 #pragma unroll
-                    for (int i = 0; i < 3; i++) stage->v[q][i] = p[i];
-                }
-                wave_sync();
+                for (int i = 0; i < 3; i++) dst[i] = rq[i];
+                if constexpr (CULL) over = is_s && (gb * t.R + q >= ns || rt(rq));
             }
-            const uint32_t row = gr * t.R + r;
-            const bool valid = cv && row < ns;
-            // the lane's row slot: a 32-bit LDS address, the lane's part
-            // formed once
-            const uint32_t k0 = k * t.R;
-            const TriS S = load_s_lds(s_base + kBnd * k0);
-            auto row_v = [&]() { return load_v_lds(v_base + 3 * k0); };
-            if (SWAP)
-                f(L, S, PairAt{cv ? col : 0u, row, word, valid}, lane_v, row_v,
-                  std::integral_constant<bool, true>{});
-            else
-                f(S, L, PairAt{row, cv ? col : 0u, word, valid}, row_v, lane_v,
-                  std::integral_constant<bool, false>{});
-            if (++k == gpb) k = 0;
+            // rows of the block rt passes over (bit per row)
+            const uint64_t passed = CULL ? __ballot(over) : 0ull;
+            wave_sync();
+            const uint32_t nb = min(gpb, t.ngr - gb);
+            uint32_t row = gb * t.R + r;
+            for (uint32_t k = 0, k0 = 0; k < nb; k++, k0 += t.R, row += t.R, word++) {
+                const bool valid = cv && row < ns;
+                // the lane's row slot: a 32-bit LDS address, the lane's part
+                // formed once
+                const bool skip = CULL && ((passed >> k0) & gmask) == gmask;
+                TriS S{};
+                if (!skip) S = load_s_lds(s_base + kBnd * k0);
+                auto row_v = [&]() { return load_v_lds(v_base + 3 * k0); };
+                if (SWAP)
+                    f(L, S, PairAt{cv ? col : 0u, row, word, valid, k}, lane_v, row_v,
+                      std::integral_constant<bool, true>{}, skip);
+                else
+                    f(S, L, PairAt{row, cv ? col : 0u, word, valid, k}, row_v, lane_v,
+                      std::integral_constant<bool, false>{}, skip);
+            }
+            fb(word - nb, nb);
         }
     }
 }
 
-template <bool VERTS, typename FC, typename F>
+template <bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB>
 __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int lane,
                                           const float4 *__restrict__ rec1,
                                           const float4 *__restrict__ bnd1,
                                           const float4 *__restrict__ rec2,
                                           const float4 *__restrict__ bnd2, RowStage *stage,
-                                          FC &&fc, F &&f) {
+                                          FC &&fc, F &&f, RT &&rt, FB &&fb) {
     if (t.swap)
-        walk_tiles<true, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
+        walk_tiles<true, VERTS, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f, rt, fb);
     else
-        walk_tiles<false, VERTS>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f);
+        walk_tiles<false, VERTS, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f, rt, fb);
 }
+// nothing at the end of a staged block
+struct NoBlockEnd {
+    __device__ void operator()(uint32_t, uint32_t) const {}
+};
+// no row test
+struct NoRowTest {
+    __device__ bool operator()(const float4 *) const { return false; }
+};
 
 // ------------------------------------------------------------------ bounds
 
@@ -411,7 +440,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     // column constants: r G, alo, ahi; the scores' -r, -(r + ph), h - r
     float crg = 0.f, calo = 0.f, cahi = 0.f, ck0 = 0.f, ck1 = 0.f, ck2 = 0.f;
     __shared__ RowStage stage[kWavesPerBlock];
-    for_tiles<false>(g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
+    for_tiles<false, false>(g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
               [&](const TriS &L, const TriV &) {
                   crg = L.r * kUG;
                   calo = L.alo;
@@ -420,8 +449,11 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                   ck1 = -(L.r + L.ph);
                   ck2 = L.h - L.r;
               },
-              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto swap) {
-                  if (!p.valid) return;
+              [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto swap,
+                  bool) {
+                  // every lane computes (cells outside the grid on in-bounds
+                  // data) and takes part only when valid: no lane mask saved
+                  // and restored per iteration
                   constexpr bool col_is_a = decltype(swap)::value;
                   const TriS &S = col_is_a ? B : A;
                   const TriS &L = col_is_a ? A : B;
@@ -448,13 +480,13 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
 #pragma unroll
                   for (int f = 0; f < 3; f++) {
                       if (!((NEED >> f) & 1u)) continue;
-                      up[f] = fminf(up[f], x[f]);
-                      if (key[f] < best[f]) {
-                          best[f] = key[f];
-                          bq[f] = q;
-                      }
+                      up[f] = p.valid ? fminf(up[f], x[f]) : up[f];
+                      const bool better = p.valid && key[f] < best[f];
+                      best[f] = better ? key[f] : best[f];
+                      bq[f] = better ? q : bq[f];
                   }
-              });
+              },
+              NoRowTest{}, NoBlockEnd{});
     up[1] = fmaxf(up[1], 0.f);
     float U[3];
     uint32_t pq[3];
@@ -671,9 +703,14 @@ __device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float u
 // compare = 1, fused multiply-add = 2, abs free): centre difference 3, dc 6,
 // margin 4, the two plane offsets 10, guard 5, sines and disc gap 16,
 // comparison 1 -- 45; T adds 5 in rounds that fold lo and hi (50).
+//
+// Called by every lane of the wave (cells outside the grid with valid =
+// false, on in-bounds data): the vertex-level bound is a wave-uniform branch,
+// taken when some lane needs it, so no lane mask is saved and restored per
+// pair.
 template <typename RV>
 __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const MarkCol &c, float T,
-                                          RV &&row_v) {
+                                          bool valid, RV &&row_v) {
 #pragma clang fp contract(fast)
     const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
     const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
@@ -690,18 +727,21 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
     const float sl = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cl, cl, 1.f + 16.f * kU), 16.f * kU));
     const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
     const float dl = g ? fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m)) : 0.f;
-    if (!(dl <= T)) return false;
-    if (!g) return true;
-    const TriV v = row_v();
-    // S's vertices lie beyond L's along d = cS - cL
-    const float s0 = dx * (v.x0 - L.cx) + dy * (v.y0 - L.cy) + dz * (v.z0 - L.cz);
-    const float s1 = dx * (v.x1 - L.cx) + dy * (v.y1 - L.cy) + dz * (v.z1 - L.cz);
-    const float s2 = dx * (v.x2 - L.cx) + dy * (v.y2 - L.cy) + dz * (v.z2 - L.cz);
-    const float l0 = dx * c.x0 + dy * c.y0 + dz * c.z0;
-    const float l1 = dx * c.x1 + dy * c.y1 + dz * c.z1;
-    const float l2 = dx * c.x2 + dy * c.y2 + dz * c.z2;
-    const float gap = (fminf(s0, fminf(s1, s2)) - fmaxf(l0, fmaxf(l1, l2))) * inv - m;
-    return gap <= T;
+    const bool keep = valid && dl <= T;
+    bool take = keep;
+    if (__ballot(keep && g)) {
+        const TriV v = row_v();
+        // S's vertices lie beyond L's along d = cS - cL
+        const float s0 = dx * (v.x0 - L.cx) + dy * (v.y0 - L.cy) + dz * (v.z0 - L.cz);
+        const float s1 = dx * (v.x1 - L.cx) + dy * (v.y1 - L.cy) + dz * (v.z1 - L.cz);
+        const float s2 = dx * (v.x2 - L.cx) + dy * (v.y2 - L.cy) + dz * (v.z2 - L.cz);
+        const float l0 = dx * c.x0 + dy * c.y0 + dz * c.z0;
+        const float l1 = dx * c.x1 + dy * c.y1 + dz * c.z1;
+        const float l2 = dx * c.x2 + dy * c.y2 + dz * c.z2;
+        const float gap = (fminf(s0, fminf(s1, s2)) - fmaxf(l0, fmaxf(l1, l2))) * inv - m;
+        take = keep && (!g || gap <= T);
+    }
+    return take;
 }
 
 // Per object pair, the smallest U_d of its tiled voxel pairs (FP32 upper
@@ -744,7 +784,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_obj_u(
 //    changes neither the object range nor which voxel pairs the eviction
 //    keeps (a voxel pair whose exact minimum equals the object's has a pair
 //    at that value, below the cap, so it is found exactly).
-template <uint32_t NEED>
+template <uint32_t NEED, bool CULL>
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
     const float4 *__restrict__ bnd1, const float4 *__restrict__ rec2,
@@ -752,12 +792,26 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const uint32_t *__restrict__ probe, const unsigned long long *__restrict__ mkey,
     const uint64_t *__restrict__ woff, const double *__restrict__ vp_min,
     const double *__restrict__ vp_max, const uint32_t *__restrict__ vp_pair,
-    const uint32_t *__restrict__ obj_u, int lo_free, const unsigned long long *__restrict__ pwb,
+    const uint32_t *__restrict__ obj_u, int caps, int lo_free,
+    const unsigned long long *__restrict__ pwb,
+    const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
+    const float4 *__restrict__ sph1, const float4 *__restrict__ sph2, int slot,
     unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
     const int lane = lane_id();
+    // Every figure of the voxel pair is requested before any is used: one
+    // wait for all of them instead of a chain of dependent waits (the cap
+    // arrays are always valid; `caps` says which caps apply).
+    const size_t w3 = 3 * (size_t)w;
     const VpPlan pl = plan[w];
+    const uint32_t vw1 = vp_v1[w], vw2 = vp_v2[w], vpp = vp_pair[w];
+    const float u0 = vpU[w3], u1 = vpU[w3 + 1], u2 = vpU[w3 + 2];
+    const unsigned long long q0 = mkey[w3], q1 = mkey[w3 + 1], q2 = mkey[w3 + 2];
+    const unsigned long long a0 = pwb[w3], a1 = pwb[w3 + 1], a2 = pwb[w3 + 2];
+    const uint64_t wo = woff[w];
+    const double vmn = vp_min[w], vmx = vp_max[w];
+    const uint32_t ou = obj_u[vpp];
     const uint32_t fl = plan_flags(pl);
     if (fl & MX_EMPTY) {
         if (lane == 0) count[w] = 0;
@@ -768,73 +822,116 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
         if (lane == 0) count[w] = g.n1 * g.n2;
         return;
     }
-    float U[3];
-    bounds_U(vpU, mkey, w, U);
-    // lo without an FP32 bound this round: only the probes' exact values
-    if (lo_free && mkey[3 * w + 1] != dkey(0.0)) U[1] = ulp_up((float)dunkey(mkey[3 * w + 1]));
-    if (obj_u) U[0] = fminf(U[0], funkey(obj_u[vp_pair[w]]));
-    if (vp_min) {
-        U[1] = fminf(U[1], ulp_up((float)vp_min[w]));
-        U[2] = fminf(U[2], ulp_up((float)vp_max[w]));
+    // U per quantity: the FP32 upper bound tightened by the exact values
+    // found so far (rounded up to FP32); -inf for lo once it is known to be
+    // +0, its smallest value; lo without an FP32 bound this round takes only
+    // the probes' exact values; then the caps
+    const bool lo0 = q1 == dkey(0.0);
+    float U0 = fminf(u0, ulp_up((float)dunkey(q0)));
+    float U1 = lo0 ? -INFINITY : (lo_free ? ulp_up((float)dunkey(q1))
+                                          : fminf(u1, ulp_up((float)dunkey(q1))));
+    float U2 = fminf(u2, ulp_up((float)dunkey(q2)));
+    if (caps & 2) U0 = fminf(U0, funkey(ou));
+    if (caps & 1) {
+        U1 = fminf(U1, ulp_up((float)vmn));
+        U2 = fminf(U2, ulp_up((float)vmx));
     }
-    const float ud = U[0], ulo_h = U[1] * kTH, uhi_h = U[2] * kTH;
+    const float ud = U0, ulo_h = U1 * kTH, uhi_h = U2 * kTH;
     const Tiling t = plan_tiling(pl);
-    const uint32_t n2 = g.n2, nw = pl.nw;
-    unsigned long long *__restrict__ out = mask + woff[w];
+    unsigned long long *__restrict__ out = mask + wo;
     // the probes' words and bits in the tiling (k_mx_eval)
     uint32_t pword[3];
     unsigned long long pbit[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        const unsigned long long at = pwb[3 * (size_t)w + f];
+        const unsigned long long at = f == 0 ? a0 : (f == 1 ? a1 : a2);
         pword[f] = at == ~0ull ? 0xffffffffu : (uint32_t)(at >> 6);
         pbit[f] = at == ~0ull ? 0ull : 1ull << (at & 63u);
     }
-    // the words of the current group of 64 iterations, stored together
-    __shared__ unsigned long long s_words[kWavesPerBlock][kWave];
-    unsigned long long *const sw = s_words[threadIdx.x / kWave];
-    uint32_t c = 0;
+    // the words of the current staged block, lane k holding word k, stored
+    // together after the block; each lane counts the bits of the words it
+    // stores
+    uint32_t wlo = 0, whi = 0;
+    uint32_t cl = 0;
     MarkCol mc;
     __shared__ RowStage stage[kWavesPerBlock];
-    for_tiles<true>(
+    // The row test against the column voxel's sphere (k_vox_sph; every
+    // column triangle of the walk lies in it): a row whose plane has the
+    // whole sphere on one side (by more than twice the margin, the disc
+    // guard with the sphere for the column triangle) and whose disc is
+    // farther from the sphere than the largest T of its pairs cannot hold a
+    // minimum with any column triangle -- each pair's FP64 TriDist value is
+    // then at least the triangles' distance (the face test shows them
+    // disjoint), which is at least the disc-to-sphere gap D - r_S sin - R.
+    // The row's T takes the column terms' extremes over the voxel (largest
+    // ph, smallest h); the fused multiply-adds are monotone, so it is at
+    // least every pair's T.
+    // The staging lanes read the sphere with their rows (a broadcast load
+    // issued beside the row loads), so it is neither waited for before the
+    // walk nor kept in registers through it.
+    typedef __attribute__((address_space(4))) const f4v cst_f4;
+    const cst_f4 *const sp =
+        (const cst_f4 *)((t.swap ? sph1 : sph2) + kSph * (5 * (size_t)(t.swap ? vw1 : vw2) + slot));
+    const f4v sa = sp[0], sb = sp[1];  // scalar loads: the sphere stays in scalar registers
+    auto row_test = [&](const float4 *pr) -> bool {
+#pragma clang fp contract(fast)
+        const float s_mc = fmaf(32.f * kU, sb.x, 1e-6f);
+        const float s_tlo = fmaf(sb.y, kTH, ulo_h), s_thi = fmaf(sb.z, -kTG, uhi_h);
+        const float dx = pr[0].x - sa.x, dy = pr[0].y - sa.y, dz = pr[0].z - sa.z;
+        const float D = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+        const float m = fmaf(32.f * kU, D + pr[1].w, s_mc);
+        const float e = pr[1].x * dx + pr[1].y * dy + pr[1].z * dz;
+        const float cs = e * __builtin_amdgcn_rcpf(D);
+        const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
+        const float lb = fmaf(-pr[0].w, ss, D - m) - sa.w;
+        float T = -INFINITY;
+        if (NEED & 1u) T = ud;
+        if (NEED & 2u) T = fmaxf(T, fmaf(pr[2].x, kTH, s_tlo));
+        if (NEED & 4u) T = fmaxf(T, fmaf(pr[2].y, -kTG, s_thi));
+        return fabsf(e) > sa.w + 2.f * m && lb > T;
+    };
+    for_tiles<true, CULL>(
         g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
         [&](const TriS &L, const TriV &LV) { mc = mark_col(L, LV, ulo_h, uhi_h); },
-        [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap) {
+        [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap,
+            bool skip) {
             constexpr bool col_is_a = decltype(swap)::value;
             const TriS &S = col_is_a ? B : A;
             const TriS &L = col_is_a ? A : B;
             bool take = false;
-            if (p.valid) {
+            if (!skip) {
                 if constexpr (col_is_a)
-                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), vc);
+                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), p.valid, vc);
                 else
-                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), va);
+                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), p.valid, va);
             }
             const unsigned long long m = __ballot(take);
-            c += (uint32_t)__popcll(m);
-            const uint32_t k = p.word & (kWave - 1);
-            if (lane == 0) sw[k] = m;
-            if (k == kWave - 1 || p.word + 1 == nw) {
-                wave_sync();
-                // the probes' bits among this group of words are cleared
-                // (once per 64 iterations instead of a test per iteration)
-                const uint32_t g0 = p.word & ~(uint32_t)(kWave - 1);
+            // word k of the block into lane k (v_writelane: the ballot's
+            // scalar registers written into one lane, no compare or select)
+            const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
+            const uint32_t kk = p.k;
+            uint32_t a = wlo, b = whi;
+            asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(a) : "s"(mlo), "{m0}"(kk));
+            asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(b) : "s"(mhi), "{m0}"(kk));
+            wlo = a;
+            whi = b;
+        },
+        row_test,
+        [&](uint32_t w0, uint32_t nb) {
+            // lane l holds word w0 + l; the probes' bits are cleared (they are
+            // evaluated already)
+            unsigned long long x =
+                ((unsigned long long)(uint32_t)whi << 32) | (unsigned long long)(uint32_t)wlo;
 #pragma unroll
-                for (int f = 0; f < 3; f++) {
-                    if (pword[f] - g0 <= k) {
-                        const unsigned long long x = sw[pword[f] - g0];
-                        if (x & pbit[f]) {
-                            c -= 1u;
-                            if (lane == 0) sw[pword[f] - g0] = x & ~pbit[f];
-                        }
-                        wave_sync();
-                    }
-                }
-                if ((uint32_t)lane <= k) out[g0 + lane] = sw[lane];
-                wave_sync();
+            for (int f = 0; f < 3; f++)
+                if (pword[f] == w0 + (uint32_t)lane) x &= ~pbit[f];
+            if ((uint32_t)lane < nb) {
+                out[w0 + lane] = x;
+                cl += (uint32_t)__popcll(x);
             }
         });
-    if (lane == 0) count[w] = c;
+    const uint64_t c = wave_sum_u64(cl);
+    if (lane == 0) count[w] = (uint32_t)c;
 }
 
 // Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One wave

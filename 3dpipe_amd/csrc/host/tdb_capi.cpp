@@ -637,7 +637,7 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
     auto same = [](const tdp_tuning &a, const tdp_tuning &b) {
         return a.small_pairs == b.small_pairs && a.max_rows == b.max_rows && a.drain == b.drain &&
                a.drain_blocks == b.drain_blocks && a.eval_blocks == b.eval_blocks &&
-               a.graph == b.graph;
+               a.graph == b.graph && a.row_cull == b.row_cull;
     };
     for (int form : {0, 1, 2})
         for (int rows : {64, 4})

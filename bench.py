@@ -46,7 +46,7 @@ PEAK_FP64_TFLOPS = 78.6
 # the runs summarised in TRAFFIC_SOURCE; reported only for that workload and
 # those tunings.
 DEFAULT_TUNING = {"small_pairs": 8, "max_rows": 64, "drain": 0, "drain_blocks": 0,
-                  "eval_blocks": 4096, "graph": 0}
+                  "eval_blocks": 4096, "graph": 0, "row_cull": 1}
 MEASURED_TRAFFIC = {  # max_rows -> (k_mx_mark bytes per launch, bytes per round)
     64: (12.57e9, 24.86e9),
     4: (12.62e9, 24.97e9),

@@ -250,6 +250,9 @@ typedef struct {
     int eval_blocks;  /* grid of the work-list FP64 stage (grid-stride) */
     int graph;        /* MIXED: 1 = each LOD round is captured in a hipGraph and launched as one
                          (implies the persistent FP64 stage; one host synchronisation per round) */
+    int row_cull;     /* MIXED: 1 = the candidate pass first bounds each row triangle against a
+                         sphere around the other voxel and skips the wave iterations whose rows
+                         are all ruled out */
 } tdp_tuning;
 /* the context's tuning (defaults at tdp_open); joins created afterwards use it */
 int tdp_get_tuning(tdp_ctx *ctx, tdp_tuning *out);
