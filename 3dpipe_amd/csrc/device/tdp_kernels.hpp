@@ -44,6 +44,28 @@ __device__ __forceinline__ R wave_min(R v) {
     for (int o = 32; o > 0; o >>= 1) v = rmin2<R>(v, __shfl_xor(v, o, kWave));
     return v;
 }
+// Wave reductions on the DPP lane network instead of LDS permutes: an
+// inclusive scan inside each row of 16 lanes (shifts right by 1, 2, 4, 8),
+// then row broadcasts 15 and 31 carry it across the rows, so lane 63 holds
+// the reduction of all 64 lanes, read into a scalar register. Lanes whose
+// source lies outside their row take the identity (bound control off). Every
+// lane of the wave must be active.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v, uint32_t id, Op op) {
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x111, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x112, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x114, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x118, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x142, 0xa, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x143, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    return wave_reduce_u32(v, 0xffffffffu, [](uint32_t a, uint32_t b) { return min(a, b); });
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    return wave_reduce_u32(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -59,6 +81,9 @@ __device__ __forceinline__ float funkey(uint32_t k) {
     uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
     return __uint_as_float(u);
 }
+// minimum of a float over the wave, as the minimum of its order-preserving
+// keys (no NaN among the values)
+__device__ __forceinline__ float wave_min_f32(float v) { return funkey(wave_min_u32(fkey(v))); }
 __device__ __forceinline__ unsigned long long dkey(double f) {
     unsigned long long u = (unsigned long long)__double_as_longlong(f);
     return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);

@@ -386,11 +386,8 @@ struct NoRowTest {
 
 // lowest q among the lanes holding the wave minimum of v (kNoProbe if none)
 __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
-    const float vmin = wave_min(v);
-    uint32_t c = (v == vmin) ? q : kNoProbe;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c = min(c, (uint32_t)__shfl_xor((int)c, o, kWave));
-    return c;
+    const float vmin = wave_min_f32(v);
+    return wave_min_u32(v == vmin ? q : kNoProbe);
 }
 
 // Upper bounds for k_mx_bounds. Every value TriDist returns is the distance
@@ -480,10 +477,13 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
 #pragma unroll
                   for (int f = 0; f < 3; f++) {
                       if (!((NEED >> f) & 1u)) continue;
-                      up[f] = p.valid ? fminf(up[f], x[f]) : up[f];
-                      const bool better = p.valid && key[f] < best[f];
-                      best[f] = better ? key[f] : best[f];
-                      bq[f] = better ? q : bq[f];
+                      // selects and exact minima: cells outside the grid
+                      // count as +inf
+                      const float xv = p.valid ? x[f] : INFINITY;
+                      const float kv = p.valid ? key[f] : INFINITY;
+                      up[f] = fminf(up[f], xv);
+                      bq[f] = kv < best[f] ? q : bq[f];
+                      best[f] = fminf(best[f], kv);
                   }
               },
               NoRowTest{}, NoBlockEnd{});
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         if (((NEED & need) >> f) & 1u) {
-            U[f] = wave_min(up[f]);
+            U[f] = wave_min_f32(up[f]);
             pq[f] = wave_argmin_q(best[f], bq[f]);
         } else {
             U[f] = -INFINITY;
@@ -930,8 +930,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
                 cl += (uint32_t)__popcll(x);
             }
         });
-    const uint64_t c = wave_sum_u64(cl);
-    if (lane == 0) count[w] = (uint32_t)c;
+    const uint32_t c = wave_sum_u32(cl);
+    if (lane == 0) count[w] = c;
 }
 
 // Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One wave
