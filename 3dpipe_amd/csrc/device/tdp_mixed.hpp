@@ -269,7 +269,8 @@ struct RowStage {
 // each column chunk with the lane's column triangle (sphere data, vertices).
 // Each lane reads its row from `stage`, which the wave fills with the records
 // of the next R * floor(32 / R) rows whenever it starts a block of row groups
-// (the row vertices too when VERTS): one coalesced load per row and chunk
+// (the row vertices too when VERTS, less their triangle's centre): one
+// coalesced load per row and chunk
 // instead of one per row and iteration, and no memory latency inside the
 // iterations. (With one row per iteration the row is the same in every lane
 // and its LDS reads are broadcasts.)
@@ -331,6 +332,19 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
                 float4 rq[3];
 #pragma unroll
                 for (int i = 0; i < 3; i++) rq[i] = p[i];
+                if (VERTS && !is_s) {
+                    // vertices staged less their triangle's centre, formed as
+                    // the bound record forms it (tri_bound)
+                    const float cx = ((rq[0].x + rq[1].x) + rq[2].x) * (1.f / 3.f);
+                    const float cy = ((rq[0].y + rq[1].y) + rq[2].y) * (1.f / 3.f);
+                    const float cz = ((rq[0].z + rq[1].z) + rq[2].z) * (1.f / 3.f);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        rq[i].x -= cx;
+                        rq[i].y -= cy;
+                        rq[i].z -= cz;
+                    }
+                }
                 float4 *dst = is_s ? &stage->s[q][0] : &stage->v[q][0];
 #pragma unroll
                 for (int i = 0; i < 3; i++) dst[i] = rq[i];
@@ -696,9 +710,10 @@ __device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float u
 //     reaching this step, already shows the triangles disjoint for TriDist,
 //     so the vertex-level plane tests are not repeated; a pair without the
 //     guard stays a candidate. The projections are taken on the unnormalised
-//     centre difference and scaled by 1/dc once; the rounding of the vertex
-//     differences, the dot products and the reciprocal is a few u (dc + r_S
-//     + r_L + cm_S + cm_L), inside the margin.
+//     centre difference d and scaled by 1/dc once, the row's as d . (v - cS)
+//     + |d|^2 from vertices staged less their centre; the rounding of the
+//     vertex differences, the dot products, |d|^2 and the reciprocal is a few
+//     u (dc + r_S + r_L + cm_S + cm_L), inside the margin.
 // Operation count per pair through step 1 (add/sub/mul/sqrt/rcp/min/max/
 // compare = 1, fused multiply-add = 2, abs free): centre difference 3, dc 6,
 // margin 4, the two plane offsets 10, guard 5, sines and disc gap 16,
@@ -713,7 +728,8 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
                                           bool valid, RV &&row_v) {
 #pragma clang fp contract(fast)
     const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
-    const float dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    const float dd = dx * dx + dy * dy + dz * dz;
+    const float dc = __builtin_amdgcn_sqrtf(dd);
     const float m = fmaf(32.f * kU, dc + S.cm, c.mc);
     const float eL = L.nx * dx + L.ny * dy + L.nz * dz;
     const float eS = S.nx * dx + S.ny * dy + S.nz * dz;
@@ -730,15 +746,17 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
     const bool keep = valid && dl <= T;
     bool take = keep;
     if (__ballot(keep && g)) {
+        // the row's vertices less its centre (as staged): S's vertices lie
+        // beyond L's along d = cS - cL, at d . (v - cS) + |d|^2 from L's centre
         const TriV v = row_v();
-        // S's vertices lie beyond L's along d = cS - cL
-        const float s0 = dx * (v.x0 - L.cx) + dy * (v.y0 - L.cy) + dz * (v.z0 - L.cz);
-        const float s1 = dx * (v.x1 - L.cx) + dy * (v.y1 - L.cy) + dz * (v.z1 - L.cz);
-        const float s2 = dx * (v.x2 - L.cx) + dy * (v.y2 - L.cy) + dz * (v.z2 - L.cz);
+        const float s0 = dx * v.x0 + dy * v.y0 + dz * v.z0;
+        const float s1 = dx * v.x1 + dy * v.y1 + dz * v.z1;
+        const float s2 = dx * v.x2 + dy * v.y2 + dz * v.z2;
         const float l0 = dx * c.x0 + dy * c.y0 + dz * c.z0;
         const float l1 = dx * c.x1 + dy * c.y1 + dz * c.z1;
         const float l2 = dx * c.x2 + dy * c.y2 + dz * c.z2;
-        const float gap = (fminf(s0, fminf(s1, s2)) - fmaxf(l0, fmaxf(l1, l2))) * inv - m;
+        const float gap =
+            ((fminf(s0, fminf(s1, s2)) + dd) - fmaxf(l0, fmaxf(l1, l2))) * inv - m;
         take = keep && (!g || gap <= T);
     }
     return take;
