@@ -1705,6 +1705,23 @@ static uint32_t fold_need(const tdp_join *j, int is_last) {
     return is_last || lv != 2 ? 1u : 6u;
 }
 
+// The bounds pass's row test (tuning row_cull) against the cap the mark pass
+// will apply: 2 = the object pairs' range max so far, in the last round (after
+// a first round). The kernel also takes the voxel pairs' ranges (caps bit 1)
+// for rounds that fold lo and hi, but there it passed over few rows and the
+// extra loads cost more than they saved (bounds 21.2 -> 23.2 ms over the four
+// lo/hi rounds of the 10-NN 200k join), so it is not used. None for the
+// intersection join, whose rounds fold their own values.
+struct BoundCull {
+    int caps = 0;
+};
+static BoundCull bound_cull(const tdp_join *j, int is_last) {
+    BoundCull b;
+    if (!j->tune.row_cull || j->prm.query == TDP_QUERY_INTERSECT) return b;
+    if (is_last && j->rounds >= 1) b.caps = 2;
+    return b;
+}
+
 // The mark pass's row test (tuning row_cull) where it pays: rounds that fold
 // d alone. In rounds that fold lo and hi it rules out few rows (their
 // thresholds carry the Hausdorff terms) and its registers cost occupancy.
@@ -1838,10 +1855,17 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         // saved.
         const bool lo_free = need == 6u && j->rounds >= 1 && j->prm.query != TDP_QUERY_INTERSECT;
         const uint32_t bneed = lo_free ? 4u : need;
+        const BoundCull bc = bound_cull(j, is_last);
         with_need(bneed, [&](auto nd) {
-            k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, bneed, j->mx_U, j->mx_probe,
-                j->mx_key);
+            auto go = [&](auto cull) {
+                k_mx_bounds<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(
+                    n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, bneed, j->mx_U, j->mx_probe,
+                    j->mx_key, j->v1[c], j->v2[c], A->sph, B->sph, slot,
+                    (const double *)j->vmin[c], (const double *)j->vmax[c], j->vpp[c],
+                    (const double *)j->rmax, bc.caps);
+            };
+            if (bc.caps) go(std::true_type{});
+            else go(std::false_type{});
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_BOUNDS) != TDP_OK) return TDP_ERR_HIP;
@@ -2030,9 +2054,17 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     launched();
     const bool lo_free = need == 6u && j->rounds >= 1 && j->prm.query != TDP_QUERY_INTERSECT;
     const uint32_t bneed = lo_free ? 4u : need;
+    const BoundCull bc = bound_cull(j, is_last);
     with_need(bneed, [&](auto nd) {
-        k_mx_bounds<decltype(nd)::value><<<gw, kBlock, 0, s>>>(
-            n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, bneed, j->mx_U, j->mx_probe, j->mx_key);
+        auto go = [&](auto cull) {
+            k_mx_bounds<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(
+                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, bneed, j->mx_U, j->mx_probe,
+                j->mx_key, j->v1[c], j->v2[c], A->sph, B->sph, slot,
+                (const double *)j->vmin[c], (const double *)j->vmax[c], j->vpp[c],
+                (const double *)j->rmax, bc.caps);
+        };
+        if (bc.caps) go(std::true_type{});
+        else go(std::false_type{});
     });
     launched();
     if (rc == TDP_OK) rc = scan_launch<uint32_t>(ctx, j->mx_nw, n, j->mx_woff, j->d_words);

@@ -426,16 +426,43 @@ constexpr float kUG = 1.f + 4.f * kU;
 // the arithmetic of the others drops out (and `need` the same bits at run
 // time). The column triangle's parts of the bounds and scores are per-lane
 // constants of a column chunk; a pair adds the row triangle's.
-template <uint32_t NEED>
+//
+// Rows may be passed over freely: the minimum over any subset of the pairs
+// is still an upper bound of the minimum, and any pair a valid probe. With
+// CULL the walk passes over the rows whose pairs cannot reach below what the
+// round's fold keeps anyway (the caps of k_mx_mark): in a round that folds
+// lo and hi, the voxel pair's range (vp_min, vp_max); in the last round, the
+// object pair's range max so far (pair_max; an estimate of its final
+// distance). The row test is the sphere gap to the column voxel's sphere
+// (k_vox_sph), with the quantities' Hausdorff terms; it needs no margins
+// (a wrong call costs a looser U, never a wrong result).
+template <uint32_t NEED, bool CULL>
 __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
     const float4 *__restrict__ bnd1, const float4 *__restrict__ rec2,
     const float4 *__restrict__ bnd2, uint32_t need, float *__restrict__ vpU,
-    uint32_t *__restrict__ probe, unsigned long long *__restrict__ mkey) {
+    uint32_t *__restrict__ probe, unsigned long long *__restrict__ mkey,
+    const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
+    const float4 *__restrict__ sph1, const float4 *__restrict__ sph2, int slot,
+    const double *__restrict__ vp_min, const double *__restrict__ vp_max,
+    const uint32_t *__restrict__ vp_pair, const double *__restrict__ pair_max, int caps) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
     const int lane = lane_id();
     const VpPlan pl = plan[w];
+    // the cull thresholds (requested with the plan; +inf where none applies)
+    float Td = INFINITY, Tlo = INFINITY, Thi = INFINITY;
+    uint32_t vw1 = 0, vw2 = 0;
+    if constexpr (CULL) {
+        vw1 = vp_v1[w];
+        vw2 = vp_v2[w];
+        const double mn = vp_min[w], mx = vp_max[w], pm = pair_max[vp_pair[w]];
+        if (caps & 1) {
+            Tlo = (float)mn;
+            Thi = (float)mx;
+        }
+        if (caps & 2) Td = (float)pm;
+    }
     if (lane < 3) {
         mkey[3 * (size_t)w + lane] = dkey(INFINITY);
         probe[3 * (size_t)w + lane] = kNoProbe;
@@ -451,7 +478,25 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
     // column constants: r G, alo, ahi; the scores' -r, -(r + ph), h - r
     float crg = 0.f, calo = 0.f, cahi = 0.f, ck0 = 0.f, ck1 = 0.f, ck2 = 0.f;
     __shared__ RowStage stage[kWavesPerBlock];
-    for_tiles<false, false>(g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
+    // the column voxel's sphere (scalar loads) and the row test
+    typedef __attribute__((address_space(4))) const f4v cst_f4;
+    f4v sa = {0.f, 0.f, 0.f, 0.f}, sb = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (CULL) {
+        const cst_f4 *const sp = (const cst_f4 *)((t.swap ? sph1 : sph2) +
+                                                  kSph * (5 * (size_t)(t.swap ? vw1 : vw2) + slot));
+        sa = sp[0];
+        sb = sp[1];
+    }
+    auto row_test = [&](const float4 *pr) -> bool {
+        const float dx = pr[0].x - sa.x, dy = pr[0].y - sa.y, dz = pr[0].z - sa.z;
+        const float lb = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - pr[0].w - sa.w;
+        bool over = true;
+        if ((NEED & need) & 1u) over = over && lb > Td;
+        if ((NEED & need) & 2u) over = over && lb - pr[2].x - sb.y > Tlo;
+        if ((NEED & need) & 4u) over = over && lb + pr[2].y + sb.z > Thi;
+        return over;
+    };
+    for_tiles<false, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
               [&](const TriS &L, const TriV &) {
                   crg = L.r * kUG;
                   calo = L.alo;
@@ -461,7 +506,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                   ck2 = L.h - L.r;
               },
               [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto swap,
-                  bool) {
+                  bool skip) {
+                  // an iteration whose rows the row test passed over adds
+                  // nothing (a wave-uniform branch)
+                  if (skip) return;
                   // every lane computes (cells outside the grid on in-bounds
                   // data) and takes part only when valid: no lane mask saved
                   // and restored per iteration
@@ -500,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                       best[f] = fminf(best[f], kv);
                   }
               },
-              NoRowTest{}, NoBlockEnd{});
+              row_test, NoBlockEnd{});
     up[1] = fmaxf(up[1], 0.f);
     float U[3];
     uint32_t pq[3];
