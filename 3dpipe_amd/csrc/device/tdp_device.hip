@@ -115,6 +115,7 @@ struct tdp_join {
     unsigned long long *mx_key = nullptr;
     VpPlan *mx_plan = nullptr;  // per voxel pair: triangles and tiling (k_mx_plan)
     unsigned long long *mx_pwb = nullptr;  // per voxel pair: its probes' mask word and bit
+    uint32_t *mx_tested = nullptr;  // per voxel pair: pairs the mark pass tested one by one
     uint32_t *mx_nw = nullptr;   // candidate-mask words per voxel pair
     uint64_t *mx_woff = nullptr; // their exclusive scan
     size_t cap_mx = 0;
@@ -1345,7 +1346,7 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q,
-                    j->mx_plan, j->mx_pwb};
+                    j->mx_plan, j->mx_pwb, j->mx_tested};
     for (void *p : ptrs) pfree(ctx, p);
     for (int b = 0; b < 2; b++) {
         pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
@@ -1563,8 +1564,10 @@ static int ensure_mx(tdp_join *j, size_t n) {
     const size_t cap = std::max(n, j->cap_vp);
     pfree(ctx, j->mx_U); pfree(ctx, j->mx_probe); pfree(ctx, j->mx_flags); pfree(ctx, j->mx_key);
     pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff); pfree(ctx, j->mx_plan); pfree(ctx, j->mx_pwb);
+    pfree(ctx, j->mx_tested);
     j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
     j->mx_nw = nullptr; j->mx_woff = nullptr; j->mx_plan = nullptr; j->mx_pwb = nullptr;
+    j->mx_tested = nullptr;
     j->cap_mx = 0;
     int rc = palloc(ctx, &j->mx_U, 3 * cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_probe, 3 * cap);
@@ -1574,6 +1577,7 @@ static int ensure_mx(tdp_join *j, size_t n) {
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_woff, cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_plan, cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_pwb, 3 * cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_tested, cap);
     if (rc == TDP_OK) j->cap_mx = cap;
     return rc;
 }
@@ -1891,7 +1895,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                     n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe,
                     j->mx_key, j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u,
                     cap.caps, lo_free ? 1 : 0, j->mx_pwb, j->v1[c], j->v2[c], A->sph, B->sph,
-                    slot, j->mx_mask, j->keep);
+                    slot, j->mx_mask, j->keep, j->mx_tested);
             };
             if (row_cull(j, need)) go(std::true_type{});
             else go(std::false_type{});
@@ -1933,7 +1937,7 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
             n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
             B->vox_tri_cnt, A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last,
             fold_level(j), j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
-            j->stats);
+            j->stats, j->mx_tested);
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_FOLD) != TDP_OK) return TDP_ERR_HIP;
     }
@@ -2082,7 +2086,7 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
                 n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
                 j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, cap.caps,
                 lo_free ? 1 : 0, j->mx_pwb, j->v1[c], j->v2[c], A->sph, B->sph, slot,
-                j->mx_mask, j->keep);
+                j->mx_mask, j->keep, j->mx_tested);
         };
         if (row_cull(j, need)) go(std::true_type{});
         else go(std::false_type{});
@@ -2095,7 +2099,7 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
         n, j->v1[c], j->v2[c], j->vpp[c], (R *)j->vmin[c], (R *)j->vmax[c], A->vox_tri_cnt,
         B->vox_tri_cnt, A->vox_obj, B->vox_obj, A->obj_hd, B->obj_hd, slot, is_last,
         fold_level(j), j->mx_flags, j->mx_key, (KT *)j->acc_min, (KT *)j->acc_max,
-        j->stats);
+        j->stats, j->mx_tested);
     launched();
     k_pair_finish<R><<<blocks_for(P, kBlock), kBlock, 0, s>>>(P, j->state, (const KT *)j->acc_min,
                                                               (const KT *)j->acc_max, rmin, rmax);
@@ -2350,6 +2354,18 @@ extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
     // MIXED: every triangle pair not evaluated in FP64 was excluded by its
     // FP32 interval
     out5[4] = j->prm.precision == TDP_PREC_MIXED && s[0] > s[2] ? s[0] - s[2] : 0;
+    return TDP_OK;
+}
+
+extern "C" int tdp_join_stats_n(tdp_join *j, uint64_t *out, int n) {
+    if (!j || !out || n < 0) return TDP_ERR_ARG;
+    uint64_t a[6] = {0, 0, 0, 0, 0, 0};
+    const int rc = tdp_join_stats(j, a);
+    if (rc != TDP_OK) return rc;
+    unsigned long long s[4];
+    TDP_CHECK(j->ctx, copy_sync(j->ctx, s, j->stats, sizeof s, hipMemcpyDeviceToHost));
+    a[5] = s[3];
+    for (int k = 0; k < n && k < 6; k++) out[k] = a[k];
     return TDP_OK;
 }
 

@@ -862,7 +862,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const unsigned long long *__restrict__ pwb,
     const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const float4 *__restrict__ sph1, const float4 *__restrict__ sph2, int slot,
-    unsigned long long *__restrict__ mask, uint32_t *__restrict__ count) {
+    unsigned long long *__restrict__ mask, uint32_t *__restrict__ count,
+    uint32_t *__restrict__ tested) {
     const uint32_t w = wave_item();
     if (w >= n_vp) return;
     const int lane = lane_id();
@@ -880,12 +881,15 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     const uint32_t ou = obj_u[vpp];
     const uint32_t fl = plan_flags(pl);
     if (fl & MX_EMPTY) {
-        if (lane == 0) count[w] = 0;
+        if (lane == 0) count[w] = tested[w] = 0;
         return;
     }
     const VpLod g = plan_lod(pl);
     if (fl & MX_SMALL) {
-        if (lane == 0) count[w] = g.n1 * g.n2;
+        if (lane == 0) {
+            count[w] = g.n1 * g.n2;
+            tested[w] = 0;
+        }
         return;
     }
     // U per quantity: the FP32 upper bound tightened by the exact values
@@ -919,6 +923,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     // stores
     uint32_t wlo = 0, whi = 0;
     uint32_t cl = 0;
+    uint32_t sk = 0;  // pairs of the iterations passed over (not tested one by one)
     MarkCol mc;
     __shared__ RowStage stage[kWavesPerBlock];
     // The row test against the column voxel's sphere (k_vox_sph; every
@@ -965,7 +970,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
             const TriS &S = col_is_a ? B : A;
             const TriS &L = col_is_a ? A : B;
             bool take = false;
-            if (!skip) {
+            if (skip) sk += (uint32_t)__popcll(__ballot(p.valid));
+            else {
                 if constexpr (col_is_a)
                     take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), p.valid, vc);
                 else
@@ -997,7 +1003,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
             }
         });
     const uint32_t c = wave_sum_u32(cl);
-    if (lane == 0) count[w] = c;
+    if (lane == 0) {
+        count[w] = c;
+        tested[w] = g.n1 * g.n2 - sk;
+    }
 }
 
 // Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One wave
@@ -1343,12 +1352,22 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
     const float *__restrict__ ohd1, const float *__restrict__ ohd2, int slot, int is_last,
     int level, const uint32_t *__restrict__ flags, const unsigned long long *__restrict__ mkey,
     typename Key<R>::type *__restrict__ acc_min, typename Key<R>::type *__restrict__ acc_max,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, const uint32_t *__restrict__ tested) {
     using K = typename Key<R>::type;
     const K none = Key<R>::enc((R)__builtin_inff());
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t seg = 0xffffffffu;
     K kmin = none, kmax = none;
+    // the mark pass's pairs tested one by one: block-reduced into stats[3]
+    __shared__ unsigned long long part[kWavesPerBlock];
+    const uint64_t tv = wave_sum_u64(w < n_vp ? (uint64_t)tested[w] : 0ull);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = tv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0;
+        for (int k = 0; k < kWavesPerBlock; k++) a += part[k];
+        if (a) atomicAdd(&stats[3], a);
+    }
     if (w < n_vp) {
         const uint32_t v1 = vp_v1[w], v2 = vp_v2[w];
         seg = vp_pair[w];

@@ -48,8 +48,8 @@ PEAK_FP64_TFLOPS = 78.6
 DEFAULT_TUNING = {"small_pairs": 8, "max_rows": 64, "drain": 0, "drain_blocks": 0,
                   "eval_blocks": 4096, "graph": 0, "row_cull": 1}
 MEASURED_TRAFFIC = {  # max_rows -> (k_mx_mark bytes per launch, bytes per round)
-    64: (12.57e9, 24.86e9),
-    4: (12.62e9, 24.97e9),
+    64: (12.70e9, 25.10e9),
+    4: (12.76e9, 25.20e9),
 }
 TRAFFIC_SOURCE = "profiles/knn10_200k_nuclei_1000_vessels_mixed_traffic_and_valu.md"
 REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",
@@ -428,14 +428,18 @@ def main():
     dom_ms = stage_ms[dom] / dom_launches
     # algorithmic work per launch: the per-pair operation count of the
     # kernel's test (k_mx_mark: F_MARK; the plain forms' k_refine: F_TRI)
-    # times the triangle pairs of a launch
+    # times the triangle pairs the kernel tests one by one (k_mx_mark passes
+    # over whole rows ruled out against a voxel sphere, and leaves voxel pairs
+    # of at most 8 triangle pairs to FP64: those pairs are not counted)
     flop_pair = {"mark": F_MARK, "direct": F_TRI}.get(dom)
-    achieved = (flop_pair * tri_pairs / (stage_ms[dom] * 1e-3) / 1e12) if flop_pair else None
+    tested = r.stats.get("mark_tested", 0) if dom == "mark" else tri_pairs
+    achieved = (flop_pair * tested / (stage_ms[dom] * 1e-3) / 1e12) if flop_pair else None
     roofline = {"bound": "valu-fp32" if dom != "direct" or args.precision != "fp64" else "valu-fp64",
                 "kernel": dom_kernel, "achieved": achieved,
                 "peak": PEAK_FP32_TFLOPS if args.precision != "fp64" else PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": None, "traffic": None,
                 "launches": dom_launches, "avg_launch_ms": dom_ms, "flop_per_tri_pair": flop_pair,
+                "tri_pairs_tested": tested, "tri_pairs_of_rounds": tri_pairs,
                 "share_of_refine": stage_ms[dom] / refine_kernel_ms if refine_kernel_ms else None}
     if achieved is not None:
         roofline["frac"] = achieved / roofline["peak"]
