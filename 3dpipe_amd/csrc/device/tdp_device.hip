@@ -679,7 +679,7 @@ static int tile_upload(tdp_ctx *ctx, const tdp_tile_desc *d, const float *const 
     if (rc == TDP_OK && d->n_tri) {
         k_tri_bounds<<<blocks_for(d->n_tri, kBlock), kBlock, 0, ctx->stream>>>(d->n_tri, t->tri, t->bnd);
         k_vox_sph<<<stride_grid(5 * d->n_vox * kWave, 4096), kBlock, 0, ctx->stream>>>(
-            d->n_vox, t->vox_tri_off, t->vox_tri_cnt, t->tri, t->sph);
+            d->n_vox, t->vox_tri_off, t->vox_tri_cnt, t->tri, t->bnd, t->sph);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
@@ -1689,7 +1689,7 @@ extern "C" int tdp_join_filter(tdp_join *j, uint64_t *pairs_out) {
                 if (t->n_vox)
                     k_vox_sph<<<stride_grid(5 * t->n_vox * kWave, 4096), kBlock, 0,
                                 j->ctx->stream>>>(t->n_vox, t->vox_tri_off, t->vox_tri_cnt,
-                                                  t->tri, t->sph);
+                                                  t->tri, t->bnd, t->sph);
                 TDP_LAUNCHED(j->ctx);
                 t->sph_stale = false;
             }
@@ -1849,8 +1849,8 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         const uint32_t need = fold_need(j, is_last);
         k_mx_plan<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
             n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
-            slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan,
-            j->mx_flags, j->mx_nw, j->stats);
+            slot, A->sph, B->sph, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows,
+            j->mx_plan, j->mx_flags, j->mx_nw, j->stats);
         TDP_LAUNCHED(ctx);
         // After the first round a lo/hi round bounds only hi: U_lo then comes
         // from the probes and the voxel pair's range min, which caps it in the
@@ -2053,8 +2053,8 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
     launched();
     k_mx_plan<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
         n, j->v1[c], j->v2[c], A->vox_tri_off, A->vox_tri_cnt, B->vox_tri_off, B->vox_tri_cnt,
-        slot, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows, j->mx_plan, j->mx_flags,
-        j->mx_nw, j->stats);
+        slot, A->sph, B->sph, (uint32_t)j->tune.small_pairs, (uint32_t)j->tune.max_rows,
+        j->mx_plan, j->mx_flags, j->mx_nw, j->stats);
     launched();
     const bool lo_free = need == 6u && j->rounds >= 1 && j->prm.query != TDP_QUERY_INTERSECT;
     const uint32_t bneed = lo_free ? 4u : need;
@@ -2289,7 +2289,7 @@ extern "C" int tdp_join_lazy_put(tdp_join *j, int lod, int side, const float *re
     }
     // the listed voxels' spheres at this slot, from their records in place
     k_vox_sph_list<<<stride_grid(n * kWave, 2048), kBlock, 0, ctx->stream>>>(
-        n, slot, t->lz_list, t->vox_tri_off, t->vox_tri_cnt, t->tri, t->sph);
+        n, slot, t->lz_list, t->vox_tri_off, t->vox_tri_cnt, t->tri, t->bnd, t->sph);
     TDP_LAUNCHED(ctx);
     t->put_records += recs;
     j->lz_ready[side - 1] = lod;

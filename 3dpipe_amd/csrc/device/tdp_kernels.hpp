@@ -626,9 +626,11 @@ __global__ void __launch_bounds__(kBlock) k_lazy_put(size_t n_list, int slot,
 //                relies on: its square is summed in FP64 from exact vertex
 //                differences, and the root is rounded upwards to FP32);
 //   {R + max |C coord| (rounded upwards), the largest proxy Hausdorff ph,
-//    the smallest Hausdorff h, 0}   the margin term and the extreme per-
-//                triangle Hausdorff values that bound the threshold of any
-//                pair with one of these triangles.
+//    the smallest Hausdorff h, the largest cm of the triangles' bound
+//    records}   the margin term of the sphere, the extreme per-triangle
+//                Hausdorff values that bound the threshold of any pair with
+//                one of these triangles, and the largest per-triangle margin
+//                term (r + max |centre coord| as k_tri_bounds stored it).
 // A voxel without triangles at the slot gets zeros (its voxel pairs are
 // EMPTY and never read it).
 constexpr int kSph = 2;
@@ -636,7 +638,8 @@ constexpr int kSph = 2;
 __device__ __forceinline__ float wave_max_f(float v) { return -wave_min<float>(-v); }
 
 // one wave: the sphere records of the cnt triangles at off
-__device__ __forceinline__ void vox_sphere(const float4 *__restrict__ tri, uint64_t off,
+__device__ __forceinline__ void vox_sphere(const float4 *__restrict__ tri,
+                                           const float4 *__restrict__ bnd, uint64_t off,
                                            uint32_t cnt, float4 *__restrict__ out) {
     const uint32_t lane = (uint32_t)lane_id();
     if (cnt == 0) {
@@ -645,8 +648,10 @@ __device__ __forceinline__ void vox_sphere(const float4 *__restrict__ tri, uint6
     }
     float lx = INFINITY, ly = INFINITY, lz = INFINITY;
     float hx = -INFINITY, hy = -INFINITY, hz = -INFINITY, mph = -INFINITY, mh = INFINITY;
+    float mcm = 0.f;
     for (uint32_t t = lane; t < cnt; t += kWave) {
         const float4 a = tri[3 * (off + t)], b = tri[3 * (off + t) + 1], c = tri[3 * (off + t) + 2];
+        mcm = fmaxf(mcm, bnd[kBnd * (off + t) + 1].w);
         lx = fminf(lx, fminf(a.x, fminf(b.x, c.x)));
         ly = fminf(ly, fminf(a.y, fminf(b.y, c.y)));
         lz = fminf(lz, fminf(a.z, fminf(b.z, c.z)));
@@ -664,6 +669,7 @@ __device__ __forceinline__ void vox_sphere(const float4 *__restrict__ tri, uint6
     hz = wave_max_f(hz);
     mph = wave_max_f(mph);
     mh = wave_min<float>(mh);
+    mcm = wave_max_f(mcm);
     const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
     // FP32 differences are exact in FP64 and so are their squares; the sum
     // and the root add a few 2^-53, which the factor below covers
@@ -681,7 +687,7 @@ __device__ __forceinline__ void vox_sphere(const float4 *__restrict__ tri, uint6
     const float cm = (r + fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)))) * (1.f + 2.f * kU);
     if (lane == 0) {
         out[0] = make_float4(cx, cy, cz, r);
-        out[1] = make_float4(cm, mph, mh, 0.f);
+        out[1] = make_float4(cm, mph, mh, mcm);
     }
 }
 
@@ -690,11 +696,12 @@ __global__ void __launch_bounds__(kBlock) k_vox_sph(size_t n_vox,
                                                     const uint64_t *__restrict__ vox_tri_off,
                                                     const uint32_t *__restrict__ vox_tri_cnt,
                                                     const float4 *__restrict__ tri,
+                                                    const float4 *__restrict__ bnd,
                                                     float4 *__restrict__ sph) {
     const size_t waves = (size_t)gridDim.x * kWavesPerBlock;
     for (size_t k = (size_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < 5 * n_vox;
          k += waves)
-        vox_sphere(tri, vox_tri_off[k], vox_tri_cnt[k], sph + kSph * k);
+        vox_sphere(tri, bnd, vox_tri_off[k], vox_tri_cnt[k], sph + kSph * k);
 }
 
 // the listed voxels at one slot (records put for a round), one wave each
@@ -703,12 +710,13 @@ __global__ void __launch_bounds__(kBlock) k_vox_sph_list(size_t n_list, int slot
                                                          const uint64_t *__restrict__ vox_tri_off,
                                                          const uint32_t *__restrict__ vox_tri_cnt,
                                                          const float4 *__restrict__ tri,
+                                                         const float4 *__restrict__ bnd,
                                                          float4 *__restrict__ sph) {
     const size_t waves = (size_t)gridDim.x * kWavesPerBlock;
     for (size_t k = (size_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; k < n_list;
          k += waves) {
         const size_t vs = 5 * (size_t)list[k] + slot;
-        vox_sphere(tri, vox_tri_off[vs], vox_tri_cnt[vs], sph + kSph * vs);
+        vox_sphere(tri, bnd, vox_tri_off[vs], vox_tri_cnt[vs], sph + kSph * vs);
     }
 }
 

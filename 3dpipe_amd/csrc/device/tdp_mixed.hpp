@@ -110,7 +110,9 @@ __device__ __forceinline__ TriV load_v(const float4 *__restrict__ rec, uint64_t 
 // other triangle's plane can pass the guard. m = 32u (dc + rA + rB + cmA +
 // cmB) + 1e-6 covers the FP32 rounding of these operations, including the
 // rounded centroid's distance from its triangle's plane (u = 2^-24). The
-// bound is evaluated by mark_pair().
+// bound is evaluated by mark_pair(), with one margin M for all pairs of a
+// voxel pair (mark_margin): a larger margin only narrows the guard and lowers
+// the bound.
 
 // ------------------------------------------------------------ pair walking
 
@@ -174,8 +176,10 @@ struct VpPlan {
     uint32_t nch, ngr;  // tiling (zero for EMPTY / SMALL)
     uint32_t shape;     // cw | R << 8 | swap << 16 | flags << 24
     uint32_t nw;        // mask words (zero for EMPTY / SMALL)
+    float M;            // the mark pass's margin (mark_margin; tiled only)
+    uint32_t pad;
 };
-static_assert(sizeof(VpPlan) == 40, "VpPlan layout");
+static_assert(sizeof(VpPlan) == 48, "VpPlan layout");
 static_assert(kBnd == 3, "a bound record and a vertex record stage alike (3 float4)");
 
 __device__ __forceinline__ VpLod plan_lod(const VpPlan &p) { return VpLod{p.o1, p.o2, p.n1, p.n2}; }
@@ -190,6 +194,32 @@ __device__ __forceinline__ Tiling plan_tiling(const VpPlan &p) {
     return t;
 }
 
+// The margin of mark_pair for every pair of a voxel pair. A pair's margin
+// m = fmaf(32u, dc + cm_S, fmaf(32u, cm_L, 1e-6)) grows with dc, cm_S and
+// cm_L (each rounded operation is monotone), so the same operations on upper
+// bounds of the three give an M at least every pair's m:
+//   cm_S, cm_L  the largest cm of each voxel's triangles (sphere record
+//               field w, k_vox_sph);
+//   dc          Dmax = ((1 + 16u) D + R_S + R_L + 16u (cmV_S + cmV_L))
+//               (1 + 16u), D the FP32 distance of the spheres' centres, R
+//               their radii, cmV their R + max |C coord|.
+// A triangle's exact centroid lies in its voxel's sphere and its FP32
+// centroid within 5u of a coordinate bound (cmV) of it on each axis, so the
+// centroids of a pair lie at most |C_S - C_L| + R_S + R_L + 9u (cmV_S +
+// cmV_L) apart; D is at most 4u below |C_S - C_L|, and a pair's FP32 dc at
+// most 4u above the distance of its centroids, which the factors cover
+// with the rounding of Dmax's own three operations. (sa, sb): the column
+// voxel's sphere records, (ra, rb) the row voxel's. (Formed per voxel pair
+// by k_mx_plan: in the mark pass the dependent sphere loads lengthened every
+// wave's set-up.)
+__device__ __forceinline__ float mark_margin(f4v sa, f4v sb, f4v ra, f4v rb) {
+    const float ex = sa.x - ra.x, ey = sa.y - ra.y, ez = sa.z - ra.z;
+    const float D = __builtin_amdgcn_sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+    const float dmax =
+        (fmaf(D, 1.f + 16.f * kU, sa.w + ra.w) + 16.f * kU * (sb.x + rb.x)) * (1.f + 16.f * kU);
+    return fmaf(32.f * kU, dmax + rb.w, fmaf(32.f * kU, sb.w, 1e-6f));
+}
+
 // Plans of the round's voxel pairs, with their flags and mask word counts:
 // EMPTY (a voxel without triangles at this LOD), SMALL (at most small_pairs
 // triangle pairs: all go to FP64, no mask), else tiled for the FP32 passes.
@@ -199,6 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_plan(
     uint64_t n_vp, const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const uint64_t *__restrict__ off1, const uint32_t *__restrict__ cnt1,
     const uint64_t *__restrict__ off2, const uint32_t *__restrict__ cnt2, int slot,
+    const float4 *__restrict__ sph1, const float4 *__restrict__ sph2,
     uint32_t small_pairs, uint32_t max_rows, VpPlan *__restrict__ plan,
     uint32_t *__restrict__ flags, uint32_t *__restrict__ nwords,
     unsigned long long *__restrict__ stats) {
@@ -206,8 +237,9 @@ __global__ void __launch_bounds__(kBlock) k_mx_plan(
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t pairs = 0;
     if (w < n_vp) {
-        const VpLod g = vp_lod(vp_v1[w], vp_v2[w], off1, cnt1, off2, cnt2, slot);
-        VpPlan p{g.o1, g.o2, g.n1, g.n2, 0u, 0u, 0u, 0u};
+        const uint32_t v1 = vp_v1[w], v2 = vp_v2[w];
+        const VpLod g = vp_lod(v1, v2, off1, cnt1, off2, cnt2, slot);
+        VpPlan p{g.o1, g.o2, g.n1, g.n2, 0u, 0u, 0u, 0u, 0.f, 0u};
         uint32_t fl = 0;
         pairs = (uint64_t)g.n1 * g.n2;
         if (pairs == 0) {
@@ -220,6 +252,11 @@ __global__ void __launch_bounds__(kBlock) k_mx_plan(
             p.ngr = t.ngr;
             p.shape = t.cw | (t.R << 8) | (t.swap ? 1u << 16 : 0u);
             p.nw = mx_words(t);
+            const float4 *c = (t.swap ? sph1 : sph2) + kSph * (5 * (size_t)(t.swap ? v1 : v2) + slot);
+            const float4 *r = (t.swap ? sph2 : sph1) + kSph * (5 * (size_t)(t.swap ? v2 : v1) + slot);
+            const float4 c0 = c[0], c1 = c[1], r0 = r[0], r1 = r[1];
+            p.M = mark_margin(f4v{c0.x, c0.y, c0.z, c0.w}, f4v{c1.x, c1.y, c1.z, c1.w},
+                              f4v{r0.x, r0.y, r0.z, r0.w}, f4v{r1.x, r1.y, r1.z, r1.w});
         }
         p.shape |= fl << 24;
         plan[w] = p;
@@ -248,7 +285,21 @@ struct PairAt {
     uint32_t word;     // iteration number
     bool valid;
     uint32_t k;        // iteration number within the staged block
+    bool last;         // the iteration is the chunk's last row group
+    uint64_t cvm, cvl; // the valid lanes of the chunk's row groups: all but
+                       // the last, the last
 };
+
+// lane masks straight from a compare (one v_cmp into a scalar register
+// pair; a ballot of a combined condition goes through a select and a
+// compare). All 64 lanes must be active.
+constexpr int kCmpOGT = 2, kCmpOLE = 5;  // ordered >, <= (LLVM predicates)
+__device__ __forceinline__ uint64_t lanes_gt(float a, float b) {
+    return __builtin_amdgcn_fcmpf(a, b, kCmpOGT);
+}
+__device__ __forceinline__ uint64_t lanes_le(float a, float b) {
+    return __builtin_amdgcn_fcmpf(a, b, kCmpOLE);
+}
 
 // Row records of the tiled walk, staged in LDS per wave: kStageRows rows of
 // bound records and of triangle records. mx_tiling never gives more than 10
@@ -280,15 +331,18 @@ struct RowStage {
 // whose rows rt passes over is not evaluated: f gets skip = true (wave-
 // uniform) and an empty row record. Without CULL, skip is always false.
 //
-// fb(w0, nb) is called by every lane after each staged block, whose
-// iterations were w0 .. w0 + nb - 1 (nb <= 32).
-template <bool SWAP, bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB>
+// fb(w0, nb, vmask) is called by every lane after each staged block, whose
+// iterations were w0 .. w0 + nb - 1 (nb <= 32), vmask the valid lanes of
+// iteration w0 + lane. sx(record) may rewrite a row's bound record (after rt
+// has seen it) before it is staged.
+template <bool SWAP, bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB,
+          typename SX>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
                                            const float4 *__restrict__ rec1,
                                            const float4 *__restrict__ bnd1,
                                            const float4 *__restrict__ rec2,
                                            const float4 *__restrict__ bnd2, RowStage *stage,
-                                           FC &&fc, F &&f, RT &&rt, FB &&fb) {
+                                           FC &&fc, F &&f, RT &&rt, FB &&fb, SX &&sx) {
     const uint32_t nl = SWAP ? g.n1 : g.n2, ns = SWAP ? g.n2 : g.n1;
     const uint64_t ol = SWAP ? g.o1 : g.o2, os = SWAP ? g.o2 : g.o1;
     const float4 *__restrict__ bl = SWAP ? bnd1 : bnd2;
@@ -314,6 +368,12 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const uint64_t lt = ol + (cv ? col : nl - 1);  // in bounds for idle lanes
         const TriS L = load_s(bl, lt);
         const TriV LV = load_v(rl, lt);
+        // the chunk's valid lanes (cvm), and those of the last row group:
+        // its first ns - (ngr - 1) R rows, held by the lanes below that
+        // many times cw
+        const uint64_t cvm = __ballot(cv);
+        const uint32_t nlast = (ns - (t.ngr - 1u) * t.R) * t.cw;
+        const uint64_t cvl = cvm & (nlast >= 64u ? ~0ull : (1ull << nlast) - 1ull);
         fc(L, LV);
         auto lane_v = [&]() { return LV; };
         // staged blocks of gpb row groups, then the block's iterations in a
@@ -345,55 +405,65 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
                         rq[i].z -= cz;
                     }
                 }
+                if constexpr (CULL) over = is_s && (gb * t.R + q >= ns || rt(rq));
+                if (is_s) sx(rq);
                 float4 *dst = is_s ? &stage->s[q][0] : &stage->v[q][0];
 #pragma unroll
                 for (int i = 0; i < 3; i++) dst[i] = rq[i];
-                if constexpr (CULL) over = is_s && (gb * t.R + q >= ns || rt(rq));
             }
-            // rows of the block rt passes over (bit per row)
-            const uint64_t passed = CULL ? __ballot(over) : 0ull;
+            // rows of the block rt does not pass over (bit per row)
+            const uint64_t kept = CULL ? ~__ballot(over) : ~0ull;
             wave_sync();
             const uint32_t nb = min(gpb, t.ngr - gb);
             uint32_t row = gb * t.R + r;
+            uint64_t gm = gmask;  // the rows of iteration k
             for (uint32_t k = 0, k0 = 0; k < nb; k++, k0 += t.R, row += t.R, word++) {
                 const bool valid = cv && row < ns;
                 // the lane's row slot: a 32-bit LDS address, the lane's part
                 // formed once
-                const bool skip = CULL && ((passed >> k0) & gmask) == gmask;
+                const bool skip = CULL && (kept & gm) == 0ull;
+                gm <<= t.R;
                 TriS S{};
                 if (!skip) S = load_s_lds(s_base + kBnd * k0);
                 auto row_v = [&]() { return load_v_lds(v_base + 3 * k0); };
+                const bool last = gb + k == t.ngr - 1u;
                 if (SWAP)
-                    f(L, S, PairAt{cv ? col : 0u, row, word, valid, k}, lane_v, row_v,
-                      std::integral_constant<bool, true>{}, skip);
+                    f(L, S, PairAt{cv ? col : 0u, row, word, valid, k, last, cvm, cvl}, lane_v,
+                      row_v, std::integral_constant<bool, true>{}, skip);
                 else
-                    f(S, L, PairAt{row, cv ? col : 0u, word, valid, k}, row_v, lane_v,
-                      std::integral_constant<bool, false>{}, skip);
+                    f(S, L, PairAt{row, cv ? col : 0u, word, valid, k, last, cvm, cvl}, row_v,
+                      lane_v, std::integral_constant<bool, false>{}, skip);
             }
-            fb(word - nb, nb);
+            fb(word - nb, nb, gb + (uint32_t)lane == t.ngr - 1u ? cvl : cvm);
         }
     }
 }
 
-template <bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB>
+template <bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB, typename SX>
 __device__ __forceinline__ void for_tiles(const VpLod &g, const Tiling &t, int lane,
                                           const float4 *__restrict__ rec1,
                                           const float4 *__restrict__ bnd1,
                                           const float4 *__restrict__ rec2,
                                           const float4 *__restrict__ bnd2, RowStage *stage,
-                                          FC &&fc, F &&f, RT &&rt, FB &&fb) {
+                                          FC &&fc, F &&f, RT &&rt, FB &&fb, SX &&sx) {
     if (t.swap)
-        walk_tiles<true, VERTS, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f, rt, fb);
+        walk_tiles<true, VERTS, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f, rt, fb,
+                                      sx);
     else
-        walk_tiles<false, VERTS, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f, rt, fb);
+        walk_tiles<false, VERTS, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, stage, fc, f, rt, fb,
+                                       sx);
 }
 // nothing at the end of a staged block
 struct NoBlockEnd {
-    __device__ void operator()(uint32_t, uint32_t) const {}
+    __device__ void operator()(uint32_t, uint32_t, uint64_t) const {}
 };
 // no row test
 struct NoRowTest {
     __device__ bool operator()(const float4 *) const { return false; }
+};
+// row bound records staged as loaded
+struct NoStageEdit {
+    __device__ void operator()(float4 *) const {}
 };
 
 // ------------------------------------------------------------------ bounds
@@ -548,7 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                       best[f] = fminf(best[f], kv);
                   }
               },
-              row_test, NoBlockEnd{});
+              row_test, NoBlockEnd{}, NoStageEdit{});
     up[1] = fmaxf(up[1], 0.f);
     float U[3];
     uint32_t pq[3];
@@ -722,16 +792,16 @@ constexpr float kTG = 1.f - 16.f * kU;
 // per-lane constants of a column triangle L
 struct MarkCol {
     float clo, chi;  // column parts of T_lo, T_hi
-    float mc;        // column part of the margin: 32u (r_L + cm_L) + 1e-6
+    float rm;        // r_L + 2M, the guard's bound for the row's plane offset
     float x0, y0, z0, x1, y1, z1, x2, y2, z2;  // L's vertices less its centre
 };
 
 __device__ __forceinline__ MarkCol mark_col(const TriS &L, const TriV &V, float ulo_h,
-                                            float uhi_h) {
+                                            float uhi_h, float M) {
     MarkCol c;
     c.clo = fmaf(L.ph, kTH, ulo_h);
     c.chi = fmaf(L.h, -kTG, uhi_h);
-    c.mc = fmaf(32.f * kU, L.cm, 1e-6f);  // cm: r + max |centre coord|
+    c.rm = L.r + 2.f * M;
     c.x0 = V.x0 - L.cx; c.y0 = V.y0 - L.cy; c.z0 = V.z0 - L.cz;
     c.x1 = V.x1 - L.cx; c.y1 = V.y1 - L.cy; c.z1 = V.z1 - L.cz;
     c.x2 = V.x2 - L.cx; c.y2 = V.y2 - L.cy; c.z2 = V.z2 - L.cz;
@@ -762,38 +832,44 @@ __device__ __forceinline__ float mark_T(const TriS &S, const MarkCol &c, float u
 //     + |d|^2 from vertices staged less their centre; the rounding of the
 //     vertex differences, the dot products, |d|^2 and the reciprocal is a few
 //     u (dc + r_S + r_L + cm_S + cm_L), inside the margin.
+// M is the voxel pair's margin (mark_margin, at least every pair's m); the
+// row's r_S + 2M comes staged in place of its cm (k_mx_mark), the column's
+// r_L + 2M in c.rm, so the guard is two compares.
 // Operation count per pair through step 1 (add/sub/mul/sqrt/rcp/min/max/
 // compare = 1, fused multiply-add = 2, abs free): centre difference 3, dc 6,
-// margin 4, the two plane offsets 10, guard 5, sines and disc gap 16,
-// comparison 1 -- 45; T adds 5 in rounds that fold lo and hi (50).
+// the two plane offsets 10, guard 3, sines and disc gap 16, comparison 1 --
+// 39; T adds 5 in rounds that fold lo and hi (44).
 //
-// Called by every lane of the wave (cells outside the grid with valid =
-// false, on in-bounds data): the vertex-level bound is a wave-uniform branch,
-// taken when some lane needs it, so no lane mask is saved and restored per
-// pair.
+// Called by every lane of the wave (cells outside the grid run on in-bounds
+// data; the caller masks their bits); returns the mask of the lanes whose
+// pair is kept. The conditions are lane masks in scalar registers
+// throughout, and the vertex-level bound is a wave-uniform branch, taken when
+// some lane needs it, so no lane mask is saved and restored per pair.
 template <typename RV>
-__device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const MarkCol &c, float T,
-                                          bool valid, RV &&row_v) {
+__device__ __forceinline__ uint64_t mark_pair(const TriS &S, const TriS &L, const MarkCol &c,
+                                              float M, float T, RV &&row_v) {
 #pragma clang fp contract(fast)
     const float dx = S.cx - L.cx, dy = S.cy - L.cy, dz = S.cz - L.cz;
     const float dd = dx * dx + dy * dy + dz * dz;
     const float dc = __builtin_amdgcn_sqrtf(dd);
-    const float m = fmaf(32.f * kU, dc + S.cm, c.mc);
     const float eL = L.nx * dx + L.ny * dy + L.nz * dz;
     const float eS = S.nx * dx + S.ny * dy + S.nz * dz;
-    const float m2 = 2.f * m;
-    const bool g = fabsf(eL) > S.r + m2 || fabsf(eS) > L.r + m2;
+    // the lanes with the guard (S.cm: r_S + 2M as staged)
+    const uint64_t g = lanes_gt(fabsf(eL), S.cm) | lanes_gt(fabsf(eS), c.rm);
     // the disc bound for every lane (no branch: nearly every iteration has a
-    // lane with the guard); without the guard the bound is 0. (dc = 0 gives
-    // no guard; its NaN cosines are dropped by fmaxf.)
+    // lane with the guard); without the guard the bound is 0 (a select on
+    // the guard's mask). (dc = 0 gives no guard; its NaN cosines are dropped
+    // by fmaxf.)
     const float inv = __builtin_amdgcn_rcpf(dc);
     const float cl = eL * inv, cs = eS * inv;
     const float sl = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cl, cl, 1.f + 16.f * kU), 16.f * kU));
     const float ss = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cs, cs, 1.f + 16.f * kU), 16.f * kU));
-    const float dl = g ? fmaf(-L.r, sl, fmaf(-S.r, ss, dc - m)) : 0.f;
-    const bool keep = valid && dl <= T;
-    bool take = keep;
-    if (__ballot(keep && g)) {
+    const float dg = fmaf(-L.r, sl, fmaf(-S.r, ss, dc - M));
+    float dl;
+    asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(dl) : "v"(dg), "s"(g));
+    const uint64_t keep = lanes_le(dl, T);
+    uint64_t take = keep;
+    if (keep & g) {
         // the row's vertices less its centre (as staged): S's vertices lie
         // beyond L's along d = cS - cL, at d . (v - cS) + |d|^2 from L's centre
         const TriV v = row_v();
@@ -804,8 +880,8 @@ __device__ __forceinline__ bool mark_pair(const TriS &S, const TriS &L, const Ma
         const float l1 = dx * c.x1 + dy * c.y1 + dz * c.z1;
         const float l2 = dx * c.x2 + dy * c.y2 + dz * c.z2;
         const float gap =
-            ((fminf(s0, fminf(s1, s2)) + dd) - fmaxf(l0, fmaxf(l1, l2))) * inv - m;
-        take = keep && (!g || gap <= T);
+            ((fminf(s0, fminf(s1, s2)) + dd) - fmaxf(l0, fmaxf(l1, l2))) * inv - M;
+        take = keep & (~g | lanes_le(gap, T));
     }
     return take;
 }
@@ -937,13 +1013,12 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     // The row's T takes the column terms' extremes over the voxel (largest
     // ph, smallest h); the fused multiply-adds are monotone, so it is at
     // least every pair's T.
-    // The staging lanes read the sphere with their rows (a broadcast load
-    // issued beside the row loads), so it is neither waited for before the
-    // walk nor kept in registers through it.
+    // The sphere comes by scalar loads and stays in scalar registers.
     typedef __attribute__((address_space(4))) const f4v cst_f4;
     const cst_f4 *const sp =
         (const cst_f4 *)((t.swap ? sph1 : sph2) + kSph * (5 * (size_t)(t.swap ? vw1 : vw2) + slot));
-    const f4v sa = sp[0], sb = sp[1];  // scalar loads: the sphere stays in scalar registers
+    const f4v sa = sp[0], sb = sp[1];
+    const float M = pl.M;  // the voxel pair's margin (k_mx_plan)
     auto row_test = [&](const float4 *pr) -> bool {
 #pragma clang fp contract(fast)
         const float s_mc = fmaf(32.f * kU, sb.x, 1e-6f);
@@ -963,21 +1038,18 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     };
     for_tiles<true, CULL>(
         g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
-        [&](const TriS &L, const TriV &LV) { mc = mark_col(L, LV, ulo_h, uhi_h); },
+        [&](const TriS &L, const TriV &LV) { mc = mark_col(L, LV, ulo_h, uhi_h, M); },
         [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap,
             bool skip) {
             constexpr bool col_is_a = decltype(swap)::value;
             const TriS &S = col_is_a ? B : A;
             const TriS &L = col_is_a ? A : B;
-            bool take = false;
-            if (skip) sk += (uint32_t)__popcll(__ballot(p.valid));
-            else {
-                if constexpr (col_is_a)
-                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), p.valid, vc);
-                else
-                    take = mark_pair(S, L, mc, mark_T<NEED>(S, mc, ud), p.valid, va);
-            }
-            const unsigned long long m = __ballot(take);
+            unsigned long long m = 0;
+            if (skip) sk += (uint32_t)__popcll(p.last ? p.cvl : p.cvm);
+            else if constexpr (col_is_a)
+                m = mark_pair(S, L, mc, M, mark_T<NEED>(S, mc, ud), vc);
+            else
+                m = mark_pair(S, L, mc, M, mark_T<NEED>(S, mc, ud), va);
             // word k of the block into lane k (v_writelane: the ballot's
             // scalar registers written into one lane, no compare or select)
             const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
@@ -989,11 +1061,12 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
             whi = b;
         },
         row_test,
-        [&](uint32_t w0, uint32_t nb) {
-            // lane l holds word w0 + l; the probes' bits are cleared (they are
-            // evaluated already)
+        [&](uint32_t w0, uint32_t nb, uint64_t vmask) {
+            // lane l holds word w0 + l, less the bits of cells outside the
+            // grid; the probes' bits are cleared (they are evaluated already)
             unsigned long long x =
-                ((unsigned long long)(uint32_t)whi << 32) | (unsigned long long)(uint32_t)wlo;
+                (((unsigned long long)(uint32_t)whi << 32) | (unsigned long long)(uint32_t)wlo) &
+                vmask;
 #pragma unroll
             for (int f = 0; f < 3; f++)
                 if (pword[f] == w0 + (uint32_t)lane) x &= ~pbit[f];
@@ -1001,7 +1074,9 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
                 out[w0 + lane] = x;
                 cl += (uint32_t)__popcll(x);
             }
-        });
+        },
+        // the row's r + 2M staged in place of its cm (the guard's bound)
+        [&](float4 *r) { r[1].w = r[0].w + 2.f * M; });
     const uint32_t c = wave_sum_u32(cl);
     if (lane == 0) {
         count[w] = c;

@@ -29,11 +29,12 @@ sys.path.insert(0, ROOT)
 # in tests/test_cpu.py).
 F_TRI = 997
 # Static FP32 operation count of the mixed form's per-pair disc test, the
-# part of k_mx_mark every triangle pair goes through (disc_lower(), the
-# threshold and the comparison; the breakdown is beside disc_lower()). 45 in
-# the round that folds d alone, 50 in the rounds that fold lo and hi; the
-# smaller count is used, so the figure is a lower bound of the kernel's work.
-F_MARK = 45
+# part of k_mx_mark every triangle pair it tests goes through (mark_pair()
+# through the comparison, with the voxel pair's margin; the breakdown is
+# beside mark_pair()). 39 in the round that folds d alone, 44 in the rounds
+# that fold lo and hi; the smaller count is used, so the figure is a lower
+# bound of the kernel's work.
+F_MARK = 39
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per launch on the default workload (seed 1, 200k nuclei x 1000
