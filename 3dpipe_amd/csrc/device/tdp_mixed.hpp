@@ -316,8 +316,9 @@ struct RowStage {
 // swap an std::integral_constant<bool> that is true when the column triangle
 // (the lane's, fixed over a column chunk) is A. All 64 lanes take part in
 // every call (p.valid = false for cells outside the grid), so f may use
-// wave-wide operations. fc(L, LV) is called by every lane at the start of
-// each column chunk with the lane's column triangle (sphere data, vertices).
+// wave-wide operations. fc(L, LV, cv) is called by every lane at the start of
+// each column chunk with the lane's column triangle (sphere data, vertices;
+// cv false for lanes outside the grid, which hold the last column's).
 // Each lane reads its row from `stage`, which the wave fills with the records
 // of the next R * floor(32 / R) rows whenever it starts a block of row groups
 // (the row vertices too when VERTS, less their triangle's centre): one
@@ -333,8 +334,9 @@ struct RowStage {
 //
 // fb(w0, nb, vmask) is called by every lane after each staged block, whose
 // iterations were w0 .. w0 + nb - 1 (nb <= 32), vmask the valid lanes of
-// iteration w0 + lane. sx(record) may rewrite a row's bound record (after rt
-// has seen it) before it is staged.
+// iteration w0 + lane. sx(record, beyond) may rewrite a row's bound record
+// (after rt has seen it) before it is staged; beyond: the row lies past the
+// smaller voxel.
 template <bool SWAP, bool VERTS, bool CULL, typename FC, typename F, typename RT, typename FB,
           typename SX>
 __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int lane,
@@ -374,7 +376,7 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
         const uint64_t cvm = __ballot(cv);
         const uint32_t nlast = (ns - (t.ngr - 1u) * t.R) * t.cw;
         const uint64_t cvl = cvm & (nlast >= 64u ? ~0ull : (1ull << nlast) - 1ull);
-        fc(L, LV);
+        fc(L, LV, cv);
         auto lane_v = [&]() { return LV; };
         // staged blocks of gpb row groups, then the block's iterations in a
         // loop of their own (no per-iteration block test)
@@ -406,7 +408,7 @@ __device__ __forceinline__ void walk_tiles(const VpLod &g, const Tiling &t, int 
                     }
                 }
                 if constexpr (CULL) over = is_s && (gb * t.R + q >= ns || rt(rq));
-                if (is_s) sx(rq);
+                if (is_s) sx(rq, gb * t.R + q >= ns);
                 float4 *dst = is_s ? &stage->s[q][0] : &stage->v[q][0];
 #pragma unroll
                 for (int i = 0; i < 3; i++) dst[i] = rq[i];
@@ -463,10 +465,19 @@ struct NoRowTest {
 };
 // row bound records staged as loaded
 struct NoStageEdit {
-    __device__ void operator()(float4 *) const {}
+    __device__ void operator()(float4 *, bool) const {}
 };
 
 // ------------------------------------------------------------------ bounds
+
+// v_min_f32 as written: fminf in a loop re-canonicalises its running value
+// (one more instruction per call); the operands here are never signalling
+// NaNs
+__device__ __forceinline__ float min_f32(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 
 // lowest q among the lanes holding the wave minimum of v (kNoProbe if none)
 __device__ __forceinline__ uint32_t wave_argmin_q(float v, uint32_t q) {
@@ -566,23 +577,26 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
         if ((NEED & need) & 4u) over = over && lb + pr[2].y + sb.z > Thi;
         return over;
     };
+    // Cells outside the grid get +inf bounds and scores, so every lane
+    // computes every cell with no select and no lane mask: a lane outside
+    // the grid has +inf column constants, and a row past the smaller voxel is
+    // staged with +inf row terms. The staged row record carries the scores'
+    // row terms (-r, -(r + ph), h - r) in place of its normal, which this
+    // pass does not read.
     for_tiles<false, CULL>(g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
-              [&](const TriS &L, const TriV &) {
-                  crg = L.r * kUG;
-                  calo = L.alo;
-                  cahi = L.ahi;
-                  ck0 = -L.r;
-                  ck1 = -(L.r + L.ph);
-                  ck2 = L.h - L.r;
+              [&](const TriS &L, const TriV &, bool cv) {
+                  crg = cv ? L.r * kUG : INFINITY;
+                  calo = cv ? L.alo : INFINITY;
+                  cahi = cv ? L.ahi : INFINITY;
+                  ck0 = cv ? -L.r : INFINITY;
+                  ck1 = cv ? -(L.r + L.ph) : INFINITY;
+                  ck2 = cv ? L.h - L.r : INFINITY;
               },
               [&](const TriS &A, const TriS &B, const PairAt &p, auto &&, auto &&, auto swap,
                   bool skip) {
                   // an iteration whose rows the row test passed over adds
                   // nothing (a wave-uniform branch)
                   if (skip) return;
-                  // every lane computes (cells outside the grid on in-bounds
-                  // data) and takes part only when valid: no lane mask saved
-                  // and restored per iteration
                   constexpr bool col_is_a = decltype(swap)::value;
                   const TriS &S = col_is_a ? B : A;
                   const TriS &L = col_is_a ? A : B;
@@ -593,32 +607,42 @@ __global__ void __launch_bounds__(kBlock) k_mx_bounds(
                       dc = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
                   }
                   float x[3], key[3];
+                  // S.nx, S.ny, S.nz: the staged score terms (see above)
                   if (NEED & 1u) {
                       x[0] = fmaf(dc, kUF, fmaf(S.r, kUG, crg));
-                      key[0] = dc + (ck0 - S.r);
+                      key[0] = dc + (ck0 + S.nx);
                   }
                   if (NEED & 2u) {
                       x[1] = fmaf(dc, kUF, S.alo + calo);
-                      key[1] = dc + (ck1 - (S.r + S.ph));
+                      key[1] = dc + (ck1 + S.ny);
                   }
                   if (NEED & 4u) {
                       x[2] = fmaf(dc, kUF, S.ahi + cahi);
-                      key[2] = dc + (ck2 + (S.h - S.r));
+                      key[2] = dc + (ck2 + S.nz);
                   }
                   const uint32_t q = p.i * n2 + p.j;
 #pragma unroll
                   for (int f = 0; f < 3; f++) {
                       if (!((NEED >> f) & 1u)) continue;
-                      // selects and exact minima: cells outside the grid
-                      // count as +inf
-                      const float xv = p.valid ? x[f] : INFINITY;
-                      const float kv = p.valid ? key[f] : INFINITY;
-                      up[f] = fminf(up[f], xv);
-                      bq[f] = kv < best[f] ? q : bq[f];
-                      best[f] = fminf(best[f], kv);
+                      up[f] = min_f32(up[f], x[f]);
+                      bq[f] = key[f] < best[f] ? q : bq[f];
+                      best[f] = min_f32(best[f], key[f]);
                   }
               },
-              row_test, NoBlockEnd{}, NoStageEdit{});
+              row_test, NoBlockEnd{},
+              [](float4 *r, bool beyond) {
+                  // the scores' row terms in place of the normal; +inf row
+                  // terms for a row past the smaller voxel
+                  const float k0 = -r[0].w, k1 = -(r[0].w + r[2].x), k2 = r[2].y - r[0].w;
+                  r[1].x = beyond ? INFINITY : k0;
+                  r[1].y = beyond ? INFINITY : k1;
+                  r[1].z = beyond ? INFINITY : k2;
+                  if (beyond) {
+                      r[0].w = INFINITY;
+                      r[2].z = INFINITY;
+                      r[2].w = INFINITY;
+                  }
+              });
     up[1] = fmaxf(up[1], 0.f);
     float U[3];
     uint32_t pq[3];
@@ -1038,7 +1062,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     };
     for_tiles<true, CULL>(
         g, t, lane, rec1, bnd1, rec2, bnd2, &stage[threadIdx.x / kWave],
-        [&](const TriS &L, const TriV &LV) { mc = mark_col(L, LV, ulo_h, uhi_h, M); },
+        [&](const TriS &L, const TriV &LV, bool) { mc = mark_col(L, LV, ulo_h, uhi_h, M); },
         [&](const TriS &A, const TriS &B, const PairAt &p, auto &&va, auto &&vc, auto swap,
             bool skip) {
             constexpr bool col_is_a = decltype(swap)::value;
@@ -1076,7 +1100,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
             }
         },
         // the row's r + 2M staged in place of its cm (the guard's bound)
-        [&](float4 *r) { r[1].w = r[0].w + 2.f * M; });
+        [&](float4 *r, bool) { r[1].w = r[0].w + 2.f * M; });
     const uint32_t c = wave_sum_u32(cl);
     if (lane == 0) {
         count[w] = c;
