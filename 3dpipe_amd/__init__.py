@@ -340,9 +340,9 @@ def _read_result(r) -> "JoinResult":
         rd = np.zeros(n, np.uint8)
         tdb.tdb_result_get(r, _ptr(t), _ptr(b), _ptr(mn), _ptr(mx), _ptr(rd))
         tm = np.zeros(10, np.float64)
-        st = np.zeros(8, np.uint64)
+        st = np.zeros(9, np.uint64)
         tdb.tdb_result_timing_n(r, _ptr(tm), 10)
-        tdb.tdb_result_stats_n(r, _ptr(st), 8)
+        tdb.tdb_result_stats_n(r, _ptr(st), 9)
         sm = np.zeros(len(STAGES), np.float64)
         sn = np.zeros(len(STAGES), np.uint64)
         tdb.tdb_result_stages(r, _ptr(sm), _ptr(sn), len(STAGES))
@@ -354,7 +354,8 @@ def _read_result(r) -> "JoinResult":
     timing = dict(zip(["total", "candidates", "filter", "refine", "filter_kernels",
                        "refine_kernels", "upload", "setup", "download", "assembly"], tm.tolist()))
     stats = dict(zip(["tri_pairs", "voxel_pairs", "fp64_evals", "filter_voxel_pairs",
-                      "object_pairs", "rounds", "fp32_culled", "mark_tested"],
+                      "object_pairs", "rounds", "fp32_culled", "mark_tested",
+                      "fp64_probe_evals"],
                      (int(x) for x in st)))
     return JoinResult(t, b, mn, mx, rd, nt, timing, stats, dict(zip(STAGES, sm.tolist())),
                       dict(zip(STAGES, (int(x) for x in sn))),

@@ -116,6 +116,7 @@ struct tdp_join {
     VpPlan *mx_plan = nullptr;  // per voxel pair: triangles and tiling (k_mx_plan)
     unsigned long long *mx_pwb = nullptr;  // per voxel pair: its probes' mask word and bit
     uint32_t *mx_tested = nullptr;  // per voxel pair: pairs the mark pass tested one by one
+    MarkPrep *mx_prep = nullptr;     // per voxel pair: the mark pass's thresholds (k_mx_prep)
     uint32_t *mx_nw = nullptr;   // candidate-mask words per voxel pair
     uint64_t *mx_woff = nullptr; // their exclusive scan
     size_t cap_mx = 0;
@@ -1346,7 +1347,7 @@ extern "C" int tdp_join_destroy(tdp_join *j) {
                     j->round, j->scratch, j->confirmed, j->acc_min, j->acc_max, j->count,
                     j->offs, j->keep, j->pos, j->stats, j->mx_U, j->mx_probe, j->mx_flags,
                     j->mx_key, j->mx_nw, j->mx_woff, j->mx_mask, j->item_vp, j->item_q,
-                    j->mx_plan, j->mx_pwb, j->mx_tested};
+                    j->mx_plan, j->mx_pwb, j->mx_tested, j->mx_prep};
     for (void *p : ptrs) pfree(ctx, p);
     for (int b = 0; b < 2; b++) {
         pfree(ctx, j->v1[b]); pfree(ctx, j->v2[b]); pfree(ctx, j->vpp[b]); pfree(ctx, j->vmin[b]); pfree(ctx, j->vmax[b]);
@@ -1448,7 +1449,7 @@ static int join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
     if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->acc_max, P * j->rsz);
     if (rc == TDP_OK) rc = palloc(ctx, &j->count, P);
     if (rc == TDP_OK) rc = palloc(ctx, &j->offs, P);
-    if (rc == TDP_OK) rc = palloc(ctx, &j->stats, 4);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->stats, 5);
     if (rc == TDP_OK) rc = palloc(ctx, &j->work, 4);
     if (rc == TDP_OK && prm->precision == TDP_PREC_MIXED) rc = palloc(ctx, &j->obj_u, P);
     if (rc != TDP_OK) {
@@ -1481,7 +1482,7 @@ static int join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
         for (auto &x : wv) x = x ? 1 : 0;
         e = copy_sync(ctx, j->within_t, wv.data(), N, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = set_sync(ctx, j->stats, 0, 32);
+    if (e == hipSuccess) e = set_sync(ctx, j->stats, 0, 40);
     if (e == hipSuccess) e = set_sync(ctx, j->confirmed, 0, 4 * (N ? N : 1));
     if (e == hipSuccess) e = set_sync(ctx, j->round, 0xff, P ? P : 1);
     if (e == hipSuccess) e = set_sync(ctx, j->state, 0, P ? P : 1);
@@ -1564,10 +1565,10 @@ static int ensure_mx(tdp_join *j, size_t n) {
     const size_t cap = std::max(n, j->cap_vp);
     pfree(ctx, j->mx_U); pfree(ctx, j->mx_probe); pfree(ctx, j->mx_flags); pfree(ctx, j->mx_key);
     pfree(ctx, j->mx_nw); pfree(ctx, j->mx_woff); pfree(ctx, j->mx_plan); pfree(ctx, j->mx_pwb);
-    pfree(ctx, j->mx_tested);
+    pfree(ctx, j->mx_tested); pfree(ctx, j->mx_prep);
     j->mx_U = nullptr; j->mx_probe = nullptr; j->mx_flags = nullptr; j->mx_key = nullptr;
     j->mx_nw = nullptr; j->mx_woff = nullptr; j->mx_plan = nullptr; j->mx_pwb = nullptr;
-    j->mx_tested = nullptr;
+    j->mx_tested = nullptr; j->mx_prep = nullptr;
     j->cap_mx = 0;
     int rc = palloc(ctx, &j->mx_U, 3 * cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_probe, 3 * cap);
@@ -1578,6 +1579,7 @@ static int ensure_mx(tdp_join *j, size_t n) {
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_plan, cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_pwb, 3 * cap);
     if (rc == TDP_OK) rc = palloc(ctx, &j->mx_tested, cap);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->mx_prep, cap);
     if (rc == TDP_OK) j->cap_mx = cap;
     return rc;
 }
@@ -1774,29 +1776,27 @@ static void launch_decide(tdp_join *j, uint8_t rnd) {
 // The kernel reads the arrays whether or not a cap applies (its loads are
 // issued together, before any branch); `caps` says which apply: 1 the range
 // caps, 2 the object cap.
-struct MarkCaps {
-    const double *vp_min = nullptr, *vp_max = nullptr;
-    const uint32_t *vp_pair = nullptr, *obj_u = nullptr;
-    int caps = 0;
-};
-static MarkCaps mark_caps(tdp_join *j, int is_last, uint64_t n) {
-    MarkCaps m;
+// The mark pass's thresholds per voxel pair (k_mx_prep) with the caps that
+// apply: in the last round the object pairs' smallest U_d (k_mx_obj_u), in a
+// round that folds lo and hi the voxel pairs' ranges; none in the
+// intersection join, whose fold keeps no range.
+static void mark_prep(tdp_join *j, int is_last, uint64_t n, int lo_free) {
     const int c = j->cur;
-    m.vp_pair = j->vpp[c];
-    m.vp_min = (const double *)j->vmin[c];
-    m.vp_max = (const double *)j->vmax[c];
-    m.obj_u = j->obj_u;
-    if (j->prm.query == TDP_QUERY_INTERSECT) return m;
     hipStream_t s = j->ctx->stream;
-    if (is_last) {
-        k_fill_u32<<<blocks_for(j->P, kBlock), kBlock, 0, s>>>(j->P, j->obj_u, 0xff800000u);
-        k_mx_obj_u<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->mx_plan, j->mx_U, j->mx_key,
-                                                             j->vpp[c], j->obj_u);
-        m.caps = 2;
-    } else if (fold_need(j, is_last) == 6u) {
-        m.caps = 1;
+    int caps = 0;
+    if (j->prm.query != TDP_QUERY_INTERSECT) {
+        if (is_last) {
+            k_fill_u32<<<blocks_for(j->P, kBlock), kBlock, 0, s>>>(j->P, j->obj_u, 0xff800000u);
+            k_mx_obj_u<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, j->mx_plan, j->mx_U,
+                                                                 j->mx_key, j->vpp[c], j->obj_u);
+            caps = 2;
+        } else if (fold_need(j, is_last) == 6u) {
+            caps = 1;
+        }
     }
-    return m;
+    k_mx_prep<<<blocks_for(n, kBlock), kBlock, 0, s>>>(
+        n, j->mx_plan, j->mx_U, j->mx_key, j->mx_pwb, (const double *)j->vmin[c],
+        (const double *)j->vmax[c], j->vpp[c], j->obj_u, caps, lo_free, j->mx_prep);
 }
 
 template <typename R, int MODE>
@@ -1887,15 +1887,14 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
         });
         TDP_LAUNCHED(ctx);
         if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
-        const MarkCaps cap = mark_caps(j, is_last, n);
+        mark_prep(j, is_last, n, lo_free ? 1 : 0);
         TDP_LAUNCHED(ctx);
         with_need(need, [&](auto nd) {
             auto go = [&](auto cull) {
                 k_mx_mark<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(
-                    n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe,
-                    j->mx_key, j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u,
-                    cap.caps, lo_free ? 1 : 0, j->mx_pwb, j->v1[c], j->v2[c], A->sph, B->sph,
-                    slot, j->mx_mask, j->keep, j->mx_tested);
+                    n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_prep, j->mx_woff,
+                    j->v1[c], j->v2[c], A->sph, B->sph, slot, j->mx_mask, j->keep,
+                    j->mx_tested);
             };
             if (row_cull(j, need)) go(std::true_type{});
             else go(std::false_type{});
@@ -2078,15 +2077,13 @@ static int join_refine_graph(tdp_join *j, int lod, int is_last, uint64_t *pairs_
             j->mx_pwb);
     });
     launched();
-    const MarkCaps cap = mark_caps(j, is_last, n);
+    mark_prep(j, is_last, n, lo_free ? 1 : 0);
     launched();
     with_need(need, [&](auto nd) {
         auto go = [&](auto cull) {
             k_mx_mark<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(
-                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_U, j->mx_probe, j->mx_key,
-                j->mx_woff, cap.vp_min, cap.vp_max, cap.vp_pair, cap.obj_u, cap.caps,
-                lo_free ? 1 : 0, j->mx_pwb, j->v1[c], j->v2[c], A->sph, B->sph, slot,
-                j->mx_mask, j->keep, j->mx_tested);
+                n, j->mx_plan, A->tri, A->bnd, B->tri, B->bnd, j->mx_prep, j->mx_woff, j->v1[c],
+                j->v2[c], A->sph, B->sph, slot, j->mx_mask, j->keep, j->mx_tested);
         };
         if (row_cull(j, need)) go(std::true_type{});
         else go(std::false_type{});
@@ -2359,13 +2356,14 @@ extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
 
 extern "C" int tdp_join_stats_n(tdp_join *j, uint64_t *out, int n) {
     if (!j || !out || n < 0) return TDP_ERR_ARG;
-    uint64_t a[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t a[7] = {0, 0, 0, 0, 0, 0, 0};
     const int rc = tdp_join_stats(j, a);
     if (rc != TDP_OK) return rc;
-    unsigned long long s[4];
+    unsigned long long s[5];
     TDP_CHECK(j->ctx, copy_sync(j->ctx, s, j->stats, sizeof s, hipMemcpyDeviceToHost));
     a[5] = s[3];
-    for (int k = 0; k < n && k < 6; k++) out[k] = a[k];
+    a[6] = s[4];
+    for (int k = 0; k < n && k < 7; k++) out[k] = a[k];
     return TDP_OK;
 }
 

@@ -775,7 +775,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
         seg_min_fold<NEED>(w, a, b, c, mkey);
         n += (uint64_t)__popcll(__ballot(act));
     }
-    if (lane_id() == 0 && n) atomicAdd(&stats[2], (unsigned long long)n);
+    if (lane_id() == 0 && n) {
+        atomicAdd(&stats[2], (unsigned long long)n);
+        if (vp_from_index3) atomicAdd(&stats[4], (unsigned long long)n);  // the probes
+    }
 }
 
 // ------------------------------------------------------------ FP32 pass
@@ -924,6 +927,62 @@ __global__ void __launch_bounds__(kBlock) k_mx_obj_u(
     atomicMin(&obj_u[vp_pair[w]], fkey(u));
 }
 
+// A tiled voxel pair's figures for the mark pass: its thresholds and where
+// its probes lie in its candidate masks (k_mx_prep).
+struct MarkPrep {
+    float ud;            // U_d
+    float ulo_h, uhi_h;  // U_lo, U_hi times (1 + 8u)
+    uint32_t pbits;      // bit of probe f in its word at bits 8f..8f+5
+    uint32_t pw[3];      // word of probe f (~0: none)
+    uint32_t pad;
+};
+static_assert(sizeof(MarkPrep) == 32, "MarkPrep layout");
+
+// The mark pass's thresholds per tiled voxel pair, one thread each (worked
+// out by each voxel pair's wave they took a chain of dependent loads and
+// about a hundred wave instructions before its walk could start).
+//
+// U per quantity: the FP32 upper bound tightened by the exact values found
+// so far (rounded up to FP32); -inf for lo once it is known to be +0, its
+// smallest value; lo without an FP32 bound this round (lo_free) takes only
+// the probes' exact values; then the caps of k_mx_mark (bit 0: the voxel
+// pair's range, bit 1: the object pair's smallest U_d). The probes' words
+// and bits come from k_mx_eval (pwb).
+__global__ void __launch_bounds__(kBlock) k_mx_prep(
+    uint64_t n_vp, const VpPlan *__restrict__ plan, const float *__restrict__ vpU,
+    const unsigned long long *__restrict__ mkey, const unsigned long long *__restrict__ pwb,
+    const double *__restrict__ vp_min, const double *__restrict__ vp_max,
+    const uint32_t *__restrict__ vp_pair, const uint32_t *__restrict__ obj_u, int caps,
+    int lo_free, MarkPrep *__restrict__ prep) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_vp || plan_flags(plan[w])) return;
+    const size_t w3 = 3 * w;
+    const unsigned long long q1 = mkey[w3 + 1];
+    float U0 = fminf(vpU[w3], ulp_up((float)dunkey(mkey[w3])));
+    float U1 = q1 == dkey(0.0) ? -INFINITY
+                               : (lo_free ? ulp_up((float)dunkey(q1))
+                                          : fminf(vpU[w3 + 1], ulp_up((float)dunkey(q1))));
+    float U2 = fminf(vpU[w3 + 2], ulp_up((float)dunkey(mkey[w3 + 2])));
+    if (caps & 2) U0 = fminf(U0, funkey(obj_u[vp_pair[w]]));
+    if (caps & 1) {
+        U1 = fminf(U1, ulp_up((float)vp_min[w]));
+        U2 = fminf(U2, ulp_up((float)vp_max[w]));
+    }
+    MarkPrep m;
+    m.ud = U0;
+    m.ulo_h = U1 * kTH;
+    m.uhi_h = U2 * kTH;
+    m.pbits = 0;
+    m.pad = 0;
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const unsigned long long at = pwb[w3 + f];
+        m.pw[f] = at == ~0ull ? 0xffffffffu : (uint32_t)(at >> 6);
+        m.pbits |= (uint32_t)(at & 63u) << (8 * f);
+    }
+    prep[w] = m;
+}
+
 // Candidate bit masks: word k of voxel pair w (at mask[woff[w] + k]) holds
 // the ballot of iteration k of the tiled walk; count[w] = number of bits.
 // A pair is a candidate when mark_pair keeps it; the probes (already
@@ -931,7 +990,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_obj_u(
 // and bit worked out once per voxel pair, a scalar test per iteration
 // instead of three vector compares per pair). SMALL voxel pairs:
 // every pair, no mask. NEED: the quantities of the round (U is -inf for the
-// others).
+// others). The thresholds come from k_mx_prep.
 //
 // Two caps tighten U where a round's fold does not need every voxel pair's
 // exact minima (both keep every pair that can change a folded value):
@@ -954,12 +1013,8 @@ template <uint32_t NEED, bool CULL>
 __global__ void __launch_bounds__(kBlock) k_mx_mark(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const float4 *__restrict__ rec1,
     const float4 *__restrict__ bnd1, const float4 *__restrict__ rec2,
-    const float4 *__restrict__ bnd2, const float *__restrict__ vpU,
-    const uint32_t *__restrict__ probe, const unsigned long long *__restrict__ mkey,
-    const uint64_t *__restrict__ woff, const double *__restrict__ vp_min,
-    const double *__restrict__ vp_max, const uint32_t *__restrict__ vp_pair,
-    const uint32_t *__restrict__ obj_u, int caps, int lo_free,
-    const unsigned long long *__restrict__ pwb,
+    const float4 *__restrict__ bnd2, const MarkPrep *__restrict__ prep,
+    const uint64_t *__restrict__ woff,
     const uint32_t *__restrict__ vp_v1, const uint32_t *__restrict__ vp_v2,
     const float4 *__restrict__ sph1, const float4 *__restrict__ sph2, int slot,
     unsigned long long *__restrict__ mask, uint32_t *__restrict__ count,
@@ -968,17 +1023,17 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     if (w >= n_vp) return;
     const int lane = lane_id();
     // Every figure of the voxel pair is requested before any is used: one
-    // wait for all of them instead of a chain of dependent waits (the cap
-    // arrays are always valid; `caps` says which caps apply).
-    const size_t w3 = 3 * (size_t)w;
+    // wait for all of them (the empty asm takes them all before the first
+    // branch; left alone, the compiler moves each load next to its first
+    // use, behind the EMPTY / SMALL branch, and the loads wait one after
+    // another). The figures of k_mx_prep are not written for EMPTY and
+    // SMALL voxel pairs, which do not use them.
     const VpPlan pl = plan[w];
-    const uint32_t vw1 = vp_v1[w], vw2 = vp_v2[w], vpp = vp_pair[w];
-    const float u0 = vpU[w3], u1 = vpU[w3 + 1], u2 = vpU[w3 + 2];
-    const unsigned long long q0 = mkey[w3], q1 = mkey[w3 + 1], q2 = mkey[w3 + 2];
-    const unsigned long long a0 = pwb[w3], a1 = pwb[w3 + 1], a2 = pwb[w3 + 2];
+    const MarkPrep pr = prep[w];
+    const uint32_t vw1 = vp_v1[w], vw2 = vp_v2[w];
     const uint64_t wo = woff[w];
-    const double vmn = vp_min[w], vmx = vp_max[w];
-    const uint32_t ou = obj_u[vpp];
+    asm volatile("" ::"s"(pl.shape), "s"(pl.o1), "s"(pl.o2), "s"(pl.n1), "s"(pl.n2), "s"(pl.M),
+                 "s"(pr.ud), "s"(pr.pw[0]), "s"(wo), "s"(vw1), "s"(vw2));
     const uint32_t fl = plan_flags(pl);
     if (fl & MX_EMPTY) {
         if (lane == 0) count[w] = tested[w] = 0;
@@ -992,31 +1047,16 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
         }
         return;
     }
-    // U per quantity: the FP32 upper bound tightened by the exact values
-    // found so far (rounded up to FP32); -inf for lo once it is known to be
-    // +0, its smallest value; lo without an FP32 bound this round takes only
-    // the probes' exact values; then the caps
-    const bool lo0 = q1 == dkey(0.0);
-    float U0 = fminf(u0, ulp_up((float)dunkey(q0)));
-    float U1 = lo0 ? -INFINITY : (lo_free ? ulp_up((float)dunkey(q1))
-                                          : fminf(u1, ulp_up((float)dunkey(q1))));
-    float U2 = fminf(u2, ulp_up((float)dunkey(q2)));
-    if (caps & 2) U0 = fminf(U0, funkey(ou));
-    if (caps & 1) {
-        U1 = fminf(U1, ulp_up((float)vmn));
-        U2 = fminf(U2, ulp_up((float)vmx));
-    }
-    const float ud = U0, ulo_h = U1 * kTH, uhi_h = U2 * kTH;
+    const float ud = pr.ud, ulo_h = pr.ulo_h, uhi_h = pr.uhi_h;
     const Tiling t = plan_tiling(pl);
     unsigned long long *__restrict__ out = mask + wo;
-    // the probes' words and bits in the tiling (k_mx_eval)
+    // the probes' words and bits in the tiling
     uint32_t pword[3];
     unsigned long long pbit[3];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        const unsigned long long at = f == 0 ? a0 : (f == 1 ? a1 : a2);
-        pword[f] = at == ~0ull ? 0xffffffffu : (uint32_t)(at >> 6);
-        pbit[f] = at == ~0ull ? 0ull : 1ull << (at & 63u);
+        pword[f] = pr.pw[f];
+        pbit[f] = pr.pw[f] == 0xffffffffu ? 0ull : 1ull << ((pr.pbits >> (8 * f)) & 63u);
     }
     // the words of the current staged block, lane k holding word k, stored
     // together after the block; each lane counts the bits of the words it

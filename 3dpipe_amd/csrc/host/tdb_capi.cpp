@@ -146,8 +146,8 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     double *const mn = (double *)pin, *const mx = mn + P;
     uint8_t *const rnd = (uint8_t *)(mx + P);
     if (rc == TDP_OK) rc = tdp_join_results(j, rnd, mn, mx);
-    uint64_t st[6] = {0, 0, 0, 0, 0, 0};
-    if (rc == TDP_OK) rc = tdp_join_stats_n(j, st, 6);
+    uint64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (rc == TDP_OK) rc = tdp_join_stats_n(j, st, 7);
     if (rc == TDP_OK) tdp_join_kernel_ms(j, &res->timing[5], &res->timing[4]);
     if (rc == TDP_OK) tdp_join_stage_ms(j, res->stage_ms, res->stage_n, TDP_STAGE_COUNT);
     if (rc != TDP_OK) {
@@ -161,6 +161,7 @@ int tdb_run_join(tdp_ctx *ctx, tdp_tile *d1, tdp_tile *d2, const tdb_join_opts *
     for (int k = 0; k < 4; k++) res->stats[k] = st[k];
     res->stats[6] = st[4];
     res->stats[7] = st[5];
+    res->stats[8] = st[6];
     // per target: reported pairs ordered by (round, list position); targets
     // in parallel, each written at its offset
     std::vector<uint64_t> at(N + 1, 0);
@@ -763,7 +764,7 @@ void tdb_result_stats(const tdb_result *r, uint64_t *out7) {
 }
 int tdb_result_stats_n(const tdb_result *r, uint64_t *out, int n) {
     if (!r || !out || n < 0) return 0;
-    n = std::min(n, 8);
+    n = std::min(n, 9);
     for (int k = 0; k < n; k++) out[k] = r->stats[k];
     return n;
 }

@@ -66,7 +66,7 @@ struct tdb_result {
     double timing[10] = {};
     double stage_ms[TDP_STAGE_COUNT] = {};
     uint64_t stage_n[TDP_STAGE_COUNT] = {};
-    uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double stream[16] = {};  // streaming figures (tdb_result_stream_n)
 };
 

@@ -718,7 +718,7 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
             R.rmax.insert(R.rmax.end(), r.rmax.begin(), r.rmax.end());
             R.round.insert(R.round.end(), r.round.begin(), r.round.end());
             for (int k = 2; k < 10; k++) if (k != 6) R.timing[k] += r.timing[k];
-            for (int k = 0; k < 8; k++)
+            for (int k = 0; k < 9; k++)
                 if (k != 4) R.stats[k] = k == 5 ? std::max(R.stats[k], r.stats[k]) : R.stats[k] + r.stats[k];
             for (int k = 0; k < TDP_STAGE_COUNT; k++) {
                 R.stage_ms[k] += r.stage_ms[k];
@@ -824,7 +824,7 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     }
     for (const tdb_result &r : parts) {
         for (int k = 2; k < 10; k++) if (k != 6) res->timing[k] += r.timing[k];
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < 9; k++)
             if (k != 4) res->stats[k] = k == 5 ? std::max(res->stats[k], r.stats[k]) : res->stats[k] + r.stats[k];
         for (int k = 0; k < TDP_STAGE_COUNT; k++) {
             res->stage_ms[k] += r.stage_ms[k];

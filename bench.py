@@ -477,6 +477,7 @@ def main():
                    "triangle_pairs": tri_pairs, "voxel_pairs": r.stats["voxel_pairs"],
                    "fp32_culled_tri_pairs": r.stats["fp32_culled"],
                    "fp64_tridist_evals": r.stats["fp64_evals"],
+                   "fp64_probe_evals": r.stats.get("fp64_probe_evals", 0),
                    "object_pairs": r.stats["object_pairs"], "results": int(len(r.target)),
                    "tuning": eng.tuning(), "autotune": tuned,
                    "timing_ms": r.timing_ms},

@@ -167,8 +167,9 @@ int tdb_result_timing_n(const tdb_result *r, double *out, int n);
  * (mixed form) */
 void tdb_result_stats(const tdb_result *r, uint64_t *out7);
 /* the same and, at [7], the triangle pairs the mixed form's candidate pass
- * tested one by one (tdp_join_stats_n); the first min(n, 8). No counterpart
- * in the reference (statistics). */
+ * tested one by one, at [8] the FP64 evaluations of the probes (part of [2];
+ * tdp_join_stats_n); the first min(n, 9). No counterpart in the reference
+ * (statistics). */
 int tdb_result_stats_n(const tdb_result *r, uint64_t *out, int n);
 /* device time (ms) and launches per refinement stage, TDP_STAGE_* order of
  * tdp.h (n up to TDP_STAGE_COUNT) */

@@ -339,8 +339,9 @@ int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin, double *r
 int tdp_join_stats(tdp_join *j, uint64_t *out5);
 /* The same figures and, at [5], the triangle pairs the MIXED form's candidate
  * pass tested one by one (the others lie in rows it ruled out whole against
- * the voxel spheres, or in voxel pairs small enough for FP64 alone); the
- * first min(n, 6). No counterpart in the reference (statistics). */
+ * the voxel spheres, or in voxel pairs small enough for FP64 alone), at [6]
+ * the FP64 TriDist evaluations of the probes (part of [2]); the first
+ * min(n, 7). No counterpart in the reference (statistics). */
 int tdp_join_stats_n(tdp_join *j, uint64_t *out, int n);
 /* device time (ms, from HIP events) spent in the refine kernels so far */
 int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
