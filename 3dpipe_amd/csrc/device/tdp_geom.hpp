@@ -198,18 +198,24 @@ __device__ __forceinline__ R tri_dist(const R *S, const R *T, bool *cert = nullp
 // voxel-pair bounds: [box min distance, core-core distance]
 // (voxel_dist_core, computing_gpu.cu:1606-1631); boxes are 9 floats
 // squared box gap s and squared core distance c of two voxel records
-// {low, high, core}; voxel_bound is (sqrt s, sqrt c). The square root being
-// monotone, min sqrt(s) over many pairs is sqrt(min s), bit for bit.
-template <typename R>
-__device__ __forceinline__ void voxel_sq(const float *a, const float *b, R &s, R &c) {
+// {low, high, core}, given in R; voxel_bound is (sqrt s, sqrt c). The square
+// root being monotone, min sqrt(s) over many pairs is sqrt(min s), bit for
+// bit. Per axis the reference adds t2^2 when t2 < 0, else t1^2 when t1 > 0
+// (t1 <= t2 always: the extents are ordered). Here the gap g = max(-t2, t1,
+// 0) is squared and always added: g is -t2 when t2 < 0, t1 when t1 > 0 and
+// +-0 otherwise, and (-t2)^2 = t2^2 while +0 added to s (>= 0) leaves it
+// unchanged, so the bits are the same, with no branch (a NaN operand of max
+// gives the other, as a false comparison skips its term).
+template <typename R, typename T>
+__device__ __forceinline__ void voxel_sq(const T *a, const T *b, R &s, R &c) {
     s = 0;
     c = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const R t1 = (R)a[i] - (R)b[i + 3];
         const R t2 = (R)a[i + 3] - (R)b[i];
-        if (t2 < 0) s += t2 * t2;
-        else if (t1 > 0) s += t1 * t1;
+        const R g = __builtin_fmax(__builtin_fmax(-t2, t1), (R)0);
+        s += g * g;
         const R dc = (R)a[i + 6] - (R)b[i + 6];
         c += dc * dc;
     }

@@ -147,15 +147,16 @@ __global__ void __launch_bounds__(kBlock) k_aggregate(
         const float *__restrict__ sb = sw ? vb2 + 9 * (size_t)b0 : vb1 + 9 * (size_t)a0;
         const uint32_t r = lane / cw, c = lane - r * cw;
         const bool in_tile = r < rows;
-        // the rows' voxel records staged in LDS when they fit (one load per
-        // voxel instead of one per voxel and iteration)
-        __shared__ float stage_all[kWavesPerBlock][kWave * 9];
-        float *const stage = stage_all[threadIdx.x / kWave];
+        // the rows' voxel records staged in LDS in R when they fit (one load
+        // and one conversion per voxel instead of one per voxel and
+        // iteration); the column's converted once per chunk
+        __shared__ R stage_all[kWavesPerBlock][kWave * 9];
+        R *const stage = stage_all[threadIdx.x / kWave];
         const bool staged = ns <= (uint32_t)kWave;
         if (staged) {
             if (lane < ns) {
 #pragma unroll
-                for (int k = 0; k < 9; k++) stage[9 * lane + k] = sb[9 * (size_t)lane + k];
+                for (int k = 0; k < 9; k++) stage[9 * lane + k] = (R)sb[9 * (size_t)lane + k];
             }
             wave_sync();
         }
@@ -165,31 +166,31 @@ __global__ void __launch_bounds__(kBlock) k_aggregate(
             for (uint32_t ch = 0; ch < nch; ch++) {
                 const uint32_t col = ch * cw + c;
                 const bool cv = in_tile && col < nl;
-                float lv[9];
+                R lv[9];
                 const float *src = lb + 9 * (size_t)(cv ? col : nl - 1);
 #pragma unroll
-                for (int k = 0; k < 9; k++) lv[k] = src[k];
+                for (int k = 0; k < 9; k++) lv[k] = (R)src[k];
                 for (uint32_t gr = 0; gr < ngr; gr++) {
                     const uint32_t row = gr * rows + r;
-                    const float *rv = row_ptr(row < ns ? row : ns - 1);
-                    float sv[9];
+                    const auto *rv = row_ptr(row < ns ? row : ns - 1);
+                    R sv[9];
 #pragma unroll
-                    for (int k = 0; k < 9; k++) sv[k] = rv[k];
+                    for (int k = 0; k < 9; k++) sv[k] = (R)rv[k];
                     R s, cd;
                     // the tile-1 voxel is always the first operand
                     if (sw)
                         voxel_sq<R>(lv, sv, s, cd);
                     else
                         voxel_sq<R>(sv, lv, s, cd);
-                    if (cv && row < ns) {
-                        smin = rmin2(smin, s);
-                        cmin = rmin2(cmin, cd);
-                    }
+                    // cells outside the grid count as +inf (a select)
+                    const bool v = cv && row < ns;
+                    smin = rmin2(smin, v ? s : (R)__builtin_inff());
+                    cmin = rmin2(cmin, v ? cd : (R)__builtin_inff());
                 }
             }
         };
         if (staged)
-            walk([&](uint32_t rw) { return (const float *)(stage + 9 * rw); });
+            walk([&](uint32_t rw) { return (const R *)(stage + 9 * rw); });
         else
             walk([&](uint32_t rw) { return sb + 9 * (size_t)rw; });
     }
