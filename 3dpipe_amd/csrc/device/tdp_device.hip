@@ -1886,9 +1886,9 @@ static int join_refine_t(tdp_join *j, int lod, int is_last, uint64_t *pairs_out)
                     j->stats, j->mx_pwb);
         });
         TDP_LAUNCHED(ctx);
-        if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
         mark_prep(j, is_last, n, lo_free ? 1 : 0);
         TDP_LAUNCHED(ctx);
+        if (mark(TDP_STAGE_PROBE) != TDP_OK) return TDP_ERR_HIP;
         with_need(need, [&](auto nd) {
             auto go = [&](auto cull) {
                 k_mx_mark<decltype(nd)::value, decltype(cull)::value><<<gw, kBlock, 0, s>>>(

@@ -49,8 +49,8 @@ PEAK_FP64_TFLOPS = 78.6
 DEFAULT_TUNING = {"small_pairs": 8, "max_rows": 64, "drain": 0, "drain_blocks": 0,
                   "eval_blocks": 4096, "graph": 0, "row_cull": 1}
 MEASURED_TRAFFIC = {  # max_rows -> (k_mx_mark bytes per launch, bytes per round)
-    64: (12.70e9, 25.10e9),
-    4: (12.76e9, 25.20e9),
+    64: (11.91e9, 25.85e9),
+    4: (11.97e9, 25.95e9),
 }
 TRAFFIC_SOURCE = "profiles/knn10_200k_nuclei_1000_vessels_mixed_traffic_and_valu.md"
 REFINE_STAGES = ["reset", "bounds", "scan", "probe", "mark", "expand", "eval", "fold", "direct",

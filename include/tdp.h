@@ -352,7 +352,7 @@ int tdp_join_kernel_ms(tdp_join *j, double *refine_ms, double *filter_ms);
 #define TDP_STAGE_RESET 0   /* per-round bookkeeping: accumulator reset, pair counts */
 #define TDP_STAGE_BOUNDS 1  /* MIXED: FP32 upper bounds and probe choice */
 #define TDP_STAGE_SCAN 2    /* MIXED: prefix sums, including the host read-back */
-#define TDP_STAGE_PROBE 3   /* MIXED: FP64 TriDist of the probes */
+#define TDP_STAGE_PROBE 3   /* MIXED: FP64 TriDist of the probes, the mark pass's thresholds */
 #define TDP_STAGE_MARK 4    /* MIXED: FP32 candidate test and bit masks */
 #define TDP_STAGE_EXPAND 5  /* MIXED: bit masks to the FP64 work list */
 #define TDP_STAGE_EVAL 6    /* MIXED: FP64 TriDist of the work list */
