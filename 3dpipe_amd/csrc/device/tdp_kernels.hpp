@@ -60,6 +60,17 @@ __device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v, uint32_t id, Op 
     v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, 0x143, 0xc, 0xf, false));
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
+// inclusive prefix sum across the wave (the scan above, lane i holding the
+// sum of lanes 0..i); every lane of the wave must be active
+__device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     return wave_reduce_u32(v, 0xffffffffu, [](uint32_t a, uint32_t b) { return min(a, b); });
 }

@@ -1148,12 +1148,20 @@ __global__ void __launch_bounds__(kBlock) k_mx_mark(
     }
 }
 
-// Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One wave
-// per kWave consecutive voxel pairs: lane i loads the plan of voxel pair
-// w0 + i (all 64 at once), then the wave expands the voxel pairs that have
-// items one after another, reading each one's figures from its lane into
-// scalar registers. (One wave per voxel pair spent most of its time on the
-// per-pair set-up.)
+// Bit masks (and SMALL voxel pairs) to the flat work list at offs[w]. One
+// wave per kWave consecutive voxel pairs. Their mask words are contiguous in
+// memory (woff is the exclusive scan of the word counts in voxel-pair order),
+// so the wave reads them as one stream, 64 words a pass, and turns the set
+// bits of each pass into items with every lane busy: lane i of an item pass
+// takes set bit number base + i of the pass -- its word by binary search over
+// the pass's prefix counts, its bit by halving the word -- and the word's
+// voxel pair by binary search over the wave's word offsets (a voxel pair
+// averages about 8 items; one pass per voxel pair left most lanes idle). A
+// voxel pair's items land at offs[w] + their rank among its items: the set
+// bits of the stream before them less those before its first word. SMALL
+// voxel pairs (every pair an item, no words) go through passes of their own
+// over the wave's concatenated pair counts. The items of a voxel pair are
+// contiguous, which is all the FP64 stage's segmented fold needs.
 __global__ void __launch_bounds__(kBlock) k_mx_expand(
     uint64_t n_vp, const VpPlan *__restrict__ plan, const uint64_t *__restrict__ woff,
     const unsigned long long *__restrict__ mask, const uint32_t *__restrict__ count,
@@ -1163,118 +1171,138 @@ __global__ void __launch_bounds__(kBlock) k_mx_expand(
     if (w0 >= n_vp) return;
     const int lane = lane_id();
     const uint64_t wl = w0 + (uint64_t)lane;
-    // lane's voxel pair: items (SMALL: all n1 * n2 pairs) or tiling
-    uint32_t c = 0, l_small = 0, l_n2 = 0, l_cw = 1, l_R = 1, l_ngr = 1, l_nw = 0, l_swap = 0;
-    uint64_t l_pos = 0, l_wo = 0;
-    if (wl < n_vp) {
-        c = count[wl];
-        if (c) {  // no items: no candidate, or empty voxels among them
-            const VpPlan pl = plan[wl];
-            l_n2 = pl.n2;
-            l_pos = offs[wl];
-            if (plan_flags(pl) & MX_SMALL) {
-                l_small = pl.n1 * pl.n2;
-            } else {
-                l_wo = woff[wl];
-                const Tiling t = plan_tiling(pl);
-                l_cw = t.cw;
-                l_R = t.R;
-                l_ngr = t.ngr;
-                l_nw = pl.nw;
-                l_swap = t.swap;
-            }
+    const uint32_t last = (uint32_t)min((uint64_t)(kWave - 1), n_vp - 1 - w0);
+    // the lane's voxel pair: its items (SMALL), word offset and tiling
+    uint32_t small = 0, nw = 0, meta = 0, n2 = 0, ngr = 1;
+    uint64_t pos = 0, wo = 0;
+    if ((uint32_t)lane <= last) {
+        const VpPlan pl = plan[wl];
+        const uint32_t c = count[wl];
+        pos = offs[wl];
+        wo = woff[wl];
+        n2 = pl.n2;
+        if (plan_flags(pl) & MX_SMALL) {
+            small = c;
+        } else if (!(plan_flags(pl) & MX_EMPTY)) {
+            const Tiling t = plan_tiling(pl);
+            nw = pl.nw;
+            ngr = t.ngr;
+            meta = t.cw | (t.R << 8) | (t.swap ? 1u << 16 : 0u);
         }
     }
-    // Per group of 64 mask words: lane k holds word k, its inclusive prefix
-    // of set bits and its tile origin (in LDS); then lane i of each pass
-    // takes set bit number base + i of the group -- its word by binary search
-    // over the prefixes, its bit by halving the word -- so consecutive lanes
-    // write consecutive items.
+    struct VpTab {
+        uint64_t pos;    // items' offset
+        uint32_t wo;     // first word, from the wave's first
+        uint32_t first;  // set bits of the stream before its first word
+        uint32_t meta;   // cw | R << 8 | swap << 16
+        uint32_t n2, ngr;
+        uint32_t sincl;  // inclusive prefix of the SMALL items
+    };
+    __shared__ VpTab s_vp[kWavesPerBlock][kWave];
     __shared__ unsigned long long s_word[kWavesPerBlock][kWave];
-    __shared__ uint32_t s_incl[kWavesPerBlock][kWave];
-    __shared__ uint32_t s_row0[kWavesPerBlock][kWave], s_col0[kWavesPerBlock][kWave];
-    // bit b of a word is row b / cw, column b % cw of its tile
-    __shared__ uint16_t s_tab[kWavesPerBlock][kWave];
+    __shared__ uint32_t s_incl[kWavesPerBlock][kWave], s_row0[kWavesPerBlock][kWave],
+        s_col0[kWavesPerBlock][kWave], s_slot[kWavesPerBlock][kWave];
     const int wv = threadIdx.x / kWave;
+    VpTab *const vt = s_vp[wv];
     unsigned long long *const sw = s_word[wv];
     uint32_t *const si = s_incl[wv];
     uint32_t *const sr = s_row0[wv];
     uint32_t *const sc = s_col0[wv];
-    uint16_t *const tab = s_tab[wv];
-    auto rl = [](uint32_t v, int L) { return (uint32_t)__builtin_amdgcn_readlane((int)v, L); };
-    auto rl64 = [&](uint64_t v, int L) {
-        return (uint64_t)rl((uint32_t)v, L) | ((uint64_t)rl((uint32_t)(v >> 32), L) << 32);
-    };
-    unsigned long long live = __ballot(c != 0);
-    while (live) {
-        const int L = __ffsll((long long)live) - 1;
-        live &= live - 1ull;
-        const uint32_t w = (uint32_t)(w0 + (uint64_t)L);
-        uint64_t pos = rl64(l_pos, L);
-        const uint32_t small = rl(l_small, L);
-        if (small) {
-            for (uint32_t q = lane; q < small; q += kWave) {
-                item_vp[pos + q] = w;
-                item_q[pos + q] = q;
-            }
-            continue;
+    uint32_t *const ss = s_slot[wv];
+    const uint64_t W0 = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)wo) |
+                        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(wo >> 32)) << 32);
+    const uint64_t wend = wo + nw;
+    const uint32_t nwt =
+        (uint32_t)(((uint64_t)__builtin_amdgcn_readlane((uint32_t)wend, last) |
+                    ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(wend >> 32), last) << 32)) -
+                   W0);
+    const uint32_t sincl = wave_incl_sum_u32(small);
+    const uint32_t stotal = (uint32_t)__builtin_amdgcn_readlane((int)sincl, kWave - 1);
+    // lanes past the last voxel pair get an offset no word reaches, so that
+    // the owner search (the last voxel pair starting at or before a word)
+    // never picks them
+    vt[lane] = VpTab{pos, (uint32_t)lane <= last ? (uint32_t)(wo - W0) : 0xffffffffu, 0u, meta,
+                     n2, ngr, sincl};
+    wave_sync();
+    // SMALL voxel pairs: item q of voxel pair s is its pair q
+    for (uint32_t b0 = 0; b0 < stotal; b0 += kWave) {
+        const uint32_t idx = b0 + (uint32_t)lane;
+        if (idx < stotal) {
+            uint32_t s = 0;  // the voxel pair: the number of prefixes <= idx
+#pragma unroll
+            for (uint32_t st = kWave / 2; st >= 1; st >>= 1)
+                if (vt[s + st - 1].sincl <= idx) s += st;
+            const uint32_t q = idx - (s ? vt[s - 1].sincl : 0u);
+            item_vp[vt[s].pos + q] = (uint32_t)(w0 + s);
+            item_q[vt[s].pos + q] = q;
         }
-        const uint32_t n2 = rl(l_n2, L), cw = rl(l_cw, L), R = rl(l_R, L), ngr = rl(l_ngr, L),
-                       nw = rl(l_nw, L);
-        const bool swap = rl(l_swap, L) != 0;
-        const unsigned long long *__restrict__ in = mask + rl64(l_wo, L);
-        {
-            // lane / cw through the reciprocal: lane < 64 and cw <= 64 keep
-            // (lane + 1/2) / cw at least 1 / 128 from an integer
-            const uint32_t r =
-                (uint32_t)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)cw));
-            tab[lane] = (uint16_t)((r << 8) | ((uint32_t)lane - r * cw));
+    }
+    // the word stream
+    uint32_t run = 0;  // set bits of the stream before the pass
+    for (uint32_t base = 0; base < nwt; base += kWave) {
+        const uint32_t k = base + (uint32_t)lane;
+        unsigned long long m = 0;
+        uint32_t s = 0, kk = 1, r0 = 0, c0 = 0;
+        if (k < nwt) {
+            m = mask[W0 + k];
+#pragma unroll
+            for (uint32_t st = kWave / 2; st >= 1; st >>= 1)
+                if (s + st < (uint32_t)kWave && vt[s + st].wo <= k) s += st;
+            kk = k - vt[s].wo;
+            const uint32_t g = vt[s].ngr, mt = vt[s].meta;
+            // kk / g through the reciprocal, then one correction each way (the
+            // quotient, a column chunk, is small enough for the estimate to be
+            // within one)
+            uint32_t ch = (uint32_t)((float)kk * __builtin_amdgcn_rcpf((float)g));
+            if (ch * g > kk) ch--;
+            else if ((ch + 1u) * g <= kk) ch++;
+            r0 = (kk - ch * g) * ((mt >> 8) & 255u);
+            c0 = ch * (mt & 255u);
         }
-        for (uint32_t wb = 0; wb < nw; wb += kWave) {
-            const uint32_t k = wb + lane;
-            const unsigned long long m = k < nw ? in[k] : 0ull;
-            const uint32_t cnt = (uint32_t)__popcll(m);
-            uint32_t incl = cnt;
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        const uint32_t incl = wave_incl_sum_u32(cnt);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        if (k < nwt && kk == 0) vt[s].first = run + incl - cnt;
+        sw[lane] = m;
+        si[lane] = incl;
+        sr[lane] = r0;
+        sc[lane] = c0;
+        ss[lane] = s;
+        wave_sync();
+        for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
+            const uint32_t idx = b0 + (uint32_t)lane;
+            if (idx < total) {
+                // word: the number of prefixes <= idx (< 64 as si[63] > idx)
+                uint32_t W = 0;
 #pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t v = (uint32_t)__shfl_up((int)incl, o, kWave);
-                if (lane >= o) incl += v;
-            }
-            const uint32_t total = rl(incl, kWave - 1);
-            const uint32_t ch = k / ngr, gr = k - ch * ngr;
-            sw[lane] = m;
-            si[lane] = incl;
-            sr[lane] = gr * R;
-            sc[lane] = ch * cw;
-            wave_sync();
-            for (uint32_t base = 0; base < total; base += kWave) {
-                const uint32_t idx = base + (uint32_t)lane;
-                if (idx < total) {
-                    // word: the number of prefixes <= idx (< 64 as si[63] > idx)
-                    uint32_t W = 0;
+                for (uint32_t st = kWave / 2; st >= 1; st >>= 1)
+                    if (si[W + st - 1] <= idx) W += st;
+                unsigned long long x = sw[W];
+                uint32_t rank = idx - (si[W] - (uint32_t)__popcll(x)), b = 0;
 #pragma unroll
-                    for (uint32_t st = kWave / 2; st >= 1; st >>= 1)
-                        if (si[W + st - 1] <= idx) W += st;
-                    unsigned long long x = sw[W];
-                    uint32_t rank = idx - (si[W] - (uint32_t)__popcll(x)), b = 0;
-#pragma unroll
-                    for (uint32_t h = 32; h >= 1; h >>= 1) {
-                        const uint32_t cb = (uint32_t)__popcll(x & ((1ull << h) - 1ull));
-                        if (rank >= cb) {
-                            rank -= cb;
-                            x >>= h;
-                            b += h;
-                        }
+                for (uint32_t h = 32; h >= 1; h >>= 1) {
+                    const uint32_t cb = (uint32_t)__popcll(x & ((1ull << h) - 1ull));
+                    if (rank >= cb) {
+                        rank -= cb;
+                        x >>= h;
+                        b += h;
                     }
-                    const uint32_t e = tab[b];
-                    const uint32_t row = sr[W] + (e >> 8), col = sc[W] + (e & 255u);
-                    item_vp[pos + idx] = w;
-                    item_q[pos + idx] = swap ? col * n2 + row : row * n2 + col;
                 }
+                const uint32_t sl = ss[W];
+                const VpTab e = vt[sl];
+                const uint32_t cw = e.meta & 255u;
+                // b / cw through the reciprocal: b < 64 and cw <= 64 keep
+                // (b + 1/2) / cw at least 1/128 from an integer
+                const uint32_t rin =
+                    (uint32_t)(((float)b + 0.5f) * __builtin_amdgcn_rcpf((float)cw));
+                const uint32_t row = sr[W] + rin, col = sc[W] + (b - rin * cw);
+                const uint64_t p = e.pos + (uint64_t)(run + idx - e.first);
+                item_vp[p] = (uint32_t)(w0 + sl);
+                item_q[p] = (e.meta >> 16) & 1u ? col * e.n2 + row : row * e.n2 + col;
             }
-            wave_sync();
-            pos += total;
         }
+        wave_sync();
+        run += total;
     }
 }
 
@@ -1497,17 +1525,9 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t seg = 0xffffffffu;
     K kmin = none, kmax = none;
-    // the mark pass's pairs tested one by one: block-reduced into stats[3]
-    __shared__ unsigned long long part[kWavesPerBlock];
-    const uint64_t tv = wave_sum_u64(w < n_vp ? (uint64_t)tested[w] : 0ull);
-    if (lane_id() == 0) part[threadIdx.x / kWave] = tv;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long a = 0;
-        for (int k = 0; k < kWavesPerBlock; k++) a += part[k];
-        if (a) atomicAdd(&stats[3], a);
-    }
+    uint32_t tl = 0;  // the mark pass's pairs tested one by one
     if (w < n_vp) {
+        tl = tested[w];
         const uint32_t v1 = vp_v1[w], v2 = vp_v2[w];
         seg = vp_pair[w];
         const R omin = vp_min[w], omax = vp_max[w];
@@ -1528,6 +1548,17 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
         }
     }
     seg_min2<K>(seg, kmin, kmax, none, acc_min, acc_max);
+    // the tested pairs, block-reduced into stats[3] (after the fold, so that
+    // the block's barrier does not hold back its loads)
+    __shared__ unsigned long long part[kWavesPerBlock];
+    const uint64_t tv = wave_sum_u64((uint64_t)tl);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = tv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0;
+        for (int k = 0; k < kWavesPerBlock; k++) a += part[k];
+        if (a) atomicAdd(&stats[3], a);
+    }
 }
 
 }  // namespace tdp
