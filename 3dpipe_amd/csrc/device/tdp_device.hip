@@ -1449,7 +1449,7 @@ static int join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
     if (rc == TDP_OK) rc = palloc(ctx, (char **)&j->acc_max, P * j->rsz);
     if (rc == TDP_OK) rc = palloc(ctx, &j->count, P);
     if (rc == TDP_OK) rc = palloc(ctx, &j->offs, P);
-    if (rc == TDP_OK) rc = palloc(ctx, &j->stats, 5);
+    if (rc == TDP_OK) rc = palloc(ctx, &j->stats, (size_t)kStatN * kStatSlots);
     if (rc == TDP_OK) rc = palloc(ctx, &j->work, 4);
     if (rc == TDP_OK && prm->precision == TDP_PREC_MIXED) rc = palloc(ctx, &j->obj_u, P);
     if (rc != TDP_OK) {
@@ -1482,7 +1482,7 @@ static int join_create(tdp_ctx *ctx, const tdp_tile *t1, const tdp_tile *t2,
         for (auto &x : wv) x = x ? 1 : 0;
         e = copy_sync(ctx, j->within_t, wv.data(), N, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = set_sync(ctx, j->stats, 0, 40);
+    if (e == hipSuccess) e = set_sync(ctx, j->stats, 0, 8 * kStatN * kStatSlots);
     if (e == hipSuccess) e = set_sync(ctx, j->confirmed, 0, 4 * (N ? N : 1));
     if (e == hipSuccess) e = set_sync(ctx, j->round, 0xff, P ? P : 1);
     if (e == hipSuccess) e = set_sync(ctx, j->state, 0, P ? P : 1);
@@ -2340,10 +2340,22 @@ extern "C" int tdp_join_results(tdp_join *j, uint8_t *report_round, double *rmin
     return TDP_OK;
 }
 
+// the statistics counters, their copies added up (stat_add)
+static int read_stats(tdp_join *j, unsigned long long s[kStatN]) {
+    unsigned long long c[kStatN * kStatSlots];
+    TDP_CHECK(j->ctx, copy_sync(j->ctx, c, j->stats, sizeof c, hipMemcpyDeviceToHost));
+    for (int k = 0; k < kStatN; k++) {
+        s[k] = 0;
+        for (int b = 0; b < kStatSlots; b++) s[k] += c[kStatN * b + k];
+    }
+    return TDP_OK;
+}
+
 extern "C" int tdp_join_stats(tdp_join *j, uint64_t *out5) {
     if (!j || !out5) return TDP_ERR_ARG;
-    unsigned long long s[4];
-    TDP_CHECK(j->ctx, copy_sync(j->ctx, s, j->stats, sizeof s, hipMemcpyDeviceToHost));
+    unsigned long long s[kStatN];
+    const int rc = read_stats(j, s);
+    if (rc != TDP_OK) return rc;
     out5[0] = s[0];
     out5[1] = s[1];
     out5[2] = s[2];
@@ -2359,8 +2371,9 @@ extern "C" int tdp_join_stats_n(tdp_join *j, uint64_t *out, int n) {
     uint64_t a[7] = {0, 0, 0, 0, 0, 0, 0};
     const int rc = tdp_join_stats(j, a);
     if (rc != TDP_OK) return rc;
-    unsigned long long s[5];
-    TDP_CHECK(j->ctx, copy_sync(j->ctx, s, j->stats, sizeof s, hipMemcpyDeviceToHost));
+    unsigned long long s[kStatN];
+    const int rs = read_stats(j, s);
+    if (rs != TDP_OK) return rs;
     a[5] = s[3];
     a[6] = s[4];
     for (int k = 0; k < n && k < 7; k++) out[k] = a[k];

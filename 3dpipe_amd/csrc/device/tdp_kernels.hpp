@@ -83,6 +83,15 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
+// A join's statistics counters: kStatN of them, each spread over kStatSlots
+// copies picked by block, so that the one-per-block atomics of a launch with
+// tens of thousands of blocks do not queue on one address (they took most of
+// the time of the short kernels that count); readers add up the copies.
+constexpr int kStatN = 8, kStatSlots = 64;
+__device__ __forceinline__ void stat_add(unsigned long long *stats, int k, unsigned long long v) {
+    atomicAdd(&stats[kStatN * (blockIdx.x & (kStatSlots - 1)) + k], v);
+}
+
 // order-preserving unsigned keys for atomic min over floating values
 __device__ __forceinline__ uint32_t fkey(float f) {
     uint32_t u = __float_as_uint(f);
@@ -1042,8 +1051,8 @@ __global__ void __launch_bounds__(kBlock) k_round_stats(
             a += part[0][k];
             b += part[1][k];
         }
-        if (a) atomicAdd(&stats[0], a);
-        if (b) atomicAdd(&stats[1], b);
+        if (a) stat_add(stats, 0, a);
+        if (b) stat_add(stats, 1, b);
     }
 }
 

@@ -275,8 +275,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_plan(
             a += part[0][k];
             b += part[1][k];
         }
-        if (a) atomicAdd(&stats[0], a);
-        if (b) atomicAdd(&stats[1], b);
+        if (a) stat_add(stats, 0, a);
+        if (b) stat_add(stats, 1, b);
     }
 }
 
@@ -776,8 +776,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_eval(
         n += (uint64_t)__popcll(__ballot(act));
     }
     if (lane_id() == 0 && n) {
-        atomicAdd(&stats[2], (unsigned long long)n);
-        if (vp_from_index3) atomicAdd(&stats[4], (unsigned long long)n);  // the probes
+        stat_add(stats, 2, (unsigned long long)n);
+        if (vp_from_index3) stat_add(stats, 4, (unsigned long long)n);  // the probes
     }
 }
 
@@ -1448,7 +1448,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_drain(
         }
         qn = rest;
     }
-    if (lane == 0 && evals) atomicAdd(&stats[2], (unsigned long long)evals);
+    if (lane == 0 && evals) stat_add(stats, 2, (unsigned long long)evals);
 }
 
 // per LOD slot: the mask words every voxel pair of the list would need --
@@ -1548,7 +1548,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
         }
     }
     seg_min2<K>(seg, kmin, kmax, none, acc_min, acc_max);
-    // the tested pairs, block-reduced into stats[3] (after the fold, so that
+    // the tested pairs, block-reduced into counter 3 (after the fold, so that
     // the block's barrier does not hold back its loads)
     __shared__ unsigned long long part[kWavesPerBlock];
     const uint64_t tv = wave_sum_u64((uint64_t)tl);
@@ -1557,7 +1557,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_fold(
     if (threadIdx.x == 0) {
         unsigned long long a = 0;
         for (int k = 0; k < kWavesPerBlock; k++) a += part[k];
-        if (a) atomicAdd(&stats[3], a);
+        if (a) stat_add(stats, 3, a);
     }
 }
 
