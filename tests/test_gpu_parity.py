@@ -155,6 +155,15 @@ def test_join_small_all_forms(tdp, eng, small_tiles, query, k, tau):
     rmx = eng.join(n, v, query=query, knn=k, within_dist=tau, precision="mixed")
     _compare(rmx, ot, oo, omn, omx)
     assert r64.stats["tri_pairs"] == rmx.stats["tri_pairs"] == ost["tri_pairs"]
+    # the mixed form's counters (each summed over its per-block copies): the
+    # probes are part of the FP64 evaluations, the pairs the mark pass tested
+    # one by one part of the pairs covered, and every pair not evaluated in
+    # FP64 was excluded by an FP32 bound
+    st = rmx.stats
+    assert st["fp64_probe_evals"] <= st["fp64_evals"] <= st["tri_pairs"]
+    assert st["mark_tested"] <= st["tri_pairs"]
+    assert st["fp32_culled"] == st["tri_pairs"] - st["fp64_evals"]
+    assert r64.stats["fp64_probe_evals"] == 0 and r64.stats["mark_tested"] == 0
 
 
 @pytest.mark.parametrize("lods", [(20,), (100,), (40, 20, 20), (20, 60, 100, 60), (50, 100)])
