@@ -295,7 +295,8 @@ class _StreamOpts(_c.Structure):
 # streaming figures of a result (tdb_result_stream_n)
 STREAM = ["chunks", "h2d_bytes", "decode_ms", "copy_ms", "join_ms", "wall_ms", "overlap",
           "peak_device_bytes", "index_ms", "candidates_ms", "objects_decoded", "splits",
-          "record_bytes_all", "record_bytes_shipped", "meta_bytes", "workers"]
+          "record_bytes_all", "record_bytes_shipped", "meta_bytes", "workers",
+          "candidate_host_targets"]
 
 # ------------------------------------------------------------------ join
 

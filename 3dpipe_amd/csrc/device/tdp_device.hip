@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1155,6 +1156,160 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
 
 // ------------------------------------------------------------ join
 
+// kNN lists for tile 2 of more than kCandMaxObj objects: tile 2 binned into a
+// grid of cubic cells of about two objects each (k_grid_*), then the targets
+// in slices of kGridSlice, each slice's lists compacted on the device and
+// kept until the total is known.
+constexpr size_t kGridSlice = (size_t)1 << 18;
+
+static int knn_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
+                               int self, const int32_t *k_target, uint32_t *prefix, uint32_t *ids,
+                               size_t ids_cap, uint64_t *n_ids, uint8_t *fallback) {
+    const uint32_t n2 = (uint32_t)t2->n_obj;
+    const uint32_t cap = kCandMaxCap;
+    hipStream_t s = ctx->stream;
+    std::vector<void *> held;
+    auto grab = [&](auto **p, size_t count) {
+        const int rc = palloc(ctx, p, count);
+        if (rc == TDP_OK) held.push_back((void *)*p);
+        return rc;
+    };
+    auto done = [&](int rc) {
+        (void)hipStreamSynchronize(s);
+        for (void *p : held) pfree(ctx, p);
+        return rc;
+    };
+    auto hip_err = [&](hipError_t e) {
+        ctx->err = std::string("tdp_knn_candidates: ") + hipGetErrorString(e);
+        return done(TDP_ERR_HIP);
+    };
+    // grid bounds and cell size
+    float *dbnd = nullptr;
+    int rc = grab(&dbnd, 8);
+    if (rc != TDP_OK) return done(rc);
+    k_grid_bounds<<<1, 1024, 0, s>>>(n2, t2->obj_box, t2->obj_tr, dbnd);
+    float hb[7];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hb, dbnd, sizeof hb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_err(e);
+    CandGrid g{};
+    double ext[3], emx = 0, amax = 0;
+    bool finite = std::isfinite(hb[6]);
+    for (int a = 0; a < 3; a++) {
+        finite &= std::isfinite(hb[a]) && std::isfinite(hb[3 + a]);
+        ext[a] = std::max(0.0, (double)hb[3 + a] - (double)hb[a]);
+        emx = std::max(emx, ext[a]);
+        amax = std::max(amax, std::max(std::fabs((double)hb[a]), std::fabs((double)hb[3 + a])));
+    }
+    if (!finite || emx == 0) {
+        // one cell (every target then finds more than a wave holds: host lists)
+        for (int a = 0; a < 3; a++) g.lo[a] = 0.f, g.n[a] = 1;
+        g.inv_h = 0.f;
+    } else {
+        const double cells = std::max(1.0, n2 / 2.0);
+        double vol = 1;
+        for (int a = 0; a < 3; a++) vol *= std::max(ext[a], emx * 1e-3);
+        double h = std::cbrt(vol / cells);
+        for (;;) {
+            double tot = 1;
+            for (int a = 0; a < 3; a++) {
+                g.n[a] = (int)std::min(std::max(1.0, std::ceil(ext[a] / h)), 1024.0);
+                tot *= g.n[a];
+            }
+            if (tot <= 2.0 * n2 + 1024) break;
+            h *= 1.25;
+        }
+        for (int a = 0; a < 3; a++) g.lo[a] = hb[a];
+        g.inv_h = (float)(1.0 / h);
+    }
+    g.emax = finite ? hb[6] : INFINITY;
+    g.tol = (float)(amax * 0x1p-18) + 1e-30f;
+    const size_t ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
+    // objects in cell order
+    uint32_t *dcell = nullptr, *dcnt = nullptr, *dcur = nullptr;
+    uint64_t *doff = nullptr;
+    float4 *gobj = nullptr;
+    if (rc == TDP_OK) rc = grab(&dcell, n2);
+    if (rc == TDP_OK) rc = grab(&dcnt, ncell + 1);
+    if (rc == TDP_OK) rc = grab(&dcur, ncell);
+    if (rc == TDP_OK) rc = grab(&doff, ncell + 1);
+    if (rc == TDP_OK) rc = grab(&gobj, 3 * (size_t)n2);
+    if (rc != TDP_OK) return done(rc);
+    e = hipMemsetAsync(dcnt, 0, 4 * (ncell + 1), s);
+    if (e == hipSuccess) e = hipMemsetAsync(dcur, 0, 4 * ncell, s);
+    if (e == hipSuccess) {
+        k_grid_count<<<blocks_for(n2, kBlock), kBlock, 0, s>>>(n2, t2->obj_tr, g, dcell, dcnt);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return hip_err(e);
+    uint64_t placed = 0;
+    rc = scan_exclusive<uint32_t>(ctx, dcnt, ncell + 1, doff, &placed);
+    if (rc != TDP_OK) return done(rc);
+    if (placed != n2) {
+        ctx->err = "tdp_knn_candidates: grid binning lost objects";
+        return done(TDP_ERR_HIP);
+    }
+    k_grid_fill<<<blocks_for(n2, kBlock), kBlock, 0, s>>>(n2, t2->obj_box, t2->obj_tr, dcell, doff,
+                                                         dcur, gobj);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_err(e);
+    // targets in slices
+    const size_t S = std::min(N, kGridSlice);
+    int32_t *dk = nullptr;
+    uint32_t *dc = nullptr, *dl = nullptr;
+    uint8_t *df = nullptr;
+    uint64_t *dso = nullptr;
+    if (rc == TDP_OK) rc = grab(&dk, N);
+    if (rc == TDP_OK) rc = grab(&dc, S);
+    if (rc == TDP_OK) rc = grab(&dl, S * cap);
+    if (rc == TDP_OK) rc = grab(&df, S);
+    if (rc == TDP_OK) rc = grab(&dso, S);
+    if (rc != TDP_OK) return done(rc);
+    if ((e = hipMemcpyAsync(dk, k_target, 4 * N, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_err(e);
+    std::vector<uint32_t> cnt(N);
+    std::vector<std::pair<uint32_t *, uint64_t>> parts;
+    uint64_t total = 0;
+    for (size_t t0 = 0; t0 < N; t0 += S) {
+        const size_t sn = std::min(S, N - t0);
+        k_knn_candidates_grid<<<wave_grid(sn), kBlock, 0, s>>>(t0 + sn, t0, t1->obj_box,
+                                                               t1->obj_tr, g, doff, gobj, self, dk,
+                                                               cap, dc, dl, df);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_err(e);
+        uint64_t st = 0;
+        rc = scan_exclusive<uint32_t>(ctx, dc, sn, dso, &st);
+        if (rc != TDP_OK) return done(rc);
+        uint32_t *dpart = nullptr;
+        if ((rc = grab(&dpart, st)) != TDP_OK) return done(rc);
+        k_cand_compact<<<wave_grid(sn), kBlock, 0, s>>>(sn, cap, dc, dso, dl, dpart);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(cnt.data() + t0, dc, 4 * sn, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(fallback + t0, df, sn, hipMemcpyDeviceToHost, s);
+        // the slice buffers are reused by the next slice's kernel on this stream
+        if (e != hipSuccess) return hip_err(e);
+        parts.push_back({dpart, st});
+        total += st;
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e);
+    *n_ids = total;
+    if (total > ids_cap) {
+        ctx->err = "tdp_knn_candidates: ids array too small";
+        return done(TDP_ERR_ARG);
+    }
+    uint64_t at = 0;
+    for (auto &pt : parts) {
+        if (pt.second &&
+            (e = hipMemcpyAsync(ids + at, pt.first, 4 * pt.second, hipMemcpyDeviceToHost, s)) !=
+                hipSuccess)
+            return hip_err(e);
+        at += pt.second;
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e);
+    for (size_t t = 0; t < N; t++) prefix[t + 1] = prefix[t] + cnt[t];
+    return done(TDP_OK);
+}
+
 extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
                                   int self, const int32_t *k_target, uint32_t *prefix,
                                   uint32_t *ids, size_t ids_cap, uint64_t *n_ids,
@@ -1169,12 +1324,15 @@ extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, co
         ctx->err = "tdp_knn_candidates: tiles uploaded without object boxes";
         return TDP_ERR_ARG;
     }
-    if (t2->n_obj > (size_t)kCandMaxObj || N > t1->n_obj) {
-        ctx->err = "tdp_knn_candidates: tile 2 has more than 1024 objects";
+    if (N > t1->n_obj || t2->n_obj >= 0xffffffffull) {
+        ctx->err = "tdp_knn_candidates: more targets than tile 1 holds, or tile 2 too large";
         return TDP_ERR_ARG;
     }
     prefix[0] = 0;
     if (N == 0) return TDP_OK;
+    if (t2->n_obj > (size_t)kCandMaxObj)
+        return knn_candidates_grid(ctx, t1, N, t2, self, k_target, prefix, ids, ids_cap, n_ids,
+                                   fallback);
     const uint32_t cap = kCandMaxCap;
     int32_t *dk = nullptr;
     uint32_t *dc = nullptr, *dl = nullptr, *dids = nullptr;

@@ -222,9 +222,10 @@ tdp_tile_desc tdb_desc_of(const tdb::Tile &t) {
     return d;
 }
 
-// Candidates with the device where it applies (kNN or within, tile 2 of at
-// most 1024 objects, both tiles resident): the device lists, and the octree
-// for the targets the device leaves to it. Elsewhere the octree for all
+// Candidates with the device where it applies (kNN with tile 2 of any size,
+// within with tile 2 of at most 1024 objects, both tiles resident): the
+// device lists, and the octree for the targets the device leaves to it
+// (built only when there are such targets). Elsewhere the octree for all
 // targets.
 int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
                            const tdb_tile *t1, const tdb_tile *t2, bool self, size_t N,
@@ -234,11 +235,10 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
     if (n_fallback) *n_fallback = 0;
     const bool knn = o->query == TDB_QUERY_NN,
                within = o->query == TDB_QUERY_WITHIN && device_within;
-    if (!ctx || !d1 || !d2 || !(knn || within) || t2->t.n_obj > 1024 || N == 0) {
+    if (!ctx || !d1 || !d2 || !(knn || (within && t2->t.n_obj <= 1024)) || N == 0) {
         tdb_gen_candidates(t1, t2, self, N, o, k_t, tau_t, prefix, ids);
         return TDP_OK;
     }
-    const tdb::Octree &tree = t2->octree();
     std::vector<int32_t> kv;
     std::vector<float> tv;
     std::vector<float> lbox;
@@ -249,7 +249,7 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
     } else {
         tv.resize(N);
         for (size_t t = 0; t < N; t++) tv[t] = tau_t ? tau_t[t] : o->within_dist;
-        tree.leaves(lbox, loff, lobj);
+        t2->octree().leaves(lbox, loff, lobj);
     }
     std::vector<uint32_t> dpre(N + 1);
     std::vector<uint8_t> fb(N);
@@ -280,6 +280,12 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
         if (fb[t]) redo.push_back((long)t);
     if (n_fallback) *n_fallback = redo.size();
     std::vector<std::vector<uint32_t>> host(redo.size());
+    if (redo.empty()) {
+        prefix.swap(dpre);
+        ids.assign(dids.get(), dids.get() + n_ids);
+        return TDP_OK;
+    }
+    const tdb::Octree &tree = t2->octree();
 #ifdef _OPENMP
     const int nt = o->threads > 0 ? o->threads : omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
@@ -294,11 +300,6 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
             tree.within(&t1->t.obj_box[6 * i], &t1->t.obj_transform[3 * i], self ? (int)i : -1,
                         tv[i], c);
         for (auto &x : c) host[r].push_back((uint32_t)x.first);
-    }
-    if (redo.empty()) {
-        prefix.swap(dpre);
-        ids.assign(dids.get(), dids.get() + n_ids);
-        return TDP_OK;
     }
     prefix.assign(N + 1, 0);
     size_t r = 0;

@@ -55,6 +55,21 @@ inline uint64_t obj_bytes(const tdb::ObjectSource &s, size_t o) {
     return 96 * s.n_tri(o) + 132 * (uint64_t)s.n_vox(o) + 48;
 }
 
+// a device tile holding the index's object boxes and proxy points only (no
+// voxels, no records): what the device candidate lists read
+int upload_index(tdp_ctx *ctx, const tdb_tile &ix, tdp_tile **out) {
+    const size_t n = ix.t.n_obj;
+    std::vector<uint32_t> vox_off(n + 1, 0u);
+    std::vector<float> hd(10 * n, 0.f);
+    tdp_tile_desc d{};
+    d.n_obj = n;
+    d.obj_vox_off = vox_off.data();
+    d.obj_hd = hd.data();
+    d.obj_box = ix.t.obj_box.data();
+    d.obj_transform = ix.t.obj_transform.data();
+    return tdp_tile_upload(ctx, &d, out);
+}
+
 struct Chunk {
     std::vector<uint32_t> targets;      // targets (ids in source 1), local 0..n-1
     std::vector<uint32_t> objs[2];      // objects of tile 1 (self: targets, then the rest) / tile 2
@@ -421,7 +436,7 @@ void tdb_result_stream(const tdb_result *r, double *out12) {
 
 int tdb_result_stream_n(const tdb_result *r, double *out, int n) {
     if (!out || n < 0) return 0;
-    n = std::min(n, 16);
+    n = std::min(n, 17);
     for (int k = 0; k < n; k++) out[k] = r ? r->stream[k] : 0.0;
     return n;
 }
@@ -478,13 +493,33 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     const tdb_tile &I2 = self ? idx1 : idx2;
     res->stream[8] = ms_since(t0);
 
-    // candidate lists of all targets (octree), in target order
+    // candidate lists of all targets, in target order: kNN lists on the
+    // device from the two sides' boxes and proxy points (the octree for the
+    // targets it leaves), within lists from the octree
     t0 = Clock::now();
     std::vector<uint32_t> order = morton_order(idx1.t.obj_box, N);
     std::vector<uint32_t> prefix, ids;
     {
         std::vector<uint32_t> p0, i0;
-        tdb_gen_candidates(&idx1, &I2, self, N, o, nullptr, nullptr, p0, i0);
+        res->stream[16] = -1;
+        if (o->query == TDB_QUERY_NN) {
+            tdp_tile *d1 = nullptr, *d2 = nullptr;
+            int crc = upload_index(e->ctx, idx1, &d1);
+            if (crc == TDP_OK) crc = self ? TDP_OK : upload_index(e->ctx, I2, &d2);
+            uint64_t n_fb = 0;
+            if (crc == TDP_OK)
+                crc = tdb_gen_candidates_dev(e->ctx, d1, self ? d1 : d2, &idx1, &I2, self, N, o,
+                                             nullptr, nullptr, p0, i0, &n_fb, e->err, false);
+            if (d2) tdp_tile_free(d2);
+            if (d1) tdp_tile_free(d1);
+            if (crc != TDP_OK) {
+                if (e->err.empty()) e->err = tdp_last_error(e->ctx);
+                return crc;
+            }
+            res->stream[16] = (double)n_fb;
+        } else {
+            tdb_gen_candidates(&idx1, &I2, self, N, o, nullptr, nullptr, p0, i0);
+        }
         // lists in chunk-plan order (position k holds target order[k])
         prefix.assign(N + 1, 0);
         for (size_t k = 0; k < N; k++) prefix[k + 1] = prefix[k] + (p0[order[k] + 1] - p0[order[k]]);

@@ -162,6 +162,7 @@ def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn, laz
         "stage_ms": {"index": st["index_ms"], "candidates": st["candidates_ms"],
                      "decode": st["decode_ms"], "copy": st["copy_ms"], "join": st["join_ms"],
                      "pipeline_wall": st["wall_ms"]},
+        "candidate_targets_from_host_octree": int(st["candidate_host_targets"]),
         "overlap_fraction": st["overlap"],
         "peak_device_bytes": int(st["peak_device_bytes"]),
         "budget_bytes": budget, "chunk_splits": int(st["splits"]),

@@ -138,10 +138,10 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
 int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
                        uint32_t *prefix, uint32_t *ids, size_t cap);
 /* the same lists as tdb_join builds them: on the engine's device for kNN
- * and within with tile 2 of at most 1024 objects (tdp_knn_candidates,
- * tdp_within_candidates), the octree for the targets the device leaves to it
- * (*n_fallback of them) and for everything else; returns the number of ids
- * or -1 */
+ * (tdp_knn_candidates, tile 2 of any size) and within with tile 2 of at most
+ * 1024 objects (tdp_within_candidates), the octree for the targets the
+ * device leaves to it (*n_fallback of them) and for everything else; returns
+ * the number of ids or -1 */
 int64_t tdb_engine_candidates(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
                               const tdb_join_opts *o, uint32_t *prefix, uint32_t *ids, size_t cap,
                               uint64_t *n_fallback);
@@ -234,11 +234,13 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *s1, const tdb_source *s2,
  * boxes), [9] candidate ms, [10] objects decoded, [11] chunk joins split to
  * fit the budget */
 void tdb_result_stream(const tdb_result *r, double *out12);
-/* the same figures and more, n of them (at most 16; returns how many): [0..11]
+/* the same figures and more, n of them (at most 17; returns how many): [0..11]
  * as above with [2] decode ms including the per-round record decodes and [3]
  * copy ms including the per-round record puts, then [12] record bytes of the
  * chunks' objects (what shipping every record would copy), [13] record bytes
- * shipped, [14] voxel and object array bytes shipped, [15] workers */
+ * shipped, [14] voxel and object array bytes shipped, [15] workers, [16]
+ * targets whose kNN candidate lists came from the host octree (the device
+ * builds the others; -1 when the octree built them all) */
 int tdb_result_stream_n(const tdb_result *r, double *out, int n);
 
 /* ---- serving: queries against one resident tile, continuously batched ----

@@ -212,8 +212,10 @@ int tdp_knn_status(tdp_ctx *ctx, size_t n_targets, const uint32_t *prefix,
  * order or the k-th place is decided by equal distances -- which the octree
  * breaks by its visiting order -- or a list is longer than 256, fallback[t]
  * = 1 and the target's list is left empty for the host octree. Both tiles
- * need obj_box and obj_transform; t2 may hold at most 1024 objects
- * (TDP_ERR_ARG otherwise). Host arrays: k_target[n_targets],
+ * need obj_box and obj_transform; t2 may hold any number of objects below
+ * 2^32 - 1 (up to 1024, one wave per target holds them all; beyond, they are
+ * binned into a grid of cells and a target reads the cells near it).
+ * Host arrays: k_target[n_targets],
  * prefix[n_targets + 1], ids[ids_cap], fallback[n_targets]; *n_ids = the
  * number of ids (if it exceeds ids_cap, nothing is written to ids and the
  * call returns TDP_ERR_ARG: call again with room for *n_ids). */

@@ -337,6 +337,48 @@ def test_device_knn_candidates_equal_octree(tdp, eng, small_tiles, tmp_path, k):
 
 
 @pytest.fixture(scope="module")
+def many_nuclei(tdp, tmp_path_factory):
+    """tiles 2 above the 1024 objects one wave holds (the grid path): 3,000
+    nuclei around 150 vessels, and 2,500 nuclei uniform in the same cube"""
+    d = tmp_path_factory.mktemp("many_nuclei")
+    n, v = tdp.generate(seed=5, n_vessels=150, nuclei_per_vessel=20, cube=4000.0)
+    u, _ = tdp.generate(seed=6, n_vessels=0, nuclei_per_vessel=2500, cube=4000.0,
+                        nucleus_r_min=6.0, nucleus_r_max=14.0)
+    paths = [str(d / f) for f in ("n.dt", "v.dt", "u.dt")]
+    for t, pth in zip((n, v, u), paths):
+        t.save(pth)
+    return paths
+
+
+@pytest.mark.parametrize("k", [1, 10, 40])
+def test_device_knn_candidates_grid_equal_octree(tdp, eng, many_nuclei, tmp_path, k):
+    """kNN lists for tile 2 of more than 1024 objects (the grid of cells)
+    equal the octree's query_knn_fast lists, order included: self-joins of
+    clustered and uniform nuclei, nuclei against other nuclei (targets in
+    cells of few objects and outside the grid), and vessels against nuclei
+    (long boxes: long lists, those past 256 left to the octree)"""
+    pn, pv, pu = many_nuclei
+    n, v, u = (tdp.Tile.load(p) for p in (pn, pv, pu))
+    assert n.n_obj > 1024 and u.n_obj > 1024
+    for t1, t2 in ((n, None), (u, None), (u, n), (n, u), (v, n)):
+        hp, hi = tdp.candidates(t1, t2, "nn", k)
+        dp, di, fb = eng.candidates(t1, t2, "nn", k)
+        assert np.array_equal(hp, dp) and np.array_equal(hi, di), (t1.n_obj, k)
+        if t1 is not v:
+            assert fb < t1.n_obj // 50  # equal distances only
+    # equal distances: object 0 duplicated -> those targets (target 0 of the
+    # same nuclei at least: both copies at distance 0) go to the octree
+    dup = str(tmp_path / "ndup.dt")
+    _dup_first_object(pn, dup)
+    n2 = tdp.Tile.load(dup)
+    assert n2.n_obj == n.n_obj + 1
+    hp, hi = tdp.candidates(n, n2, "nn", k)
+    dp, di, fb = eng.candidates(n, n2, "nn", k)
+    assert np.array_equal(hp, dp) and np.array_equal(hi, di)
+    assert fb > 0
+
+
+@pytest.fixture(scope="module")
 def many_vessels(tdp, tmp_path_factory):
     """300 vessels (an octree of many leaves, long vessels spanning several)
     and 2 nuclei per vessel in a 3000^3 cube"""
