@@ -58,7 +58,48 @@ Octree::Octree(const float *boxes, const float *transform, size_t n, long leaf_s
             if (space[3 + d] < boxes[6 * i + 3 + d]) space[3 + d] = boxes[6 * i + 3 + d];
         }
     root_ = new Node(space, 0, leaf_size);
-    for (size_t i = 0; i < n; i++) add(root_, (int)i);
+    std::vector<int> seq(n);
+    for (size_t i = 0; i < n; i++) seq[i] = (int)i;
+    // subtrees side by side below the first two levels (64 of them) where the
+    // tile is large; small tiles in turn
+    const int par = n >= 50000 ? 2 : 0;
+#ifdef _OPENMP
+#pragma omp parallel if (par > 0)
+#pragma omp single
+#endif
+    build(root_, seq, 0, par);
+}
+
+// Adds seq[from..] to n in order, as add() one by one would. A node's subtree
+// depends only on the objects it receives, in their order: once n has split,
+// every later object goes to the children its box overlaps, so each child's
+// part of the rest is filtered out of seq in order and the children are built
+// independently (side by side for par_depth more levels).
+void Octree::build(Node *n, const std::vector<int> &seq, size_t from, int par_depth) {
+    if (par_depth <= 0) {
+        for (size_t j = from; j < seq.size(); j++) add(n, seq[j]);
+        return;
+    }
+    size_t i = from;
+    while (i < seq.size() && n->leaf) add(n, seq[i++]);
+    if (i == seq.size()) return;
+    n->size += (uint32_t)(seq.size() - i);
+    for (int c = 0; c < 8; c++) {
+        auto part = [this, n, c, i, &seq, par_depth]() {
+            Node *ch = n->child[c];
+            std::vector<int> sub;
+            for (size_t j = i; j < seq.size(); j++)
+                if (overlaps(ch->box, boxes_ + 6 * seq[j])) sub.push_back(seq[j]);
+            build(ch, sub, 0, par_depth - 1);
+        };
+#ifdef _OPENMP
+#pragma omp task default(none) firstprivate(part)
+#endif
+        part();
+    }
+#ifdef _OPENMP
+#pragma omp taskwait
+#endif
 }
 
 Octree::~Octree() { delete root_; }

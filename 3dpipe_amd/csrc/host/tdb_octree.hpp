@@ -50,6 +50,7 @@ private:
     const float *boxes_;
     const float *transform_;
     void add(Node *n, int id);
+    void build(Node *n, const std::vector<int> &seq, size_t from, int par_depth);
     void within_rec(const Node *n, const float *qbox, const float *qt, int self, float tau,
                     std::vector<std::pair<int, Range>> &out) const;
     void intersect_rec(const Node *n, const float *qbox, int self, std::vector<int> &out) const;
