@@ -296,7 +296,7 @@ class _StreamOpts(_c.Structure):
 STREAM = ["chunks", "h2d_bytes", "decode_ms", "copy_ms", "join_ms", "wall_ms", "overlap",
           "peak_device_bytes", "index_ms", "candidates_ms", "objects_decoded", "splits",
           "record_bytes_all", "record_bytes_shipped", "meta_bytes", "workers",
-          "candidate_host_targets"]
+          "candidate_host_targets", "plan_ms", "workers_setup_ms"]
 
 # ------------------------------------------------------------------ join
 

@@ -67,7 +67,7 @@ struct tdb_result {
     double stage_ms[TDP_STAGE_COUNT] = {};
     uint64_t stage_n[TDP_STAGE_COUNT] = {};
     uint64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double stream[17] = {};  // streaming figures (tdb_result_stream_n)
+    double stream[19] = {};  // streaming figures (tdb_result_stream_n)
 };
 
 using Clock = std::chrono::steady_clock;

@@ -436,7 +436,7 @@ void tdb_result_stream(const tdb_result *r, double *out12) {
 
 int tdb_result_stream_n(const tdb_result *r, double *out, int n) {
     if (!out || n < 0) return 0;
-    n = std::min(n, 17);
+    n = std::min(n, 19);
     for (int k = 0; k < n; k++) out[k] = r ? r->stream[k] : 0.0;
     return n;
 }
@@ -544,6 +544,7 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     const bool lazy = so->lazy != 0;
     // fewer workers (larger tile slots) when a target with its candidates
     // does not fit the slot of as many as were asked for
+    t0 = Clock::now();
     int n_workers = so->overlap ? std::max(1, so->workers > 0 ? so->workers : 4) : 1;
     const uint64_t max_targets = so->max_chunk_targets ? so->max_chunk_targets : UINT64_MAX;
     std::vector<Chunk> chunks;
@@ -563,6 +564,8 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     }
     std::vector<uint32_t>().swap(prefix);
     std::vector<uint32_t>().swap(ids);
+    res->stream[17] = ms_since(t0);
+    t0 = Clock::now();
     const int n_tiles = self ? 1 : 2;
     size_t cap_obj[2] = {1, 1}, cap_vox[2] = {1, 1}, cap_tri[2] = {1, 1};
     for (const Chunk &c : chunks)
@@ -620,6 +623,7 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         return rc;
     }
     for (Worker &w : W) tdp_set_memory_limit(w.ctx, tiles_bytes / n_workers + (budget - tiles_bytes) / n_workers);
+    res->stream[18] = ms_since(t0);
 
     const size_t nc = chunks.size();
     Progress pg;

@@ -160,6 +160,7 @@ def stream_bench(tdp, n_nuclei, cube, budget_gib, knn, seed, tuning, log_fn, laz
         "end_to_end_gbps": h2d / (st["wall_ms"] * 1e6) if st["wall_ms"] else None,
         "decode_gbps": h2d / (st["decode_ms"] * 1e6) if st["decode_ms"] else None,
         "stage_ms": {"index": st["index_ms"], "candidates": st["candidates_ms"],
+                     "plan": st["plan_ms"], "workers_setup": st["workers_setup_ms"],
                      "decode": st["decode_ms"], "copy": st["copy_ms"], "join": st["join_ms"],
                      "pipeline_wall": st["wall_ms"]},
         "candidate_targets_from_host_octree": int(st["candidate_host_targets"]),

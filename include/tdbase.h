@@ -234,13 +234,14 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *s1, const tdb_source *s2,
  * boxes), [9] candidate ms, [10] objects decoded, [11] chunk joins split to
  * fit the budget */
 void tdb_result_stream(const tdb_result *r, double *out12);
-/* the same figures and more, n of them (at most 17; returns how many): [0..11]
+/* the same figures and more, n of them (at most 19; returns how many): [0..11]
  * as above with [2] decode ms including the per-round record decodes and [3]
  * copy ms including the per-round record puts, then [12] record bytes of the
  * chunks' objects (what shipping every record would copy), [13] record bytes
  * shipped, [14] voxel and object array bytes shipped, [15] workers, [16]
  * targets whose kNN candidate lists came from the host octree (the device
- * builds the others; -1 when the octree built them all) */
+ * builds the others; -1 when the octree built them all), [17] chunk plan ms,
+ * [18] worker set-up ms (contexts, tile slots, pinned staging) */
 int tdb_result_stream_n(const tdb_result *r, double *out, int n);
 
 /* ---- serving: queries against one resident tile, continuously batched ----
