@@ -587,6 +587,34 @@ def test_host_octree_matches_oracle(tdp, small_tiles):
                 assert ids[prefix[i]:prefix[i + 1]].tolist() == ci[:c].tolist(), (query, i)
 
 
+def test_host_octree_built_by_subtrees_matches_oracle(tdp, tmp_path):
+    """a tile of 60,000 objects (octree subtrees built side by side): kNN and
+    within lists, order included, equal the oracle octree's, which inserts
+    the objects one by one"""
+    n, _ = tdp.generate(seed=11, n_vessels=0, nuclei_per_vessel=60000, cube=12000.0, subdiv=0,
+                        nucleus_r_min=6.0, nucleus_r_max=14.0)
+    assert n.n_obj >= 50000
+    pth = str(tmp_path / "n60k.dt")
+    n.save(pth)
+    on = orc.OracleTile(pth)
+    cap = 4096
+    box = np.zeros(9, np.float32)
+    ci = np.zeros(cap, np.int32); mn = np.zeros(cap, np.float32); mx = np.zeros(cap, np.float32)
+    for query, k, tau in (("nn", 10, 0.0), ("within", 1, 150.0)):
+        prefix, ids = tdp.candidates(n, None, query=query, knn=k, within_dist=tau)
+        for i in range(0, n.n_obj, 211):
+            orc.lib().orc_tile_object_box(on.h, i, orc.ptr(box))
+            if query == "nn":
+                c = orc.lib().orc_query_knn(on.h, orc.ptr(box[:3]), orc.ptr(box[3:6]),
+                                            orc.ptr(box[6:]), i, k, orc.ptr(ci), orc.ptr(mn),
+                                            orc.ptr(mx), cap)
+            else:
+                c = orc.lib().orc_query_within(on.h, orc.ptr(box[:3]), orc.ptr(box[3:6]),
+                                               orc.ptr(box[6:]), i, tau, orc.ptr(ci),
+                                               orc.ptr(mn), orc.ptr(mx), cap)
+            assert ids[prefix[i]:prefix[i + 1]].tolist() == ci[:c].tolist(), (query, i)
+
+
 def test_cli_without_gpu_fails_loudly(small_tiles):
     if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
         pytest.skip("a GPU is present")
