@@ -834,33 +834,32 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
         return TDP_ERR_HIP;
     }
 
-    // results in target order (each target's pairs stay in report order)
+    // results in target order, each target's pairs in chunk order and in
+    // report order within a chunk: a counting sort by target (stable)
     size_t total = 0;
-    for (const tdb_result &r : parts) total += r.target.size();
-    std::vector<std::pair<int32_t, uint32_t>> key;  // (target, chunk-major position)
-    key.reserve(total);
-    std::vector<std::pair<uint32_t, uint32_t>> where;  // (chunk, index)
-    where.reserve(total);
-    for (uint32_t i = 0; i < nc; i++)
-        for (uint32_t q = 0; q < parts[i].target.size(); q++) {
-            key.push_back({parts[i].target[q], (uint32_t)where.size()});
-            where.push_back({i, q});
-        }
-    std::sort(key.begin(), key.end());
-    res->target.reserve(total);
-    res->obj2.reserve(total);
-    res->rmin.reserve(total);
-    res->rmax.reserve(total);
-    res->round.reserve(total);
-    for (auto &kq : key) {
-        const auto [i, q] = where[kq.second];
-        const tdb_result &r = parts[i];
-        res->target.push_back(r.target[q]);
-        res->obj2.push_back(r.obj2[q]);
-        res->rmin.push_back(r.rmin[q]);
-        res->rmax.push_back(r.rmax[q]);
-        res->round.push_back(r.round[q]);
+    int32_t t_hi = -1;
+    for (const tdb_result &r : parts) {
+        total += r.target.size();
+        for (int32_t t : r.target) t_hi = std::max(t_hi, t);
     }
+    std::vector<uint64_t> at((size_t)(t_hi + 1) + 1, 0);
+    for (const tdb_result &r : parts)
+        for (int32_t t : r.target) at[(size_t)t + 1]++;
+    for (size_t t = 1; t < at.size(); t++) at[t] += at[t - 1];
+    res->target.resize(total);
+    res->obj2.resize(total);
+    res->rmin.resize(total);
+    res->rmax.resize(total);
+    res->round.resize(total);
+    for (const tdb_result &r : parts)
+        for (size_t q = 0; q < r.target.size(); q++) {
+            const uint64_t w = at[r.target[q]]++;
+            res->target[w] = r.target[q];
+            res->obj2[w] = r.obj2[q];
+            res->rmin[w] = r.rmin[q];
+            res->rmax[w] = r.rmax[q];
+            res->round[w] = r.round[q];
+        }
     for (const tdb_result &r : parts) {
         for (int k = 2; k < 10; k++) if (k != 6) res->timing[k] += r.timing[k];
         for (int k = 0; k < 9; k++)
