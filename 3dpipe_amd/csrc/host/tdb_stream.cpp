@@ -559,8 +559,19 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     {
         std::vector<uint32_t> kpos(s1.n_obj(), 0);
         for (size_t k = 0; k < N; k++) kpos[order[k]] = (uint32_t)k;
-        std::vector<int32_t> loc(std::max(s1.n_obj(), s2.n_obj()), -1);
-        for (Chunk &c : chunks) finish_chunk(c, self, kpos, prefix, ids, s1, s2, loc);
+        // chunks side by side, each thread with its own renumbering scratch
+        // (finish_chunk leaves it as it found it)
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+        {
+            std::vector<int32_t> loc(std::max(s1.n_obj(), s2.n_obj()), -1);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 2)
+#endif
+            for (long i = 0; i < (long)chunks.size(); i++)
+                finish_chunk(chunks[i], self, kpos, prefix, ids, s1, s2, loc);
+        }
     }
     std::vector<uint32_t>().swap(prefix);
     std::vector<uint32_t>().swap(ids);
