@@ -20,6 +20,7 @@ namespace tdp {
 constexpr int kCandMaxObj = 1024;              // tile-2 objects on the device path
 constexpr int kCandPerLane = kCandMaxObj / kWave;
 constexpr uint32_t kCandMaxCap = 256;          // longest list ranked on the device
+constexpr int kWithinMaxObj = 16384;           // tile-2 objects of the within lists
 
 // aab::distance min part (himesh_box.cpp:149-168) with a = object, b = query
 __device__ __forceinline__ float cand_box_min(const float *a, const float *b) {
@@ -156,19 +157,21 @@ __global__ void __launch_bounds__(kBlock) k_knn_candidates(
 // box passes: the walk is the leaves in depth-first order, each tested on its
 // own. One wave per target tests 64 leaves at a time, then walks the passing
 // ones in order, 64 objects at a time, with an LDS bit set of the objects
-// taken. Lists longer than cap are flagged for the host octree.
+// taken (kWithinMaxObj bits per wave). Lists longer than cap are flagged for
+// the host octree.
 __global__ void __launch_bounds__(kBlock) k_within_candidates(
     uint64_t N, const float *__restrict__ box1, uint32_t n2, const float *__restrict__ box2,
     int self, const float *__restrict__ tau_t, uint32_t n_leaves,
     const float *__restrict__ leaf_box, const uint32_t *__restrict__ leaf_off,
     const uint32_t *__restrict__ leaf_obj, uint32_t cap, uint32_t *__restrict__ count,
     uint32_t *__restrict__ ids, uint8_t *__restrict__ fallback) {
-    __shared__ uint32_t seen_all[kWavesPerBlock][kCandMaxObj / 32];
+    __shared__ uint32_t seen_all[kWavesPerBlock][kWithinMaxObj / 32];
     const uint32_t t = wave_item();
     if (t >= N) return;
     const int lane = lane_id();
     uint32_t *const seen = seen_all[threadIdx.x / kWave];
-    for (uint32_t i = (uint32_t)lane; i < kCandMaxObj / 32; i += kWave) seen[i] = 0u;
+    const uint32_t words = (n2 + 31) / 32;
+    for (uint32_t i = (uint32_t)lane; i < words; i += kWave) seen[i] = 0u;
     wave_sync();
     float qb[6];
 #pragma unroll

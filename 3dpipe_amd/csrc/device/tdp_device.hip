@@ -1409,8 +1409,8 @@ extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
         ctx->err = "tdp_within_candidates: tiles uploaded without object boxes";
         return TDP_ERR_ARG;
     }
-    if (t2->n_obj > (size_t)kCandMaxObj || N > t1->n_obj || n_leaves > 0xffffffffu) {
-        ctx->err = "tdp_within_candidates: tile 2 has more than 1024 objects";
+    if (t2->n_obj > (size_t)kWithinMaxObj || N > t1->n_obj || n_leaves > 0xffffffffu) {
+        ctx->err = "tdp_within_candidates: tile 2 has more than 16384 objects";
         return TDP_ERR_ARG;
     }
     const size_t n_lobj = n_leaves ? leaf_off[n_leaves] : 0;

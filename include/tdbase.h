@@ -139,7 +139,7 @@ int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_op
                        uint32_t *prefix, uint32_t *ids, size_t cap);
 /* the same lists as tdb_join builds them: on the engine's device for kNN
  * (tdp_knn_candidates, tile 2 of any size) and within with tile 2 of at most
- * 1024 objects (tdp_within_candidates), the octree for the targets the
+ * 16384 objects (tdp_within_candidates), the octree for the targets the
  * device leaves to it (*n_fallback of them) and for everything else; returns
  * the number of ids or -1 */
 int64_t tdb_engine_candidates(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,

@@ -378,6 +378,24 @@ def test_device_knn_candidates_grid_equal_octree(tdp, eng, many_nuclei, tmp_path
     assert fb > 0
 
 
+@pytest.mark.parametrize("tau", [0.0, 60.0, 250.0])
+def test_device_within_candidates_larger_tile_equal_octree(tdp, eng, many_nuclei, tau):
+    """within lists for tile 2 of 1025..16384 objects equal the octree's
+    query_within lists, order included: self-joins of clustered and uniform
+    nuclei, each against the other, and vessels against nuclei (long lists
+    past 256 left to the octree)"""
+    pn, pv, pu = many_nuclei
+    n, v, u = (tdp.Tile.load(p) for p in (pn, pv, pu))
+    for t1, t2 in ((n, None), (u, None), (u, n), (n, u), (v, n)):
+        hp, hi = tdp.candidates(t1, t2, "within", within_dist=tau)
+        dp, di, fb = eng.candidates(t1, t2, "within", within_dist=tau)
+        assert np.array_equal(hp, dp) and np.array_equal(hi, di), (t1.n_obj, tau)
+        if t1 is not v and tau <= 60.0:
+            assert fb == 0
+    if tau > 0:
+        assert len(hi) > v.n_obj
+
+
 @pytest.fixture(scope="module")
 def many_vessels(tdp, tmp_path_factory):
     """300 vessels (an octree of many leaves, long vessels spanning several)
