@@ -520,6 +520,103 @@ __global__ void __launch_bounds__(kBlock) k_knn_candidates_grid(
     }
 }
 
+// ----------------------------------------------------- within lists on a grid
+//
+// For tile 2 of more than kWithinMaxObj objects: the lists of
+// k_within_candidates (objects, not the target, whose box distance is at
+// most tau, in the order of the octree's depth-first walk, each at the first
+// leaf that takes it), from the grid of k_knn_candidates_grid. An object with
+// box distance at most tau has its transform point within tau + E of the
+// target's box per axis, so the cells of the target's box grown by tau + E
+// hold every such object. Each object's leaves (depth-first rank and its
+// place in the leaf, in rank order: mem[mem_off[o] ..]) give its place in
+// the walk: the first of them whose box passes (a leaf is reached exactly
+// when its own box passes). An object in no leaf that passes (none: a box
+// without volume overlaps no leaf) is not listed, as the walk never meets it.
+// The list is ranked by (leaf rank, place); longer than cap: to the host.
+__global__ void __launch_bounds__(kBlock) k_within_candidates_grid(
+    uint64_t t_end, uint64_t t0, const float *__restrict__ box1, CandGrid g,
+    const uint64_t *__restrict__ off, const float4 *__restrict__ gobj, int self,
+    const float *__restrict__ tau_t, const uint32_t *__restrict__ mem_off,
+    const uint2 *__restrict__ mem, const float *__restrict__ leaf_box, uint32_t cap,
+    uint32_t *__restrict__ count, uint32_t *__restrict__ ids, uint8_t *__restrict__ fallback) {
+    __shared__ uint64_t lk_all[kWavesPerBlock][kCandMaxCap];
+    __shared__ uint32_t lo_all[kWavesPerBlock][kCandMaxCap];
+    __shared__ uint32_t lc_all[kWavesPerBlock];
+    const uint32_t w = wave_item();
+    const uint64_t t = t0 + w;
+    if (t >= t_end) return;
+    const int lane = lane_id();
+    const int wv = threadIdx.x / kWave;
+    uint64_t *const lk = lk_all[wv];
+    uint32_t *const lo = lo_all[wv];
+    uint32_t *const lc = &lc_all[wv];
+    float qb[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) qb[i] = box1[6 * t + i];
+    const float tau = tau_t[t];
+    const uint32_t skip = self ? (uint32_t)t : 0xffffffffu;
+    bool ok = tau >= 0.f && __builtin_isfinite(tau);
+#pragma unroll
+    for (int i = 0; i < 6; i++) ok &= __builtin_isfinite(qb[i]);
+    uint32_t cn = 0;
+    if (ok) {
+        int c0[3], c1[3];
+        const float W = (tau + g.emax) * (1.f + 0x1p-12f);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float m = 0x1p-18f * fmaxf(fabsf(qb[a]), fabsf(qb[3 + a])) + g.tol;
+            c0[a] = grid_cell(g, a, qb[a] - (W + m));
+            c1[a] = grid_cell(g, a, qb[3 + a] + (W + m));
+        }
+        if (lane == 0) *lc = 0u;
+        wave_sync();
+        const uint32_t ny = (uint32_t)(c1[1] - c0[1] + 1);
+        const uint32_t rows = ny * (uint32_t)(c1[2] - c0[2] + 1);
+        for (uint32_t row = (uint32_t)lane; row < rows; row += kWave) {
+            const uint32_t yy = (uint32_t)c0[1] + row % ny, zz = (uint32_t)c0[2] + row / ny;
+            const size_t base = ((size_t)zz * (uint32_t)g.n[1] + yy) * (uint32_t)g.n[0];
+            const uint64_t e = off[base + (uint32_t)c1[0] + 1];
+            for (uint64_t p = off[base + (uint32_t)c0[0]]; p < e; p++) {
+                const uint32_t o = __float_as_uint(gobj[3 * p].w);
+                if (o == skip || !(cand_box_min4(gobj[3 * p + 1], gobj[3 * p + 2], qb) <= tau))
+                    continue;
+                // its first leaf on the walk
+                uint64_t key = ~0ull;
+                for (uint32_t k = mem_off[o]; k < mem_off[o + 1]; k++) {
+                    const uint2 mk = mem[k];
+                    if (cand_box_min(leaf_box + 6 * (size_t)mk.x, qb) <= tau) {
+                        key = ((uint64_t)mk.x << 32) | mk.y;
+                        break;
+                    }
+                }
+                if (key == ~0ull) continue;
+                const uint32_t at = atomicAdd(lc, 1u);
+                if (at < kCandMaxCap) {
+                    lk[at] = key;
+                    lo[at] = o;
+                }
+            }
+        }
+        wave_sync();
+        cn = __builtin_amdgcn_readfirstlane(*lc);
+        ok = cn <= cap && cn <= kCandMaxCap;
+    }
+    // rank by place on the walk (keys differ: one object per place)
+    if (ok) {
+        for (uint32_t i = (uint32_t)lane; i < cn; i += kWave) {
+            const uint64_t k = lk[i];
+            uint32_t rank = 0;
+            for (uint32_t q = 0; q < cn; q++) rank += lk[q] < k;
+            ids[(size_t)w * cap + rank] = lo[i];
+        }
+    }
+    if (lane == 0) {
+        fallback[w] = ok ? 0 : 1;
+        count[w] = ok ? cn : 0;
+    }
+}
+
 // per-target lists (stride cap) to one array at offs[t]
 __global__ void __launch_bounds__(kBlock) k_cand_compact(uint64_t N, uint32_t cap,
                                                         const uint32_t *__restrict__ count,

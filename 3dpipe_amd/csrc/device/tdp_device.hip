@@ -1156,44 +1156,28 @@ extern "C" int tdp_knn_status(tdp_ctx *ctx, size_t N, const uint32_t *prefix,
 
 // ------------------------------------------------------------ join
 
-// kNN lists for tile 2 of more than kCandMaxObj objects: tile 2 binned into a
-// grid of cubic cells of about two objects each (k_grid_*), then the targets
-// in slices of kGridSlice, each slice's lists compacted on the device and
-// kept until the total is known.
-constexpr size_t kGridSlice = (size_t)1 << 18;
-
-static int knn_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
-                               int self, const int32_t *k_target, uint32_t *prefix, uint32_t *ids,
-                               size_t ids_cap, uint64_t *n_ids, uint8_t *fallback) {
+// tile 2's objects binned into a grid of cubic cells of about two objects
+// each, in cell order (k_grid_*): the grid, its cell offsets (doff) and the
+// objects' records (gobj); device buffers from grab(), held by the caller
+template <typename Grab>
+static int build_cand_grid(tdp_ctx *ctx, const tdp_tile *t2, Grab &&grab, CandGrid &g,
+                           uint64_t *&doff, float4 *&gobj) {
     const uint32_t n2 = (uint32_t)t2->n_obj;
-    const uint32_t cap = kCandMaxCap;
     hipStream_t s = ctx->stream;
-    std::vector<void *> held;
-    auto grab = [&](auto **p, size_t count) {
-        const int rc = palloc(ctx, p, count);
-        if (rc == TDP_OK) held.push_back((void *)*p);
-        return rc;
-    };
-    auto done = [&](int rc) {
-        (void)hipStreamSynchronize(s);
-        for (void *p : held) pfree(ctx, p);
-        return rc;
-    };
-    auto hip_err = [&](hipError_t e) {
-        ctx->err = std::string("tdp_knn_candidates: ") + hipGetErrorString(e);
-        return done(TDP_ERR_HIP);
+    auto grid_err = [&](hipError_t e) {
+        ctx->err = std::string("candidate grid: ") + hipGetErrorString(e);
+        return TDP_ERR_HIP;
     };
     // grid bounds and cell size
     float *dbnd = nullptr;
     int rc = grab(&dbnd, 8);
-    if (rc != TDP_OK) return done(rc);
+    if (rc != TDP_OK) return rc;
     k_grid_bounds<<<1, 1024, 0, s>>>(n2, t2->obj_box, t2->obj_tr, dbnd);
     float hb[7];
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(hb, dbnd, sizeof hb, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_err(e);
-    CandGrid g{};
+    if (e != hipSuccess) return grid_err(e);
     double ext[3], emx = 0, amax = 0;
     bool finite = std::isfinite(hb[6]);
     for (int a = 0; a < 3; a++) {
@@ -1228,59 +1212,74 @@ static int knn_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const
     const size_t ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
     // objects in cell order
     uint32_t *dcell = nullptr, *dcnt = nullptr, *dcur = nullptr;
-    uint64_t *doff = nullptr;
-    float4 *gobj = nullptr;
     if (rc == TDP_OK) rc = grab(&dcell, n2);
     if (rc == TDP_OK) rc = grab(&dcnt, ncell + 1);
     if (rc == TDP_OK) rc = grab(&dcur, ncell);
     if (rc == TDP_OK) rc = grab(&doff, ncell + 1);
     if (rc == TDP_OK) rc = grab(&gobj, 3 * (size_t)n2);
-    if (rc != TDP_OK) return done(rc);
+    if (rc != TDP_OK) return rc;
     e = hipMemsetAsync(dcnt, 0, 4 * (ncell + 1), s);
     if (e == hipSuccess) e = hipMemsetAsync(dcur, 0, 4 * ncell, s);
     if (e == hipSuccess) {
         k_grid_count<<<blocks_for(n2, kBlock), kBlock, 0, s>>>(n2, t2->obj_tr, g, dcell, dcnt);
         e = hipGetLastError();
     }
-    if (e != hipSuccess) return hip_err(e);
+    if (e != hipSuccess) return grid_err(e);
     uint64_t placed = 0;
     rc = scan_exclusive<uint32_t>(ctx, dcnt, ncell + 1, doff, &placed);
-    if (rc != TDP_OK) return done(rc);
+    if (rc != TDP_OK) return rc;
     if (placed != n2) {
-        ctx->err = "tdp_knn_candidates: grid binning lost objects";
-        return done(TDP_ERR_HIP);
+        ctx->err = "candidate grid: binning lost objects";
+        return TDP_ERR_HIP;
     }
     k_grid_fill<<<blocks_for(n2, kBlock), kBlock, 0, s>>>(n2, t2->obj_box, t2->obj_tr, dcell, doff,
                                                          dcur, gobj);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_err(e);
-    // targets in slices
+    if ((e = hipGetLastError()) != hipSuccess) return grid_err(e);
+    return TDP_OK;
+}
+
+// kNN lists for tile 2 of more than kCandMaxObj objects: tile 2 binned into a
+// grid of cubic cells of about two objects each (k_grid_*), then the targets
+// in slices of kGridSlice, each slice's lists compacted on the device and
+// kept until the total is known.
+constexpr size_t kGridSlice = (size_t)1 << 18;
+
+// Targets in slices of kGridSlice: launch(t0, sn, count, lists, flags) fills
+// the slice's counts, lists (stride cap) and flags; each slice's lists are
+// compacted on the device and kept until the total is known, then copied out
+// as CSR (nothing is written to ids when they do not fit: *n_ids says how
+// many there are).
+template <typename Grab, typename Launch>
+static int cand_slices(tdp_ctx *ctx, size_t N, uint32_t cap, Grab &&grab, Launch &&launch,
+                       const char *who, uint32_t *prefix, uint32_t *ids, size_t ids_cap,
+                       uint64_t *n_ids, uint8_t *fallback) {
+    hipStream_t s = ctx->stream;
+    auto hip_err = [&](hipError_t e) {
+        ctx->err = std::string(who) + ": " + hipGetErrorString(e);
+        return TDP_ERR_HIP;
+    };
     const size_t S = std::min(N, kGridSlice);
-    int32_t *dk = nullptr;
     uint32_t *dc = nullptr, *dl = nullptr;
     uint8_t *df = nullptr;
     uint64_t *dso = nullptr;
-    if (rc == TDP_OK) rc = grab(&dk, N);
-    if (rc == TDP_OK) rc = grab(&dc, S);
+    int rc = grab(&dc, S);
     if (rc == TDP_OK) rc = grab(&dl, S * cap);
     if (rc == TDP_OK) rc = grab(&df, S);
     if (rc == TDP_OK) rc = grab(&dso, S);
-    if (rc != TDP_OK) return done(rc);
-    if ((e = hipMemcpyAsync(dk, k_target, 4 * N, hipMemcpyHostToDevice, s)) != hipSuccess)
-        return hip_err(e);
+    if (rc != TDP_OK) return rc;
+    hipError_t e = hipSuccess;
     std::vector<uint32_t> cnt(N);
     std::vector<std::pair<uint32_t *, uint64_t>> parts;
     uint64_t total = 0;
     for (size_t t0 = 0; t0 < N; t0 += S) {
         const size_t sn = std::min(S, N - t0);
-        k_knn_candidates_grid<<<wave_grid(sn), kBlock, 0, s>>>(t0 + sn, t0, t1->obj_box,
-                                                               t1->obj_tr, g, doff, gobj, self, dk,
-                                                               cap, dc, dl, df);
+        launch(t0, sn, dc, dl, df);
         if ((e = hipGetLastError()) != hipSuccess) return hip_err(e);
         uint64_t st = 0;
         rc = scan_exclusive<uint32_t>(ctx, dc, sn, dso, &st);
-        if (rc != TDP_OK) return done(rc);
+        if (rc != TDP_OK) return rc;
         uint32_t *dpart = nullptr;
-        if ((rc = grab(&dpart, st)) != TDP_OK) return done(rc);
+        if ((rc = grab(&dpart, st)) != TDP_OK) return rc;
         k_cand_compact<<<wave_grid(sn), kBlock, 0, s>>>(sn, cap, dc, dso, dl, dpart);
         e = hipGetLastError();
         if (e == hipSuccess)
@@ -1294,8 +1293,8 @@ static int knn_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e);
     *n_ids = total;
     if (total > ids_cap) {
-        ctx->err = "tdp_knn_candidates: ids array too small";
-        return done(TDP_ERR_ARG);
+        ctx->err = std::string(who) + ": ids array too small";
+        return TDP_ERR_ARG;
     }
     uint64_t at = 0;
     for (auto &pt : parts) {
@@ -1307,7 +1306,121 @@ static int knn_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const
     }
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_err(e);
     for (size_t t = 0; t < N; t++) prefix[t + 1] = prefix[t] + cnt[t];
-    return done(TDP_OK);
+    return TDP_OK;
+}
+
+static int knn_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
+                               int self, const int32_t *k_target, uint32_t *prefix, uint32_t *ids,
+                               size_t ids_cap, uint64_t *n_ids, uint8_t *fallback) {
+    const uint32_t n2 = (uint32_t)t2->n_obj;
+    const uint32_t cap = kCandMaxCap;
+    hipStream_t s = ctx->stream;
+    std::vector<void *> held;
+    auto grab = [&](auto **p, size_t count) {
+        const int rc = palloc(ctx, p, count);
+        if (rc == TDP_OK) held.push_back((void *)*p);
+        return rc;
+    };
+    auto done = [&](int rc) {
+        (void)hipStreamSynchronize(s);
+        for (void *p : held) pfree(ctx, p);
+        return rc;
+    };
+    auto hip_err = [&](hipError_t e) {
+        ctx->err = std::string("tdp_knn_candidates: ") + hipGetErrorString(e);
+        return done(TDP_ERR_HIP);
+    };
+    CandGrid g{};
+    uint64_t *doff = nullptr;
+    float4 *gobj = nullptr;
+    int rc = build_cand_grid(ctx, t2, grab, g, doff, gobj);
+    if (rc != TDP_OK) return done(rc);
+    hipError_t e = hipSuccess;
+    int32_t *dk = nullptr;
+    if ((rc = grab(&dk, N)) != TDP_OK) return done(rc);
+    if ((e = hipMemcpyAsync(dk, k_target, 4 * N, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_err(e);
+    return done(cand_slices(
+        ctx, N, cap, grab,
+        [&](size_t t0, size_t sn, uint32_t *dc, uint32_t *dl, uint8_t *df) {
+            k_knn_candidates_grid<<<wave_grid(sn), kBlock, 0, s>>>(
+                t0 + sn, t0, t1->obj_box, t1->obj_tr, g, doff, gobj, self, dk, cap, dc, dl, df);
+        },
+        "tdp_knn_candidates", prefix, ids, ids_cap, n_ids, fallback));
+}
+
+// Within lists for tile 2 of more than kWithinMaxObj objects: the grid of
+// tile 2, each object's leaves in walk order (depth-first rank, place in the
+// leaf) from the caller's leaf arrays, then k_within_candidates_grid over the
+// targets in slices.
+static int within_candidates_grid(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
+                                  int self, const float *tau_target, size_t n_leaves,
+                                  const float *leaf_box, const uint32_t *leaf_off,
+                                  const uint32_t *leaf_obj, uint32_t *prefix, uint32_t *ids,
+                                  size_t ids_cap, uint64_t *n_ids, uint8_t *fallback) {
+    const uint32_t n2 = (uint32_t)t2->n_obj;
+    const size_t n_lobj = n_leaves ? leaf_off[n_leaves] : 0;
+    if (!t2->obj_tr || n_lobj >= 0xffffffffull) {
+        ctx->err = "tdp_within_candidates: tile 2 without proxy points, or too many leaf entries";
+        return TDP_ERR_ARG;
+    }
+    const uint32_t cap = kCandMaxCap;
+    hipStream_t s = ctx->stream;
+    std::vector<void *> held;
+    auto grab = [&](auto **p, size_t count) {
+        const int rc = palloc(ctx, p, count);
+        if (rc == TDP_OK) held.push_back((void *)*p);
+        return rc;
+    };
+    auto done = [&](int rc) {
+        (void)hipStreamSynchronize(s);
+        for (void *p : held) pfree(ctx, p);
+        return rc;
+    };
+    // each object's leaves, in walk order
+    std::vector<uint32_t> moff(n2 + 1, 0u);
+    for (size_t k = 0; k < n_lobj; k++) moff[leaf_obj[k] + 1]++;
+    for (uint32_t o = 0; o < n2; o++) moff[o + 1] += moff[o];
+    std::vector<uint2> mem(std::max<size_t>(n_lobj, 1));
+    {
+        std::vector<uint32_t> cur(moff.begin(), moff.end() - 1);
+        for (size_t l = 0; l < n_leaves; l++)
+            for (uint32_t k = leaf_off[l]; k < leaf_off[l + 1]; k++)
+                mem[cur[leaf_obj[k]]++] = make_uint2((uint32_t)l, k - leaf_off[l]);
+    }
+    uint32_t *dmoff = nullptr;
+    uint2 *dmem = nullptr;
+    float *dlbox = nullptr, *dtau = nullptr;
+    int rc = grab(&dmoff, n2 + 1);
+    if (rc == TDP_OK) rc = grab(&dmem, mem.size());
+    if (rc == TDP_OK) rc = grab(&dlbox, 6 * std::max<size_t>(n_leaves, 1));
+    if (rc == TDP_OK) rc = grab(&dtau, N);
+    if (rc != TDP_OK) return done(rc);
+    hipError_t e = hipMemcpyAsync(dmoff, moff.data(), 4 * (n2 + 1), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dmem, mem.data(), sizeof(uint2) * mem.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && n_leaves)
+        e = hipMemcpyAsync(dlbox, leaf_box, 24 * n_leaves, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtau, tau_target, 4 * N, hipMemcpyHostToDevice, s);
+    // the host vectors must outlive the copies
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        ctx->err = std::string("tdp_within_candidates: ") + hipGetErrorString(e);
+        return done(TDP_ERR_HIP);
+    }
+    CandGrid g{};
+    uint64_t *doff = nullptr;
+    float4 *gobj = nullptr;
+    rc = build_cand_grid(ctx, t2, grab, g, doff, gobj);
+    if (rc != TDP_OK) return done(rc);
+    return done(cand_slices(
+        ctx, N, cap, grab,
+        [&](size_t t0, size_t sn, uint32_t *dc, uint32_t *dl, uint8_t *df) {
+            k_within_candidates_grid<<<wave_grid(sn), kBlock, 0, s>>>(
+                t0 + sn, t0, t1->obj_box, g, doff, gobj, self, dtau, dmoff, dmem, dlbox, cap, dc,
+                dl, df);
+        },
+        "tdp_within_candidates", prefix, ids, ids_cap, n_ids, fallback));
 }
 
 extern "C" int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N, const tdp_tile *t2,
@@ -1409,8 +1522,8 @@ extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
         ctx->err = "tdp_within_candidates: tiles uploaded without object boxes";
         return TDP_ERR_ARG;
     }
-    if (t2->n_obj > (size_t)kWithinMaxObj || N > t1->n_obj || n_leaves > 0xffffffffu) {
-        ctx->err = "tdp_within_candidates: tile 2 has more than 16384 objects";
+    if (N > t1->n_obj || t2->n_obj >= 0xffffffffull || n_leaves > 0xffffffffu) {
+        ctx->err = "tdp_within_candidates: more targets than tile 1 holds, or tile 2 too large";
         return TDP_ERR_ARG;
     }
     const size_t n_lobj = n_leaves ? leaf_off[n_leaves] : 0;
@@ -1426,6 +1539,9 @@ extern "C" int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t N,
         }
     prefix[0] = 0;
     if (N == 0) return TDP_OK;
+    if (t2->n_obj > (size_t)kWithinMaxObj)
+        return within_candidates_grid(ctx, t1, N, t2, self, tau_target, n_leaves, leaf_box,
+                                      leaf_off, leaf_obj, prefix, ids, ids_cap, n_ids, fallback);
     const uint32_t cap = kCandMaxCap;
     float *dtau = nullptr, *dlbox = nullptr;
     uint32_t *dloff = nullptr, *dlobj = nullptr, *dc = nullptr, *dl = nullptr, *dids = nullptr;

@@ -222,8 +222,8 @@ tdp_tile_desc tdb_desc_of(const tdb::Tile &t) {
     return d;
 }
 
-// Candidates with the device where it applies (kNN with tile 2 of any size,
-// within with tile 2 of at most 16384 objects, both tiles resident): the
+// Candidates with the device where it applies (kNN and within, tile 2 of any
+// size, both tiles resident): the
 // device lists, and the octree for the targets the device leaves to it
 // (built only when there are such targets). Elsewhere the octree for all
 // targets.
@@ -235,7 +235,7 @@ int tdb_gen_candidates_dev(tdp_ctx *ctx, const tdp_tile *d1, const tdp_tile *d2,
     if (n_fallback) *n_fallback = 0;
     const bool knn = o->query == TDB_QUERY_NN,
                within = o->query == TDB_QUERY_WITHIN && device_within;
-    if (!ctx || !d1 || !d2 || !(knn || (within && t2->t.n_obj <= 16384)) || N == 0) {
+    if (!ctx || !d1 || !d2 || !(knn || within) || N == 0) {
         tdb_gen_candidates(t1, t2, self, N, o, k_t, tau_t, prefix, ids);
         return TDP_OK;
     }

@@ -493,23 +493,24 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
     const tdb_tile &I2 = self ? idx1 : idx2;
     res->stream[8] = ms_since(t0);
 
-    // candidate lists of all targets, in target order: kNN lists on the
-    // device from the two sides' boxes and proxy points (the octree for the
-    // targets it leaves), within lists from the octree
+    // candidate lists of all targets, in target order: kNN and within lists
+    // on the device from the two sides' boxes and proxy points (the octree
+    // for the targets it leaves, and the within walk's leaves), intersection
+    // lists from the octree
     t0 = Clock::now();
     std::vector<uint32_t> order = morton_order(idx1.t.obj_box, N);
     std::vector<uint32_t> prefix, ids;
     {
         std::vector<uint32_t> p0, i0;
         res->stream[16] = -1;
-        if (o->query == TDB_QUERY_NN) {
+        if (o->query == TDB_QUERY_NN || o->query == TDB_QUERY_WITHIN) {
             tdp_tile *d1 = nullptr, *d2 = nullptr;
             int crc = upload_index(e->ctx, idx1, &d1);
             if (crc == TDP_OK) crc = self ? TDP_OK : upload_index(e->ctx, I2, &d2);
             uint64_t n_fb = 0;
             if (crc == TDP_OK)
                 crc = tdb_gen_candidates_dev(e->ctx, d1, self ? d1 : d2, &idx1, &I2, self, N, o,
-                                             nullptr, nullptr, p0, i0, &n_fb, e->err, false);
+                                             nullptr, nullptr, p0, i0, &n_fb, e->err, true);
             if (d2) tdp_tile_free(d2);
             if (d1) tdp_tile_free(d1);
             if (crc != TDP_OK) {

@@ -138,8 +138,8 @@ int tdb_engine_autotune(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
 int64_t tdb_candidates(const tdb_tile *t1, const tdb_tile *t2, const tdb_join_opts *o,
                        uint32_t *prefix, uint32_t *ids, size_t cap);
 /* the same lists as tdb_join builds them: on the engine's device for kNN
- * (tdp_knn_candidates, tile 2 of any size) and within with tile 2 of at most
- * 16384 objects (tdp_within_candidates), the octree for the targets the
+ * (tdp_knn_candidates) and within (tdp_within_candidates), tile 2 of any
+ * size, the octree for the targets the
  * device leaves to it (*n_fallback of them) and for everything else; returns
  * the number of ids or -1 */
 int64_t tdb_engine_candidates(tdb_engine *e, const tdb_tile *t1, const tdb_tile *t2,
@@ -239,8 +239,8 @@ void tdb_result_stream(const tdb_result *r, double *out12);
  * copy ms including the per-round record puts, then [12] record bytes of the
  * chunks' objects (what shipping every record would copy), [13] record bytes
  * shipped, [14] voxel and object array bytes shipped, [15] workers, [16]
- * targets whose kNN candidate lists came from the host octree (the device
- * builds the others; -1 when the octree built them all), [17] chunk plan ms,
+ * targets whose kNN or within candidate lists came from the host octree (the
+ * device builds the others; -1 when the octree built them all), [17] chunk plan ms,
  * [18] worker set-up ms (contexts, tile slots, pinned staging) */
 int tdb_result_stream_n(const tdb_result *r, double *out, int n);
 

@@ -231,8 +231,10 @@ int tdp_knn_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t n_targets, const
  * depth-first order: leaf_box[6 n_leaves] ({low.xyz, high.xyz}), leaf_off
  * [n_leaves + 1] and leaf_obj[leaf_off[n_leaves]] (each leaf's objects in
  * its order). Lists longer than 256 set fallback[t] = 1 and are left empty
- * for the host octree. Both tiles need obj_box; t2 may hold at most 16384
- * objects (TDP_ERR_ARG otherwise). Host arrays; ids, ids_cap and n_ids as
+ * for the host octree. Both tiles need obj_box; t2 may hold any number of
+ * objects below 2^32 - 1 (beyond 16384 it needs obj_transform as well: its
+ * objects are binned into a grid of cells by proxy point, and each listed
+ * object's place comes from its first leaf on the walk whose box passes). Host arrays; ids, ids_cap and n_ids as
  * for tdp_knn_candidates. */
 int tdp_within_candidates(tdp_ctx *ctx, const tdp_tile *t1, size_t n_targets, const tdp_tile *t2,
                           int self, const float *tau_target, size_t n_leaves,

@@ -396,6 +396,26 @@ def test_device_within_candidates_larger_tile_equal_octree(tdp, eng, many_nuclei
         assert len(hi) > v.n_obj
 
 
+@pytest.mark.parametrize("tau", [0.0, 80.0, 300.0])
+def test_device_within_candidates_grid_equal_octree(tdp, eng, tau):
+    """within lists for tile 2 above 16,384 objects (the grid of cells, each
+    object placed by its first passing leaf on the octree's walk) equal the
+    octree's query_within lists, order included: a self-join of 20,000 small
+    nuclei, and 2,000 other nuclei against them"""
+    a, _ = tdp.generate(seed=12, n_vessels=0, nuclei_per_vessel=20000, cube=8000.0, subdiv=0,
+                        nucleus_r_min=6.0, nucleus_r_max=14.0)
+    b, _ = tdp.generate(seed=13, n_vessels=0, nuclei_per_vessel=2000, cube=9000.0, subdiv=0,
+                        nucleus_r_min=6.0, nucleus_r_max=40.0)
+    assert a.n_obj > 16384
+    for t1, t2 in ((a, None), (b, a)):
+        hp, hi = tdp.candidates(t1, t2, "within", within_dist=tau)
+        dp, di, fb = eng.candidates(t1, t2, "within", within_dist=tau)
+        assert np.array_equal(hp, dp) and np.array_equal(hi, di), (t1.n_obj, tau)
+        assert fb == 0
+        if tau >= 300.0:
+            assert len(hi) > t1.n_obj  # several per target on average
+
+
 @pytest.fixture(scope="module")
 def many_vessels(tdp, tmp_path_factory):
     """300 vessels (an octree of many leaves, long vessels spanning several)

@@ -62,11 +62,12 @@ def test_stream_self_join_equals_join(tdp, nuclei, query, k, tau, overlap, share
     assert got.n_targets == want.n_targets == nuclei.n_obj
     assert len(got.target) > 0
     assert _same(got, want)
-    # kNN candidate lists from the device, within lists from the octree
-    if query == "nn":
-        assert 0 <= st["candidate_host_targets"] < nuclei.n_obj // 50
-    else:
+    # kNN and within candidate lists from the device, intersection lists from
+    # the octree
+    if query == "intersect":
         assert st["candidate_host_targets"] == -1
+    else:
+        assert 0 <= st["candidate_host_targets"] < nuclei.n_obj // 50
     # what went over: the voxel and object arrays plus the records shipped;
     # lazily, only those of the voxels some round read
     assert st["h2d_bytes"] == st["meta_bytes"] + st["record_bytes_shipped"]
