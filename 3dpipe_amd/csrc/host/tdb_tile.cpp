@@ -52,7 +52,7 @@ bool read_raw(const std::string &path, Tile &t, std::string &err) {
     size_t got = sz > 0 ? std::fread(buf.data(), 1, buf.size(), f) : 0;
     std::fclose(f);
     if (sz < 1 || got != buf.size()) { err = "short read: " + path; return false; }
-    if (buf[0] != 1) { err = "not a RAW tile (type byte " + std::to_string((int)buf[0]) + "); COMPRESSED tiles need the PPMC decoder, which this build does not include"; return false; }
+    if (buf[0] != 1) { err = "not a RAW tile (type byte " + std::to_string((int)buf[0]) + "); COMPRESSED tiles need the PPMC decoder, which this library does not include"; return false; }
     t = Tile();
     t.obj_vox_off.push_back(0);
     size_t pos = 1;

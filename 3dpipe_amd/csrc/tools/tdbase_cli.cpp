@@ -114,7 +114,7 @@ int cmd_join(int argc, char **argv) {
         {'l', "lods", true},         {0, "tile1", true},          {0, "tile2", true},
         {0, "max_objects1", true},   {0, "max_objects2", true},   {'q', "query", true},
         {'k', "knn", true},          {'w', "within_dist", true},
-        // additions of this build
+        // options beyond the reference tool's
         {0, "precision", true},      {0, "device", true},         {0, "print_distance", false},
     };
     Parsed p = parse(argc, argv, specs);

@@ -325,7 +325,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the join shards by "
-              f"target objects across GPUs but multi-GPU runs are not part of this build",
+              f"target objects across GPUs but multi-GPU runs are not supported by this benchmark",
               file=sys.stderr)
         sys.exit(2)
 
