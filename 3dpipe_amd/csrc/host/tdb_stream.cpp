@@ -715,6 +715,11 @@ int tdb_stream_join(tdb_engine *e, const tdb_source *src1, const tdb_source *src
                         s.decode_slot(objs[o], slot, out + 12 * off[a], sto);
                         continue;
                     }
+                    if (sto) {  // some voxels: each decoded alone from the object's state
+                        for (size_t k = a; k < b; k++)
+                            s.decode_slot_voxel(objs[o], slot, vox[k] - v0, out + 12 * off[k], sto);
+                        continue;
+                    }
                     const uint64_t base = st.vox_tri_off[5 * (size_t)v0 + slot];
                     const size_t last = 5 * (size_t)(v0 + nv - 1) + slot;
                     tmp.resize(12 * (st.vox_tri_off[last] + st.vox_tri_cnt[last] - base));

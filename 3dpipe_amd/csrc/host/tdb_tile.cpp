@@ -588,11 +588,13 @@ void emit_counts_hd(const Voxelized &vz, double scale, double margin, uint32_t *
 }
 
 // the triangle records of slot s (voxel -> face order)
-void emit_slot(const LodMesh &m, const float *vf, const Voxelized &vz, double scale,
-               double margin, int s, float *rec) {
+// the records of faces first .. first + n - 1 of slot s's record order
+void emit_slot_range(const LodMesh &m, const float *vf, const Voxelized &vz, double scale,
+                     double margin, int s, size_t first, size_t n, float *rec) {
     auto P = [&](int i, int d) { return vf[3 * i + d]; };
     size_t w = 0;
-    for (int i : vz.order[s]) {
+    for (size_t q = first; q < first + n; q++) {
+        const int i = vz.order[s][q];
         const auto &f = m.f[s][i];
         float ph, h;
         face_hd(vz, s, i, scale, margin, ph, h);
@@ -601,6 +603,11 @@ void emit_slot(const LodMesh &m, const float *vf, const Voxelized &vz, double sc
         std::memcpy(rec + 12 * w, r, sizeof r);
         w++;
     }
+}
+
+void emit_slot(const LodMesh &m, const float *vf, const Voxelized &vz, double scale,
+               double margin, int s, float *rec) {
+    emit_slot_range(m, vf, vz, scale, margin, s, 0, vz.order[s].size(), rec);
 }
 
 // Triangle records (slot -> voxel -> face), per-voxel counts and the object
@@ -1056,6 +1063,21 @@ class NucleusSource final : public ObjectSource {
         nucleus_vertices(b_, o, vf);
         emit_slot(b_.protos[k], vf.data(), b_.pvz[k], scale, record_margin(vf, scale), slot, rec);
     }
+    void decode_slot_voxel(size_t o, int slot, uint32_t v, float *rec,
+                           const float *state) const override {
+        if (!state) {
+            ObjectSource::decode_slot_voxel(o, slot, v, rec, state);
+            return;
+        }
+        const int k = b_.inst[o].proto;
+        const Voxelized &vz = b_.pvz[k];
+        size_t first = 0;
+        for (uint32_t u = 0; u < v; u++) first += vz.vcnt[slot][u];
+        double margin;
+        std::memcpy(&margin, state + 3 * b_.protos[k].v.size(), sizeof margin);
+        emit_slot_range(b_.protos[k], state, vz, b_.inst[o].radius, margin, slot, first,
+                        vz.vcnt[slot][v], rec);
+    }
 
   private:
     NucleusBank b_;
@@ -1117,6 +1139,18 @@ class TileSource final : public ObjectSource {
 void ObjectSource::decode_meta(size_t o, const ObjArrays &out, float *) const {
     std::vector<float> rec(12 * n_tri(o));
     decode(o, ObjArrays{out.vox_box, out.cnt, out.hd, rec.data()});
+}
+
+void ObjectSource::decode_slot_voxel(size_t o, int slot, uint32_t v, float *rec,
+                                     const float *state) const {
+    const uint32_t nv = n_vox(o);
+    std::vector<float> box(9 * (size_t)nv), hd(10), all(12 * n_tri(o));
+    std::vector<uint32_t> cnt(5 * (size_t)nv);
+    decode_meta(o, ObjArrays{box.data(), cnt.data(), hd.data(), nullptr});
+    size_t at = 0;
+    for (uint32_t u = 0; u < v; u++) at += cnt[5 * u + slot];
+    decode_slot(o, slot, all.data(), state);
+    std::memcpy(rec, all.data() + 12 * at, 48 * (size_t)cnt[5 * v + slot]);
 }
 
 void ObjectSource::decode_slot(size_t o, int slot, float *rec, const float *) const {

@@ -101,6 +101,10 @@ class ObjectSource {
     // the records of LOD slot `slot` alone, voxel after voxel, as decode
     // writes that slot's part of out.rec (state: from decode_meta, or null)
     virtual void decode_slot(size_t o, int slot, float *rec, const float *state = nullptr) const;
+    // the records of voxel v (of the object's) at LOD slot `slot` alone, as
+    // decode_slot writes that voxel's part
+    virtual void decode_slot_voxel(size_t o, int slot, uint32_t v, float *rec,
+                                   const float *state = nullptr) const;
     virtual size_t state_floats(size_t o) const { return 0; }
 };
 
